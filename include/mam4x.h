@@ -1,0 +1,165 @@
+/*
+ * mam4x.h -- C ABI of the MI355X MAM4 aerosol-microphysics chain.
+ *
+ * The chain is the per-cell sequence of mam4::microphysics::
+ * modal_aero_amicphys_intr (eagles-project/mam4xx, src/mam4xx/
+ * mam4_amicphys.hpp:2119-2376): gas-aerosol exchange (H2SO4 + SOAG), Aitken->
+ * accumulation renaming, new-particle nucleation, coagulation and
+ * primary-carbon aging, followed by the modal size recalculation of
+ * CalcSize::compute_tendencies (calcsize.hpp:962-1218).
+ *
+ * In the reference the chain runs inside one Kokkos team per column, behind
+ * AeroProcess<Impl>::compute_tendencies(team, t, dt, atm, sfc, progs, diags,
+ * tends) (src/validation/aero_process.hpp:113-122).  A GPU kernel cannot be
+ * launched from inside a device team, so this boundary sits one level up: one
+ * host call covers every column of a domain, with the same argument set.  The
+ * team argument becomes a HIP stream.
+ *
+ * Conventions
+ *  - Every function returns an int status (MAM4X_OK = 0); no C++ exception
+ *    crosses this boundary.  mam4x_last_error() describes the last failure.
+ *  - All array arguments are caller-owned DEVICE pointers to double.  A
+ *    "plane" is one field on [nlev][ldc] with the column index fastest; ldc >=
+ *    ncol is the column stride (0 means ncol).  Field tables list one pointer
+ *    per field, so the prognostic fields may live in separate allocations (as
+ *    EAMxx fields do) or in one [field][nlev][ldc] block.
+ *  - Work is enqueued on the given stream (a hipStream_t; NULL = the null
+ *    stream).  Calls are stream-ordered and never synchronize the host,
+ *    except mam4x_validate, which returns a host-side count.
+ *  - One host thread per handle at a time; distinct handles are independent.
+ *    The handle owns its device workspace.
+ */
+#ifndef MAM4X_H
+#define MAM4X_H
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define MAM4X_ABI_VERSION 1
+
+/* status codes */
+#define MAM4X_OK 0
+#define MAM4X_ERR_ARG 1         /* invalid argument (NULL pointer, size)   */
+#define MAM4X_ERR_HIP 2         /* a HIP runtime call failed               */
+#define MAM4X_ERR_UNSUPPORTED 3 /* option not implemented by this build    */
+#define MAM4X_ERR_NOMEM 4       /* device workspace allocation failed      */
+
+/* ---------------------------------------------------------------------- */
+/* State layout: 52 prognostic fields per cell, AeroProcess units.         */
+/*   [0,4)   interstitial number, modes acc, ait, coarse, pca    [#/kg]    */
+/*   [4,8)   cloud-borne number                                  [#/kg]    */
+/*   [8,29)  interstitial species mass, mode-major; species in   [kg/kg]   */
+/*           mode_aero_species order (aero_modes.hpp:214-243):             */
+/*           acc: so4 pom soa bc dst ncl mom; ait: so4 soa ncl mom;        */
+/*           coarse: dst ncl so4 bc pom soa mom; pca: pom bc mom           */
+/*   [29,50) cloud-borne species mass, same order                [kg/kg]   */
+/*   50      H2SO4 gas, 51 SOAG gas                              [kg/kg]   */
+/* Mass <-> molar conversion uses the E3SM molecular weights               */
+/* (gas_chem_mechanism.hpp:31-37) and M_air = 28.966 g/mol; numbers        */
+/* convert as #/kmol = #/kg * 28.966.                                      */
+/* ---------------------------------------------------------------------- */
+#define MAM4X_NSTATE 52
+#define MAM4X_NMODE 4
+#define MAM4X_F_NUM_I 0
+#define MAM4X_F_NUM_C 4
+#define MAM4X_F_Q_I 8
+#define MAM4X_F_Q_C 29
+#define MAM4X_F_H2SO4 50
+#define MAM4X_F_SOAG 51
+
+/* precision of a handle */
+#define MAM4X_PREC_FP64 0  /* everything in FP64 (the drop-in)             */
+#define MAM4X_PREC_MIXED 1 /* FP32 bulk, FP64 kept where stated in DESIGN  */
+
+/* Process switches.  Mirrors microphysics::AmicPhysConfig
+ * (mam4_amicphys.hpp:223-237), Aging's n_so4_monolayers_pcage
+ * (aging.hpp:20-32) and CalcSize::Config (calcsize.hpp:835-846). */
+typedef struct mam4x_config {
+  int do_cond;
+  int do_rename;
+  int do_newnuc;
+  int do_coag;
+  int gaexch_h2so4_uptake_optaa; /* 1 sequential, 2 simultaneous           */
+  int newnuc_h2so4_conc_optaa;
+  unsigned n_so4_monolayers_pcage; /* default 8                            */
+  int do_calcsize;                 /* run calcsize after the chain         */
+  int calcsize_do_aitacc_transfer;
+  int calcsize_do_adjust;
+  int precision; /* MAM4X_PREC_*                                          */
+} mam4x_config;
+
+/* Atmosphere (atmosphere.hpp:16-137) and modal diagnostics the chain reads. */
+typedef struct mam4x_inputs {
+  int ncol;
+  int nlev;
+  int ldc;                            /* column stride of every plane       */
+  const double *state[MAM4X_NSTATE];  /* prognostics, one plane each        */
+  const double *temperature;          /* [K]                                */
+  const double *pressure;             /* mid-level pressure [Pa]            */
+  const double *height;               /* mid-level height above surface [m] */
+  const double *vapor_mixing_ratio;   /* qv [kg/kg]                         */
+  const double *cloud_fraction;       /* [-]                                */
+  const double *pblh;                 /* per column [ncol], PBL height [m]  */
+  const double *dgncur_a[MAM4X_NMODE];    /* dry geo-mean diameter [m]      */
+  const double *dgncur_awet[MAM4X_NMODE]; /* wet geo-mean diameter [m]      */
+  const double *wetdens[MAM4X_NMODE];     /* wet density [kg/m3]            */
+  /* Optional pre-chemistry states (mam4_amicphys.hpp:2136-2137).  NULL
+   * entries mean "equal to the current state", the pure-microphysics step.
+   * pregaschem_h2so4/soag: gases before gas chemistry; precldchem: all 52
+   * fields before cloud chemistry (interstitial and cloud-borne). */
+  const double *pregaschem_h2so4;
+  const double *pregaschem_soag;
+  const double *precldchem[MAM4X_NSTATE];
+} mam4x_inputs;
+
+typedef struct mam4x_outputs {
+  /* d(state)/dt of the whole step [units/s]: (chain result - state)/dt plus
+   * the calcsize tendencies evaluated on the chain result.  Required. */
+  double *tendencies[MAM4X_NSTATE];
+  /* CalcSize diagnostics dry_geometric_mean_diameter_{i,c} (aero_config.hpp:
+   * 209-213).  Optional (NULL to skip). */
+  double *dgncur_i[MAM4X_NMODE];
+  double *dgncur_c[MAM4X_NMODE];
+  /* Optional: state after the step, state + dt * tendencies.  May alias the
+   * input state planes (in-place update). */
+  double *state_out[MAM4X_NSTATE];
+} mam4x_outputs;
+
+typedef struct mam4x_handle_s *mam4x_handle;
+
+/* Library / handle lifetime.  mam4x_create replaces AeroProcess<Impl>'s
+ * constructor + Impl::init (aero_process.hpp:34-43). */
+int mam4x_abi_version(void);
+int mam4x_create(const mam4x_config *config, mam4x_handle *out);
+int mam4x_destroy(mam4x_handle h);
+int mam4x_get_config(mam4x_handle h, mam4x_config *out);
+int mam4x_set_precision(mam4x_handle h, int precision);
+const char *mam4x_last_error(mam4x_handle h);
+int mam4x_default_config(mam4x_config *out);
+
+/* AeroProcess<Impl>::validate (aero_process.hpp:90-95; checks of
+ * GasAerExch::validate, gasaerexch.hpp:96-120): counts cells with T < 0,
+ * p < 0, qv < 0 or any negative prognostic.  Synchronizes the stream and
+ * writes the count to *violations. */
+int mam4x_validate(mam4x_handle h, const mam4x_inputs *in, long long *violations,
+                   void *stream);
+
+/* AeroProcess<Impl>::compute_tendencies (aero_process.hpp:113-122) for all
+ * columns: one chain step of length dt [s] followed by calcsize.  t is the
+ * model time, unused (as in the reference). */
+int mam4x_compute_tendencies(mam4x_handle h, double t, double dt,
+                             const mam4x_inputs *in, const mam4x_outputs *out,
+                             void *stream);
+
+/* Number of SOA sub-step iterations the last compute_tendencies call took,
+ * summed over cells (a cost diagnostic).  Synchronizes the stream.  Returns
+ * MAM4X_ERR_UNSUPPORTED unless the handle was created with stats enabled via
+ * mam4x_enable_stats. */
+int mam4x_enable_stats(mam4x_handle h, int enable);
+int mam4x_last_stats(mam4x_handle h, long long *soa_iterations, void *stream);
+
+#ifdef __cplusplus
+}
+#endif
+#endif /* MAM4X_H */

@@ -1,0 +1,115 @@
+"""ctypes mirror of include/mam4x.h.
+
+The structures below must match the C declarations field for field; the
+``test_abi`` CPU tests check sizes against the header's layout rules.
+"""
+import ctypes
+import os
+
+NSTATE = 52
+NMODE = 4
+
+OK = 0
+ERR_ARG = 1
+ERR_HIP = 2
+ERR_UNSUPPORTED = 3
+ERR_NOMEM = 4
+
+PREC_FP64 = 0
+PREC_MIXED = 1
+
+_dp = ctypes.POINTER(ctypes.c_double)
+_vp = ctypes.c_void_p
+
+
+class Config(ctypes.Structure):
+    _fields_ = [
+        ("do_cond", ctypes.c_int),
+        ("do_rename", ctypes.c_int),
+        ("do_newnuc", ctypes.c_int),
+        ("do_coag", ctypes.c_int),
+        ("gaexch_h2so4_uptake_optaa", ctypes.c_int),
+        ("newnuc_h2so4_conc_optaa", ctypes.c_int),
+        ("n_so4_monolayers_pcage", ctypes.c_uint),
+        ("do_calcsize", ctypes.c_int),
+        ("calcsize_do_aitacc_transfer", ctypes.c_int),
+        ("calcsize_do_adjust", ctypes.c_int),
+        ("precision", ctypes.c_int),
+    ]
+
+
+class Inputs(ctypes.Structure):
+    _fields_ = [
+        ("ncol", ctypes.c_int),
+        ("nlev", ctypes.c_int),
+        ("ldc", ctypes.c_int),
+        ("state", _vp * NSTATE),
+        ("temperature", _vp),
+        ("pressure", _vp),
+        ("height", _vp),
+        ("vapor_mixing_ratio", _vp),
+        ("cloud_fraction", _vp),
+        ("pblh", _vp),
+        ("dgncur_a", _vp * NMODE),
+        ("dgncur_awet", _vp * NMODE),
+        ("wetdens", _vp * NMODE),
+        ("pregaschem_h2so4", _vp),
+        ("pregaschem_soag", _vp),
+        ("precldchem", _vp * NSTATE),
+    ]
+
+
+class Outputs(ctypes.Structure):
+    _fields_ = [
+        ("tendencies", _vp * NSTATE),
+        ("dgncur_i", _vp * NMODE),
+        ("dgncur_c", _vp * NMODE),
+        ("state_out", _vp * NSTATE),
+    ]
+
+
+# entry points declared in include/mam4x.h, with ctypes signatures
+SIGNATURES = {
+    "mam4x_abi_version": (ctypes.c_int, []),
+    "mam4x_default_config": (ctypes.c_int, [ctypes.POINTER(Config)]),
+    "mam4x_create": (ctypes.c_int, [ctypes.POINTER(Config), ctypes.POINTER(_vp)]),
+    "mam4x_destroy": (ctypes.c_int, [_vp]),
+    "mam4x_get_config": (ctypes.c_int, [_vp, ctypes.POINTER(Config)]),
+    "mam4x_set_precision": (ctypes.c_int, [_vp, ctypes.c_int]),
+    "mam4x_last_error": (ctypes.c_char_p, [_vp]),
+    "mam4x_validate": (ctypes.c_int, [_vp, ctypes.POINTER(Inputs),
+                                      ctypes.POINTER(ctypes.c_longlong), _vp]),
+    "mam4x_compute_tendencies": (ctypes.c_int, [_vp, ctypes.c_double, ctypes.c_double,
+                                                ctypes.POINTER(Inputs), ctypes.POINTER(Outputs),
+                                                _vp]),
+    "mam4x_enable_stats": (ctypes.c_int, [_vp, ctypes.c_int]),
+    "mam4x_last_stats": (ctypes.c_int, [_vp, ctypes.POINTER(ctypes.c_longlong), _vp]),
+}
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(HERE, "lib", "libmam4x.so")
+
+_lib = None
+
+
+class LibraryMissing(RuntimeError):
+    pass
+
+
+def load(path=None):
+    """Load libmam4x.so (never falls back to anything else)."""
+    global _lib
+    if _lib is not None and path is None:
+        return _lib
+    p = path or LIB_PATH
+    if not os.path.exists(p):
+        raise LibraryMissing(
+            "%s not built: run __graft_entry__.build() (hipcc, gfx950) first" % p)
+    lib = ctypes.CDLL(p)
+    for name, (res, args) in SIGNATURES.items():
+        fn = getattr(lib, name)
+        fn.restype = res
+        fn.argtypes = args
+    if path is None:
+        _lib = lib
+    return lib

@@ -1,0 +1,144 @@
+"""Python host interface over the C ABI (include/mam4x.h).
+
+``Chain`` owns one library handle; ``compute_tendencies`` takes torch tensors
+resident on the GPU in the ``[field][level][column]`` layout and enqueues the
+step on torch's current stream.  There is no CPU path: the HIP library is
+required, and a missing library is an error.
+"""
+import ctypes
+
+import torch
+
+from . import _abi
+from . import layout as L
+
+
+class ChainError(RuntimeError):
+    pass
+
+
+def _ptr(t):
+    return ctypes.c_void_p(t.data_ptr()) if t is not None else None
+
+
+def _plane_ptrs(t, n):
+    """n plane pointers of a [n][nlev][ncol] tensor."""
+    assert t.dim() == 3 and t.shape[0] == n, t.shape
+    assert t.dtype == torch.float64 and t.is_cuda
+    assert t.stride(2) == 1 and t.stride(1) == t.shape[2]
+    base = t.data_ptr()
+    step = t.stride(0) * 8
+    return [base + i * step for i in range(n)]
+
+
+def default_config(**kw):
+    cfg = _abi.Config()
+    _abi.load().mam4x_default_config(ctypes.byref(cfg))
+    for k, v in kw.items():
+        setattr(cfg, k, v)
+    return cfg
+
+
+class Chain:
+    """MAM4 microphysics chain on one GPU (AeroProcess-style interface)."""
+
+    def __init__(self, config=None, precision=_abi.PREC_FP64, **kw):
+        self.lib = _abi.load()
+        cfg = config or default_config(**kw)
+        cfg.precision = precision
+        h = ctypes.c_void_p()
+        rc = self.lib.mam4x_create(ctypes.byref(cfg), ctypes.byref(h))
+        if rc != _abi.OK:
+            raise ChainError("mam4x_create failed (%d)" % rc)
+        self.h = h
+        self.config = cfg
+
+    def close(self):
+        if getattr(self, "h", None):
+            self.lib.mam4x_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def _check(self, rc, what):
+        if rc != _abi.OK:
+            msg = self.lib.mam4x_last_error(self.h)
+            raise ChainError("%s failed (%d): %s" % (what, rc, msg.decode() if msg else ""))
+
+    @staticmethod
+    def make_inputs(scene, precldchem=None, pregaschem=None):
+        """Build an _abi.Inputs from a dict of CUDA float64 tensors (see
+        scenes.generate for the keys).  Returns (Inputs, keepalive list)."""
+        st = scene["state"]
+        nf, nlev, ncol = st.shape
+        assert nf == L.NSTATE
+        inp = _abi.Inputs()
+        inp.ncol, inp.nlev, inp.ldc = ncol, nlev, ncol
+        for i, p in enumerate(_plane_ptrs(st, L.NSTATE)):
+            inp.state[i] = p
+        for key, fld in (("temperature", "temperature"), ("pressure", "pressure"),
+                         ("height", "height"), ("qv", "vapor_mixing_ratio"),
+                         ("cloud_fraction", "cloud_fraction"), ("pblh", "pblh")):
+            t = scene[key]
+            assert t.dtype == torch.float64 and t.is_cuda and t.is_contiguous()
+            setattr(inp, fld, t.data_ptr())
+        for key in ("dgncur_a", "dgncur_awet", "wetdens"):
+            arr = getattr(inp, key)
+            for m, p in enumerate(_plane_ptrs(scene[key], 4)):
+                arr[m] = p
+        if precldchem is not None:
+            for i, p in enumerate(_plane_ptrs(precldchem, L.NSTATE)):
+                inp.precldchem[i] = p
+        if pregaschem is not None:
+            pg = _plane_ptrs(pregaschem, 2)
+            inp.pregaschem_h2so4, inp.pregaschem_soag = pg[0], pg[1]
+        return inp
+
+    def compute_tendencies(self, scene, dt, tend, dgn_i=None, dgn_c=None, state_out=None,
+                           precldchem=None, pregaschem=None, t=0.0, stream=None):
+        inp = self.make_inputs(scene, precldchem, pregaschem)
+        out = _abi.Outputs()
+        for i, p in enumerate(_plane_ptrs(tend, L.NSTATE)):
+            out.tendencies[i] = p
+        if dgn_i is not None:
+            for m, p in enumerate(_plane_ptrs(dgn_i, 4)):
+                out.dgncur_i[m] = p
+        if dgn_c is not None:
+            for m, p in enumerate(_plane_ptrs(dgn_c, 4)):
+                out.dgncur_c[m] = p
+        if state_out is not None:
+            for i, p in enumerate(_plane_ptrs(state_out, L.NSTATE)):
+                out.state_out[i] = p
+        s = stream if stream is not None else torch.cuda.current_stream()
+        rc = self.lib.mam4x_compute_tendencies(self.h, t, dt, ctypes.byref(inp),
+                                               ctypes.byref(out), ctypes.c_void_p(s.cuda_stream))
+        self._check(rc, "mam4x_compute_tendencies")
+
+    def validate(self, scene, stream=None):
+        inp = self.make_inputs(scene)
+        v = ctypes.c_longlong(0)
+        s = stream if stream is not None else torch.cuda.current_stream()
+        rc = self.lib.mam4x_validate(self.h, ctypes.byref(inp), ctypes.byref(v),
+                                     ctypes.c_void_p(s.cuda_stream))
+        self._check(rc, "mam4x_validate")
+        return int(v.value)
+
+    def enable_stats(self, on=True):
+        self._check(self.lib.mam4x_enable_stats(self.h, int(on)), "mam4x_enable_stats")
+
+    def last_stats(self, stream=None):
+        v = ctypes.c_longlong(0)
+        s = stream if stream is not None else torch.cuda.current_stream()
+        self._check(self.lib.mam4x_last_stats(self.h, ctypes.byref(v),
+                                              ctypes.c_void_p(s.cuda_stream)), "mam4x_last_stats")
+        return int(v.value)
+
+
+def scene_to_device(scene, device="cuda"):
+    """numpy scene dict -> dict of contiguous float64 CUDA tensors."""
+    return {k: torch.as_tensor(v, dtype=torch.float64).contiguous().to(device)
+            for k, v in scene.items()}

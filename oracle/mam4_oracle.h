@@ -1,0 +1,137 @@
+/*
+ * CPU FP64 oracle for the MAM4 aerosol-microphysics chain.
+ *
+ * TEST INFRASTRUCTURE ONLY.  This is a plain-C restatement of the reference
+ * algorithm (eagles-project/mam4xx, src/mam4xx/mam4_amicphys.hpp and the
+ * process headers it calls, plus calcsize.hpp), written to be read side by
+ * side with it.  It is the checker for the HIP library and the CPU baseline
+ * that bench.py times; nothing in the shipped library links or calls it.
+ *
+ * Units inside the chain are the reference's: gases and aerosol masses in
+ * mol/mol-air, numbers in #/kmol-air.  The composed step (oracle_chain_step)
+ * takes the AeroProcess prognostic units (kg/kg, #/kg) and converts.
+ */
+#ifndef MAM4X_ORACLE_H
+#define MAM4X_ORACLE_H
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define ORC_NMODE 4
+#define ORC_NAER 7
+#define ORC_NGAS 2
+#define ORC_GAS_PCNST 31
+#define ORC_NQTENDAA 4
+#define ORC_NSTATE 52
+
+typedef struct {
+  int do_cond;
+  int do_rename;
+  int do_newnuc;
+  int do_coag;
+  int gaexch_h2so4_uptake_optaa;
+  int newnuc_h2so4_conc_optaa;
+  unsigned n_so4_monolayers_pcage;
+  int do_calcsize;
+  int calcsize_do_aitacc_transfer;
+  int calcsize_do_adjust;
+} orc_config;
+
+/* Per-cell chain driver: modal_aero_amicphys_intr (mam4_amicphys.hpp:2119).
+ * qq/qqcw are gas_pcnst(31)-vectors in chain units, updated in place.
+ * qgcm_tendaa / qqcwgcm_tendaa (may be NULL) receive the per-process
+ * tendency diagnostics [31][4] (cond, rename, newnuc, coag) and [31]
+ * (cloud-borne rename), as written to the optional View2Ds at :2365-2374.
+ * stats (may be NULL): [0] SOA sub-step iterations, [1] nsubarea. */
+void orc_amicphys_intr(const orc_config *cfg, double deltat, double temp,
+                       double pmid, double pdel, double zm, double pblh,
+                       double qv, double cld, double qq[31], double qqcw[31],
+                       const double q_pregaschem[31],
+                       const double q_precldchem[31],
+                       const double qqcw_precldchem[31],
+                       const double dgncur_a[4], const double dgncur_awet[4],
+                       const double wetdens_host[4], double *qgcm_tendaa,
+                       double *qqcwgcm_tendaa, int stats[2]);
+
+/* Per-level calcsize: the body of CalcSize::compute_tendencies
+ * (calcsize.hpp:962-1218) for one level k.  Inputs are prognostics in the
+ * AeroProcess layout (n[mode], q[mode][species-in-mode]); tendencies must be
+ * zeroed by the caller (the reference accumulates into them). */
+void orc_calcsize_level(const orc_config *cfg, double dt, const double n_i[4],
+                        const double n_c[4], const double q_i[4][7],
+                        const double q_c[4][7], double dn_i[4], double dn_c[4],
+                        double dq_i[4][7], double dq_c[4][7], double dgn_i[4],
+                        double dgn_c[4]);
+
+/* Composed microphysics step on [field][level][column] buffers (column
+ * fastest, leading dimension ncol).  state: ORC_NSTATE fields in kg/kg and
+ * #/kg, ordered as documented in include/mam4x.h.  Outputs: tend (ORC_NSTATE
+ * fields, per second), dgn_i/dgn_c (4 fields each).  Optional extended
+ * pre-chemistry inputs (NULL = use the current state).  nthreads <= 0: use
+ * the OpenMP default.  Returns total SOA sub-step iterations. */
+long orc_chain_step(const orc_config *cfg, int ncol, int nlev, double dt,
+                    const double *state, const double *temperature,
+                    const double *pressure, const double *height,
+                    const double *qv, const double *cloud_fraction,
+                    const double *pblh, const double *dgncur_a,
+                    const double *dgncur_awet, const double *wetdens,
+                    const double *pregaschem_gas, const double *precldchem,
+                    double *tend, double *dgn_i, double *dgn_c, int nthreads);
+
+/* ---- process-level entry points used by the known-answer tests ---- */
+
+/* H2SO4 non-volatile analytic update (gasaerexch.hpp:479-537; identical math
+ * to mam_gasaerexch_1subarea_1gas_nonvolatile, gasaerexch.hpp:139-218). */
+void orc_nonvolatile_update(double dt, double netprod, const double uptkaer[4],
+                            double *qgas_cur, double *qgas_avg,
+                            double qaer_cur[4]);
+
+/* Coagulation coefficient wrapper getcoags_wrapper_f (coagulation.hpp:751). */
+void orc_getcoags_wrapper(double airtemp, double airprs, double dgatk,
+                          double dgacc, double sgatk, double sgacc,
+                          double xxlsgat, double xxlsgac, double pdensat,
+                          double pdensac, double out[4]);
+double orc_bm0ij(int n1, int n2a, int n2n);
+double orc_bm3i(int n1, int n2a, int n2n);
+
+/* Vehkamaki (2002) eqs 11-15 (vehkamaki2002.hpp:53-188). */
+double orc_vehk_xcrit(double c_h2so4, double temp, double rh);
+double orc_vehk_logj(double c_h2so4, double temp, double rh, double xcrit);
+double orc_vehk_ntot(double c_h2so4, double temp, double rh, double xcrit);
+double orc_vehk_radius(double xcrit, double ntot);
+double orc_vehk_threshold(double temp, double rh);
+
+/* Nucleation box model mer07_veh02_nuc_mosaic_1box (nucleation.hpp:346). */
+void orc_nuc_mosaic_1box(int flag, double dtnuc, double temp, double rh,
+                         double press, double zm, double pblh,
+                         double qh2so4_cur, double qh2so4_avg, double qnh3_cur,
+                         double h2so4_uptkrate, double mw_so4a_host, int nsize,
+                         double dp_lo, double dp_hi, double rgas, double avogad,
+                         double mw_nh4a, double mw_so4a, double pi,
+                         double out[8]);
+
+/* Aging transfer helpers (aging.hpp:166-231), arrays of length 4. */
+void orc_transfer_aged_pcarbon_to_accum(int nsrc, int ndest, double xferfrac,
+                                        double frac_cond, double frac_coag,
+                                        double q[4], double dcond[4],
+                                        double dcoag[4]);
+void orc_transfer_cond_coag_mass_to_accum(int nsrc, int ndest, double q[4],
+                                          double dcond[4], double dcoag[4]);
+
+/* Full pcarbon aging of one subarea (aging.hpp:238-358). */
+void orc_pcarbon_aging_1subarea(unsigned nmono, const double dgn_a[4],
+                                double qnum[4], double qnum_del_cond[4],
+                                double qnum_del_coag[4], double qaer[7][4],
+                                double qaer_del_cond[7][4],
+                                double qaer_del_coag[7][4],
+                                double qaer_del_coag_in[7][1]);
+
+/* Lognormal conversions (conversions.hpp:299-317). */
+double orc_mean_particle_diameter_from_volume(double vol, double sigma);
+double orc_mean_particle_volume_from_diameter(double dgn, double sigma);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

@@ -1,0 +1,127 @@
+"""ctypes binding and build recipe for the C oracle (test infrastructure)."""
+import ctypes
+import os
+import subprocess
+
+import numpy as np
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+SRC = os.path.join(HERE, "mam4_oracle.c")
+LIB = os.path.join(HERE, "libmam4x_oracle.so")
+
+_dp = ctypes.POINTER(ctypes.c_double)
+
+
+def lib_path():
+    return LIB
+
+
+def build(force=True):
+    """Compile the oracle with gcc (-O3, OpenMP, no FMA contraction)."""
+    if not force and os.path.exists(LIB) and os.path.getmtime(LIB) >= os.path.getmtime(SRC):
+        return LIB
+    cmd = ["gcc", "-O3", "-std=gnu11", "-fopenmp", "-fPIC", "-shared", "-ffp-contract=off",
+           "-Wall", "-Wno-unused-variable", "-o", LIB + ".tmp", SRC, "-lm"]
+    subprocess.run(cmd, check=True)
+    os.replace(LIB + ".tmp", LIB)
+    return LIB
+
+
+class OrcConfig(ctypes.Structure):
+    _fields_ = [
+        ("do_cond", ctypes.c_int),
+        ("do_rename", ctypes.c_int),
+        ("do_newnuc", ctypes.c_int),
+        ("do_coag", ctypes.c_int),
+        ("gaexch_h2so4_uptake_optaa", ctypes.c_int),
+        ("newnuc_h2so4_conc_optaa", ctypes.c_int),
+        ("n_so4_monolayers_pcage", ctypes.c_uint),
+        ("do_calcsize", ctypes.c_int),
+        ("calcsize_do_aitacc_transfer", ctypes.c_int),
+        ("calcsize_do_adjust", ctypes.c_int),
+    ]
+
+
+def default_config(**kw):
+    c = dict(do_cond=1, do_rename=1, do_newnuc=1, do_coag=1, gaexch_h2so4_uptake_optaa=2,
+             newnuc_h2so4_conc_optaa=2, n_so4_monolayers_pcage=8, do_calcsize=1,
+             calcsize_do_aitacc_transfer=1, calcsize_do_adjust=1)
+    c.update(kw)
+    return OrcConfig(**c)
+
+
+def _p(a):
+    if a is None:
+        return None
+    assert a.dtype == np.float64 and a.flags["C_CONTIGUOUS"]
+    return a.ctypes.data_as(_dp)
+
+
+class Oracle:
+    """Loads the oracle shared library (building it when missing)."""
+
+    def __init__(self, path=None):
+        path = path or LIB
+        if not os.path.exists(path):
+            build()
+        self.lib = ctypes.CDLL(path)
+        L = self.lib
+        L.orc_chain_step.restype = ctypes.c_long
+        L.orc_chain_step.argtypes = [ctypes.POINTER(OrcConfig), ctypes.c_int, ctypes.c_int,
+                                     ctypes.c_double] + [_dp] * 15 + [ctypes.c_int]
+        L.orc_nonvolatile_update.argtypes = [ctypes.c_double, ctypes.c_double, _dp, _dp, _dp, _dp]
+        L.orc_getcoags_wrapper.argtypes = [ctypes.c_double] * 10 + [_dp]
+        for fn in ("orc_bm0ij", "orc_bm3i"):
+            getattr(L, fn).restype = ctypes.c_double
+            getattr(L, fn).argtypes = [ctypes.c_int] * 3
+        for fn, n in (("orc_vehk_xcrit", 3), ("orc_vehk_logj", 4), ("orc_vehk_ntot", 4),
+                      ("orc_vehk_radius", 2), ("orc_vehk_threshold", 2),
+                      ("orc_mean_particle_diameter_from_volume", 2),
+                      ("orc_mean_particle_volume_from_diameter", 2)):
+            getattr(L, fn).restype = ctypes.c_double
+            getattr(L, fn).argtypes = [ctypes.c_double] * n
+        L.orc_nuc_mosaic_1box.argtypes = ([ctypes.c_int] + [ctypes.c_double] * 11 + [ctypes.c_int]
+                                          + [ctypes.c_double] * 7 + [_dp])
+        L.orc_transfer_aged_pcarbon_to_accum.argtypes = [ctypes.c_int, ctypes.c_int] + \
+            [ctypes.c_double] * 3 + [_dp] * 3
+        L.orc_transfer_cond_coag_mass_to_accum.argtypes = [ctypes.c_int, ctypes.c_int] + [_dp] * 3
+        L.orc_pcarbon_aging_1subarea.argtypes = [ctypes.c_uint] + [_dp] * 8
+        L.orc_amicphys_intr.argtypes = [ctypes.POINTER(OrcConfig)] + [ctypes.c_double] * 8 + \
+            [_dp] * 10 + [_dp, _dp, ctypes.POINTER(ctypes.c_int)]
+        L.orc_calcsize_level.argtypes = [ctypes.POINTER(OrcConfig), ctypes.c_double] + [_dp] * 12
+
+    def chain_step(self, scene, dt=1800.0, config=None, nthreads=0, pregaschem_gas=None,
+                   precldchem=None):
+        """Run the composed step on a scene dict; returns (tend, dgn_i, dgn_c, soa_iters)."""
+        cfg = config or default_config()
+        st = np.ascontiguousarray(scene["state"], dtype=np.float64)
+        nf, nlev, ncol = st.shape
+        tend = np.empty_like(st)
+        dgn_i = np.empty((4, nlev, ncol))
+        dgn_c = np.empty((4, nlev, ncol))
+        arrs = [np.ascontiguousarray(scene[k], dtype=np.float64) for k in
+                ("temperature", "pressure", "height", "qv", "cloud_fraction", "pblh",
+                 "dgncur_a", "dgncur_awet", "wetdens")]
+        pg = None if pregaschem_gas is None else np.ascontiguousarray(pregaschem_gas)
+        pc = None if precldchem is None else np.ascontiguousarray(precldchem)
+        iters = self.lib.orc_chain_step(ctypes.byref(cfg), ncol, nlev, dt, _p(st),
+                                        *[_p(a) for a in arrs], _p(pg), _p(pc), _p(tend),
+                                        _p(dgn_i), _p(dgn_c), int(nthreads))
+        return tend, dgn_i, dgn_c, int(iters)
+
+    # ---- process-level helpers for the known-answer tests ----
+    def nonvolatile_update(self, dt, netprod, uptkaer, qgas_cur, qaer_cur):
+        up = np.ascontiguousarray(uptkaer, dtype=np.float64)
+        qa = np.array(qaer_cur, dtype=np.float64)
+        qg = ctypes.c_double(qgas_cur)
+        qavg = ctypes.c_double(0.0)
+        self.lib.orc_nonvolatile_update(dt, netprod, _p(up), ctypes.byref(qg), ctypes.byref(qavg),
+                                        _p(qa))
+        return qg.value, qavg.value, qa
+
+    def getcoags(self, T, p, dgatk, dgacc, sgatk, sgacc, rhoat, rhoac):
+        import math
+        out = np.zeros(4)
+        self.lib.orc_getcoags_wrapper(T, p, dgatk, dgacc, sgatk, sgacc, math.log(sgatk),
+                                      math.log(sgacc), rhoat, rhoac, _p(out))
+        return out
