@@ -18,6 +18,9 @@ HIP_FLAGS = [
     "-O3", "-std=c++17", "--offload-arch=gfx950", "-fPIC", "-shared",
     # packed-FP32 VALU forms are kept out of every kernel
     "-Xclang", "-target-feature", "-Xclang", "-packed-fp32-ops",
+    # the FP64 drop-in performs the reference's IEEE operations one by one
+    # (no a*b+c fusion), so it differs from the CPU build only in libm bits
+    "-ffp-contract=off",
     "-Wall", "-Wno-unused-variable", "-Wno-unused-but-set-variable",
 ]
 

@@ -128,7 +128,10 @@ static void fill_args(mam4x_handle h, const mam4x_inputs *in, ChainArgs &a) {
   a.cfg.do_rename = c.do_rename;
   a.cfg.do_newnuc = c.do_newnuc;
   a.cfg.do_coag = c.do_coag;
-  a.cfg.gaexch_h2so4_uptake_optaa = c.gaexch_h2so4_uptake_optaa;
+  // QUIRK (mam4_amicphys.hpp:1129 vs :1914): the two optaa switches arrive
+  // swapped in mam_amicphys_1subarea, so the H2SO4 production treatment
+  // follows newnuc_h2so4_conc_optaa.
+  a.cfg.gaexch_h2so4_uptake_optaa = c.newnuc_h2so4_conc_optaa;
   a.cfg.nmono = c.n_so4_monolayers_pcage;
   a.do_calcsize = c.do_calcsize;
   a.do_xfer = c.calcsize_do_aitacc_transfer;

@@ -31,6 +31,39 @@ static inline double min_max_bound(double lo, double hi, double x) {
   return kmax(lo, kmin(hi, x));
 }
 
+
+/* ---------------- libm-rounding noise (conditioning probe) ----------------
+ * When enabled (orc_set_libm_noise(seed != 0)), every exp/log/pow/cbrt/erfc/
+ * sqrt result is moved by -1, 0 or +1 ulp at random (per-cell seeded xorshift).
+ * Running the oracle under a few seeds measures, cell by cell, how far the
+ * chain's outputs move when only the last bit of its transcendental
+ * functions changes -- the spread two correct libm implementations (glibc on
+ * the host, the GPU's device libm) can legitimately produce. */
+static unsigned long long g_noise_seed = 0;
+static _Thread_local unsigned long long t_rng = 0;
+void orc_set_libm_noise(unsigned long long seed) { g_noise_seed = seed; }
+static void noise_cell(long cell) {
+  unsigned long long z = g_noise_seed + 0x9E3779B97F4A7C15ULL * (unsigned long long)(cell + 1);
+  z = (z ^ (z >> 30)) * 0xBF58476D1CE4E5B9ULL;
+  z = (z ^ (z >> 27)) * 0x94D049BB133111EBULL;
+  t_rng = (z ^ (z >> 31)) | 1ULL;
+}
+static inline double nz(double v) {
+  if (!g_noise_seed) return v;
+  t_rng ^= t_rng << 13;
+  t_rng ^= t_rng >> 7;
+  t_rng ^= t_rng << 17;
+  const unsigned r = (unsigned)(t_rng % 3ULL);
+  if (r == 0 || !isfinite(v)) return v;
+  return nextafter(v, r == 1 ? INFINITY : -INFINITY);
+}
+#define M_exp(x) nz(exp(x))
+#define M_log(x) nz(log(x))
+#define M_pow(x, y) nz(pow((x), (y)))
+#define M_cbrt(x) nz(cbrt(x))
+#define M_erfc(x) nz(erfc(x))
+#define M_sqrt(x) nz(sqrt(x))
+
 /* ---------------- constants (mam4_constants.hpp, aero_modes.hpp) -------- */
 static const double PI = 3.14159265358979323846264;
 #define PI_SIXTH (PI / 6)
@@ -114,19 +147,19 @@ static const double MASS_2_VOL[7] = {0.15,
 /* conversions.hpp:299-305 */
 double orc_mean_particle_diameter_from_volume(double vol, double sigma) {
   const double pio6 = PI_SIXTH;
-  return cbrt(vol / pio6) * exp(-1.5 * sq(log(sigma)));
+  return M_cbrt(vol / pio6) * M_exp(-1.5 * sq(M_log(sigma)));
 }
 /* conversions.hpp:312-317 */
 double orc_mean_particle_volume_from_diameter(double dgn, double sigma) {
   const double pio6 = PI_SIXTH;
-  return cube(dgn) * exp(4.5 * sq(log(sigma))) * pio6;
+  return cube(dgn) * M_exp(4.5 * sq(M_log(sigma))) * pio6;
 }
 /* conversions.hpp:148-180 */
 static double vapor_saturation_pressure_magnus(double T, double P) {
   const double e0 = 610.94, exp_num = 17.625, exp_den = 234.04;
   const double celsius = T - FREEZING_PT_H2O;
-  const double ew = e0 * exp(exp_num * celsius / (exp_den + celsius));
-  return 1.00071 * exp(4.5e-8 * P) * ew;
+  const double ew = e0 * M_exp(exp_num * celsius / (exp_den + celsius));
+  return 1.00071 * M_exp(4.5e-8 * P) * ew;
 }
 
 /* ======================= subarea set-up (mam4_amicphys.hpp) ============== */
@@ -337,13 +370,13 @@ static void set_subarea_gases_and_aerosols(
 /* gasaerexch.hpp:222-236 */
 static double gas_diffusivity(double T, double p_atm, double mw, double vd) {
   const double onethird = 1.0 / 3.0;
-  const double dgas = (1.0e-3 * pow(T, 1.75) * sqrt(1. / mw + 0.035)) /
-                      (p_atm * pow((pow(vd, onethird) + 2.7189), 2.0));
+  const double dgas = (1.0e-3 * M_pow(T, 1.75) * M_sqrt(1. / mw + 0.035)) /
+                      (p_atm * M_pow((M_pow(vd, onethird) + 2.7189), 2.0));
   return dgas * 1.0e-4;
 }
 /* gasaerexch.hpp:240-250 */
 static double mean_molecular_speed(double temp, double rmw) {
-  return 145.5 * sqrt(temp / rmw);
+  return 145.5 * M_sqrt(temp / rmw);
 }
 /* gasaerexch.hpp:253-265 */
 static double fuchs_sutugin(double D_p, double gasfreepath, double a283,
@@ -356,14 +389,14 @@ static void gas_aer_uptkrates_1box1gas(double accom, double gasdiffus,
                                        double gasfreepath, double beta_inp,
                                        const double dgncur_awet[4],
                                        const double lnsg[4], double uptkaer[4]) {
-  const double tworootpi = 2 * sqrt(PI);
-  const double root2 = sqrt(2.0);
+  const double tworootpi = 2 * M_sqrt(PI);
+  const double root2 = M_sqrt(2.0);
   const double xghq[2] = {7.071067690849304E-01, -7.071067690849304E-01};
   const double wghq[2] = {8.862269520759583E-01, 8.862269520759583E-01};
   const double a283 = accom * 0.283;
   const double a75 = accom * 0.75;
   for (int n = 0; n < 4; ++n) {
-    const double lndpgn = log(dgncur_awet[n]);
+    const double lndpgn = M_log(dgncur_awet[n]);
     double beta = 0;
     if (fabs(beta_inp - 1.5) > 0.5) {
       const double D_p = dgncur_awet[n];
@@ -376,13 +409,13 @@ static void gas_aer_uptkrates_1box1gas(double accom, double gasdiffus,
       beta = beta_inp;
     }
     const double constant =
-        tworootpi * exp(beta * lndpgn + 0.5 * pow(beta * lnsg[n], 2.0));
+        tworootpi * M_exp(beta * lndpgn + 0.5 * M_pow(beta * lnsg[n], 2.0));
     double sumghq = 0.0;
     for (int iq = 0; iq < 2; ++iq) {
       const double lndp = lndpgn + beta * lnsg[n] * lnsg[n] + root2 * lnsg[n] * xghq[iq];
-      const double D_p = exp(lndp);
+      const double D_p = M_exp(lndp);
       const double hh = fuchs_sutugin(D_p, gasfreepath, a283, a75);
-      sumghq += wghq[iq] * D_p * hh / pow(D_p, beta);
+      sumghq += wghq[iq] * D_p * hh / M_pow(D_p, beta);
     }
     uptkaer[n] = constant * gasdiffus * sumghq;
   }
@@ -421,7 +454,7 @@ static int mam_soaexch_1subarea(double dtsubstep, double temp, double pmid,
   memset(qaer_prv, 0, sizeof(qaer_prv));
 
   const double p0_soa =
-      p0_soa_298 * exp(-(delh_vap_soa / rgas) * ((1.0 / temp) - (1.0 / 298.0)));
+      p0_soa_298 * M_exp(-(delh_vap_soa / rgas) * ((1.0 / temp) - (1.0 / 298.0)));
   const double g0_soa = 1.01325e5 * p0_soa / pmid;
   qgas_avg[0] = 0.0;
   dtsum_qgas_avg = 0.0;
@@ -534,8 +567,8 @@ void orc_nonvolatile_update(double dt, double netprod, const double uptkaer[4],
     double tmp_q3, tmp_q4;
     if (tmp_kxt > 0.001) {
       const double tmp_pok = tmp_pxt / tmp_kxt;
-      tmp_q3 = (tmp_q1 - tmp_pok) * exp(-tmp_kxt) + tmp_pok;
-      tmp_q4 = (tmp_q1 - tmp_pok) * (1.0 - exp(-tmp_kxt)) / tmp_kxt + tmp_pok;
+      tmp_q3 = (tmp_q1 - tmp_pok) * M_exp(-tmp_kxt) + tmp_pok;
+      tmp_q4 = (tmp_q1 - tmp_pok) * (1.0 - M_exp(-tmp_kxt)) / tmp_kxt + tmp_pok;
     } else {
       const double k2 = tmp_kxt * tmp_kxt;
       tmp_q3 = tmp_q1 * (1.0 - tmp_kxt + k2 * 0.5) +
@@ -568,7 +601,7 @@ static int mam_gasaerexch_1subarea(int jtsubstep, double dtsubstep, double temp,
                                    double *uptkrate_h2so4) {
   const int mode_aging_optaa[4] = {0, 0, 0, 1};
   double alnsg_aer[4];
-  for (int m = 0; m < 4; ++m) alnsg_aer[m] = log(MODE_SIGMA[m]);
+  for (int m = 0; m < 4; ++m) alnsg_aer[m] = M_log(MODE_SIGMA[m]);
   const double mw_gas[2] = {1.500000000000000E+02, 9.807840000000000E+01};
   const double vol_molar_gas[2] = {6.563265306122449E+01, 4.288000000000000E+01};
   const double accom_coef_gas[2] = {6.5000E-01, 6.5000E-01};
@@ -612,7 +645,7 @@ static int mam_gasaerexch_1subarea(int jtsubstep, double dtsubstep, double temp,
 
 /* vehkamaki2002.hpp:53-66 */
 double orc_vehk_xcrit(double c, double t, double rh) {
-  const double lnN = log(c), lnrh = log(rh);
+  const double lnN = M_log(c), lnrh = M_log(rh);
   return 0.740997 - 0.00266379 * t - 0.00349998 * lnN + 0.0000504022 * t * lnN +
          0.00201048 * lnrh - 0.000183289 * t * lnrh + 0.00157407 * sq(lnrh) -
          0.0000179059 * t * sq(lnrh) + 0.000184403 * cube(lnrh) -
@@ -630,7 +663,7 @@ double orc_vehk_logj(double c, double t, double rh, double x) {
   const double h = 9.71682 - 0.115048 * t + 0.000157098 * sq(t) + 4.00914e-7 * cube(t) + 0.71186 / x;
   const double i = -1.05611 + 0.00903378 * t - 0.0000198417 * sq(t) + 2.46048e-8 * cube(t) - 0.0579087 / x;
   const double j = -0.148712 + 0.00283508 * t - 9.24619e-6 * sq(t) + 5.00427e-9 * cube(t) - 0.0127081 / x;
-  const double lnN = log(c), lnrh = log(rh);
+  const double lnN = M_log(c), lnrh = M_log(rh);
   return a + b * lnrh + cc * sq(lnrh) + d * cube(lnrh) + e * lnN +
          f * lnrh * lnN + g * sq(lnrh) * lnN + h * sq(lnN) +
          i * lnrh * sq(lnN) + j * cube(lnN);
@@ -647,21 +680,21 @@ double orc_vehk_ntot(double c, double t, double rh, double x) {
   const double H = -0.0619974 + 0.000906958 * t - 9.11728e-7 * sq(t) - 5.36796e-9 * cube(t) - 0.00774234 / x;
   const double I = +0.0121827 - 0.00010665 * t + 2.5346e-7 * sq(t) - 3.63519e-10 * cube(t) + 0.000610065 / x;
   const double J = +0.000320184 - 0.0000174762 * t + 6.06504e-8 * sq(t) - 1.4177e-11 * cube(t) + 0.000135751 / x;
-  const double lnN = log(c), lnrh = log(rh);
-  return exp(A + B * lnrh + C * sq(lnrh) + D * cube(lnrh) + E * lnN +
+  const double lnN = M_log(c), lnrh = M_log(rh);
+  return M_exp(A + B * lnrh + C * sq(lnrh) + D * cube(lnrh) + E * lnN +
              F * lnrh * lnN + G * sq(lnrh) * lnN + H * sq(lnN) +
              I * lnrh * sq(lnN) + J * cube(lnN));
 }
 /* vehkamaki2002.hpp:175-179 */
 double orc_vehk_radius(double x, double ntot) {
-  return exp(-1.6524245 + 0.42316402 * x + 0.3346648 * log(ntot));
+  return M_exp(-1.6524245 + 0.42316402 * x + 0.3346648 * M_log(ntot));
 }
 /* vehkamaki2002.hpp:186-193: eq 15, H2SO4 [cm-3] giving J = 1 cm-3 s-1 */
 double orc_vehk_threshold(double t, double rh) {
-  return exp(-279.243 + 11.7344 * rh + 22700.9 / t - 1088.64 * rh / t +
+  return M_exp(-279.243 + 11.7344 * rh + 22700.9 / t - 1088.64 * rh / t +
              1.14436 * t - 0.0302331 * rh * t - 0.00130254 * sq(t) -
-             6.38697 * log(rh) + 854.98 * log(rh) / t +
-             0.00879662 * t * log(rh));
+             6.38697 * M_log(rh) + 854.98 * M_log(rh) / t +
+             0.00879662 * t * M_log(rh));
 }
 
 /* nucleation.hpp:127-165 */
@@ -671,7 +704,7 @@ static void binary_nuc_vehk2002(double temp, double rh, double so4vol,
                                 double *radius_cluster) {
   const double x = orc_vehk_xcrit(so4vol, temp, rh);
   *rateloge = orc_vehk_logj(so4vol, temp, rh, x);
-  *ratenucl = exp(kmin(*rateloge, log(1e38)));
+  *ratenucl = M_exp(kmin(*rateloge, M_log(1e38)));
   *cnum_tot = orc_vehk_ntot(so4vol, temp, rh, x);
   *cnum_h2so4 = *cnum_tot * x;
   *radius_cluster = orc_vehk_radius(x, *cnum_tot);
@@ -694,7 +727,7 @@ static void pbl_nuc_wang2008(double so4vol, double pi, int choice,
   else
     return;
   tmp_ratenucl = tmp_ratenucl * adjust;
-  const double tmp_rateloge = log(kmax(1.0e-38, tmp_ratenucl));
+  const double tmp_rateloge = M_log(kmax(1.0e-38, tmp_ratenucl));
   if (tmp_rateloge <= *rateloge) return;
   *actual = choice;
   *rateloge = tmp_rateloge;
@@ -739,7 +772,7 @@ void orc_nuc_mosaic_1box(int flag, double dtnuc, double temp_in, double rh_in,
   const double so4vol_in = qh2so4_avg * cair * avogad * 1.0e-6;
   const double nh3ppt = qnh3_cur * 1.0e12;
   double ratenuclt = 1.0e-38;
-  double rateloge = log(ratenuclt);
+  double rateloge = M_log(ratenuclt);
   double cnum_h2so4 = 0.0, cnum_tot = 0.0, radius_cluster = 0.0, cnum_nh3 = 0.0;
   int flag2 = -1;
   if ((flag != 2) && (nh3ppt >= 0.1)) {
@@ -755,7 +788,7 @@ void orc_nuc_mosaic_1box(int flag, double dtnuc, double temp_in, double rh_in,
     cnum_nh3 = 0.0;
     flag2 = 2;
   }
-  rateloge = rateloge + log(kmax(1.0e-38, adjust_bin));
+  rateloge = rateloge + M_log(kmax(1.0e-38, adjust_bin));
   if ((flag == 11) || (flag == 12)) {
     if (zm_in <= kmax(pblh_in, 100.0)) {
       pbl_nuc_wang2008(so4vol_in, pi, flag, adjust_pbl, &flag2, &ratenuclt,
@@ -768,15 +801,15 @@ void orc_nuc_mosaic_1box(int flag, double dtnuc, double temp_in, double rh_in,
     ORC_NUC_OUT();
     return;
   }
-  ratenuclt = exp(rateloge);
+  ratenuclt = M_exp(rateloge);
   const double ratenuclt_bb = ratenuclt * 1.0e6;
   dnclusterdt = ratenuclt_bb;
   double tmpa = kmax(0.10, kmin(0.95, rh_in));
-  const double wetvol_dryvol = 1.0 - 0.56 / log(tmpa);
+  const double wetvol_dryvol = 1.0 - 0.56 / M_log(tmpa);
   double voldry_clus = (kmax(cnum_h2so4, 1.0) * mw_so4a + cnum_nh3 * mw_nh4a) /
                        (1.0e3 * dens_sulfacid * avogad);
   voldry_clus = voldry_clus * (mw_so4a_host / mw_so4a);
-  const double dpdry_clus = pow(voldry_clus * 6.0 / pi, onethird);
+  const double dpdry_clus = M_pow(voldry_clus * 6.0 / pi, onethird);
   int igrow;
   isize_nuc = 1;
   double dpdry_part = dp_lo_sect;
@@ -797,7 +830,7 @@ void orc_nuc_mosaic_1box(int flag, double dtnuc, double temp_in, double rh_in,
       }
     }
   }
-  const double voldry_part = (pi / 6.0) * pow(dpdry_part, 3);
+  const double voldry_part = (pi / 6.0) * M_pow(dpdry_part, 3);
   double n1 = 0.0, n2 = 0.0, n3 = 0.0;
   if (igrow <= 0) {
     n1 = 0.0; n2 = 0.0; n3 = 1.0;
@@ -826,22 +859,22 @@ void orc_nuc_mosaic_1box(int flag, double dtnuc, double temp_in, double rh_in,
   if (igrow <= 0) {
     factor_kk = 1.0;
   } else {
-    const double tmp_spd = 14.7 * sqrt(temp_in);
+    const double tmp_spd = 14.7 * M_sqrt(temp_in);
     const double gr_kk = 3.0e-9 * tmp_spd * mw_sulfacid * so4vol_in /
                          (dens_part * wet_volfrac_so4a);
-    const double dfin_kk = 1.0e9 * dpdry_part * pow(wetvol_dryvol, onethird);
+    const double dfin_kk = 1.0e9 * dpdry_part * M_pow(wetvol_dryvol, onethird);
     double dnuc_kk = 2.0 * radius_cluster;
     dnuc_kk = kmax(dnuc_kk, 1.0);
-    const double gamma_kk = 0.23 * pow(dnuc_kk, 0.2) * pow(dfin_kk / 3.0, 0.075) *
-                            pow(dens_part * 1.0e-3, -0.33) *
-                            pow(temp_in / 293.0, -0.75);
+    const double gamma_kk = 0.23 * M_pow(dnuc_kk, 0.2) * M_pow(dfin_kk / 3.0, 0.075) *
+                            M_pow(dens_part * 1.0e-3, -0.33) *
+                            M_pow(temp_in / 293.0, -0.75);
     tmpa = h2so4_uptkrate * 3600.0;
     tmpa = kmax(tmpa, 0.0);
-    tmpb = 6.7037e-6 * pow(temp_in, 0.75) / cair;
+    tmpb = 6.7037e-6 * M_pow(temp_in, 0.75) / cair;
     tmpb = tmpb * 3600.0;
     const double cs_prime_kk = tmpa / (4.0 * pi * tmpb * accom_coef_h2so4);
     const double nu_kk = gamma_kk * cs_prime_kk / gr_kk;
-    factor_kk = exp((nu_kk / dfin_kk) - (nu_kk / dnuc_kk));
+    factor_kk = M_exp((nu_kk / dfin_kk) - (nu_kk / dnuc_kk));
   }
   const double ratenuclt_kk = ratenuclt_bb * factor_kk;
   tmpa = kmax(0.0, (ratenuclt_kk * dtnuc * mass_part));
@@ -884,12 +917,12 @@ static void mam_newnuc_1subarea(double deltat, double temp, double pmid,
   const double tmp_uptkrate = uptkrate_h2so4;
   if (qh2so4_avg <= qh2so4_cutoff) return;
   const double qnh3_cur = 0;
-  const double dplom = exp(0.67 * log(MODE_DMIN[NAIT]) + 0.33 * log(MODE_DNOM[NAIT]));
+  const double dplom = M_exp(0.67 * M_log(MODE_DMIN[NAIT]) + 0.33 * M_log(MODE_DNOM[NAIT]));
   const double dphim = MODE_DMAX[NAIT];
   const double dens_so4a_host = 1770;
   double tmpa = dens_so4a_host * PI / 6.0;
-  const double mass1p_aitlo = tmpa * (pow(dplom, 3.0));
-  const double mass1p_aithi = tmpa * (pow(dphim, 3.0));
+  const double mass1p_aitlo = tmpa * (M_pow(dplom, 3.0));
+  const double mass1p_aithi = tmpa * (M_pow(dphim, 3.0));
   const double relhumnn = kmax(0.01, kmin(0.99, relhum));
   const double mw_so4a_host = 115, mwnh4 = 18, mwso4 = 96;
   const double rgas = 8.31446759100000, avogadro = 6.022140000000000E+023;
@@ -998,12 +1031,12 @@ static void getcoags(double lamda, double kfmatac, double kfmat, double kfmac,
                      double sgacc, double xxlsgat, double xxlsgac, double *qn11,
                      double *qn22, double *qn12, double *qv12) {
   const double a = 1.246;
-  const double esat01 = exp(0.125 * xxlsgat * xxlsgat);
-  const double esac01 = exp(0.125 * xxlsgac * xxlsgac);
-  const double sqrttwo = sqrt(2.0);
-  const double dlgsqt2 = 1.0 / log(sqrttwo);
-  const double esat04 = pow(esat01, 4.0);
-  const double esac04 = pow(esac01, 4.0);
+  const double esat01 = M_exp(0.125 * xxlsgat * xxlsgat);
+  const double esac01 = M_exp(0.125 * xxlsgac * xxlsgac);
+  const double sqrttwo = M_sqrt(2.0);
+  const double dlgsqt2 = 1.0 / M_log(sqrttwo);
+  const double esat04 = M_pow(esat01, 4.0);
+  const double esac04 = M_pow(esac01, 4.0);
   const double esat05 = esat04 * esat01, esac05 = esac04 * esac01;
   const double esat08 = esat04 * esat04, esac08 = esac04 * esac04;
   const double esat09 = esat08 * esat01, esac09 = esac08 * esac01;
@@ -1016,7 +1049,7 @@ static void getcoags(double lamda, double kfmatac, double kfmat, double kfmac,
   const double esat64 = esat20 * esat20 * esat24;
   const double esat100 = esat64 * esat36;
   const double dgat3 = dgatk * dgatk * dgatk;
-  const double sqdgat = sqrt(dgatk), sqdgac = sqrt(dgacc);
+  const double sqdgat = M_sqrt(dgatk), sqdgac = M_sqrt(dgacc);
   const double sqdgat7 = dgat3 * sqdgat;
   const double r1 = sqdgac / sqdgat;
   const double r2 = r1 * r1;
@@ -1029,7 +1062,7 @@ static void getcoags(double lamda, double kfmatac, double kfmat, double kfmac,
   const double rat = dgacc / dgatk;
   const int n2n = (int)(kmax(1, kmin(10, round(4.0 * (sgatk - 0.75)))) - 1);
   const int n2a = (int)(kmax(1, kmin(10, round(4.0 * (sgacc - 0.75)))) - 1);
-  const int n1 = (int)(kmax(1, kmin(10, 1 + round(dlgsqt2 * log(rat)))) - 1);
+  const int n1 = (int)(kmax(1, kmin(10, 1 + round(dlgsqt2 * M_log(rat)))) - 1);
   *qn12 = intermodal_0th(a, r1, r2, rx4, ri1, ri2, ri3, knc, kngat, kngac,
                          kfmatac, sqdgat, esat01, esat04, esat09, esat16,
                          esac01, esac04, esac09, esac16, n1, n2a, n2n);
@@ -1049,21 +1082,21 @@ void orc_getcoags_wrapper(double airtemp, double airprs, double dgatk,
                           double xxlsgat, double xxlsgac, double pdensat,
                           double pdensac, double out[4]) {
   const double t0 = FREEZING_PT_H2O + 15.0;
-  const double sqrt_temp = sqrt(airtemp);
+  const double sqrt_temp = M_sqrt(airtemp);
   const double lamda = 6.6328e-8 * PRESSURE_STP * airtemp / (t0 * airprs);
   const double amu = 1.458e-6 * airtemp * sqrt_temp / (airtemp + 110.4);
   const double boltzmann = 1.3806500000000000e-023;
   const double knc = (2.0 / 3.0) * boltzmann * airtemp / amu;
-  const double kfmat = sqrt(3.0 * boltzmann * airtemp / pdensat);
-  const double kfmac = sqrt(3.0 * boltzmann * airtemp / pdensac);
-  const double kfmatac = sqrt(6.0 * boltzmann * airtemp / (pdensat + pdensac));
+  const double kfmat = M_sqrt(3.0 * boltzmann * airtemp / pdensat);
+  const double kfmac = M_sqrt(3.0 * boltzmann * airtemp / pdensac);
+  const double kfmatac = M_sqrt(6.0 * boltzmann * airtemp / (pdensat + pdensac));
   double qn11, qn22, qn12, qv12;
   getcoags(lamda, kfmatac, kfmat, kfmac, knc, dgatk, dgacc, sgatk, sgacc,
            xxlsgat, xxlsgac, &qn11, &qn22, &qn12, &qv12);
   out[2] = kmax(0.0, qn11);
   out[3] = kmax(0.0, qn22);
   out[0] = kmax(0.0, qn12);
-  const double dumatk3 = (cube(dgatk) * exp(4.5 * xxlsgat * xxlsgat));
+  const double dumatk3 = (cube(dgatk) * M_exp(4.5 * xxlsgat * xxlsgat));
   out[1] = kmax(0.0, qv12 / dumatk3);
 }
 
@@ -1082,7 +1115,7 @@ static void mam_coag_aer_update(const double ybetaij3[3], double deltat,
   if (decay_factor > epsilonx2) {
     const double prtn2 = bijqnumj2 / decay_const;
     const double prtn1 = 1.0 - prtn2;
-    const double tmp_xf = 1.0 - exp(-decay_factor);
+    const double tmp_xf = 1.0 - M_exp(-decay_factor);
     for (int s = 0; s < 7; ++s) {
       const double tmp_dq = tmp_xf * qaer_bgn[s][NAIT];
       qaer_end[s][NAIT] -= tmp_dq;
@@ -1094,7 +1127,7 @@ static void mam_coag_aer_update(const double ybetaij3[3], double deltat,
   decay_const = kmax(0.0, ybetaij3[1] * qnum_tavg[NACC]);
   decay_factor = deltat * decay_const;
   if (decay_factor > epsilonx2) {
-    const double tmp_xf = 1.0 - exp(-decay_factor);
+    const double tmp_xf = 1.0 - M_exp(-decay_factor);
     for (int s = 0; s < 7; ++s) {
       const double tmp_dq = tmp_xf * qaer_bgn[s][NPCA];
       qaer_end[s][NPCA] -= tmp_dq;
@@ -1111,7 +1144,7 @@ static void update_qnum_intra_inter(double bijdtqnumj, double biidt,
     *qnumi_end = qnumi_bgn / (1.0 + (bijdtqnumj + biidt * qnumi_bgn) *
                                         (1.0 + 0.5 * bijdtqnumj));
   } else {
-    const double tmp_exp = exp(-bijdtqnumj);
+    const double tmp_exp = M_exp(-bijdtqnumj);
     *qnumi_end = qnumi_bgn * tmp_exp /
                  (1.0 + (biidt * qnumi_bgn / bijdtqnumj) * (1.0 - tmp_exp));
   }
@@ -1159,7 +1192,7 @@ static void mam_coag_1subarea(double deltat, double temp, double pmid,
     const double ss = MODE_SIGMA[src[ip]], sd = MODE_SIGMA[dst[ip]];
     double o[4];
     orc_getcoags_wrapper(temp, pmid, dgn_awet[src[ip]], dgn_awet[dst[ip]], ss,
-                         sd, log(ss), log(sd), wetdens[src[ip]],
+                         sd, M_log(ss), M_log(sd), wetdens[src[ip]],
                          wetdens[dst[ip]], o);
     ij0[ip] = o[0];
     ij3[ip] = o[1];
@@ -1203,7 +1236,7 @@ static void mam_pcarbon_aging_frac(unsigned nmono, const double dgn_a[4],
   double vol_core = 0.0;
   for (int s = 0; s < 7; ++s)
     if (LMAP_AER[s][NPCA] > 0) vol_core += qaer_cur[s][NPCA] * MASS_2_VOL[s];
-  const double fac_volsfc = exp(2.5 * sq(log(MODE_SIGMA[NPCA])));
+  const double fac_volsfc = M_exp(2.5 * sq(M_log(MODE_SIGMA[NPCA])));
   const double xferfrac_max = 1.0 - 10.0 * DBL_EPSILON;
   const double t1 = vol_shell * dgn_a[NPCA] * fac_volsfc;
   const double dr = nmono * 4.76e-10;
@@ -1279,7 +1312,7 @@ typedef struct {
 
 /* rename.hpp:392-474 */
 static void find_renaming_pairs(const int dest_mode_of_mode[4], rename_consts *c) {
-  const double sqrt_half = sqrt(0.5);
+  const double sqrt_half = M_sqrt(0.5);
   const double frelax = 27.0;
   c->num_pairs = 0;
   for (int m = 0; m < 4; ++m) {
@@ -1294,17 +1327,17 @@ static void find_renaming_pairs(const int dest_mode_of_mode[4], rename_consts *c
       c->ln_dia_cutoff[m] = 0;
       c->diameter_threshold[m] = 0;
     } else {
-      const double alnsg = log(MODE_SIGMA[m]);
+      const double alnsg = M_log(MODE_SIGMA[m]);
       c->mean_std_dev[m] = MODE_SIGMA[m];
       c->fmode_dist_tail_fac[m] = sqrt_half / alnsg;
       c->v2n_lo_rlx[m] = 1.0 / orc_mean_particle_volume_from_diameter(MODE_DMIN[m], MODE_SIGMA[m]) * frelax;
       c->v2n_hi_rlx[m] = 1.0 / orc_mean_particle_volume_from_diameter(MODE_DMAX[m], MODE_SIGMA[m]) / frelax;
       c->ln_diameter_tail_fac[m] = 3.0 * sq(alnsg);
       c->num_pairs += 1;
-      const double alnsg_d = log(MODE_SIGMA[dest]);
-      c->diameter_cutoff[m] = sqrt(MODE_DNOM[m] * exp(1.5 * sq(alnsg)) *
-                                   MODE_DNOM[dest] * exp(1.5 * sq(alnsg_d)));
-      c->ln_dia_cutoff[m] = log(c->diameter_cutoff[m]);
+      const double alnsg_d = M_log(MODE_SIGMA[dest]);
+      c->diameter_cutoff[m] = M_sqrt(MODE_DNOM[m] * M_exp(1.5 * sq(alnsg)) *
+                                   MODE_DNOM[dest] * M_exp(1.5 * sq(alnsg_d)));
+      c->ln_dia_cutoff[m] = M_log(c->diameter_cutoff[m]);
       c->diameter_threshold[m] = 0.99 * c->diameter_cutoff[m];
     }
   }
@@ -1333,10 +1366,10 @@ static void compute_dryvol_change_in_src_mode(const int dest_mode_of_mode[4],
 static double tail_fraction(double diameter, double log_dia_cutoff,
                             double tail_dist_fac, double log_dia_tail_fac,
                             int with_tail_fac) {
-  double logd = log(diameter);
+  double logd = M_log(diameter);
   if (with_tail_fac) logd = logd + log_dia_tail_fac;
   const double tail = (log_dia_cutoff - logd) * tail_dist_fac;
-  return 0.5 * erfc(tail);
+  return 0.5 * M_erfc(tail);
 }
 
 /* rename.hpp:225-389 */
@@ -2041,7 +2074,7 @@ static void aitken_accum_exchange(const calcsize_consts *c, double adj_tscale_in
   double xfertend_num[2][2] = {{0, 0}, {0, 0}};
   double xfn_a2c = 0, xfv_a2c = 0, xfn_c2a = 0, xfv_c2a = 0;
   double drv_i_noxf = 0, drv_c_noxf = 0;
-  const double geomean = sqrt(voltonum_ait) * sqrt(voltonum_acc);
+  const double geomean = M_sqrt(voltonum_ait) * M_sqrt(voltonum_acc);
   /* compute_coef_ait_acc_transfer (calcsize.hpp:313-378) */
   {
     double fn = zero, fv = zero;
@@ -2280,6 +2313,7 @@ long orc_chain_step(const orc_config *cfg, int ncol, int nlev, double dt,
   for (int icol = 0; icol < ncol; ++icol) {
     for (int k = 0; k < nlev; ++k) {
       const long c = (long)k * ncol + icol;
+      if (g_noise_seed) noise_cell(c);
 #define AT(ptr, f) (ptr)[(long)(f) * ncell + c]
       double qq[31], qqcw[31], q1[31], q2[31], qcw2[31];
       for (int i = 0; i < 31; ++i) qq[i] = qqcw[i] = q1[i] = q2[i] = qcw2[i] = 0.0;

@@ -79,6 +79,10 @@ long orc_chain_step(const orc_config *cfg, int ncol, int nlev, double dt,
                     const double *pregaschem_gas, const double *precldchem,
                     double *tend, double *dgn_i, double *dgn_c, int nthreads);
 
+/* Conditioning probe: seed != 0 perturbs every transcendental result by a
+ * random -1/0/+1 ulp (per-cell reproducible); 0 switches it off. */
+void orc_set_libm_noise(unsigned long long seed);
+
 /* ---- process-level entry points used by the known-answer tests ---- */
 
 /* H2SO4 non-volatile analytic update (gasaerexch.hpp:479-537; identical math

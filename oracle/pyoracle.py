@@ -89,6 +89,40 @@ class Oracle:
         L.orc_amicphys_intr.argtypes = [ctypes.POINTER(OrcConfig)] + [ctypes.c_double] * 8 + \
             [_dp] * 10 + [_dp, _dp, ctypes.POINTER(ctypes.c_int)]
         L.orc_calcsize_level.argtypes = [ctypes.POINTER(OrcConfig), ctypes.c_double] + [_dp] * 12
+        L.orc_set_libm_noise.argtypes = [ctypes.c_ulonglong]
+        L.orc_set_libm_noise.restype = None
+
+    def set_libm_noise(self, seed):
+        """seed != 0: randomly move each exp/log/pow/cbrt/erfc result by one
+        ulp (conditioning probe); 0: exact libm."""
+        self.lib.orc_set_libm_noise(int(seed))
+
+    def envelope(self, scene, dt=1800.0, config=None, seeds=(1, 2, 3, 4, 5, 6), return_dgn=False,
+                 **kw):
+        """Per-element spread of the tendencies under libm-rounding noise,
+        in the state-relative measure of tests/parity.py.  Returns
+        (base tendencies, envelope array [NSTATE, nlev, ncol]) and, with
+        return_dgn, the relative spread of the calcsize diameters [2, 4, nlev, ncol]."""
+        import numpy as np
+        base, bdi, bdc, _ = self.chain_step(scene, dt=dt, config=config, **kw)
+        st = scene["state"]
+        post = st + dt * base
+        fl = np.where(np.arange(st.shape[0]) < 8, 1e-10, 1e-30)[:, None, None]
+        den = np.maximum(np.maximum(np.abs(st), np.abs(post)), fl)
+        env = np.zeros_like(base)
+        env_dgn = np.zeros((2,) + bdi.shape)
+        try:
+            for s in seeds:
+                self.set_libm_noise(s)
+                t, di, dc, _ = self.chain_step(scene, dt=dt, config=config, **kw)
+                env = np.maximum(env, np.abs(t - base) * dt / den)
+                env_dgn[0] = np.maximum(env_dgn[0], np.abs(di - bdi) / np.abs(bdi))
+                env_dgn[1] = np.maximum(env_dgn[1], np.abs(dc - bdc) / np.abs(bdc))
+        finally:
+            self.set_libm_noise(0)
+        if return_dgn:
+            return base, env, env_dgn
+        return base, env
 
     def chain_step(self, scene, dt=1800.0, config=None, nthreads=0, pregaschem_gas=None,
                    precldchem=None):

@@ -1,11 +1,22 @@
 """FP64 HIP chain vs the CPU oracle on identical seeded inputs (GPU).
 
-Bar: every state-relative error (tests/parity.py) <= 1e-10, diameters <= 1e-10
-relative.  Cases follow the reference's own process switches
-(AmicPhysConfig, mam4_amicphys.hpp:223-237; CalcSize::Config,
-calcsize.hpp:835-846), the cloud categories of setup_subareas
-(mam4_amicphys.hpp:241-317) and the edge inputs the reference's tests use
-(zero aerosol, mam4_aging_unit_tests.cpp:118-180).
+Bar (error measure of tests/parity.py, relative to the state):
+* every cell the reference computes reproducibly agrees to <= 1e-10;
+* a cell is "libm-sensitive" when the oracle itself moves by more than 1e-11
+  once its exp/log/pow/cbrt/erfc results are perturbed by random single ulps
+  (oracle.Oracle.envelope; the SOA sub-stepping of
+  gasaerexch_soaexch.hpp:170-335 chooses dt = 0.05 / sum(k |phi|) with phi a
+  near-cancelling difference, so last-bit noise changes the step sequence).
+  There the GPU, whose device libm rounds differently from glibc, must stay
+  within 10x the oracle's own spread; cells the oracle cannot reproduce to
+  1e-10 at all must stay rare (< 1 %);
+* calcsize diameters: <= 1e-10 relative, or 10x the oracle's own relative
+  spread where that exceeds 1e-11 (modes emptied down to rounding residues).
+Cases follow the reference's process switches (AmicPhysConfig,
+mam4_amicphys.hpp:223-237; CalcSize::Config, calcsize.hpp:835-846), the
+cloud categories of setup_subareas (mam4_amicphys.hpp:241-317) and edge
+inputs (zero aerosol as in mam4_aging_unit_tests.cpp:118-180, negative
+values clipped at mam4_amicphys.hpp:2258-2267).
 """
 import numpy as np
 import pytest
@@ -52,15 +63,40 @@ def run_orc(orc, scene, dt=DT, **cfg):
     return t, di, dc
 
 
-def check(orc, torch_cuda, scene, dt=DT, **cfg):
+SENSITIVE = 1e-11  # oracle spread above which a cell counts as libm-sensitive
+ENV_FACTOR = 10.0
+
+
+def check(orc, torch_cuda, scene, dt=DT, max_sensitive=0.01, **cfg):
+    import oracle.pyoracle as P
     tg, dig, dcg = run_gpu(torch_cuda, scene, dt, **cfg)
-    to, dio, dco = run_orc(orc, scene, dt, **cfg)
+    oc = P.default_config(**{k: v for k, v in cfg.items() if k != "precision"})
+    to, dio, dco, _ = orc.chain_step(scene, dt=dt, config=oc)
+    _, env, env_dgn = orc.envelope(scene, dt=dt, config=oc, return_dgn=True)
     assert np.all(np.isfinite(tg))
-    mx, _ = state_rel_err(tg, to, scene["state"], dt)
-    print("worst fields:", report(tg, to, scene["state"], dt))
-    assert mx.max() <= TOL, report(tg, to, scene["state"], dt)
-    de = max(np.max(np.abs(dig - dio) / np.abs(dio)), np.max(np.abs(dcg - dco) / np.abs(dco)))
-    assert de <= TOL, de
+    _, err = state_rel_err(tg, to, scene["state"], dt)
+    cell_env = env.max(axis=0)
+    sens = cell_env > SENSITIVE
+    bound = np.broadcast_to(np.where(sens, np.maximum(TOL, ENV_FACTOR * cell_env), TOL)[None],
+                            err.shape)
+    bad = np.argwhere(err > bound)
+    unrepro = cell_env > TOL
+    print("worst fields:", report(tg, to, scene["state"], dt),
+          "| libm-sensitive cells %d/%d (not reproducible at 1e-10: %d), worst error on the "
+          "other cells %.2e" % (sens.sum(), sens.size, unrepro.sum(),
+                                err[:, ~sens].max() if (~sens).any() else 0.0))
+    assert len(bad) == 0, ("%d elements out of bound, first %s err %.3e bound %.3e env %.3e" %
+                           (len(bad), bad[0], err[tuple(bad[0])], bound[tuple(bad[0])],
+                            cell_env[tuple(bad[0][1:])]))
+    assert unrepro.mean() <= max_sensitive, unrepro.mean()
+    # diameters of nearly emptied modes are ratios of cancellation residues;
+    # they get the same treatment, with their own (relative) envelope
+    for g_, o_, e_ in ((dig, dio, env_dgn[0]), (dcg, dco, env_dgn[1])):
+        de = np.abs(g_ - o_) / np.abs(o_)
+        db = np.where(e_ > SENSITIVE, np.maximum(TOL, ENV_FACTOR * e_), TOL)
+        bad = np.argwhere(de > db)
+        assert len(bad) == 0, ("diameter %s err %.3e env %.3e" %
+                               (bad[0], de[tuple(bad[0])], e_[tuple(bad[0])]))
     return tg, to
 
 
@@ -116,7 +152,52 @@ def test_zero_and_negative_inputs(orc, torch_cuda):
     check(orc, torch_cuda, sc)
 
 
+def test_optaa_switches(orc, torch_cuda):
+    """Both H2SO4 treatments, including the swapped-argument path
+    (mam4_amicphys.hpp:1129 vs :1914)."""
+    sc = scenes.generate(64, 72, seed=9)
+    check(orc, torch_cuda, sc, gaexch_h2so4_uptake_optaa=1)
+    check(orc, torch_cuda, sc, newnuc_h2so4_conc_optaa=1)
+
+
 def test_nucleation_heavy(orc, torch_cuda):
     """High H2SO4 so most cells nucleate (nucleation.hpp:346-809)."""
     sc = scenes.generate(64, 72, seed=8, h2so4_range=(1e-11, 1e-9))
     check(orc, torch_cuda, sc)
+
+
+def test_precldchem_inputs(orc, torch_cuda):
+    """Extended pre-chemistry inputs: cloud-chemistry growth drives cloudy-
+    subarea renaming, gas production drives simultaneous H2SO4 uptake."""
+    import torch
+    from mam4xx_amd.chain import Chain, scene_to_device
+    import oracle.pyoracle as P
+    sc = scenes.generate(64, 72, seed=12)
+    rng = np.random.default_rng(13)
+    pre = sc["state"] * rng.uniform(0.7, 1.0, sc["state"].shape)
+    pregas = np.stack([sc["state"][L.F_H2SO4] * rng.uniform(0.2, 1.2, sc["pblh"].shape),
+                       sc["state"][L.F_SOAG]])
+    ch = Chain()
+    d = scene_to_device(sc)
+    tend = torch.empty((L.NSTATE, 72, 64), dtype=torch.float64, device="cuda")
+    ch.compute_tendencies(d, DT, tend, precldchem=torch.as_tensor(pre).cuda(),
+                          pregaschem=torch.as_tensor(pregas).cuda().contiguous())
+    torch.cuda.synchronize()
+    tg = tend.cpu().numpy()
+    oc = P.default_config()
+    to, _, _, _ = orc.chain_step(sc, DT, config=oc, pregaschem_gas=pregas, precldchem=pre)
+    _, env = orc.envelope(sc, DT, config=oc, pregaschem_gas=pregas, precldchem=pre)
+    _, err = state_rel_err(tg, to, sc["state"], DT)
+    cell_env = env.max(axis=0)
+    bound = np.where(cell_env > SENSITIVE, np.maximum(TOL, ENV_FACTOR * cell_env), TOL)[None]
+    assert (err <= bound).all(), report(tg, to, sc["state"], DT)
+
+
+def test_validate_counts_violations(torch_cuda):
+    from mam4xx_amd.chain import Chain, scene_to_device
+    sc = scenes.generate(32, 8, seed=14)
+    ch = Chain()
+    assert ch.validate(scene_to_device(sc)) == 0
+    sc["temperature"][0, :3] = -1.0
+    sc["state"][5, 2, 10] = -1e-20
+    assert ch.validate(scene_to_device(sc)) == 4
