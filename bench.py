@@ -1,0 +1,186 @@
+#!/usr/bin/env python3
+"""Benchmark: MAM4 microphysics column-steps/s on one MI355X.
+
+Workload: 21,600 columns x 72 levels (the precision-study configuration),
+seeded synthetic inputs (mam4xx_amd.scenes), chained 1800 s steps: each step
+runs the whole chain + calcsize through the C ABI and writes the tendencies,
+the calcsize diameters and the updated state in place.  W untimed warm-up
+steps, then K timed steps bracketed by device synchronisation.
+
+Roofline: the step is one kernel launch, bound by FP64 VALU arithmetic (no
+dense contraction; HBM traffic is ~1.4 KB per cell).  Algorithmic work per
+launch = A * cells + B * (SOA sub-steps taken), with A and B the FP64 flop
+counts per cell and per SOA sub-step measured by the op-counting build of the
+CPU restatement (oracle/opcount.cpp, transcendentals weighted by their gfx950
+libm flops) on a 216-column sample of this workload.  The GPU counts the SOA
+sub-steps itself.  Launch time comes from HIP events on the launch stream.
+
+cpu_baseline: the CPU restatement (oracle/, OpenMP over columns) timed on the
+host's cores for one full step of the same scene.
+"""
+import argparse
+import json
+import os
+import sys
+import time
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+
+NCOL, NLEV, DT = 21600, 72, 1800.0
+# FP64 flops per cell (A) and per SOA sub-step (B); see module docstring.
+OPS_PER_CELL = 11850.2
+OPS_PER_SOA_STEP = 130.87
+HBM_PEAK_GBS = 8000.0
+FP64_PEAK_TFLOPS = 78.6
+# algorithmic HBM bytes per cell per launch: reads 52 state + 5 met + 12 modal
+# doubles, writes 52 tendencies + 52 updated state + 8 diameters (pblh per
+# column is negligible)
+BYTES_PER_CELL = 8 * (52 + 5 + 12 + 52 + 52 + 8)
+
+
+def cpu_baseline(scene, nthreads):
+    import numpy as np
+    import oracle
+    orc = oracle.Oracle()
+    orc.chain_step(scene, dt=DT, nthreads=nthreads)  # warm-up
+    best = 1e30
+    for _ in range(3):
+        t0 = time.perf_counter()
+        orc.chain_step(scene, dt=DT, nthreads=nthreads)
+        best = min(best, time.perf_counter() - t0)
+    ncol1 = 216
+    sub = {k: np.ascontiguousarray(v[..., :ncol1] if v.ndim >= 2 else v[:ncol1])
+           for k, v in scene.items()}
+    t0 = time.perf_counter()
+    orc.chain_step(sub, dt=DT, nthreads=1)
+    t1 = time.perf_counter() - t0
+    return {
+        "value": NCOL / best,
+        "unit": "column-steps/s",
+        "cores": nthreads,
+        "kind": "port",
+        "sample": "one 1800 s step of the full %d x %d scene, best of 3 after a warm-up; "
+                  "single-core figure from 216 columns" % (NCOL, NLEV),
+        "single_core_value": ncol1 / t1,
+        "cpu": _cpu_model(),
+    }
+
+
+def _cpu_model():
+    try:
+        for line in open("/proc/cpuinfo"):
+            if line.startswith("model name"):
+                return line.split(":", 1)[1].strip()
+    except OSError:
+        pass
+    return "unknown"
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=20)
+    ap.add_argument("--warmup", type=int, default=3)
+    ap.add_argument("--precision", choices=["fp64", "mixed"], default="fp64")
+    ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline")
+    args = ap.parse_args()
+    if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) > 1:
+        print("bench.py measures one GPU: the chain has no cross-GPU step and this build "
+              "runs on one MI355X only (--gpus 1).", file=sys.stderr)
+        sys.exit(2)
+
+    import torch
+    from mam4xx_amd import _abi
+    from mam4xx_amd import layout as L
+    from mam4xx_amd import scenes
+    from mam4xx_amd.chain import Chain, scene_to_device
+
+    assert torch.cuda.is_available(), "bench.py needs a GPU"
+    scene = scenes.generate(NCOL, NLEV, seed=0)
+    d = scene_to_device(scene)
+    # chained steps: calcsize diameters feed the next step's dry and wet
+    # diameters (no water uptake in this workload)
+    d["dgncur_awet"] = d["dgncur_a"]
+    prec = _abi.PREC_FP64 if args.precision == "fp64" else _abi.PREC_MIXED
+    ch = Chain(precision=prec)
+    tend = torch.empty((L.NSTATE, NLEV, NCOL), dtype=torch.float64, device="cuda")
+    dgc = torch.empty((4, NLEV, NCOL), dtype=torch.float64, device="cuda")
+    state = d["state"]
+
+    def step():
+        ch.compute_tendencies(d, DT, tend, dgn_i=d["dgncur_a"], dgn_c=dgc, state_out=state)
+
+    for _ in range(args.warmup):
+        step()
+    torch.cuda.synchronize()
+    ch.enable_stats(True)
+    ch.last_stats()  # reset the counter
+    stream = torch.cuda.current_stream()
+    e0 = torch.cuda.Event(enable_timing=True)
+    e1 = torch.cuda.Event(enable_timing=True)
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    e0.record(stream)
+    for _ in range(args.steps):
+        step()
+    e1.record(stream)
+    torch.cuda.synchronize()
+    wall = time.perf_counter() - t0
+    iters = ch.last_stats()
+    kernel_ms = e0.elapsed_time(e1) / args.steps
+    ncell = NCOL * NLEV
+    ops = OPS_PER_CELL * ncell + OPS_PER_SOA_STEP * iters / args.steps
+    achieved = ops / (kernel_ms * 1e-3) / 1e12
+    ms_per_step = wall * 1e3 / args.steps
+    finite = bool(torch.isfinite(state).all().item())
+    out = {
+        "metric": "microphysics column-steps/sec at 72 lev: mixed vs FP64 GPU at <=1e-5 rel err",
+        "value": NCOL * args.steps / wall,
+        "unit": "column-steps/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": ms_per_step,
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f64" if args.precision == "fp64" else "f32+f64",
+        "data": "synthetic (seeded generator of the reference study's value distributions)",
+        "config": {
+            "workload": "%d columns x %d levels, chained %g s microphysics steps "
+                        "(gasaerexch, rename, newnuc, coag, pcarbon aging, calcsize)"
+                        % (NCOL, NLEV, DT),
+            "columns": NCOL, "levels": NLEV, "dt_s": DT, "precision": args.precision,
+            "soa_substeps_per_cell": iters / args.steps / ncell,
+            "state_finite": finite,
+        },
+        "roofline": {
+            "bound": "mfma",
+            "compute_unit": "FP64 VALU (MI355X FP64 vector and matrix peaks coincide)",
+            "achieved": achieved,
+            "peak": FP64_PEAK_TFLOPS,
+            "unit": "TFLOP/s",
+            "frac": achieved / FP64_PEAK_TFLOPS,
+            "traffic": None,
+            "kernel_ms": kernel_ms,
+            "algorithmic_gflop_per_launch": ops / 1e9,
+            "hbm_algorithmic_gbs": BYTES_PER_CELL * ncell / (kernel_ms * 1e-3) / 1e9,
+        },
+    }
+    # HBM bytes per launch from the FETCH_SIZE/WRITE_SIZE counter passes,
+    # scaled by a copy-kernel calibration (summary under profiles/)
+    traffic_file = os.path.join(ROOT, "profiles", "traffic_chain_%s_%dx%d.txt" %
+                                (args.precision, NCOL, NLEV))
+    if os.path.exists(traffic_file):
+        for line in open(traffic_file):
+            if line.startswith("bytes_per_launch"):
+                out["roofline"]["traffic"] = float(line.split("=")[1])
+    if not args.no_cpu:
+        nthreads = int(os.environ.get("OMP_NUM_THREADS", "0")) or os.cpu_count()
+        out["cpu_baseline"] = cpu_baseline(scene, nthreads)
+    print(json.dumps(out), flush=True)
+
+
+if __name__ == "__main__":
+    main()

@@ -152,10 +152,11 @@ int mam4x_compute_tendencies(mam4x_handle h, double t, double dt,
                              const mam4x_inputs *in, const mam4x_outputs *out,
                              void *stream);
 
-/* Number of SOA sub-step iterations the last compute_tendencies call took,
- * summed over cells (a cost diagnostic).  Synchronizes the stream.  Returns
- * MAM4X_ERR_UNSUPPORTED unless the handle was created with stats enabled via
- * mam4x_enable_stats. */
+/* Cost diagnostic: with stats enabled, every compute_tendencies call adds
+ * the SOA sub-steps (gasaerexch_soaexch.hpp:170-335) its cells took to a
+ * device counter.  mam4x_last_stats returns the sum since stats were enabled
+ * or last read, resets it, and synchronizes the stream; it returns
+ * MAM4X_ERR_UNSUPPORTED while stats are disabled. */
 int mam4x_enable_stats(mam4x_handle h, int enable);
 int mam4x_last_stats(mam4x_handle h, long long *soa_iterations, void *stream);
 

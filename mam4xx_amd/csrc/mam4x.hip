@@ -65,7 +65,8 @@ struct mam4x_handle_s {
   mam4x_config cfg;
   int device = 0;
   ChainConsts *d_consts = nullptr;
-  unsigned long long *d_count = nullptr;  // validate / stats counters
+  unsigned long long *d_count = nullptr;  // validate counter
+  unsigned long long *d_stats = nullptr;  // SOA sub-steps since the last read
   bool stats = false;
   std::string err;
 };
@@ -178,14 +179,18 @@ int mam4x_create(const mam4x_config *config, mam4x_handle *out) {
   ChainConsts k;
   chain_consts_init(&k);
   if (hipMalloc(&h->d_consts, sizeof(ChainConsts)) != hipSuccess ||
-      hipMalloc(&h->d_count, sizeof(unsigned long long)) != hipSuccess) {
+      hipMalloc(&h->d_count, sizeof(unsigned long long)) != hipSuccess ||
+      hipMalloc(&h->d_stats, sizeof(unsigned long long)) != hipSuccess) {
     if (h->d_consts) (void)hipFree(h->d_consts);
+    if (h->d_count) (void)hipFree(h->d_count);
     delete h;
     return MAM4X_ERR_NOMEM;
   }
-  if (hipMemcpy(h->d_consts, &k, sizeof(k), hipMemcpyHostToDevice) != hipSuccess) {
+  if (hipMemcpy(h->d_consts, &k, sizeof(k), hipMemcpyHostToDevice) != hipSuccess ||
+      hipMemset(h->d_stats, 0, sizeof(unsigned long long)) != hipSuccess) {
     (void)hipFree(h->d_consts);
     (void)hipFree(h->d_count);
+    (void)hipFree(h->d_stats);
     delete h;
     return MAM4X_ERR_HIP;
   }
@@ -197,6 +202,7 @@ int mam4x_destroy(mam4x_handle h) {
   if (!h) return MAM4X_ERR_ARG;
   if (h->d_consts) (void)hipFree(h->d_consts);
   if (h->d_count) (void)hipFree(h->d_count);
+  if (h->d_stats) (void)hipFree(h->d_stats);
   delete h;
   return MAM4X_OK;
 }
@@ -219,6 +225,8 @@ const char *mam4x_last_error(mam4x_handle h) { return h ? h->err.c_str() : "NULL
 
 int mam4x_enable_stats(mam4x_handle h, int enable) {
   if (!h) return MAM4X_ERR_ARG;
+  if (enable && !h->stats)
+    HIP_TRY(h, hipMemset(h->d_stats, 0, sizeof(unsigned long long)));
   h->stats = enable != 0;
   return MAM4X_OK;
 }
@@ -227,9 +235,10 @@ int mam4x_last_stats(mam4x_handle h, long long *soa_iterations, void *stream) {
   if (!h || !soa_iterations) return MAM4X_ERR_ARG;
   if (!h->stats) return fail(h, MAM4X_ERR_UNSUPPORTED, "stats not enabled");
   unsigned long long v = 0;
-  HIP_TRY(h, hipMemcpyAsync(&v, h->d_count, sizeof(v), hipMemcpyDeviceToHost,
-                            (hipStream_t)stream));
-  HIP_TRY(h, hipStreamSynchronize((hipStream_t)stream));
+  hipStream_t s = (hipStream_t)stream;
+  HIP_TRY(h, hipMemcpyAsync(&v, h->d_stats, sizeof(v), hipMemcpyDeviceToHost, s));
+  HIP_TRY(h, hipMemsetAsync(h->d_stats, 0, sizeof(v), s));
+  HIP_TRY(h, hipStreamSynchronize(s));
   *soa_iterations = (long long)v;
   return MAM4X_OK;
 }
@@ -287,10 +296,7 @@ int mam4x_compute_tendencies(mam4x_handle h, double t, double dt, const mam4x_in
   }
   a.dt = dt;
   hipStream_t s = (hipStream_t)stream;
-  if (h->stats) {
-    HIP_TRY(h, hipMemsetAsync(h->d_count, 0, sizeof(unsigned long long), s));
-    a.stats = h->d_count;
-  }
+  if (h->stats) a.stats = h->d_stats;
   bool pre = in->pregaschem_h2so4 || in->pregaschem_soag;
   for (int f = 0; f < NSTATE; ++f) pre = pre || in->precldchem[f];
   const long ncell = (long)in->ncol * in->nlev;
