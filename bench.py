@@ -83,6 +83,7 @@ def main():
     ap.add_argument("--steps", type=int, default=20)
     ap.add_argument("--warmup", type=int, default=3)
     ap.add_argument("--precision", choices=["fp64", "mixed"], default="fp64")
+    ap.add_argument("--schedule", choices=["auto", "fused", "queue"], default="auto")
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline")
     args = ap.parse_args()
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) > 1:
@@ -103,7 +104,9 @@ def main():
     # diameters (no water uptake in this workload)
     d["dgncur_awet"] = d["dgncur_a"]
     prec = _abi.PREC_FP64 if args.precision == "fp64" else _abi.PREC_MIXED
-    ch = Chain(precision=prec)
+    sched = {"auto": _abi.SCHED_AUTO, "fused": _abi.SCHED_FUSED,
+             "queue": _abi.SCHED_QUEUE}[args.schedule]
+    ch = Chain(precision=prec, schedule=sched)
     tend = torch.empty((L.NSTATE, NLEV, NCOL), dtype=torch.float64, device="cuda")
     dgc = torch.empty((4, NLEV, NCOL), dtype=torch.float64, device="cuda")
     state = d["state"]

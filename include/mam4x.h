@@ -72,6 +72,12 @@ extern "C" {
 #define MAM4X_PREC_FP64 0  /* everything in FP64 (the drop-in)             */
 #define MAM4X_PREC_MIXED 1 /* FP32 bulk, FP64 kept where stated in DESIGN  */
 
+/* schedule of a step (performance only; results are identical) */
+#define MAM4X_SCHED_AUTO 0  /* fused below 16,384 cells, else work queue   */
+#define MAM4X_SCHED_FUSED 1 /* one lane runs one cell's whole step         */
+#define MAM4X_SCHED_QUEUE 2 /* per-cell phases around a work-queue kernel  */
+                            /* that runs the SOA sub-stepping              */
+
 /* Process switches.  Mirrors microphysics::AmicPhysConfig
  * (mam4_amicphys.hpp:223-237), Aging's n_so4_monolayers_pcage
  * (aging.hpp:20-32) and CalcSize::Config (calcsize.hpp:835-846). */
@@ -87,6 +93,7 @@ typedef struct mam4x_config {
   int calcsize_do_aitacc_transfer;
   int calcsize_do_adjust;
   int precision; /* MAM4X_PREC_*                                          */
+  int schedule;  /* MAM4X_SCHED_*                                         */
 } mam4x_config;
 
 /* Atmosphere (atmosphere.hpp:16-137) and modal diagnostics the chain reads. */

@@ -18,6 +18,10 @@ ERR_NOMEM = 4
 PREC_FP64 = 0
 PREC_MIXED = 1
 
+SCHED_AUTO = 0
+SCHED_FUSED = 1
+SCHED_QUEUE = 2
+
 _dp = ctypes.POINTER(ctypes.c_double)
 _vp = ctypes.c_void_p
 
@@ -35,6 +39,7 @@ class Config(ctypes.Structure):
         ("calcsize_do_aitacc_transfer", ctypes.c_int),
         ("calcsize_do_adjust", ctypes.c_int),
         ("precision", ctypes.c_int),
+        ("schedule", ctypes.c_int),
     ]
 
 

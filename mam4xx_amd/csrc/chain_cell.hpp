@@ -2,6 +2,13 @@
 // clear and cloudy subareas, recombination, calcsize, tendencies.
 // Follows modal_aero_amicphys_intr (mam4_amicphys.hpp:2119-2376) and
 // CalcSize::compute_tendencies (calcsize.hpp:962-1218).
+//
+// The step is organised in three phases so the data-dependent SOA loop can
+// run in a work-queue kernel of its own:
+//   cell_setup + clear_build + clear_pre  -> SoaIn    (phase 1)
+//   soa_init / soa_tick ...               -> results  (phase 2)
+//   cell_setup + clear_build + clear_post + cell_finish (phase 3)
+// chain_cell_fused runs the three phases back to back in one lane.
 #pragma once
 
 #include "chain_device.hpp"
@@ -19,7 +26,7 @@ struct ChainArgs {
   double *dgi[4], *dgc[4];
   double *sout[NSTATE];
   const ChainConsts *K;
-  unsigned long long *stats;  // optional: SOA sub-step count accumulator
+  unsigned long long *stats;  // SOA sub-step counter (NULL: off)
   int ncol, nlev, ldc;
   int do_calcsize, do_xfer, do_adjust, has_dgn, has_sout;
   double dt;
@@ -40,193 +47,212 @@ __device__ __forceinline__ void partition_factors(double qi, double qc, double f
   fc = (1 - r) / fcldy;
 }
 
+// grid-cell means in chain units, clipped at 0 (mam4_amicphys.hpp:2258-2267)
+__device__ __forceinline__ double ld3(const ChainArgs &a, long off, int f, double c) {
+  return kmax(0.0, a.state[f][off] * c);
+}
 template <bool PRE>
-__device__ __forceinline__ int chain_cell(const ChainArgs &a, long off, int icol) {
-  const ChainConsts &K = *a.K;
-  const Cfg cfg = a.cfg;
-  const double dt = a.dt;
-  const double T = a.T[off], p = a.p[off], z = a.z[off], qv = a.qv[off];
-  const double cld = a.cld[off], pblh = a.pblh[icol];
+__device__ __forceinline__ double ld2(const ChainArgs &a, long off, int f, double c) {
+  return PRE ? kmax(0.0, (a.precld[f] ? a.precld[f][off] : a.state[f][off]) * c)
+             : ld3(a, off, f, c);
+}
 
+// Per-cell context shared by the phases.
+struct CellCtx {
+  double T, p, z, pblh;
+  int cat;  // setup_subareas (mam4_amicphys.hpp:241-317): 0 clear, 1 overcast, 2 partly
+  double fcldy, fclea, rh_clear;
+  double dga[4], dgw[4], wd[4];
+  double h1, s1, h2, s2, h3, s3;  // gases before gas chem / before cloud chem / now
+  double fa_n[4], fc_n[4], fa_q[4], fc_q[4];
+  bool map_gas;
+};
+
+template <bool PRE>
+__device__ __forceinline__ void cell_setup(const ChainArgs &a, long off, int icol, CellCtx &X) {
+  X.T = a.T[off];
+  X.p = a.p[off];
+  X.z = a.z[off];
+  X.pblh = a.pblh[icol];
+  const double qv = a.qv[off], cld = a.cld[off];
   // relative humidity from the improved Magnus formula (conversions.hpp:148-180)
   const double epsqs = MW_H2O / MW_DRY_AIR;
-  const double tc = T - 273.15;
+  const double tc = X.T - 273.15;
   const double ew = 610.94 * exp(17.625 * tc / (234.04 + tc));
-  const double ev = 1.00071 * exp(4.5e-8 * p) * ew;
-  const double qvs = epsqs * ev / (p - (1 - epsqs) * ev);
+  const double ev = 1.00071 * exp(4.5e-8 * X.p) * ew;
+  const double qvs = epsqs * ev / (X.p - (1 - epsqs) * ev);
   const double rhgcm = kmax(0.0, kmin(1.0, qv / qvs));
-
-  // setup_subareas (mam4_amicphys.hpp:241-317): 0 clear, 1 overcast, 2 partly
-  int cat;
-  double fcldy;
   if (cld < 1.0e-5) {
-    cat = 0;
-    fcldy = 0;
+    X.cat = 0;
+    X.fcldy = 0;
   } else if (cld > 0.999) {
-    cat = 1;
-    fcldy = 1;
+    X.cat = 1;
+    X.fcldy = 1;
   } else {
-    cat = 2;
-    fcldy = cld;
+    X.cat = 2;
+    X.fcldy = cld;
   }
-  const double fclea = 1.0 - fcldy;
-  const double rh_clear = (cat == 2) ? mmb(0.0, 1.0, (rhgcm - fcldy) / fclea) : rhgcm;
-
-  double dga[4], dgw[4], wd[4];
+  X.fclea = 1.0 - X.fcldy;
+  X.rh_clear = (X.cat == 2) ? mmb(0.0, 1.0, (rhgcm - X.fcldy) / X.fclea) : rhgcm;
 #pragma unroll
   for (int m = 0; m < 4; ++m) {
-    dga[m] = a.dga[m][off];
-    dgw[m] = a.dgw[m][off];
-    wd[m] = kmax(1000.0, a.wd[m][off]);
+    X.dga[m] = a.dga[m][off];
+    X.dgw[m] = a.dgw[m][off];
+    X.wd[m] = kmax(1000.0, a.wd[m][off]);
   }
-
-  // grid-cell means in chain units, clipped at 0 (mam4_amicphys.hpp:2258-2267)
-  auto ld3 = [&](int f, double c) { return kmax(0.0, a.state[f][off] * c); };
-  auto ld2 = [&](int f, double c) {
-    return PRE ? kmax(0.0, (a.precld[f] ? a.precld[f][off] : a.state[f][off]) * c)
-               : ld3(f, c);
-  };
-  double h1, s1;
   if (PRE) {
-    h1 = kmax(0.0, (a.pre_h2so4 ? a.pre_h2so4[off] : a.state[F_H2SO4][off]) * TO_VMR_H2SO4);
-    s1 = kmax(0.0, (a.pre_soag ? a.pre_soag[off] : a.state[F_SOAG][off]) * TO_VMR_SOAG);
+    X.h1 = kmax(0.0, (a.pre_h2so4 ? a.pre_h2so4[off] : a.state[F_H2SO4][off]) * TO_VMR_H2SO4);
+    X.s1 = kmax(0.0, (a.pre_soag ? a.pre_soag[off] : a.state[F_SOAG][off]) * TO_VMR_SOAG);
   } else {
-    h1 = ld3(F_H2SO4, TO_VMR_H2SO4);
-    s1 = ld3(F_SOAG, TO_VMR_SOAG);
+    X.h1 = ld3(a, off, F_H2SO4, TO_VMR_H2SO4);
+    X.s1 = ld3(a, off, F_SOAG, TO_VMR_SOAG);
   }
-  const double h3 = ld3(F_H2SO4, TO_VMR_H2SO4), s3 = ld3(F_SOAG, TO_VMR_SOAG);
-  const double h2 = ld2(F_H2SO4, TO_VMR_H2SO4), s2 = ld2(F_SOAG, TO_VMR_SOAG);
-
+  X.h3 = ld3(a, off, F_H2SO4, TO_VMR_H2SO4);
+  X.s3 = ld3(a, off, F_SOAG, TO_VMR_SOAG);
+  X.h2 = ld2<PRE>(a, off, F_H2SO4, TO_VMR_H2SO4);
+  X.s2 = ld2<PRE>(a, off, F_SOAG, TO_VMR_SOAG);
   // partition factors of the interstitial aerosol (partly cloudy cells)
-  double fa_n[4], fc_n[4], fa_q[4], fc_q[4];
 #pragma unroll
   for (int m = 0; m < 4; ++m) {
-    fa_n[m] = fc_n[m] = fa_q[m] = fc_q[m] = 0.0;
-    if (cat == 2) {
-      partition_factors(ld2(f_ni(m), TO_VMR_NUM), ld2(f_nc(m), TO_VMR_NUM), fcldy, fclea,
-                        fa_n[m], fc_n[m]);
+    X.fa_n[m] = X.fc_n[m] = X.fa_q[m] = X.fc_q[m] = 0.0;
+    if (X.cat == 2) {
+      partition_factors(ld2<PRE>(a, off, f_ni(m), TO_VMR_NUM),
+                        ld2<PRE>(a, off, f_nc(m), TO_VMR_NUM), X.fcldy, X.fclea, X.fa_n[m],
+                        X.fc_n[m]);
       double qi = 0.0, qc = 0.0;
 #pragma unroll
       for (int s = 0; s < 7; ++s) {
         if (s >= NSPEC_MODE[m]) continue;
-        qi = qi + ld2(f_qi(m, s), to_vmr_mass(m, s));
-        qc = qc + ld2(f_qc(m, s), to_vmr_mass(m, s));
+        qi = qi + ld2<PRE>(a, off, f_qi(m, s), to_vmr_mass(m, s));
+        qc = qc + ld2<PRE>(a, off, f_qc(m, s), to_vmr_mass(m, s));
       }
-      partition_factors(qi, qc, fcldy, fclea, fa_q[m], fc_q[m]);
+      partition_factors(qi, qc, X.fcldy, X.fclea, X.fa_q[m], X.fc_q[m]);
     }
   }
+  X.map_gas = a.cfg.do_cond || a.cfg.do_newnuc;
+}
 
-  // subarea results in chain units before the area weighting:
-  // interstitial + gases of the clear (oc_*) and cloudy (od_*) subareas
-  int niter = 0;
+// Clear-subarea state (set_subarea_gases_and_aerosols, mam4_amicphys.hpp:
+// 678-911, and the mapping of mam_amicphys_1gridcell, :1784-1851).
+struct ClearGas {
+  double g1[2], g[2];  // {SOAG, H2SO4} before gas chem / current, chain units
+  double hcl, scl;     // clear-subarea gases before mapping
+  double hcd, scd;     // cloudy-subarea gases (partly cloudy cells)
+};
+
+__device__ __forceinline__ void clear_build(const ChainArgs &a, long off, const CellCtx &X,
+                                            Aer &S, ClearGas &G) {
+#pragma unroll
+  for (int m = 0; m < 4; ++m) {
+    const double nn = ld3(a, off, f_ni(m), TO_VMR_NUM);
+    S.n[m] = (X.cat == 2 ? nn * X.fa_n[m] : nn) * 1.0;
+#pragma unroll
+    for (int s = 0; s < 7; ++s) {
+      if (s >= NSPEC_MODE[m]) continue;
+      const int sp = mode_spec(m, s);
+      const double qq = ld3(a, off, f_qi(m, s), to_vmr_mass(m, s));
+      S.q[sp][m] = (X.cat == 2 ? qq * X.fa_q[m] : qq) * fcvt_aer(sp);
+    }
+  }
+  S.q[A_SO4][NPCA] = 0.0;
+  S.q[A_SOA][NPCA] = 0.0;
+  S.q[A_NCL][NPCA] = 0.0;
+  G.hcl = (X.cat == 2) ? X.h2 : X.h3;
+  G.scl = (X.cat == 2) ? X.s2 : X.s3;
+  G.hcd = 0.0;
+  G.scd = 0.0;
+  if (X.cat == 2) {
+    G.hcd = (X.h3 - X.fclea * G.hcl) / X.fcldy;
+    if (G.hcd < 0) {
+      G.hcd = 0;
+      G.hcl = X.h3 / X.fclea;
+    }
+    G.scd = (X.s3 - X.fclea * G.scl) / X.fcldy;
+    if (G.scd < 0) {
+      G.scd = 0;
+      G.scl = X.s3 / X.fclea;
+    }
+  }
+  G.g1[0] = X.map_gas ? X.s1 * FCVT_SOAG : 0.0;
+  G.g1[1] = X.map_gas ? X.h1 * 1.0 : 0.0;
+  G.g[0] = X.map_gas ? G.scl * FCVT_SOAG : 0.0;
+  G.g[1] = X.map_gas ? G.hcl * 1.0 : 0.0;
+}
+
+// Cloudy subarea (renaming only), recombination, calcsize and the writes.
+// S/G: the clear subarea after clear_post (unused for overcast cells).
+template <bool PRE>
+__device__ __forceinline__ void cell_finish(const ChainArgs &a, long off, const CellCtx &X,
+                                            const Aer &S, const ClearGas &G) {
+  const ChainConsts &K = *a.K;
+  const double dt = a.dt;
   double gi_out[NSTATE];  // grid-cell mean after the chain, chain units
-  const bool map_gas = cfg.do_cond || cfg.do_newnuc;
-
-  if (cat != 1) {
-    // ---- clear subarea ----
-    Aer S;
-#pragma unroll
-    for (int m = 0; m < 4; ++m) {
-      const double nn = ld3(f_ni(m), TO_VMR_NUM);
-      S.n[m] = (cat == 2 ? nn * fa_n[m] : nn) * 1.0;
-#pragma unroll
-      for (int s = 0; s < 7; ++s) {
-        if (s >= NSPEC_MODE[m]) continue;
-        const int sp = mode_spec(m, s);
-        const double qq = ld3(f_qi(m, s), to_vmr_mass(m, s));
-        S.q[sp][m] = (cat == 2 ? qq * fa_q[m] : qq) * fcvt_aer(sp);
-      }
-    }
-    S.q[A_SO4][NPCA] = 0.0;
-    S.q[A_SOA][NPCA] = 0.0;
-    S.q[A_NCL][NPCA] = 0.0;
-    // gases: before gas chem (1) and current (3) in the clear subarea
-    double hcl = (cat == 2) ? h2 : h3, scl = (cat == 2) ? s2 : s3;
-    double hcd = 0.0, scd = 0.0;  // cloudy-subarea gases (cat 2)
-    if (cat == 2) {
-      hcd = (h3 - fclea * hcl) / fcldy;
-      if (hcd < 0) {
-        hcd = 0;
-        hcl = h3 / fclea;
-      }
-      scd = (s3 - fclea * scl) / fcldy;
-      if (scd < 0) {
-        scd = 0;
-        scl = s3 / fclea;
-      }
-    }
-    double g[2] = {0.0, 0.0};
-    const double g1[2] = {map_gas ? s1 * FCVT_SOAG : 0.0, map_gas ? h1 * 1.0 : 0.0};
-    if (map_gas) {
-      g[0] = scl * FCVT_SOAG;
-      g[1] = hcl * 1.0;
-    }
-    niter = clear_subarea(K, cfg, dt, T, p, z, pblh, rh_clear, dga, dgw, wd, g1, g, S);
+  if (X.cat != 1) {
     // map back (mam4_amicphys.hpp:1940-1969) and weight by the clear fraction
-    const double hout = map_gas ? g[1] / 1.0 : hcl;
-    const double sout = map_gas ? g[0] / FCVT_SOAG : scl;
-    gi_out[F_H2SO4] = hout * fclea;
-    gi_out[F_SOAG] = sout * fclea;
-    if (cat == 2) {
-      gi_out[F_H2SO4] += hcd * fcldy;
-      gi_out[F_SOAG] += scd * fcldy;
+    const double hout = X.map_gas ? G.g[1] / 1.0 : G.hcl;
+    const double sout = X.map_gas ? G.g[0] / FCVT_SOAG : G.scl;
+    gi_out[F_H2SO4] = hout * X.fclea;
+    gi_out[F_SOAG] = sout * X.fclea;
+    if (X.cat == 2) {
+      gi_out[F_H2SO4] += G.hcd * X.fcldy;
+      gi_out[F_SOAG] += G.scd * X.fcldy;
     }
 #pragma unroll
     for (int m = 0; m < 4; ++m) {
-      gi_out[f_ni(m)] = (S.n[m] / 1.0) * fclea;
+      gi_out[f_ni(m)] = (S.n[m] / 1.0) * X.fclea;
 #pragma unroll
       for (int s = 0; s < 7; ++s) {
         if (s >= NSPEC_MODE[m]) continue;
         const int sp = mode_spec(m, s);
-        gi_out[f_qi(m, s)] = (S.q[sp][m] / fcvt_aer(sp)) * fclea;
+        gi_out[f_qi(m, s)] = (S.q[sp][m] / fcvt_aer(sp)) * X.fclea;
       }
     }
   } else {
-    gi_out[F_H2SO4] = h3 * 1.0;
-    gi_out[F_SOAG] = s3 * 1.0;
+    gi_out[F_H2SO4] = X.h3 * 1.0;
+    gi_out[F_SOAG] = X.s3 * 1.0;
   }
 
-  if (cat != 0) {
+  if (X.cat != 0) {
     // ---- cloudy subarea: renaming of interstitial + cloud-borne only ----
     Aer C, Cc;
     double gi[7] = {0, 0, 0, 0, 0, 0, 0}, gc[7] = {0, 0, 0, 0, 0, 0, 0};
 #pragma unroll
     for (int m = 0; m < 4; ++m) {
-      const double n3 = ld3(f_ni(m), TO_VMR_NUM);
-      const double nc3 = ld3(f_nc(m), TO_VMR_NUM);
-      C.n[m] = (cat == 2 ? n3 * fc_n[m] : n3) * 1.0;
-      Cc.n[m] = (cat == 2 ? nc3 / fcldy : nc3) * 1.0;
+      const double n3 = ld3(a, off, f_ni(m), TO_VMR_NUM);
+      const double nc3 = ld3(a, off, f_nc(m), TO_VMR_NUM);
+      C.n[m] = (X.cat == 2 ? n3 * X.fc_n[m] : n3) * 1.0;
+      Cc.n[m] = (X.cat == 2 ? nc3 / X.fcldy : nc3) * 1.0;
 #pragma unroll
       for (int s = 0; s < 7; ++s) {
         if (s >= NSPEC_MODE[m]) continue;
         const int sp = mode_spec(m, s);
-        const double q3 = ld3(f_qi(m, s), to_vmr_mass(m, s));
-        const double qc3 = ld3(f_qc(m, s), to_vmr_mass(m, s));
-        C.q[sp][m] = (cat == 2 ? q3 * fc_q[m] : q3) * fcvt_aer(sp);
-        Cc.q[sp][m] = (cat == 2 ? qc3 / fcldy : qc3) * fcvt_aer(sp);
+        const double q3 = ld3(a, off, f_qi(m, s), to_vmr_mass(m, s));
+        const double qc3 = ld3(a, off, f_qc(m, s), to_vmr_mass(m, s));
+        C.q[sp][m] = (X.cat == 2 ? q3 * X.fc_q[m] : q3) * fcvt_aer(sp);
+        Cc.q[sp][m] = (X.cat == 2 ? qc3 / X.fcldy : qc3) * fcvt_aer(sp);
         if (m == NAIT) {
-          const double q2 = ld2(f_qi(m, s), to_vmr_mass(m, s));
-          const double qc2 = ld2(f_qc(m, s), to_vmr_mass(m, s));
-          const double a2 = (cat == 2 ? q2 * fc_q[m] : q2) * fcvt_aer(sp);
-          const double c2 = (cat == 2 ? qc2 / fcldy : qc2) * fcvt_aer(sp);
+          const double q2 = ld2<PRE>(a, off, f_qi(m, s), to_vmr_mass(m, s));
+          const double qc2 = ld2<PRE>(a, off, f_qc(m, s), to_vmr_mass(m, s));
+          const double a2 = (X.cat == 2 ? q2 * X.fc_q[m] : q2) * fcvt_aer(sp);
+          const double c2 = (X.cat == 2 ? qc2 / X.fcldy : qc2) * fcvt_aer(sp);
           gi[sp] = (C.q[sp][m] - a2) / 1;
           gc[sp] = (Cc.q[sp][m] - c2) / 1;
         }
       }
     }
-    if (cfg.do_rename) rename_ait_to_acc(K, true, C, gi, Cc, gc);
-    const double w = fcldy;
+    if (a.cfg.do_rename) rename_ait_to_acc(K, true, C, gi, Cc, gc);
+    const double w = X.fcldy;
 #pragma unroll
     for (int m = 0; m < 4; ++m) {
       const double ni = (C.n[m] / 1.0) * w;
-      gi_out[f_ni(m)] = (cat == 2) ? gi_out[f_ni(m)] + ni : ni;
+      gi_out[f_ni(m)] = (X.cat == 2) ? gi_out[f_ni(m)] + ni : ni;
       gi_out[f_nc(m)] = (Cc.n[m] / 1.0) * w;
 #pragma unroll
       for (int s = 0; s < 7; ++s) {
         if (s >= NSPEC_MODE[m]) continue;
         const int sp = mode_spec(m, s);
         const double qi = (C.q[sp][m] / fcvt_aer(sp)) * w;
-        gi_out[f_qi(m, s)] = (cat == 2) ? gi_out[f_qi(m, s)] + qi : qi;
+        gi_out[f_qi(m, s)] = (X.cat == 2) ? gi_out[f_qi(m, s)] + qi : qi;
         gi_out[f_qc(m, s)] = (Cc.q[sp][m] / fcvt_aer(sp)) * w;
       }
     }
@@ -234,11 +260,11 @@ __device__ __forceinline__ int chain_cell(const ChainArgs &a, long off, int icol
     // no cloudy subarea: cloud-borne aerosol keeps its (clipped) value
 #pragma unroll
     for (int m = 0; m < 4; ++m) {
-      gi_out[f_nc(m)] = ld3(f_nc(m), TO_VMR_NUM);
+      gi_out[f_nc(m)] = ld3(a, off, f_nc(m), TO_VMR_NUM);
 #pragma unroll
       for (int s = 0; s < 7; ++s) {
         if (s >= NSPEC_MODE[m]) continue;
-        gi_out[f_qc(m, s)] = ld3(f_qc(m, s), to_vmr_mass(m, s));
+        gi_out[f_qc(m, s)] = ld3(a, off, f_qc(m, s), to_vmr_mass(m, s));
       }
     }
   }
@@ -294,14 +320,17 @@ __device__ __forceinline__ int chain_cell(const ChainArgs &a, long off, int icol
     }
   } else {
 #pragma unroll
-    for (int m = 0; m < 4; ++m) dgi[m] = dgc[m] = dga[m];
+    for (int m = 0; m < 4; ++m) dgi[m] = dgc[m] = X.dga[m];
+  }
+  if (a.has_sout) {
+    double st[NSTATE];
+#pragma unroll
+    for (int f = 0; f < NSTATE; ++f) st[f] = a.state[f][off];
+#pragma unroll
+    for (int f = 0; f < NSTATE; ++f) a.sout[f][off] = st[f] + dt * tend[f];
   }
 #pragma unroll
   for (int f = 0; f < NSTATE; ++f) a.tend[f][off] = tend[f];
-  if (a.has_sout) {
-#pragma unroll
-    for (int f = 0; f < NSTATE; ++f) a.sout[f][off] = a.state[f][off] + dt * tend[f];
-  }
   if (a.has_dgn) {
 #pragma unroll
     for (int m = 0; m < 4; ++m) {
@@ -309,6 +338,38 @@ __device__ __forceinline__ int chain_cell(const ChainArgs &a, long off, int icol
       if (a.dgc[m]) a.dgc[m][off] = dgc[m];
     }
   }
+}
+
+// The whole step for one cell in one lane.  Returns the SOA sub-step count.
+template <bool PRE>
+__device__ __forceinline__ int chain_cell_fused(const ChainArgs &a, long off, int icol) {
+  const ChainConsts &K = *a.K;
+  CellCtx X;
+  cell_setup<PRE>(a, off, icol, X);
+  Aer S;
+  ClearGas G;
+  int niter = 0;
+  if (X.cat != 1) {
+    clear_build(a, off, X, S, G);
+    SoaIn in = {};
+    double sg = 0.0, sgavg = 0.0, sa[4] = {0, 0, 0, 0};
+    if (a.cfg.do_cond) {
+      clear_pre(K, X.T, X.p, X.dgw, S, G.g[0], in);
+      SoaState st;
+      soa_init(st, in);
+      const double dtfull = (double)(int)(a.dt / 1);
+      while (soa_tick(st, dtfull)) {
+      }
+      niter = st.niter;
+      sg = st.g;
+      sgavg = soa_gavg(st);
+#pragma unroll
+      for (int n = 0; n < 4; ++n) sa[n] = st.a[n];
+    }
+    clear_post(K, a.cfg, a.dt, X.T, X.p, X.z, X.pblh, X.rh_clear, X.dga, X.dgw, X.wd, G.g1,
+               G.g, S, in.uptk, sg, sgavg, sa);
+  }
+  cell_finish<PRE>(a, off, X, S, G);
   return niter;
 }
 

@@ -88,109 +88,145 @@ __device__ __forceinline__ void h2so4_uptake(const ChainConsts &K, double temp,
 }
 
 // ------------------------------------------------------------------ SOA
-// mam_soaexch_1subarea (gasaerexch_soaexch.hpp:79-341), one SOA species.
-// Returns the number of sub-steps.  qsoa[n]: SOA in mode n; pom[n]: POM in
-// mode n (for the oxygenated-POM term; 0.1 x POM except the pca mode).
-__device__ __forceinline__ int soa_exchange(const ChainConsts &K, double dtfull,
-                                            double temp, double pmid,
-                                            const double uptk_soag[4], double &gsoa,
-                                            double &gsoa_avg, double qsoa[4],
-                                            const double pom[4]) {
-  const double alpha_astem = 0.05, a_min1 = 1.0e-20, g_min1 = 1.0e-20;
-  const double p0_soa = 1.0e-10 * exp(-(K.soa_c1) * ((1.0 / temp) - K.soa_inv298));
-  const double g0_soa = 1.01325e5 * p0_soa / pmid;
+// mam_soaexch_1subarea (gasaerexch_soaexch.hpp:79-341), one SOA species, as
+// a resumable state machine: soa_tick advances one adaptive sub-step, so the
+// persistent work-queue kernel can hand a lane a new cell as soon as its
+// current one has converged.
+struct SoaIn {        // inputs of the SOA loop of one clear subarea
+  double g;           // SOAG [mol/mol, fcvt applied]
+  double a[4];        // SOA in the four modes
+  double uptk[4];     // H2SO4 uptake rates [1/s]; SOAG's are 0.81x
+  double aopoa[4];    // oxygenated POM (0.1 x POM, 0 in the pcarbon mode)
+  double g0;          // equilibrium SOAG mixing ratio
+};
+
+struct SoaState {
+  double g, a[4], uk[4], aopoa[4], g0;
+  double tcur, avg, dtsum;
+  int niter;
   bool skip[4];
-  double uk[4], a_opoa[4], a_soa[4];
+};
+
+__device__ __forceinline__ double soa_g0(const ChainConsts &K, double temp, double pmid) {
+  const double p0_soa = 1.0e-10 * exp(-(K.soa_c1) * ((1.0 / temp) - K.soa_inv298));
+  return 1.01325e5 * p0_soa / pmid;
+}
+
+__device__ __forceinline__ void soa_init(SoaState &s, const SoaIn &in) {
+  s.g = in.g;
+  s.g0 = in.g0;
 #pragma unroll
   for (int n = 0; n < 4; ++n) {
-    skip[n] = !(uptk_soag[n] > 1.0e-15);
-    uk[n] = skip[n] ? 0.0 : uptk_soag[n];
-    // opoa_frac = 0.1 (0 for the pcarbon mode), gasaerexch_soaexch.hpp:123-138
-    a_opoa[n] = (n == NPCA) ? 0.0 * kmax(pom[n], 0.0) : 0.1 * kmax(pom[n], 0.0);
+    const double uk = in.uptk[n] * 0.81;  // gasaerexch.hpp:458-465
+    s.skip[n] = !(uk > 1.0e-15);
+    s.uk[n] = s.skip[n] ? 0.0 : uk;
+    s.a[n] = in.a[n];
+    s.aopoa[n] = in.aopoa[n];
+  }
+  s.tcur = 0.0;
+  s.avg = 0.0;
+  s.dtsum = 0.0;
+  s.niter = 0;
+}
+
+// rate of the first sub-step times dtfull / alpha: the sub-step count the loop
+// would need at its initial rate (used to order the work queue longest-first)
+__device__ __forceinline__ double soa_predict(const SoaState &s, double dtfull) {
+  const double g_soa = kmax(s.g, 0.0);
+  double tmpb = 0.0;
+#pragma unroll
+  for (int n = 0; n < 4; ++n) {
+    if (s.skip[n]) continue;
+    const double a_soa = kmax(s.a[n], 0.0);
+    const double sat = s.g0 / kmax(s.aopoa[n] + a_soa, 1.0e-20);
+    const double gstar = sat * a_soa;
+    tmpb += s.uk[n] * fabs((g_soa - gstar) / kmax(g_soa, kmax(gstar, 1.0e-20)));
+  }
+  return dtfull * tmpb / 0.05;
+}
+
+// One pass of the while loop (gasaerexch_soaexch.hpp:170-335).  Returns false
+// once the loop has ended (time reached or 1000 sub-steps taken).
+__device__ __forceinline__ bool soa_tick(SoaState &s, double dtfull) {
+  const double alpha_astem = 0.05, a_min1 = 1.0e-20, g_min1 = 1.0e-20;
+  if (!(s.tcur < dtfull - 1.0e-3)) return false;
+  s.niter++;
+  if (s.niter > 1000) return false;
+  const double gprv = s.g;
+  double g_soa = kmax(gprv, 0.0);
+  double tot = g_soa;
+  double a_soa[4], sat[4], gstar[4], beta[4];
+#pragma unroll
+  for (int n = 0; n < 4; ++n) {
     a_soa[n] = 0.0;
+    if (!s.skip[n]) {
+      a_soa[n] = kmax(s.a[n], 0.0);
+      tot += a_soa[n];
+    }
   }
-  double tcur = 0.0, avg = 0.0, dtsum = 0.0;
-  double g = gsoa;
-  double aprv[4];
+  double tmpb = 0.0;
 #pragma unroll
-  for (int n = 0; n < 4; ++n) aprv[n] = qsoa[n];
-  int niter = 0;
-  while (tcur < dtfull - 1.0e-3) {
-    niter++;
-    if (niter > 1000) break;
-    const double gprv = g;
-    double g_soa = kmax(gprv, 0.0);
-    double tot = g_soa;
-    double sat[4], gstar[4], beta[4];
-    double tmpb = 0.0;
-#pragma unroll
-    for (int n = 0; n < 4; ++n) {
-      if (!skip[n]) {
-        a_soa[n] = kmax(aprv[n], 0.0);
-        tot += a_soa[n];
-      }
+  for (int n = 0; n < 4; ++n) {
+    sat[n] = 0.0;
+    gstar[n] = 0.0;
+    beta[n] = 0.0;
+    if (!s.skip[n]) {
+      const double sumooa = s.aopoa[n] + a_soa[n];
+      sat[n] = s.g0 / kmax(sumooa, a_min1);
+      gstar[n] = sat[n] * a_soa[n];
+      const double phi = (g_soa - gstar[n]) / kmax(g_soa, kmax(gstar[n], g_min1));
+      tmpb += s.uk[n] * fabs(phi);
     }
-#pragma unroll
-    for (int n = 0; n < 4; ++n) {
-      sat[n] = 0.0;
-      gstar[n] = 0.0;
-      beta[n] = 0.0;
-      if (!skip[n]) {
-        const double sumooa = a_opoa[n] + a_soa[n];
-        sat[n] = g0_soa / kmax(sumooa, a_min1);
-        gstar[n] = sat[n] * a_soa[n];
-        const double phi = (g_soa - gstar[n]) / kmax(g_soa, kmax(gstar[n], g_min1));
-        tmpb += uk[n] * fabs(phi);
-      }
-    }
-    const double tmpa = kmax(0.0, tmpb);
-    double dtcur;
-    const double dtmax = dtfull - tcur;
-    if (dtmax * tmpa <= alpha_astem) {
-      dtcur = dtmax;
-      tcur = dtfull;
-    } else {
-      dtcur = alpha_astem / tmpa;
-      tcur += dtcur;
-    }
-    // step 1: explicit estimate of condensing modes' a_soa and sat
-#pragma unroll
-    for (int n = 0; n < 4; ++n) {
-      if (!skip[n]) {
-        beta[n] = dtcur * uk[n];
-        const double del = g_soa - gstar[n];
-        if (del > 0.0) {
-          const double at = a_soa[n] + beta[n] * del;
-          sat[n] = g0_soa / kmax(a_opoa[n] + at, a_min1);
-        }
-      }
-    }
-    // step 2: implicit in g, semi-implicit in a
-    double sa = 0.0, sb = 0.0;
-#pragma unroll
-    for (int n = 0; n < 4; ++n) {
-      if (!skip[n]) {
-        const double den = 1.0 + beta[n] * sat[n];
-        sa += a_soa[n] / den;
-        sb += beta[n] / den;
-      }
-    }
-    g_soa = (tot - sa) / (1.0 + sb);
-    g_soa = kmax(0.0, g_soa);
-#pragma unroll
-    for (int n = 0; n < 4; ++n) {
-      if (!skip[n]) a_soa[n] = (a_soa[n] + beta[n] * g_soa) / (1.0 + beta[n] * sat[n]);
-      aprv[n] = a_soa[n];  // skipped modes: a_soa never set -> 0 (reference quirk)
-    }
-    g = g_soa;
-    avg += dtcur * (gprv + 0.5 * (g - gprv));
-    dtsum += dtcur;
   }
-  gsoa = g;
+  const double tmpa = kmax(0.0, tmpb);
+  double dtcur;
+  const double dtmax = dtfull - s.tcur;
+  if (dtmax * tmpa <= alpha_astem) {
+    dtcur = dtmax;
+    s.tcur = dtfull;
+  } else {
+    dtcur = alpha_astem / tmpa;
+    s.tcur += dtcur;
+  }
+  // step 1: explicit estimate of condensing modes' a_soa and sat
 #pragma unroll
-  for (int n = 0; n < 4; ++n) qsoa[n] = aprv[n];
-  gsoa_avg = kmax(0.0, avg / dtsum);
-  return niter;
+  for (int n = 0; n < 4; ++n) {
+    if (!s.skip[n]) {
+      beta[n] = dtcur * s.uk[n];
+      const double del = g_soa - gstar[n];
+      if (del > 0.0) {
+        const double at = a_soa[n] + beta[n] * del;
+        sat[n] = s.g0 / kmax(s.aopoa[n] + at, a_min1);
+      }
+    }
+  }
+  // step 2: implicit in g, semi-implicit in a
+  double sa = 0.0, sb = 0.0;
+#pragma unroll
+  for (int n = 0; n < 4; ++n) {
+    if (!s.skip[n]) {
+      const double den = 1.0 + beta[n] * sat[n];
+      sa += a_soa[n] / den;
+      sb += beta[n] / den;
+    }
+  }
+  g_soa = (tot - sa) / (1.0 + sb);
+  g_soa = kmax(0.0, g_soa);
+#pragma unroll
+  for (int n = 0; n < 4; ++n) {
+    // skipped modes: a_soa is never set, so their SOA becomes 0 (quirk of
+    // gasaerexch_soaexch.hpp:327-331)
+    s.a[n] = s.skip[n] ? 0.0 : (a_soa[n] + beta[n] * g_soa) / (1.0 + beta[n] * sat[n]);
+  }
+  s.g = g_soa;
+  s.avg += dtcur * (gprv + 0.5 * (g_soa - gprv));
+  s.dtsum += dtcur;
+  return true;
+}
+
+// qgas_avg = max(0, sum(q dt) / sum(dt)) (gasaerexch_soaexch.hpp:337-340)
+__device__ __forceinline__ double soa_gavg(const SoaState &s) {
+  return kmax(0.0, s.avg / s.dtsum);
 }
 
 // ------------------------------------------------------------------ H2SO4
@@ -702,16 +738,39 @@ __device__ __forceinline__ void age_pcarbon(const ChainConsts &K, unsigned nmono
 }
 
 // ------------------------------------------------------------------ subarea
-// mam_amicphys_1subarea (mam4_amicphys.hpp:1126-1694) for a clear subarea
-// (condensation, renaming, nucleation, coagulation, aging).  g1/g: gases
-// {SOAG, H2SO4} before gas chemistry / current (chain units, fcvt applied).
-// Returns the SOA sub-step count.
-__device__ __forceinline__ int clear_subarea(const ChainConsts &K, const Cfg &cfg,
-                                             double deltat, double temp, double pmid,
-                                             double zmid, double pblh, double relhum,
-                                             const double dgn_a[4], const double dgn_awet[4],
-                                             const double wetdens[4], const double g1[2],
-                                             double g[2], Aer &S) {
+// mam_amicphys_1subarea (mam4_amicphys.hpp:1126-1694) for a clear subarea,
+// split around the SOA loop.
+//
+// clear_pre: the part of mam_gasaerexch_1subarea before the SOA loop
+// (gasaerexch.hpp:387-477): H2SO4 uptake rates and the loop's inputs.
+__device__ __forceinline__ void clear_pre(const ChainConsts &K, double temp, double pmid,
+                                          const double dgn_awet[4], const Aer &S,
+                                          double g_soag, SoaIn &o) {
+  const double aircon = pmid / (8314.46759100000 * temp);
+  h2so4_uptake(K, temp, pmid, aircon, dgn_awet, S.n, o.uptk);
+  o.g = g_soag;
+#pragma unroll
+  for (int n = 0; n < 4; ++n) o.a[n] = S.q[A_SOA][n];
+  // opoa_frac = 0.1, 0 for the pcarbon mode (gasaerexch_soaexch.hpp:123-138)
+  o.aopoa[NACC] = 0.1 * kmax(S.q[A_POM][NACC], 0.0);
+  o.aopoa[NAIT] = 0.1 * kmax(0.0, 0.0);
+  o.aopoa[NCOA] = 0.1 * kmax(S.q[A_POM][NCOA], 0.0);
+  o.aopoa[NPCA] = 0.0 * kmax(S.q[A_POM][NPCA], 0.0);
+  o.g0 = soa_g0(K, temp, pmid);
+}
+
+// clear_post: everything after the SOA loop -- the H2SO4 update, renaming,
+// nucleation, coagulation and aging.  uptk: H2SO4 uptake rates from
+// clear_pre; soa_g/soa_gavg/soa_a: SOA loop results (ignored without
+// condensation).  g1/g: gases {SOAG, H2SO4} before gas chemistry / current.
+__device__ __forceinline__ void clear_post(const ChainConsts &K, const Cfg &cfg,
+                                           double deltat, double temp, double pmid,
+                                           double zmid, double pblh, double relhum,
+                                           const double dgn_a[4], const double dgn_awet[4],
+                                           const double wetdens[4], const double g1[2],
+                                           double g[2], Aer &S, const double uptk[4],
+                                           double soa_g, double soa_gavg,
+                                           const double soa_a[4]) {
   const double aircon = pmid / (8314.46759100000 * temp);
   double netprod = 0.0;
   if (cfg.do_cond && cfg.gaexch_h2so4_uptake_optaa == 2) {
@@ -722,26 +781,19 @@ __device__ __forceinline__ int clear_subarea(const ChainConsts &K, const Cfg &cf
   double uptkrate_h2so4 = 0;
   double gavg[2];
   double dc_so4[4] = {0, 0, 0, 0}, dc_soa[4] = {0, 0, 0, 0};
-  int niter = 0;
   if (cfg.do_cond) {
-    double uptk[4];
-    h2so4_uptake(K, temp, pmid, aircon, dgn_awet, S.n, uptk);
-    double uptk_soag[4];
-#pragma unroll
-    for (int n = 0; n < 4; ++n) uptk_soag[n] = uptk[n] * 0.81;
 #pragma unroll
     for (int n = 0; n < 4; ++n) uptkrate_h2so4 += uptk[n];
-    double soa[4] = {S.q[A_SOA][0], S.q[A_SOA][1], S.q[A_SOA][2], S.q[A_SOA][3]};
+    g[0] = soa_g;
+    gavg[0] = soa_gavg;
     double so4[4] = {S.q[A_SO4][0], S.q[A_SO4][1], S.q[A_SO4][2], S.q[A_SO4][3]};
-    const double pom[4] = {S.q[A_POM][NACC], 0.0, S.q[A_POM][NCOA], S.q[A_POM][NPCA]};
-    niter = soa_exchange(K, dtsub, temp, pmid, uptk_soag, g[0], gavg[0], soa, pom);
     const double so4_0[4] = {so4[0], so4[1], so4[2], so4[3]};
     h2so4_update(dtsub, netprod, uptk, g[1], gavg[1], so4);
 #pragma unroll
     for (int n = 0; n < 4; ++n) {
-      dc_soa[n] = soa[n] - S.q[A_SOA][n];
+      dc_soa[n] = soa_a[n] - S.q[A_SOA][n];
       dc_so4[n] = so4[n] - so4_0[n];
-      S.q[A_SOA][n] = soa[n];
+      S.q[A_SOA][n] = soa_a[n];
       S.q[A_SO4][n] = so4[n];
     }
   } else {
@@ -762,7 +814,6 @@ __device__ __forceinline__ int clear_subarea(const ChainConsts &K, const Cfg &cf
   if (cfg.do_coag)
     coagulate(K, dtsub, temp, pmid, aircon, dgn_awet, wetdens, S, cin_so4, cin_soa);
   age_pcarbon(K, cfg.nmono, dgn_a[NPCA], S, dc_so4[NPCA], dc_soa[NPCA], cin_so4, cin_soa);
-  return niter;
 }
 
 // ------------------------------------------------------------------ calcsize

@@ -1,12 +1,26 @@
 // MI355X (gfx950) implementation of the MAM4 microphysics chain and its C ABI
 // (include/mam4x.h).
 //
-// Launch shape: one lane per (column, level) cell, cells enumerated level-
-// major / column-minor so consecutive lanes read consecutive addresses of each
-// [level][column] plane (fully coalesced 64-lane loads).  The chain is VALU
-// bound (no dense contraction anywhere), so there is no MFMA or LDS tiling;
-// the sigma-only constants come from a small device buffer through the
-// scalar cache.
+// Cells are enumerated level-major / column-minor (c = k * ncol + i), so
+// consecutive lanes read consecutive addresses of each [level][column] plane
+// (fully coalesced 64-lane loads).  The chain is FP64 VALU work with no dense
+// contraction, so there is no MFMA or LDS tiling; the sigma-only constants
+// come from a small device buffer through the scalar cache.
+//
+// The cost of a cell is dominated by a data-dependent loop: the adaptive SOA
+// sub-stepping (gasaerexch_soaexch.hpp:170-335) takes 1 to 1000 sub-steps, a
+// heavy-tailed count.  One lane per cell would let every wave run as long as
+// its slowest lane, so a step is four launches:
+//   1. chain_pre_kernel   per cell: subareas, clear-subarea state, H2SO4
+//                         uptake rates and the SOA loop inputs -> workspace;
+//                         a predicted sub-step count -> cost bucket
+//   2. queue_scatter_kernel  cells with SOA work -> a queue ordered by
+//                         bucket, longest first
+//   3. soa_queue_kernel   persistent waves drain the queue; a lane takes the
+//                         next cell as soon as its current one has converged
+//   4. chain_post_kernel  per cell: the rest of the chain, recombination,
+//                         calcsize, tendencies
+// chain_fused_kernel runs everything in one lane per cell (small domains).
 #include <hip/hip_runtime.h>
 
 #include <cstdio>
@@ -22,22 +36,228 @@ using namespace mam4x;
 namespace {
 
 constexpr int kBlock = 256;
+constexpr int kNBucket = 24;  // half-octave cost buckets: 2 log2(1 + predicted sub-steps)
+constexpr int kChunk = 64;    // queue entries a wave takes per refill
+constexpr unsigned char kNoSoa = 0xff;
+
+// workspace planes over the compact cell index; phase 3 reads the SOA loop
+// results back from W_G (SOAG), W_A.. (SOA per mode) and W_GAVG
+enum { W_G = 0, W_A = 1, W_UPTK = 5, W_AOPOA = 9, W_G0 = 13, W_NPLANE = 14 };
+constexpr int W_GAVG = W_G0;
+
+struct QueueCtl {
+  unsigned hist[kNBucket];  // cells per bucket
+  unsigned fill[kNBucket];  // scatter cursor per bucket
+  unsigned head;            // next free queue slot
+  unsigned pad;
+};
+
+struct Work {
+  double *ws;              // W_NPLANE planes of ncell doubles
+  unsigned char *bucket;   // per cell, kNoSoa when the cell has no SOA loop
+  int *queue;              // cell indices, longest first
+  QueueCtl *ctl;
+  long ncell;
+  double dtfull;           // SOA loop length: the integer sub-step (:1319)
+};
+
+__device__ __forceinline__ void cell_index(const ChainArgs &a, long c, int &icol, long &off) {
+  const int k = (int)(c / a.ncol);
+  icol = (int)(c - (long)k * a.ncol);
+  off = (long)k * a.ldc + icol;
+}
+
+__device__ __forceinline__ void add_stats(unsigned long long *stats, unsigned long long v) {
+  if (!stats) return;  // uniform branch: wave-reduce, one atomic per wave
+  for (int d = 32; d > 0; d >>= 1) v += __shfl_xor(v, d, 64);
+  if ((threadIdx.x & 63) == 0 && v) atomicAdd(stats, v);
+}
 
 template <bool PRE>
-__global__ void __launch_bounds__(kBlock) chain_fp64_kernel(ChainArgs a) {
+__global__ void __launch_bounds__(kBlock) chain_fused_kernel(ChainArgs a) {
   const long ncell = (long)a.ncol * a.nlev;
   const long c = (long)blockIdx.x * kBlock + threadIdx.x;
   unsigned long long niter = 0;
   if (c < ncell) {
-    const int k = (int)(c / a.ncol);
-    const int icol = (int)(c - (long)k * a.ncol);
-    const long off = (long)k * a.ldc + icol;
-    niter = (unsigned long long)dev::chain_cell<PRE>(a, off, icol);
+    int icol;
+    long off;
+    cell_index(a, c, icol, off);
+    niter = (unsigned long long)dev::chain_cell_fused<PRE>(a, off, icol);
   }
-  if (a.stats) {  // uniform branch: wave-reduce, one atomic per wave
-    for (int d = 32; d > 0; d >>= 1) niter += __shfl_xor(niter, d, 64);
-    if ((threadIdx.x & 63) == 0) atomicAdd(a.stats, niter);
+  add_stats(a.stats, niter);
+}
+
+__global__ void queue_reset_kernel(QueueCtl *q) {
+  const int t = threadIdx.x;
+  if (t < kNBucket) {
+    q->hist[t] = 0;
+    q->fill[t] = 0;
   }
+  if (t == 0) q->head = 0;
+}
+
+// Phase 1.
+template <bool PRE>
+__global__ void __launch_bounds__(kBlock) chain_pre_kernel(ChainArgs a, Work w) {
+  __shared__ unsigned lh[kNBucket];
+  if (threadIdx.x < kNBucket) lh[threadIdx.x] = 0;
+  __syncthreads();
+  const long c = (long)blockIdx.x * kBlock + threadIdx.x;
+  if (c < w.ncell) {
+    int icol;
+    long off;
+    cell_index(a, c, icol, off);
+    dev::CellCtx X;
+    dev::cell_setup<PRE>(a, off, icol, X);
+    unsigned char b = kNoSoa;
+    if (X.cat != 1) {
+      dev::Aer S;
+      dev::ClearGas G;
+      dev::clear_build(a, off, X, S, G);
+      dev::SoaIn in;
+      dev::clear_pre(*a.K, X.T, X.p, X.dgw, S, G.g[0], in);
+      double *ws = w.ws + c;
+      const long n = w.ncell;
+      ws[W_G * n] = in.g;
+#pragma unroll
+      for (int m = 0; m < 4; ++m) {
+        ws[(W_A + m) * n] = in.a[m];
+        ws[(W_UPTK + m) * n] = in.uptk[m];
+        ws[(W_AOPOA + m) * n] = in.aopoa[m];
+      }
+      ws[W_G0 * n] = in.g0;
+      dev::SoaState st;
+      dev::soa_init(st, in);
+      const float pred = (float)dev::soa_predict(st, w.dtfull);
+      int ib = kNBucket - 1;
+      if (pred < 1.0e6f) ib = min(kNBucket - 1, (int)(2.0f * log2f(1.0f + pred)));
+      b = (unsigned char)ib;
+      atomicAdd(&lh[ib], 1u);
+    }
+    w.bucket[c] = b;
+  }
+  __syncthreads();
+  if (threadIdx.x < kNBucket && lh[threadIdx.x]) atomicAdd(&w.ctl->hist[threadIdx.x], lh[threadIdx.x]);
+}
+
+// Phase 2: queue = cells grouped by bucket, highest bucket first.  Order
+// within a bucket is arbitrary; every cell's result is independent of it.
+__global__ void __launch_bounds__(kBlock) queue_scatter_kernel(Work w) {
+  __shared__ unsigned cnt[kNBucket], base[kNBucket];
+  if (threadIdx.x < kNBucket) cnt[threadIdx.x] = 0;
+  __syncthreads();
+  const long c = (long)blockIdx.x * kBlock + threadIdx.x;
+  unsigned char b = kNoSoa;
+  unsigned r = 0;
+  if (c < w.ncell) {
+    b = w.bucket[c];
+    if (b != kNoSoa) r = atomicAdd(&cnt[b], 1u);
+  }
+  __syncthreads();
+  if (threadIdx.x < kNBucket) {
+    const int t = threadIdx.x;
+    unsigned start = 0;  // cells in buckets above t
+    for (int j = kNBucket - 1; j > t; --j) start += w.ctl->hist[j];
+    base[t] = cnt[t] ? start + atomicAdd(&w.ctl->fill[t], cnt[t]) : 0;
+  }
+  __syncthreads();
+  if (b != kNoSoa) w.queue[base[b] + r] = (int)c;
+}
+
+// Phase 3: persistent SOA loop.  Each wave takes kChunk queue entries at a
+// time; a lane whose cell has converged takes the next entry of the wave's
+// chunk before the following sub-step.  A wave exits once the queue is
+// exhausted and all its lanes are idle; the queue is finite and every cell
+// ends after at most 1000 sub-steps, so every wave exits.
+__global__ void __launch_bounds__(kBlock) soa_queue_kernel(Work w, unsigned long long *stats) {
+  unsigned nq = 0;
+  for (int j = 0; j < kNBucket; ++j) nq += w.ctl->hist[j];
+  const int lane = threadIdx.x & 63;
+  const long n = w.ncell;
+  unsigned cbeg = 0, cend = 0;  // the wave's current chunk (wave-uniform)
+  bool exhausted = false;
+  bool active = false;
+  int cell = 0;
+  dev::SoaState s;
+  unsigned long long niter = 0;
+  while (true) {
+    const unsigned long long need = __ballot(!active);
+    if (need && cbeg == cend && !exhausted) {
+      unsigned b0 = 0;
+      if (lane == 0) b0 = atomicAdd(&w.ctl->head, (unsigned)kChunk);
+      b0 = __shfl(b0, 0, 64);
+      if (b0 >= nq) {
+        exhausted = true;
+      } else {
+        cbeg = b0;
+        cend = min(b0 + (unsigned)kChunk, nq);
+      }
+    }
+    if (need && cbeg < cend) {
+      const unsigned rank = (unsigned)__popcll(need & ((1ull << lane) - 1ull));
+      const unsigned avail = cend - cbeg;
+      if (!active && rank < avail) {
+        cell = w.queue[cbeg + rank];
+        const double *ws = w.ws + cell;
+        dev::SoaIn in;
+        in.g = ws[W_G * n];
+#pragma unroll
+        for (int m = 0; m < 4; ++m) {
+          in.a[m] = ws[(W_A + m) * n];
+          in.uptk[m] = ws[(W_UPTK + m) * n];
+          in.aopoa[m] = ws[(W_AOPOA + m) * n];
+        }
+        in.g0 = ws[W_G0 * n];
+        dev::soa_init(s, in);
+        active = true;
+      }
+      const unsigned took = min((unsigned)__popcll(need), avail);
+      cbeg += took;
+    }
+    if (!__any(active)) break;
+    if (active && !dev::soa_tick(s, w.dtfull)) {
+      double *ws = w.ws + cell;
+      ws[W_G * n] = s.g;
+#pragma unroll
+      for (int m = 0; m < 4; ++m) ws[(W_A + m) * n] = s.a[m];
+      ws[W_GAVG * n] = dev::soa_gavg(s);
+      niter += (unsigned long long)s.niter;
+      active = false;
+    }
+  }
+  add_stats(stats, niter);
+}
+
+// Phase 4.
+template <bool PRE>
+__global__ void __launch_bounds__(kBlock) chain_post_kernel(ChainArgs a, Work w) {
+  const long c = (long)blockIdx.x * kBlock + threadIdx.x;
+  if (c >= w.ncell) return;
+  int icol;
+  long off;
+  cell_index(a, c, icol, off);
+  dev::CellCtx X;
+  dev::cell_setup<PRE>(a, off, icol, X);
+  dev::Aer S;
+  dev::ClearGas G;
+  if (X.cat != 1) {
+    dev::clear_build(a, off, X, S, G);
+    double uptk[4] = {0, 0, 0, 0}, sa[4] = {0, 0, 0, 0}, sg = 0.0, sgavg = 0.0;
+    if (a.cfg.do_cond) {
+      const double *ws = w.ws + c;
+      const long n = w.ncell;
+      sg = ws[W_G * n];
+      sgavg = ws[W_GAVG * n];
+#pragma unroll
+      for (int m = 0; m < 4; ++m) {
+        sa[m] = ws[(W_A + m) * n];
+        uptk[m] = ws[(W_UPTK + m) * n];
+      }
+    }
+    dev::clear_post(*a.K, a.cfg, a.dt, X.T, X.p, X.z, X.pblh, X.rh_clear, X.dga, X.dgw, X.wd,
+                    G.g1, G.g, S, uptk, sg, sgavg, sa);
+  }
+  dev::cell_finish<PRE>(a, off, X, S, G);
 }
 
 // AeroProcess::validate checks (gasaerexch.hpp:96-120): negative T, p, qv or
@@ -68,8 +288,30 @@ struct mam4x_handle_s {
   unsigned long long *d_count = nullptr;  // validate counter
   unsigned long long *d_stats = nullptr;  // SOA sub-steps since the last read
   bool stats = false;
+  // work-queue schedule: workspace for up to `cap` cells (grown on demand)
+  long cap = 0;
+  double *ws = nullptr;
+  unsigned char *bucket = nullptr;
+  int *queue = nullptr;
+  QueueCtl *ctl = nullptr;
+  unsigned soa_grid = 0;  // resident blocks of soa_queue_kernel on the device
   std::string err;
 };
+
+// cells below which the fused one-lane-per-cell kernel is used by
+// MAM4X_SCHED_AUTO: small domains do not fill the chip with persistent waves
+// and pay four launches for little divergence saving
+static constexpr long kFusedBelow = 16384;
+
+static void free_work(mam4x_handle h) {
+  if (h->ws) (void)hipFree(h->ws);
+  if (h->bucket) (void)hipFree(h->bucket);
+  if (h->queue) (void)hipFree(h->queue);
+  h->ws = nullptr;
+  h->bucket = nullptr;
+  h->queue = nullptr;
+  h->cap = 0;
+}
 
 static int fail(mam4x_handle h, int code, const std::string &msg) {
   if (h) h->err = msg;
@@ -83,13 +325,31 @@ static int fail(mam4x_handle h, int code, const std::string &msg) {
       return fail((h), MAM4X_ERR_HIP, std::string(#expr) + ": " + hipGetErrorString(e_)); \
   } while (0)
 
+// Workspace for ncell cells.  Growing frees the old buffers only after the
+// stream has drained, since earlier launches may still use them.
+static int reserve_work(mam4x_handle h, long ncell, hipStream_t s) {
+  if (ncell <= h->cap) return MAM4X_OK;
+  if (h->cap) HIP_TRY(h, hipStreamSynchronize(s));
+  free_work(h);
+  if (hipMalloc(&h->ws, sizeof(double) * W_NPLANE * ncell) != hipSuccess ||
+      hipMalloc(&h->bucket, (size_t)ncell) != hipSuccess ||
+      hipMalloc(&h->queue, sizeof(int) * ncell) != hipSuccess) {
+    (void)hipGetLastError();
+    free_work(h);
+    return fail(h, MAM4X_ERR_NOMEM, "workspace allocation failed (" +
+                std::to_string(ncell) + " cells)");
+  }
+  h->cap = ncell;
+  return MAM4X_OK;
+}
+
 static int check_inputs(mam4x_handle h, const mam4x_inputs *in) {
   if (!in) return fail(h, MAM4X_ERR_ARG, "inputs is NULL");
   if (in->ncol < 0 || in->nlev < 0) return fail(h, MAM4X_ERR_ARG, "negative ncol or nlev");
   const int ldc = in->ldc ? in->ldc : in->ncol;
   if (ldc < in->ncol) return fail(h, MAM4X_ERR_ARG, "ldc < ncol");
-  if ((long long)ldc * in->nlev > 0x7fffffffLL * 8)
-    return fail(h, MAM4X_ERR_ARG, "domain too large");
+  if ((long long)ldc * in->nlev > 0x7fffffffLL)
+    return fail(h, MAM4X_ERR_ARG, "domain too large (ldc * nlev must fit in int32)");
   for (int f = 0; f < MAM4X_NSTATE; ++f)
     if (!in->state[f]) return fail(h, MAM4X_ERR_ARG, "state plane " + std::to_string(f) + " is NULL");
   if (!in->temperature || !in->pressure || !in->height || !in->vapor_mixing_ratio ||
@@ -156,6 +416,7 @@ int mam4x_default_config(mam4x_config *out) {
   out->calcsize_do_aitacc_transfer = 1;
   out->calcsize_do_adjust = 1;
   out->precision = MAM4X_PREC_FP64;
+  out->schedule = MAM4X_SCHED_AUTO;
   return MAM4X_OK;
 }
 
@@ -168,30 +429,34 @@ int mam4x_create(const mam4x_config *config, mam4x_handle *out) {
     h->cfg = *config;
   else
     mam4x_default_config(&h->cfg);
-  if (h->cfg.precision != MAM4X_PREC_FP64 && h->cfg.precision != MAM4X_PREC_MIXED) {
+  if ((h->cfg.precision != MAM4X_PREC_FP64 && h->cfg.precision != MAM4X_PREC_MIXED) ||
+      h->cfg.schedule < MAM4X_SCHED_AUTO || h->cfg.schedule > MAM4X_SCHED_QUEUE) {
     delete h;
     return MAM4X_ERR_ARG;
   }
-  if (hipGetDevice(&h->device) != hipSuccess) {
+  int ncu = 0, per_cu = 0;
+  if (hipGetDevice(&h->device) != hipSuccess ||
+      hipDeviceGetAttribute(&ncu, hipDeviceAttributeMultiprocessorCount, h->device) !=
+          hipSuccess ||
+      hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, soa_queue_kernel, kBlock, 0) !=
+          hipSuccess) {
     delete h;
     return MAM4X_ERR_HIP;
   }
+  h->soa_grid = (unsigned)(ncu * (per_cu > 0 ? per_cu : 1));
   ChainConsts k;
   chain_consts_init(&k);
   if (hipMalloc(&h->d_consts, sizeof(ChainConsts)) != hipSuccess ||
       hipMalloc(&h->d_count, sizeof(unsigned long long)) != hipSuccess ||
-      hipMalloc(&h->d_stats, sizeof(unsigned long long)) != hipSuccess) {
-    if (h->d_consts) (void)hipFree(h->d_consts);
-    if (h->d_count) (void)hipFree(h->d_count);
-    delete h;
+      hipMalloc(&h->d_stats, sizeof(unsigned long long)) != hipSuccess ||
+      hipMalloc(&h->ctl, sizeof(QueueCtl)) != hipSuccess) {
+    (void)hipGetLastError();
+    mam4x_destroy(h);
     return MAM4X_ERR_NOMEM;
   }
   if (hipMemcpy(h->d_consts, &k, sizeof(k), hipMemcpyHostToDevice) != hipSuccess ||
       hipMemset(h->d_stats, 0, sizeof(unsigned long long)) != hipSuccess) {
-    (void)hipFree(h->d_consts);
-    (void)hipFree(h->d_count);
-    (void)hipFree(h->d_stats);
-    delete h;
+    mam4x_destroy(h);
     return MAM4X_ERR_HIP;
   }
   *out = h;
@@ -203,6 +468,8 @@ int mam4x_destroy(mam4x_handle h) {
   if (h->d_consts) (void)hipFree(h->d_consts);
   if (h->d_count) (void)hipFree(h->d_count);
   if (h->d_stats) (void)hipFree(h->d_stats);
+  if (h->ctl) (void)hipFree(h->ctl);
+  free_work(h);
   delete h;
   return MAM4X_OK;
 }
@@ -302,10 +569,39 @@ int mam4x_compute_tendencies(mam4x_handle h, double t, double dt, const mam4x_in
   const long ncell = (long)in->ncol * in->nlev;
   if (ncell == 0) return MAM4X_OK;
   const unsigned grid = (unsigned)((ncell + kBlock - 1) / kBlock);
+  bool fused = h->cfg.schedule == MAM4X_SCHED_FUSED ||
+               (h->cfg.schedule == MAM4X_SCHED_AUTO && ncell < kFusedBelow);
+  if (fused) {
+    if (pre)
+      hipLaunchKernelGGL(chain_fused_kernel<true>, dim3(grid), dim3(kBlock), 0, s, a);
+    else
+      hipLaunchKernelGGL(chain_fused_kernel<false>, dim3(grid), dim3(kBlock), 0, s, a);
+    HIP_TRY(h, hipGetLastError());
+    return MAM4X_OK;
+  }
+  Work w;
+  std::memset(&w, 0, sizeof(w));
+  w.ncell = ncell;
+  w.ctl = h->ctl;
+  w.dtfull = (double)(int)(dt / 1);  // QUIRK (mam4_amicphys.hpp:1319): int sub-step
+  if (a.cfg.do_cond) {
+    rc = reserve_work(h, ncell, s);
+    if (rc) return rc;
+    w.ws = h->ws;
+    w.bucket = h->bucket;
+    w.queue = h->queue;
+    hipLaunchKernelGGL(queue_reset_kernel, dim3(1), dim3(64), 0, s, h->ctl);
+    if (pre)
+      hipLaunchKernelGGL(chain_pre_kernel<true>, dim3(grid), dim3(kBlock), 0, s, a, w);
+    else
+      hipLaunchKernelGGL(chain_pre_kernel<false>, dim3(grid), dim3(kBlock), 0, s, a, w);
+    hipLaunchKernelGGL(queue_scatter_kernel, dim3(grid), dim3(kBlock), 0, s, w);
+    hipLaunchKernelGGL(soa_queue_kernel, dim3(h->soa_grid), dim3(kBlock), 0, s, w, a.stats);
+  }
   if (pre)
-    hipLaunchKernelGGL(chain_fp64_kernel<true>, dim3(grid), dim3(kBlock), 0, s, a);
+    hipLaunchKernelGGL(chain_post_kernel<true>, dim3(grid), dim3(kBlock), 0, s, a, w);
   else
-    hipLaunchKernelGGL(chain_fp64_kernel<false>, dim3(grid), dim3(kBlock), 0, s, a);
+    hipLaunchKernelGGL(chain_post_kernel<false>, dim3(grid), dim3(kBlock), 0, s, a, w);
   HIP_TRY(h, hipGetLastError());
   return MAM4X_OK;
 }

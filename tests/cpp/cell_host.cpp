@@ -40,6 +40,6 @@ extern "C" long cell_host_run(const int *cfgv, int ncol, int nlev, double dt,
   a.has_dgn = 1; a.has_sout = 0; a.dt = dt;
   long iters = 0;
   for (int k = 0; k < nlev; ++k)
-    for (int i = 0; i < ncol; ++i) iters += dev::chain_cell<false>(a, (long)k * ncol + i, i);
+    for (int i = 0; i < ncol; ++i) iters += dev::chain_cell_fused<false>(a, (long)k * ncol + i, i);
   return iters;
 }

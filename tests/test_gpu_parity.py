@@ -41,8 +41,13 @@ def torch_cuda():
 
 
 def run_gpu(torch_cuda, scene, dt=DT, **cfg):
+    """One step on the GPU.  The work-queue schedule is forced (the default
+    picks the fused kernel below 16,384 cells); test_schedules_agree covers
+    the fused kernel."""
+    from mam4xx_amd import _abi
     from mam4xx_amd.chain import Chain, scene_to_device
     torch = torch_cuda
+    cfg.setdefault("schedule", _abi.SCHED_QUEUE)
     ch = Chain(**cfg)
     d = scene_to_device(scene)
     nlev, ncol = scene["temperature"].shape
@@ -56,9 +61,13 @@ def run_gpu(torch_cuda, scene, dt=DT, **cfg):
     return out
 
 
-def run_orc(orc, scene, dt=DT, **cfg):
+def _orc_cfg(cfg):
     import oracle.pyoracle as P
-    c = P.default_config(**{k: v for k, v in cfg.items() if k != "precision"})
+    return P.default_config(**{k: v for k, v in cfg.items() if k not in ("precision", "schedule")})
+
+
+def run_orc(orc, scene, dt=DT, **cfg):
+    c = _orc_cfg(cfg)
     t, di, dc, _ = orc.chain_step(scene, dt=dt, config=c)
     return t, di, dc
 
@@ -69,8 +78,8 @@ ENV_FACTOR = 10.0
 
 def check(orc, torch_cuda, scene, dt=DT, max_sensitive=0.01, **cfg):
     import oracle.pyoracle as P
-    tg, dig, dcg = run_gpu(torch_cuda, scene, dt, **cfg)
-    oc = P.default_config(**{k: v for k, v in cfg.items() if k != "precision"})
+    tg, dig, dcg = run_gpu(torch_cuda, scene, dt, **dict(cfg))
+    oc = _orc_cfg(cfg)
     to, dio, dco, _ = orc.chain_step(scene, dt=dt, config=oc)
     _, env, env_dgn = orc.envelope(scene, dt=dt, config=oc, return_dgn=True)
     assert np.all(np.isfinite(tg))
@@ -177,7 +186,8 @@ def test_precldchem_inputs(orc, torch_cuda):
     pre = sc["state"] * rng.uniform(0.7, 1.0, sc["state"].shape)
     pregas = np.stack([sc["state"][L.F_H2SO4] * rng.uniform(0.2, 1.2, sc["pblh"].shape),
                        sc["state"][L.F_SOAG]])
-    ch = Chain()
+    from mam4xx_amd import _abi
+    ch = Chain(schedule=_abi.SCHED_QUEUE)
     d = scene_to_device(sc)
     tend = torch.empty((L.NSTATE, 72, 64), dtype=torch.float64, device="cuda")
     ch.compute_tendencies(d, DT, tend, precldchem=torch.as_tensor(pre).cuda(),
@@ -191,6 +201,18 @@ def test_precldchem_inputs(orc, torch_cuda):
     cell_env = env.max(axis=0)
     bound = np.where(cell_env > SENSITIVE, np.maximum(TOL, ENV_FACTOR * cell_env), TOL)[None]
     assert (err <= bound).all(), report(tg, to, sc["state"], DT)
+
+
+@pytest.mark.parametrize("dt", [1800.0, 0.5])
+def test_schedules_agree(torch_cuda, dt):
+    """Fused one-lane-per-cell kernel and the work-queue pipeline perform the
+    same operations on every cell: results are bitwise identical."""
+    from mam4xx_amd import _abi
+    sc = scenes.generate(320, 72, seed=15)
+    a = run_gpu(torch_cuda, sc, dt, schedule=_abi.SCHED_FUSED)
+    b = run_gpu(torch_cuda, sc, dt, schedule=_abi.SCHED_QUEUE)
+    for x, y in zip(a, b):
+        assert np.array_equal(x, y, equal_nan=True)
 
 
 def test_validate_counts_violations(torch_cuda):
