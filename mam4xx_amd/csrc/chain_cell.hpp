@@ -286,11 +286,9 @@ __device__ __forceinline__ void cell_finish(const ChainArgs &a, long off, const 
   post[F_H2SO4] = kmax(0.0, gi_out[F_H2SO4]) * TO_MMR_H2SO4;
   post[F_SOAG] = kmax(0.0, gi_out[F_SOAG]) * TO_MMR_SOAG;
 
-  // tendencies of the chain, plus calcsize on the post-chain state
-  double tend[NSTATE];
-#pragma unroll
-  for (int f = 0; f < NSTATE; ++f) tend[f] = (post[f] - a.state[f][off]) / dt;
-  double dgi[4], dgc[4];
+  // calcsize on the post-chain state, then the tendencies: (post - state)/dt
+  // of the chain plus calcsize's own (zero where calcsize does not act)
+  CalcOut o;
   if (a.do_calcsize) {
     double n_i[4], n_c[4], q_i[4][7], q_c[4][7];
 #pragma unroll
@@ -303,39 +301,33 @@ __device__ __forceinline__ void cell_finish(const ChainArgs &a, long off, const 
         q_c[m][s] = s < NSPEC_MODE[m] ? post[f_qc(m, s)] : 0.0;
       }
     }
-    CalcOut o;
     calcsize(K, a.do_xfer != 0, a.do_adjust != 0, dt, n_i, n_c, q_i, q_c, o);
+  }
 #pragma unroll
-    for (int m = 0; m < 4; ++m) {
-      tend[f_ni(m)] += o.dn_i[m];
-      tend[f_nc(m)] += o.dn_c[m];
-#pragma unroll
-      for (int s = 0; s < 7; ++s) {
-        if (s >= NSPEC_MODE[m]) continue;
-        tend[f_qi(m, s)] += o.dq_i[m][s];
-        tend[f_qc(m, s)] += o.dq_c[m][s];
+  for (int f = 0; f < NSTATE; ++f) {
+    const double st = a.state[f][off];
+    double t = (post[f] - st) / dt;
+    if (a.do_calcsize) {
+      if (f < F_Q_I) {
+        t += f < F_NUM_C ? o.dn_i[f] : o.dn_c[f - F_NUM_C];
+      } else if (f < F_H2SO4) {
+        const bool cw = f >= F_Q_C;
+        const int r = f - (cw ? F_Q_C : F_Q_I);
+        const int m = r < MODE_OFF[1] ? 0 : r < MODE_OFF[2] ? 1 : r < MODE_OFF[3] ? 2 : 3;
+        const int sp = r - MODE_OFF[m];
+        t += cw ? o.dq_c[m][sp] : o.dq_i[m][sp];
       }
-      dgi[m] = o.dgn_i[m];
-      dgc[m] = o.dgn_c[m];
     }
-  } else {
-#pragma unroll
-    for (int m = 0; m < 4; ++m) dgi[m] = dgc[m] = X.dga[m];
+    a.tend[f][off] = t;
+    if (a.has_sout) a.sout[f][off] = st + dt * t;
   }
-  if (a.has_sout) {
-    double st[NSTATE];
-#pragma unroll
-    for (int f = 0; f < NSTATE; ++f) st[f] = a.state[f][off];
-#pragma unroll
-    for (int f = 0; f < NSTATE; ++f) a.sout[f][off] = st[f] + dt * tend[f];
-  }
-#pragma unroll
-  for (int f = 0; f < NSTATE; ++f) a.tend[f][off] = tend[f];
   if (a.has_dgn) {
 #pragma unroll
     for (int m = 0; m < 4; ++m) {
-      if (a.dgi[m]) a.dgi[m][off] = dgi[m];
-      if (a.dgc[m]) a.dgc[m][off] = dgc[m];
+      const double di = a.do_calcsize ? o.dgn_i[m] : X.dga[m];
+      const double dc = a.do_calcsize ? o.dgn_c[m] : X.dga[m];
+      if (a.dgi[m]) a.dgi[m][off] = di;
+      if (a.dgc[m]) a.dgc[m][off] = dc;
     }
   }
 }

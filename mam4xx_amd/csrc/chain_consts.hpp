@@ -122,6 +122,9 @@ __host__ __device__ constexpr int f_ni(int m) { return m; }
 __host__ __device__ constexpr int f_nc(int m) { return 4 + m; }
 __host__ __device__ constexpr int f_qi(int m, int s) { return 8 + MODE_OFF[m] + s; }
 __host__ __device__ constexpr int f_qc(int m, int s) { return 29 + MODE_OFF[m] + s; }
+constexpr int F_NUM_C = 4;
+constexpr int F_Q_I = 8;
+constexpr int F_Q_C = 29;
 constexpr int F_H2SO4 = 50;
 constexpr int F_SOAG = 51;
 

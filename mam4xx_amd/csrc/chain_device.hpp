@@ -147,6 +147,11 @@ __device__ __forceinline__ double soa_predict(const SoaState &s, double dtfull) 
 
 // One pass of the while loop (gasaerexch_soaexch.hpp:170-335).  Returns false
 // once the loop has ended (time reached or 1000 sub-steps taken).
+//
+// Written without branches on the mode: a skipped mode (uptake <= 1e-15) has
+// uk = 0 and a_soa = 0, so its terms add exactly +0 to every sum and its
+// denominators are exactly 1 -- the same values the reference's skip produces
+// -- and the divisions of all four modes issue as straight-line code.
 __device__ __forceinline__ bool soa_tick(SoaState &s, double dtfull) {
   const double alpha_astem = 0.05, a_min1 = 1.0e-20, g_min1 = 1.0e-20;
   if (!(s.tcur < dtfull - 1.0e-3)) return false;
@@ -155,60 +160,42 @@ __device__ __forceinline__ bool soa_tick(SoaState &s, double dtfull) {
   const double gprv = s.g;
   double g_soa = kmax(gprv, 0.0);
   double tot = g_soa;
-  double a_soa[4], sat[4], gstar[4], beta[4];
+  double a_soa[4], sat[4], gstar[4];
 #pragma unroll
   for (int n = 0; n < 4; ++n) {
-    a_soa[n] = 0.0;
-    if (!s.skip[n]) {
-      a_soa[n] = kmax(s.a[n], 0.0);
-      tot += a_soa[n];
-    }
+    a_soa[n] = s.skip[n] ? 0.0 : kmax(s.a[n], 0.0);
+    tot = s.skip[n] ? tot : tot + a_soa[n];
   }
   double tmpb = 0.0;
 #pragma unroll
   for (int n = 0; n < 4; ++n) {
-    sat[n] = 0.0;
-    gstar[n] = 0.0;
-    beta[n] = 0.0;
-    if (!s.skip[n]) {
-      const double sumooa = s.aopoa[n] + a_soa[n];
-      sat[n] = s.g0 / kmax(sumooa, a_min1);
-      gstar[n] = sat[n] * a_soa[n];
-      const double phi = (g_soa - gstar[n]) / kmax(g_soa, kmax(gstar[n], g_min1));
-      tmpb += s.uk[n] * fabs(phi);
-    }
+    sat[n] = s.g0 / kmax(s.aopoa[n] + a_soa[n], a_min1);
+    gstar[n] = sat[n] * a_soa[n];
+    const double phi = (g_soa - gstar[n]) / kmax(g_soa, kmax(gstar[n], g_min1));
+    tmpb = s.skip[n] ? tmpb : tmpb + s.uk[n] * fabs(phi);
   }
   const double tmpa = kmax(0.0, tmpb);
-  double dtcur;
   const double dtmax = dtfull - s.tcur;
-  if (dtmax * tmpa <= alpha_astem) {
-    dtcur = dtmax;
-    s.tcur = dtfull;
-  } else {
-    dtcur = alpha_astem / tmpa;
-    s.tcur += dtcur;
-  }
+  const bool last = dtmax * tmpa <= alpha_astem;
+  const double dtcur = last ? dtmax : alpha_astem / tmpa;
+  s.tcur = last ? dtfull : s.tcur + dtcur;
   // step 1: explicit estimate of condensing modes' a_soa and sat
+  double beta[4];
 #pragma unroll
   for (int n = 0; n < 4; ++n) {
-    if (!s.skip[n]) {
-      beta[n] = dtcur * s.uk[n];
-      const double del = g_soa - gstar[n];
-      if (del > 0.0) {
-        const double at = a_soa[n] + beta[n] * del;
-        sat[n] = s.g0 / kmax(s.aopoa[n] + at, a_min1);
-      }
-    }
+    beta[n] = dtcur * s.uk[n];
+    const double del = g_soa - gstar[n];
+    const double at = a_soa[n] + beta[n] * del;
+    const double sat1 = s.g0 / kmax(s.aopoa[n] + at, a_min1);
+    sat[n] = del > 0.0 ? sat1 : sat[n];
   }
   // step 2: implicit in g, semi-implicit in a
-  double sa = 0.0, sb = 0.0;
+  double sa = 0.0, sb = 0.0, den[4];
 #pragma unroll
   for (int n = 0; n < 4; ++n) {
-    if (!s.skip[n]) {
-      const double den = 1.0 + beta[n] * sat[n];
-      sa += a_soa[n] / den;
-      sb += beta[n] / den;
-    }
+    den[n] = 1.0 + beta[n] * sat[n];
+    sa = s.skip[n] ? sa : sa + a_soa[n] / den[n];
+    sb = s.skip[n] ? sb : sb + beta[n] / den[n];
   }
   g_soa = (tot - sa) / (1.0 + sb);
   g_soa = kmax(0.0, g_soa);
@@ -216,7 +203,7 @@ __device__ __forceinline__ bool soa_tick(SoaState &s, double dtfull) {
   for (int n = 0; n < 4; ++n) {
     // skipped modes: a_soa is never set, so their SOA becomes 0 (quirk of
     // gasaerexch_soaexch.hpp:327-331)
-    s.a[n] = s.skip[n] ? 0.0 : (a_soa[n] + beta[n] * g_soa) / (1.0 + beta[n] * sat[n]);
+    s.a[n] = s.skip[n] ? 0.0 : (a_soa[n] + beta[n] * g_soa) / den[n];
   }
   s.g = g_soa;
   s.avg += dtcur * (gprv + 0.5 * (g_soa - gprv));

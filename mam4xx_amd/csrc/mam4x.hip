@@ -37,7 +37,18 @@ namespace {
 
 constexpr int kBlock = 256;
 constexpr int kNBucket = 24;  // half-octave cost buckets: 2 log2(1 + predicted sub-steps)
-constexpr int kChunk = 64;    // queue entries a wave takes per refill
+#ifndef MAM4X_QUEUE_CHUNK
+#define MAM4X_QUEUE_CHUNK 64
+#endif
+constexpr int kChunk = MAM4X_QUEUE_CHUNK;  // queue entries a wave takes per refill
+// Waves whose first chunk comes from buckets >= kLongBucket (predicted >= ~180
+// sub-steps) raise their issue priority: a cell's sub-steps are a serial
+// chain, so the longest cells set the kernel's critical path and should not
+// yield issue slots to the bulk.
+#ifndef MAM4X_SOA_PRIO
+#define MAM4X_SOA_PRIO 1
+#endif
+constexpr int kLongBucket = 15;
 constexpr unsigned char kNoSoa = 0xff;
 
 // workspace planes over the compact cell index; phase 3 reads the SOA loop
@@ -170,8 +181,11 @@ __global__ void __launch_bounds__(kBlock) queue_scatter_kernel(Work w) {
 // exhausted and all its lanes are idle; the queue is finite and every cell
 // ends after at most 1000 sub-steps, so every wave exits.
 __global__ void __launch_bounds__(kBlock) soa_queue_kernel(Work w, unsigned long long *stats) {
-  unsigned nq = 0;
-  for (int j = 0; j < kNBucket; ++j) nq += w.ctl->hist[j];
+  unsigned nq = 0, nlong = 0;
+  for (int j = 0; j < kNBucket; ++j) {
+    nq += w.ctl->hist[j];
+    if (j >= kLongBucket) nlong += w.ctl->hist[j];
+  }
   const int lane = threadIdx.x & 63;
   const long n = w.ncell;
   unsigned cbeg = 0, cend = 0;  // the wave's current chunk (wave-uniform)
@@ -185,7 +199,8 @@ __global__ void __launch_bounds__(kBlock) soa_queue_kernel(Work w, unsigned long
     if (need && cbeg == cend && !exhausted) {
       unsigned b0 = 0;
       if (lane == 0) b0 = atomicAdd(&w.ctl->head, (unsigned)kChunk);
-      b0 = __shfl(b0, 0, 64);
+      b0 = __builtin_amdgcn_readfirstlane(b0);
+      if (MAM4X_SOA_PRIO && b0 < nlong) __builtin_amdgcn_s_setprio(2);
       if (b0 >= nq) {
         exhausted = true;
       } else {

@@ -551,6 +551,28 @@ static int mam_soaexch_1subarea(double dtsubstep, double temp, double pmid,
   return niter;
 }
 
+/* SOA exchange of one subarea (gasaerexch_soaexch.hpp:79-341) on its own,
+ * for the closed-form and brute-force checks: q_soag [mol/mol, fcvt applied],
+ * soa[4] and pom[4] per mode, uptk_soag[4] the SOAG uptake rates [1/s].
+ * Returns the sub-step count; *q_soag_avg receives the time average. */
+int orc_soaexch(double dt, double temp, double pmid, double *q_soag, double *q_soag_avg,
+                double soa[4], const double pom[4], const double uptk_soag[4]) {
+  double qgas[2] = {*q_soag, 0.0}, qavg[2] = {0.0, 0.0};
+  double qaer[7][4], up[2][4];
+  memset(qaer, 0, sizeof(qaer));
+  for (int n = 0; n < 4; ++n) {
+    qaer[0][n] = soa[n];
+    qaer[2][n] = pom[n];
+    up[0][n] = uptk_soag[n];
+    up[1][n] = 0.0;
+  }
+  const int it = mam_soaexch_1subarea(dt, temp, pmid, qgas, qavg, qaer, up);
+  *q_soag = qgas[0];
+  *q_soag_avg = qavg[0];
+  for (int n = 0; n < 4; ++n) soa[n] = qaer[0][n];
+  return it;
+}
+
 /* H2SO4 block of mam_gasaerexch_1subarea (gasaerexch.hpp:479-537). */
 void orc_nonvolatile_update(double dt, double netprod, const double uptkaer[4],
                             double *qgas_cur, double *qgas_avg,

@@ -83,6 +83,12 @@ long orc_chain_step(const orc_config *cfg, int ncol, int nlev, double dt,
  * random -1/0/+1 ulp (per-cell reproducible); 0 switches it off. */
 void orc_set_libm_noise(unsigned long long seed);
 
+/* SOA exchange of one subarea (gasaerexch_soaexch.hpp:79-341) in isolation;
+ * species index 0 = SOA, 2 = POM in the reference's qaer.  Returns the
+ * sub-step count. */
+int orc_soaexch(double dt, double temp, double pmid, double *q_soag, double *q_soag_avg,
+                double soa[4], const double pom[4], const double uptk_soag[4]);
+
 /* ---- process-level entry points used by the known-answer tests ---- */
 
 /* H2SO4 non-volatile analytic update (gasaerexch.hpp:479-537; identical math

@@ -89,6 +89,8 @@ class Oracle:
         L.orc_amicphys_intr.argtypes = [ctypes.POINTER(OrcConfig)] + [ctypes.c_double] * 8 + \
             [_dp] * 10 + [_dp, _dp, ctypes.POINTER(ctypes.c_int)]
         L.orc_calcsize_level.argtypes = [ctypes.POINTER(OrcConfig), ctypes.c_double] + [_dp] * 12
+        L.orc_soaexch.restype = ctypes.c_int
+        L.orc_soaexch.argtypes = [ctypes.c_double] * 3 + [_dp, _dp, _dp, _dp, _dp]
         L.orc_set_libm_noise.argtypes = [ctypes.c_ulonglong]
         L.orc_set_libm_noise.restype = None
 
@@ -144,6 +146,16 @@ class Oracle:
         return tend, dgn_i, dgn_c, int(iters)
 
     # ---- process-level helpers for the known-answer tests ----
+    def soaexch(self, dt, temp, pmid, q_soag, soa, pom, uptk_soag):
+        """SOA exchange of one subarea; returns (q_soag, q_soag_avg, soa[4], niter)."""
+        g = np.array([q_soag], dtype=np.float64)
+        gavg = np.zeros(1)
+        a = np.array(soa, dtype=np.float64)
+        po = np.ascontiguousarray(pom, dtype=np.float64)
+        up = np.ascontiguousarray(uptk_soag, dtype=np.float64)
+        it = self.lib.orc_soaexch(dt, temp, pmid, _p(g), _p(gavg), _p(a), _p(po), _p(up))
+        return g[0], gavg[0], a, it
+
     def nonvolatile_update(self, dt, netprod, uptkaer, qgas_cur, qaer_cur):
         up = np.ascontiguousarray(uptkaer, dtype=np.float64)
         qa = np.array(qaer_cur, dtype=np.float64)
