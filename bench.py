@@ -5,18 +5,27 @@ Workload: 21,600 columns x 72 levels (the precision-study configuration),
 seeded synthetic inputs (mam4xx_amd.scenes), chained 1800 s steps: each step
 runs the whole chain + calcsize through the C ABI and writes the tendencies,
 the calcsize diameters and the updated state in place.  W untimed warm-up
-steps, then K timed steps bracketed by device synchronisation.
+steps, then K timed steps bracketed by device synchronisation.  Inputs are
+resident in HBM before the timed region starts.
 
-Roofline: the step is one kernel launch, bound by FP64 VALU arithmetic (no
-dense contraction; HBM traffic is ~1.4 KB per cell).  Algorithmic work per
-launch = A * cells + B * (SOA sub-steps taken), with A and B the FP64 flop
-counts per cell and per SOA sub-step measured by the op-counting build of the
-CPU restatement (oracle/opcount.cpp, transcendentals weighted by their gfx950
-libm flops) on a 216-column sample of this workload.  The GPU counts the SOA
-sub-steps itself.  Launch time comes from HIP events on the launch stream.
+A step is four kernels (work-queue schedule, include/mam4x.h):
+  set-up (per cell) -> queue build -> SOA work-queue kernel -> finish (per cell)
+The library records HIP events on the launch stream between them while stats
+are enabled (mam4x_last_phase_times), so each phase is timed live over the
+timed steps.
+
+Roofline (dominant kernel): the SOA work-queue kernel.  Its algorithmic work
+is B FP64 flops per SOA sub-step times the sub-steps the GPU reports for the
+timed launches; the rest of the step costs A flops per cell.  A and B are the
+least-squares fit flops = A + B * substeps over a 216-column sample of this
+workload, counted by the op-counting build of the CPU restatement
+(oracle/opcount.cpp: + - * / sqrt = 1, transcendentals at their gfx950
+device-libm flop counts).  Bound: FP64 VALU -- no step is a dense contraction;
+MI355X's FP64 vector and matrix peaks coincide (78.6 TFLOP/s).
 
 cpu_baseline: the CPU restatement (oracle/, OpenMP over columns) timed on the
-host's cores for one full step of the same scene.
+host's cores for one full step of the same scene, best of 5 after a warm-up,
+plus a single-core figure on 216 columns.
 """
 import argparse
 import json
@@ -33,10 +42,12 @@ OPS_PER_CELL = 11850.2
 OPS_PER_SOA_STEP = 130.87
 HBM_PEAK_GBS = 8000.0
 FP64_PEAK_TFLOPS = 78.6
-# algorithmic HBM bytes per cell per launch: reads 52 state + 5 met + 12 modal
-# doubles, writes 52 tendencies + 52 updated state + 8 diameters (pblh per
-# column is negligible)
+# algorithmic HBM bytes per cell of a whole step: reads 52 state + 5 met + 12
+# modal doubles, writes 52 tendencies + 52 updated state + 8 diameters
 BYTES_PER_CELL = 8 * (52 + 5 + 12 + 52 + 52 + 8)
+# algorithmic bytes of the SOA kernel per queued cell: 14 doubles of loop
+# inputs and a 4-byte queue entry read, 6 doubles of results written
+SOA_BYTES_PER_CELL = 14 * 8 + 4 + 6 * 8
 
 
 def cpu_baseline(scene, nthreads):
@@ -45,7 +56,7 @@ def cpu_baseline(scene, nthreads):
     orc = oracle.Oracle()
     orc.chain_step(scene, dt=DT, nthreads=nthreads)  # warm-up
     best = 1e30
-    for _ in range(3):
+    for _ in range(5):
         t0 = time.perf_counter()
         orc.chain_step(scene, dt=DT, nthreads=nthreads)
         best = min(best, time.perf_counter() - t0)
@@ -60,8 +71,9 @@ def cpu_baseline(scene, nthreads):
         "unit": "column-steps/s",
         "cores": nthreads,
         "kind": "port",
-        "sample": "one 1800 s step of the full %d x %d scene, best of 3 after a warm-up; "
-                  "single-core figure from 216 columns" % (NCOL, NLEV),
+        "sample": "one 1800 s step of the full %d x %d scene (first step of the chained "
+                  "workload), best of 5 after a warm-up; single-core figure from 216 columns"
+                  % (NCOL, NLEV),
         "single_core_value": ncol1 / t1,
         "cpu": _cpu_model(),
     }
@@ -75,6 +87,16 @@ def _cpu_model():
     except OSError:
         pass
     return "unknown"
+
+
+def _traffic(name):
+    """HBM bytes per launch from a committed counter summary, or None."""
+    path = os.path.join(ROOT, "profiles", name)
+    if os.path.exists(path):
+        for line in open(path):
+            if line.startswith("bytes_per_launch"):
+                return float(line.split("=")[1])
+    return None
 
 
 def main():
@@ -118,7 +140,8 @@ def main():
         step()
     torch.cuda.synchronize()
     ch.enable_stats(True)
-    ch.last_stats()  # reset the counter
+    ch.last_stats()  # reset the counters
+    ch.last_phase_times()
     stream = torch.cuda.current_stream()
     e0 = torch.cuda.Event(enable_timing=True)
     e1 = torch.cuda.Event(enable_timing=True)
@@ -131,12 +154,24 @@ def main():
     torch.cuda.synchronize()
     wall = time.perf_counter() - t0
     iters = ch.last_stats()
-    kernel_ms = e0.elapsed_time(e1) / args.steps
+    phases, ncalls = ch.last_phase_times()
+    assert ncalls == args.steps
+    gpu_ms = e0.elapsed_time(e1) / args.steps
+    ph = {k: v / args.steps for k, v in phases.items()}
     ncell = NCOL * NLEV
-    ops = OPS_PER_CELL * ncell + OPS_PER_SOA_STEP * iters / args.steps
-    achieved = ops / (kernel_ms * 1e-3) / 1e12
+    substeps = iters / args.steps
     ms_per_step = wall * 1e3 / args.steps
     finite = bool(torch.isfinite(state).all().item())
+    step_flops = OPS_PER_CELL * ncell + OPS_PER_SOA_STEP * substeps
+    queue = ph["soa"] > 0.0
+    if queue:
+        soa_flops = OPS_PER_SOA_STEP * substeps
+        kern = {"name": "soa_queue_kernel", "ms": ph["soa"], "flops": soa_flops,
+                "traffic": _traffic("traffic_soa_queue_kernel_%dx%d.txt" % (NCOL, NLEV))}
+    else:
+        kern = {"name": "chain_fused_kernel", "ms": ph["finish"], "flops": step_flops,
+                "traffic": _traffic("traffic_chain_fp64_%dx%d.txt" % (NCOL, NLEV))}
+    achieved = kern["flops"] / (kern["ms"] * 1e-3) / 1e12
     out = {
         "metric": "microphysics column-steps/sec at 72 lev: mixed vs FP64 GPU at <=1e-5 rel err",
         "value": NCOL * args.steps / wall,
@@ -155,30 +190,37 @@ def main():
                         "(gasaerexch, rename, newnuc, coag, pcarbon aging, calcsize)"
                         % (NCOL, NLEV, DT),
             "columns": NCOL, "levels": NLEV, "dt_s": DT, "precision": args.precision,
-            "soa_substeps_per_cell": iters / args.steps / ncell,
+            "schedule": args.schedule,
+            "soa_substeps_per_cell": substeps / ncell,
             "state_finite": finite,
         },
         "roofline": {
             "bound": "mfma",
             "compute_unit": "FP64 VALU (MI355X FP64 vector and matrix peaks coincide)",
+            "kernel": kern["name"],
             "achieved": achieved,
             "peak": FP64_PEAK_TFLOPS,
             "unit": "TFLOP/s",
             "frac": achieved / FP64_PEAK_TFLOPS,
-            "traffic": None,
-            "kernel_ms": kernel_ms,
-            "algorithmic_gflop_per_launch": ops / 1e9,
-            "hbm_algorithmic_gbs": BYTES_PER_CELL * ncell / (kernel_ms * 1e-3) / 1e9,
+            "traffic": kern["traffic"],
+            "kernel_ms": kern["ms"],
+            "algorithmic_gflop_per_launch": kern["flops"] / 1e9,
+            "step": {
+                "gpu_ms": gpu_ms,
+                "phase_ms": ph,
+                "algorithmic_gflop": step_flops / 1e9,
+                "tflops": step_flops / (gpu_ms * 1e-3) / 1e12,
+                "hbm_algorithmic_gbs": BYTES_PER_CELL * ncell / (gpu_ms * 1e-3) / 1e9,
+            },
         },
     }
-    # HBM bytes per launch from the FETCH_SIZE/WRITE_SIZE counter passes,
-    # scaled by a copy-kernel calibration (summary under profiles/)
-    traffic_file = os.path.join(ROOT, "profiles", "traffic_chain_%s_%dx%d.txt" %
-                                (args.precision, NCOL, NLEV))
-    if os.path.exists(traffic_file):
-        for line in open(traffic_file):
-            if line.startswith("bytes_per_launch"):
-                out["roofline"]["traffic"] = float(line.split("=")[1])
+    if queue:
+        # overcast cells (cloud fraction > 0.999) have no clear subarea and
+        # never enter the queue (mam4_amicphys.hpp:241-317)
+        nqueued = int((scene["cloud_fraction"] <= 0.999).sum())
+        out["roofline"]["queued_cells"] = nqueued
+        out["roofline"]["hbm_algorithmic_gbs"] = (SOA_BYTES_PER_CELL * nqueued /
+                                                  (kern["ms"] * 1e-3) / 1e9)
     if not args.no_cpu:
         nthreads = int(os.environ.get("OMP_NUM_THREADS", "0")) or os.cpu_count()
         out["cpu_baseline"] = cpu_baseline(scene, nthreads)

@@ -167,6 +167,19 @@ int mam4x_compute_tendencies(mam4x_handle h, double t, double dt,
 int mam4x_enable_stats(mam4x_handle h, int enable);
 int mam4x_last_stats(mam4x_handle h, long long *soa_iterations, void *stream);
 
+/* Phase timing, also while stats are enabled: HIP events recorded on the
+ * launch stream between the kernels of every compute_tendencies call.
+ * mam4x_last_phase_times synchronizes the stream and returns, per phase, the
+ * milliseconds summed over the calls since stats were enabled or the times
+ * last read (which resets them), and the number of calls in *calls.
+ * The fused schedule counts its single kernel as MAM4X_PHASE_FINISH. */
+#define MAM4X_NPHASE 3
+#define MAM4X_PHASE_SETUP 0  /* per-cell set-up and the work-queue build     */
+#define MAM4X_PHASE_SOA 1    /* work-queue kernel of the SOA sub-stepping    */
+#define MAM4X_PHASE_FINISH 2 /* per-cell finish: rest of the chain, calcsize */
+int mam4x_last_phase_times(mam4x_handle h, double ms[MAM4X_NPHASE], long long *calls,
+                           void *stream);
+
 #ifdef __cplusplus
 }
 #endif

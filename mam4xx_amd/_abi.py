@@ -22,6 +22,9 @@ SCHED_AUTO = 0
 SCHED_FUSED = 1
 SCHED_QUEUE = 2
 
+NPHASE = 3
+PHASE_NAMES = ("setup", "soa", "finish")
+
 _dp = ctypes.POINTER(ctypes.c_double)
 _vp = ctypes.c_void_p
 
@@ -89,6 +92,8 @@ SIGNATURES = {
                                                 _vp]),
     "mam4x_enable_stats": (ctypes.c_int, [_vp, ctypes.c_int]),
     "mam4x_last_stats": (ctypes.c_int, [_vp, ctypes.POINTER(ctypes.c_longlong), _vp]),
+    "mam4x_last_phase_times": (ctypes.c_int, [_vp, ctypes.POINTER(ctypes.c_double),
+                                              ctypes.POINTER(ctypes.c_longlong), _vp]),
 }
 
 HERE = os.path.dirname(os.path.abspath(__file__))

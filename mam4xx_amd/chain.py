@@ -138,6 +138,18 @@ class Chain:
         return int(v.value)
 
 
+    def last_phase_times(self, stream=None):
+        """{phase: milliseconds} summed over the calls since the last read,
+        and the call count (stats must be enabled)."""
+        ms = (ctypes.c_double * _abi.NPHASE)()
+        n = ctypes.c_longlong(0)
+        s = stream if stream is not None else torch.cuda.current_stream()
+        self._check(self.lib.mam4x_last_phase_times(self.h, ms, ctypes.byref(n),
+                                                    ctypes.c_void_p(s.cuda_stream)),
+                    "mam4x_last_phase_times")
+        return dict(zip(_abi.PHASE_NAMES, list(ms))), int(n.value)
+
+
 def scene_to_device(scene, device="cuda"):
     """numpy scene dict -> dict of contiguous float64 CUDA tensors."""
     return {k: torch.as_tensor(v, dtype=torch.float64).contiguous().to(device)

@@ -25,8 +25,10 @@
 
 #include <cstdio>
 #include <cstring>
+#include <array>
 #include <new>
 #include <string>
+#include <vector>
 
 #include "../../include/mam4x.h"
 #include "chain_cell.hpp"
@@ -310,6 +312,8 @@ struct mam4x_handle_s {
   int *queue = nullptr;
   QueueCtl *ctl = nullptr;
   unsigned soa_grid = 0;  // resident blocks of soa_queue_kernel on the device
+  // phase timing (stats enabled): 4 events per call, recycled through a pool
+  std::vector<std::array<hipEvent_t, 4>> pending, pool;
   std::string err;
 };
 
@@ -317,6 +321,14 @@ struct mam4x_handle_s {
 // MAM4X_SCHED_AUTO: small domains do not fill the chip with persistent waves
 // and pay four launches for little divergence saving
 static constexpr long kFusedBelow = 16384;
+
+static void destroy_events(mam4x_handle h) {
+  for (auto *v : {&h->pending, &h->pool})
+    for (auto &q : *v)
+      for (hipEvent_t e : q) (void)hipEventDestroy(e);
+  h->pending.clear();
+  h->pool.clear();
+}
 
 static void free_work(mam4x_handle h) {
   if (h->ws) (void)hipFree(h->ws);
@@ -485,6 +497,7 @@ int mam4x_destroy(mam4x_handle h) {
   if (h->d_stats) (void)hipFree(h->d_stats);
   if (h->ctl) (void)hipFree(h->ctl);
   free_work(h);
+  destroy_events(h);
   delete h;
   return MAM4X_OK;
 }
@@ -507,8 +520,11 @@ const char *mam4x_last_error(mam4x_handle h) { return h ? h->err.c_str() : "NULL
 
 int mam4x_enable_stats(mam4x_handle h, int enable) {
   if (!h) return MAM4X_ERR_ARG;
-  if (enable && !h->stats)
+  if (enable && !h->stats) {
     HIP_TRY(h, hipMemset(h->d_stats, 0, sizeof(unsigned long long)));
+    for (auto &q : h->pending) h->pool.push_back(q);
+    h->pending.clear();
+  }
   h->stats = enable != 0;
   return MAM4X_OK;
 }
@@ -522,6 +538,25 @@ int mam4x_last_stats(mam4x_handle h, long long *soa_iterations, void *stream) {
   HIP_TRY(h, hipMemsetAsync(h->d_stats, 0, sizeof(v), s));
   HIP_TRY(h, hipStreamSynchronize(s));
   *soa_iterations = (long long)v;
+  return MAM4X_OK;
+}
+
+int mam4x_last_phase_times(mam4x_handle h, double ms[MAM4X_NPHASE], long long *calls,
+                           void *stream) {
+  if (!h || !ms || !calls) return MAM4X_ERR_ARG;
+  if (!h->stats) return fail(h, MAM4X_ERR_UNSUPPORTED, "stats not enabled");
+  HIP_TRY(h, hipStreamSynchronize((hipStream_t)stream));
+  for (int i = 0; i < MAM4X_NPHASE; ++i) ms[i] = 0.0;
+  for (auto &q : h->pending) {
+    for (int i = 0; i < MAM4X_NPHASE; ++i) {
+      float t = 0.0f;
+      HIP_TRY(h, hipEventElapsedTime(&t, q[i], q[i + 1]));
+      ms[i] += t;
+    }
+    h->pool.push_back(q);
+  }
+  *calls = (long long)h->pending.size();
+  h->pending.clear();
   return MAM4X_OK;
 }
 
@@ -586,38 +621,56 @@ int mam4x_compute_tendencies(mam4x_handle h, double t, double dt, const mam4x_in
   const unsigned grid = (unsigned)((ncell + kBlock - 1) / kBlock);
   bool fused = h->cfg.schedule == MAM4X_SCHED_FUSED ||
                (h->cfg.schedule == MAM4X_SCHED_AUTO && ncell < kFusedBelow);
-  if (fused) {
-    if (pre)
-      hipLaunchKernelGGL(chain_fused_kernel<true>, dim3(grid), dim3(kBlock), 0, s, a);
-    else
-      hipLaunchKernelGGL(chain_fused_kernel<false>, dim3(grid), dim3(kBlock), 0, s, a);
-    HIP_TRY(h, hipGetLastError());
-    return MAM4X_OK;
-  }
   Work w;
   std::memset(&w, 0, sizeof(w));
   w.ncell = ncell;
   w.ctl = h->ctl;
   w.dtfull = (double)(int)(dt / 1);  // QUIRK (mam4_amicphys.hpp:1319): int sub-step
-  if (a.cfg.do_cond) {
+  const bool queue = !fused && a.cfg.do_cond;
+  if (queue) {
     rc = reserve_work(h, ncell, s);
     if (rc) return rc;
     w.ws = h->ws;
     w.bucket = h->bucket;
     w.queue = h->queue;
+  }
+  hipEvent_t *ev = nullptr;
+  if (h->stats) {
+    if (h->pool.empty()) {
+      std::array<hipEvent_t, 4> q{};
+      for (auto &e : q) HIP_TRY(h, hipEventCreate(&e));
+      h->pool.push_back(q);
+    }
+    h->pending.push_back(h->pool.back());
+    h->pool.pop_back();
+    ev = h->pending.back().data();
+    HIP_TRY(h, hipEventRecord(ev[0], s));
+  }
+  if (queue) {
     hipLaunchKernelGGL(queue_reset_kernel, dim3(1), dim3(64), 0, s, h->ctl);
     if (pre)
       hipLaunchKernelGGL(chain_pre_kernel<true>, dim3(grid), dim3(kBlock), 0, s, a, w);
     else
       hipLaunchKernelGGL(chain_pre_kernel<false>, dim3(grid), dim3(kBlock), 0, s, a, w);
     hipLaunchKernelGGL(queue_scatter_kernel, dim3(grid), dim3(kBlock), 0, s, w);
-    hipLaunchKernelGGL(soa_queue_kernel, dim3(h->soa_grid), dim3(kBlock), 0, s, w, a.stats);
   }
-  if (pre)
-    hipLaunchKernelGGL(chain_post_kernel<true>, dim3(grid), dim3(kBlock), 0, s, a, w);
-  else
-    hipLaunchKernelGGL(chain_post_kernel<false>, dim3(grid), dim3(kBlock), 0, s, a, w);
+  if (ev) HIP_TRY(h, hipEventRecord(ev[1], s));
+  if (queue)
+    hipLaunchKernelGGL(soa_queue_kernel, dim3(h->soa_grid), dim3(kBlock), 0, s, w, a.stats);
+  if (ev) HIP_TRY(h, hipEventRecord(ev[2], s));
+  if (fused) {
+    if (pre)
+      hipLaunchKernelGGL(chain_fused_kernel<true>, dim3(grid), dim3(kBlock), 0, s, a);
+    else
+      hipLaunchKernelGGL(chain_fused_kernel<false>, dim3(grid), dim3(kBlock), 0, s, a);
+  } else {
+    if (pre)
+      hipLaunchKernelGGL(chain_post_kernel<true>, dim3(grid), dim3(kBlock), 0, s, a, w);
+    else
+      hipLaunchKernelGGL(chain_post_kernel<false>, dim3(grid), dim3(kBlock), 0, s, a, w);
+  }
   HIP_TRY(h, hipGetLastError());
+  if (ev) HIP_TRY(h, hipEventRecord(ev[3], s));
   return MAM4X_OK;
 }
 

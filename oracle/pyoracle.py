@@ -88,7 +88,7 @@ class Oracle:
         L.orc_pcarbon_aging_1subarea.argtypes = [ctypes.c_uint] + [_dp] * 8
         L.orc_amicphys_intr.argtypes = [ctypes.POINTER(OrcConfig)] + [ctypes.c_double] * 8 + \
             [_dp] * 10 + [_dp, _dp, ctypes.POINTER(ctypes.c_int)]
-        L.orc_calcsize_level.argtypes = [ctypes.POINTER(OrcConfig), ctypes.c_double] + [_dp] * 12
+        L.orc_calcsize_level.argtypes = [ctypes.POINTER(OrcConfig), ctypes.c_double] + [_dp] * 10
         L.orc_soaexch.restype = ctypes.c_int
         L.orc_soaexch.argtypes = [ctypes.c_double] * 3 + [_dp, _dp, _dp, _dp, _dp]
         L.orc_set_libm_noise.argtypes = [ctypes.c_ulonglong]
