@@ -19,10 +19,14 @@
  *  - Every function returns an int status (MAM4X_OK = 0); no C++ exception
  *    crosses this boundary.  mam4x_last_error() describes the last failure.
  *  - All array arguments are caller-owned DEVICE pointers to double.  A
- *    "plane" is one field on [nlev][ldc] with the column index fastest; ldc >=
- *    ncol is the column stride (0 means ncol).  Field tables list one pointer
- *    per field, so the prognostic fields may live in separate allocations (as
- *    EAMxx fields do) or in one [field][nlev][ldc] block.
+ *    "plane" is one field over all cells, in one of two layouts:
+ *      MAM4X_LAYOUT_LEV_COL  [nlev][ld], column index fastest (ld >= ncol);
+ *      MAM4X_LAYOUT_COL_LEV  [ncol][ld], level index fastest (ld >= nlev),
+ *                            the layout of EAMxx fields (ncol x nlev).
+ *    ld = 0 means the fastest extent.  Field tables list one pointer per
+ *    field, so the prognostic fields may live in separate allocations (as
+ *    EAMxx fields do) or in one [field][...][ld] block.  Cells are processed
+ *    in memory order of the chosen layout, so both are read coalesced.
  *  - Work is enqueued on the given stream (a hipStream_t; NULL = the null
  *    stream).  Calls are stream-ordered and never synchronize the host,
  *    except mam4x_validate, which returns a host-side count.
@@ -36,7 +40,7 @@
 extern "C" {
 #endif
 
-#define MAM4X_ABI_VERSION 1
+#define MAM4X_ABI_VERSION 2
 
 /* status codes */
 #define MAM4X_OK 0
@@ -72,6 +76,10 @@ extern "C" {
 #define MAM4X_PREC_FP64 0  /* everything in FP64 (the drop-in)             */
 #define MAM4X_PREC_MIXED 1 /* FP32 bulk, FP64 kept where stated in DESIGN  */
 
+/* plane layouts (see Conventions) */
+#define MAM4X_LAYOUT_LEV_COL 0
+#define MAM4X_LAYOUT_COL_LEV 1
+
 /* schedule of a step (performance only; results are identical) */
 #define MAM4X_SCHED_AUTO 0  /* fused below 16,384 cells, else work queue   */
 #define MAM4X_SCHED_FUSED 1 /* one lane runs one cell's whole step         */
@@ -100,7 +108,8 @@ typedef struct mam4x_config {
 typedef struct mam4x_inputs {
   int ncol;
   int nlev;
-  int ldc;                            /* column stride of every plane       */
+  int ldc;                            /* leading dimension of every plane   */
+  int layout;                         /* MAM4X_LAYOUT_*                     */
   const double *state[MAM4X_NSTATE];  /* prognostics, one plane each        */
   const double *temperature;          /* [K]                                */
   const double *pressure;             /* mid-level pressure [Pa]            */

@@ -6,6 +6,8 @@ The structures below must match the C declarations field for field; the
 import ctypes
 import os
 
+ABI_VERSION = 2
+
 NSTATE = 52
 NMODE = 4
 
@@ -21,6 +23,9 @@ PREC_MIXED = 1
 SCHED_AUTO = 0
 SCHED_FUSED = 1
 SCHED_QUEUE = 2
+
+LAYOUT_LEV_COL = 0
+LAYOUT_COL_LEV = 1
 
 NPHASE = 3
 PHASE_NAMES = ("setup", "soa", "finish")
@@ -51,6 +56,7 @@ class Inputs(ctypes.Structure):
         ("ncol", ctypes.c_int),
         ("nlev", ctypes.c_int),
         ("ldc", ctypes.c_int),
+        ("layout", ctypes.c_int),
         ("state", _vp * NSTATE),
         ("temperature", _vp),
         ("pressure", _vp),

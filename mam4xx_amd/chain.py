@@ -70,14 +70,19 @@ class Chain:
             raise ChainError("%s failed (%d): %s" % (what, rc, msg.decode() if msg else ""))
 
     @staticmethod
-    def make_inputs(scene, precldchem=None, pregaschem=None):
+    def make_inputs(scene, precldchem=None, pregaschem=None, layout=_abi.LAYOUT_LEV_COL):
         """Build an _abi.Inputs from a dict of CUDA float64 tensors (see
-        scenes.generate for the keys).  Returns (Inputs, keepalive list)."""
+        scenes.generate for the keys): planes [field][nlev][ncol] for
+        LAYOUT_LEV_COL, [field][ncol][nlev] for LAYOUT_COL_LEV."""
         st = scene["state"]
-        nf, nlev, ncol = st.shape
+        nf = st.shape[0]
         assert nf == L.NSTATE
         inp = _abi.Inputs()
-        inp.ncol, inp.nlev, inp.ldc = ncol, nlev, ncol
+        if layout == _abi.LAYOUT_LEV_COL:
+            nlev, ncol = st.shape[1:]
+        else:
+            ncol, nlev = st.shape[1:]
+        inp.ncol, inp.nlev, inp.ldc, inp.layout = ncol, nlev, st.shape[2], layout
         for i, p in enumerate(_plane_ptrs(st, L.NSTATE)):
             inp.state[i] = p
         for key, fld in (("temperature", "temperature"), ("pressure", "pressure"),
@@ -99,8 +104,9 @@ class Chain:
         return inp
 
     def compute_tendencies(self, scene, dt, tend, dgn_i=None, dgn_c=None, state_out=None,
-                           precldchem=None, pregaschem=None, t=0.0, stream=None):
-        inp = self.make_inputs(scene, precldchem, pregaschem)
+                           precldchem=None, pregaschem=None, t=0.0, stream=None,
+                           layout=_abi.LAYOUT_LEV_COL):
+        inp = self.make_inputs(scene, precldchem, pregaschem, layout)
         out = _abi.Outputs()
         for i, p in enumerate(_plane_ptrs(tend, L.NSTATE)):
             out.tendencies[i] = p

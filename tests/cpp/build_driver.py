@@ -1,0 +1,32 @@
+"""Build recipe of the C++ host-interface test driver (tests/cpp/adapter_driver.cpp).
+
+Produces tests/_build/libadapter_driver.so, linked against
+mam4xx_amd/lib/libmam4x.so (build that first).  Called by
+__graft_entry__.build(); the GPU test loads the result and fails when it is
+missing.
+"""
+import os
+import subprocess
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+ROOT = os.path.dirname(os.path.dirname(HERE))
+OUT = os.path.join(ROOT, "tests", "_build", "libadapter_driver.so")
+HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")
+
+
+def build(verbose=False):
+    os.makedirs(os.path.dirname(OUT), exist_ok=True)
+    cmd = [HIPCC, "-O2", "-std=c++17", "-fPIC", "-shared", "--offload-arch=gfx950",
+           "-Xclang", "-target-feature", "-Xclang", "-packed-fp32-ops",
+           "-o", OUT + ".tmp", os.path.join(HERE, "adapter_driver.cpp"),
+           "-L" + os.path.join(ROOT, "mam4xx_amd", "lib"), "-lmam4x",
+           "-Wl,-rpath,$ORIGIN/../../mam4xx_amd/lib"]
+    if verbose:
+        print(" ".join(cmd), flush=True)
+    subprocess.run(cmd, check=True)
+    os.replace(OUT + ".tmp", OUT)
+    return OUT
+
+
+if __name__ == "__main__":
+    build(verbose=True)

@@ -32,7 +32,7 @@ extern "C" long cell_host_run(const int *cfgv, int ncol, int nlev, double dt,
     a.dgc[m] = dgc + m * n;
   }
   a.K = &K;
-  a.ncol = ncol; a.nlev = nlev; a.ldc = ncol;
+  a.ncol = ncol; a.nlev = nlev; a.ldc = ncol; a.layout = 0;
   a.cfg.do_cond = cfgv[0]; a.cfg.do_rename = cfgv[1]; a.cfg.do_newnuc = cfgv[2];
   a.cfg.do_coag = cfgv[3]; a.cfg.gaexch_h2so4_uptake_optaa = cfgv[4];
   a.cfg.nmono = (unsigned)cfgv[5];

@@ -215,6 +215,28 @@ def test_schedules_agree(torch_cuda, dt):
         assert np.array_equal(x, y, equal_nan=True)
 
 
+def test_column_major_layout(torch_cuda):
+    """Planes stored [ncol][nlev] (level fastest, EAMxx's field layout) give
+    bitwise the same results as [nlev][ncol]."""
+    from mam4xx_amd import _abi
+    from mam4xx_amd.chain import Chain, scene_to_device
+    torch = torch_cuda
+    sc = scenes.generate(300, 72, seed=16)
+    a = run_gpu(torch_cuda, sc)
+    sct = {k: (np.ascontiguousarray(np.swapaxes(v, -1, -2)) if v.ndim >= 2 else v)
+           for k, v in sc.items()}
+    d = scene_to_device(sct)
+    ch = Chain(schedule=_abi.SCHED_QUEUE)
+    tend = torch.empty((L.NSTATE, 300, 72), dtype=torch.float64, device="cuda")
+    dgi = torch.empty((4, 300, 72), dtype=torch.float64, device="cuda")
+    dgc = torch.empty((4, 300, 72), dtype=torch.float64, device="cuda")
+    ch.compute_tendencies(d, DT, tend, dgi, dgc, layout=_abi.LAYOUT_COL_LEV)
+    torch.cuda.synchronize()
+    for x, y in zip(a, (tend, dgi, dgc)):
+        assert np.array_equal(x, np.swapaxes(y.cpu().numpy(), -1, -2))
+    ch.close()
+
+
 def test_validate_counts_violations(torch_cuda):
     from mam4xx_amd.chain import Chain, scene_to_device
     sc = scenes.generate(32, 8, seed=14)
