@@ -142,6 +142,17 @@ typedef struct mam4x_outputs {
   double *state_out[MAM4X_NSTATE];
 } mam4x_outputs;
 
+/* Forward-mode tangent: the direction of the inputs and the derivative of
+ * the tendencies along it (mam4x_compute_tangent).  NULL input planes are
+ * zero directions. */
+typedef struct mam4x_tangent {
+  const double *state[MAM4X_NSTATE];  /* d(prognostic), units of the state   */
+  const double *temperature;          /* dT [K]                              */
+  const double *pressure;             /* dp [Pa]                             */
+  const double *vapor_mixing_ratio;   /* dqv [kg/kg]                         */
+  double *tendencies[MAM4X_NSTATE];   /* out: d(tendency) along the direction */
+} mam4x_tangent;
+
 typedef struct mam4x_handle_s *mam4x_handle;
 
 /* Library / handle lifetime.  mam4x_create replaces AeroProcess<Impl>'s
@@ -167,6 +178,16 @@ int mam4x_validate(mam4x_handle h, const mam4x_inputs *in, long long *violations
 int mam4x_compute_tendencies(mam4x_handle h, double t, double dt,
                              const mam4x_inputs *in, const mam4x_outputs *out,
                              void *stream);
+
+/* Directional derivative of mam4x_compute_tendencies (no counterpart in the
+ * reference, which has no derivative code): the same step, evaluated in
+ * forward-mode dual numbers, writes the tendencies and diameters to *out
+ * and their derivative along *dir to dir->tendencies.  The derivative is the
+ * one of the branch each cell takes (the chain is piecewise smooth: clips,
+ * regime switches and the SOA sub-step count are held at the primal's
+ * choice).  FP64 only; one lane runs one cell's whole step. */
+int mam4x_compute_tangent(mam4x_handle h, double t, double dt, const mam4x_inputs *in,
+                          const mam4x_tangent *dir, const mam4x_outputs *out, void *stream);
 
 /* Cost diagnostic: with stats enabled, every compute_tendencies call adds
  * the SOA sub-steps (gasaerexch_soaexch.hpp:170-335) its cells took to a

@@ -73,6 +73,16 @@ class Inputs(ctypes.Structure):
     ]
 
 
+class Tangent(ctypes.Structure):
+    _fields_ = [
+        ("state", _vp * NSTATE),
+        ("temperature", _vp),
+        ("pressure", _vp),
+        ("vapor_mixing_ratio", _vp),
+        ("tendencies", _vp * NSTATE),
+    ]
+
+
 class Outputs(ctypes.Structure):
     _fields_ = [
         ("tendencies", _vp * NSTATE),
@@ -96,6 +106,9 @@ SIGNATURES = {
     "mam4x_compute_tendencies": (ctypes.c_int, [_vp, ctypes.c_double, ctypes.c_double,
                                                 ctypes.POINTER(Inputs), ctypes.POINTER(Outputs),
                                                 _vp]),
+    "mam4x_compute_tangent": (ctypes.c_int, [_vp, ctypes.c_double, ctypes.c_double,
+                                             ctypes.POINTER(Inputs), ctypes.POINTER(Tangent),
+                                             ctypes.POINTER(Outputs), _vp]),
     "mam4x_enable_stats": (ctypes.c_int, [_vp, ctypes.c_int]),
     "mam4x_last_stats": (ctypes.c_int, [_vp, ctypes.POINTER(ctypes.c_longlong), _vp]),
     "mam4x_last_phase_times": (ctypes.c_int, [_vp, ctypes.POINTER(ctypes.c_double),

@@ -1,0 +1,100 @@
+"""torch.autograd.Function over the microphysics step.
+
+``tendencies(state, temperature, pressure, qv, scene, dt)`` returns the
+[52, nlev, ncol] tendencies of one step (mam4x_compute_tendencies) and is
+differentiable in the prognostic state, temperature, pressure and water
+vapour.
+
+* Forward mode (``torch.func.jvp``, ``torch.autograd.forward_ad``): one call of
+  the tangent kernel (mam4x_compute_tangent) per direction.
+* Reverse mode (``backward``): cells are independent, so each cell's Jacobian
+  is a 52 x 55 block.  The vector-Jacobian product is assembled from one
+  tangent call per differentiable input field, each along the unit direction
+  of that field in every cell at once:
+  grad_x_j(cell) = sum_f g_f(cell) * d tend_f(cell) / d x_j(cell).
+  This is exact (the same derivative as forward mode) and costs one tangent
+  pass per input field that requires a gradient -- up to 55 -- not the few
+  forward passes a hand-written reverse sweep would take.
+
+Derivatives are those of the branch each cell takes (include/mam4x.h,
+mam4x_compute_tangent).
+"""
+import torch
+
+from . import layout as L
+from .chain import Chain
+
+_chains = {}
+
+
+def _chain():
+    dev = torch.cuda.current_device()
+    if dev not in _chains:
+        _chains[dev] = Chain()
+    return _chains[dev]
+
+
+def _scene(scene, state, temperature, pressure, qv):
+    d = dict(scene)
+    d["state"] = state.detach().contiguous()
+    d["temperature"] = temperature.detach().contiguous()
+    d["pressure"] = pressure.detach().contiguous()
+    d["qv"] = qv.detach().contiguous()
+    return d
+
+
+def _tangent(d, dt, dstate, dT, dp, dqv):
+    nlev, ncol = d["temperature"].shape
+    tend = torch.empty((L.NSTATE, nlev, ncol), dtype=torch.float64, device="cuda")
+    dtend = torch.empty_like(tend)
+    _chain().compute_tangent(d, dt, tend, dtend, dstate=dstate, dtemperature=dT, dpressure=dp,
+                             dqv=dqv)
+    return tend, dtend
+
+
+class _Tendencies(torch.autograd.Function):
+    @staticmethod
+    def forward(ctx, state, temperature, pressure, qv, scene, dt):
+        d = _scene(scene, state, temperature, pressure, qv)
+        nlev, ncol = d["temperature"].shape
+        tend = torch.empty((L.NSTATE, nlev, ncol), dtype=torch.float64, device="cuda")
+        _chain().compute_tendencies(d, dt, tend)
+        ctx.scene, ctx.dt = d, dt
+        return tend
+
+    @staticmethod
+    def jvp(ctx, dstate, dT, dp, dqv, _dscene, _ddt):
+        c = lambda x: None if x is None else x.contiguous()
+        _, dtend = _tangent(ctx.scene, ctx.dt, c(dstate), c(dT), c(dp), c(dqv))
+        return dtend
+
+    @staticmethod
+    def backward(ctx, g):
+        d, dt = ctx.scene, ctx.dt
+        g = g.contiguous()
+        nlev, ncol = d["temperature"].shape
+        one = torch.ones((nlev, ncol), dtype=torch.float64, device="cuda")
+        grads = [None, None, None, None]
+        if ctx.needs_input_grad[0]:
+            gs = torch.empty((L.NSTATE, nlev, ncol), dtype=torch.float64, device="cuda")
+            unit = torch.zeros((L.NSTATE, nlev, ncol), dtype=torch.float64, device="cuda")
+            for j in range(L.NSTATE):
+                unit.zero_()
+                unit[j] = 1.0
+                _, dtend = _tangent(d, dt, unit, None, None, None)
+                gs[j] = (g * dtend).sum(dim=0)
+            grads[0] = gs
+        for k, key in ((1, "T"), (2, "p"), (3, "qv")):
+            if ctx.needs_input_grad[k]:
+                args = [None, None, None]
+                args[k - 1] = one
+                _, dtend = _tangent(d, dt, None, *args)
+                grads[k] = (g * dtend).sum(dim=0)
+        return grads[0], grads[1], grads[2], grads[3], None, None
+
+
+def tendencies(state, temperature, pressure, qv, scene, dt):
+    """Differentiable tendencies of one step; scene supplies the other
+    inputs (height, cloud fraction, pblh, diameters, wet density) as CUDA
+    float64 tensors in the [field][level][column] layout."""
+    return _Tendencies.apply(state, temperature, pressure, qv, scene, dt)

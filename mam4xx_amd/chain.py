@@ -124,6 +124,37 @@ class Chain:
                                                ctypes.byref(out), ctypes.c_void_p(s.cuda_stream))
         self._check(rc, "mam4x_compute_tendencies")
 
+    def compute_tangent(self, scene, dt, tend, dtend, dstate=None, dtemperature=None,
+                        dpressure=None, dqv=None, dgn_i=None, dgn_c=None, t=0.0, stream=None,
+                        layout=_abi.LAYOUT_LEV_COL):
+        """Tendencies and their derivative along (dstate, dT, dp, dqv) (None =
+        zero direction), through mam4x_compute_tangent."""
+        inp = self.make_inputs(scene, None, None, layout)
+        out = _abi.Outputs()
+        for i, p in enumerate(_plane_ptrs(tend, L.NSTATE)):
+            out.tendencies[i] = p
+        if dgn_i is not None:
+            for m, p in enumerate(_plane_ptrs(dgn_i, 4)):
+                out.dgncur_i[m] = p
+        if dgn_c is not None:
+            for m, p in enumerate(_plane_ptrs(dgn_c, 4)):
+                out.dgncur_c[m] = p
+        tg = _abi.Tangent()
+        for i, p in enumerate(_plane_ptrs(dtend, L.NSTATE)):
+            tg.tendencies[i] = p
+        if dstate is not None:
+            for i, p in enumerate(_plane_ptrs(dstate, L.NSTATE)):
+                tg.state[i] = p
+        for name, tt in (("temperature", dtemperature), ("pressure", dpressure),
+                         ("vapor_mixing_ratio", dqv)):
+            if tt is not None:
+                assert tt.dtype == torch.float64 and tt.is_cuda and tt.is_contiguous()
+                setattr(tg, name, tt.data_ptr())
+        s = stream if stream is not None else torch.cuda.current_stream()
+        rc = self.lib.mam4x_compute_tangent(self.h, t, dt, ctypes.byref(inp), ctypes.byref(tg),
+                                            ctypes.byref(out), ctypes.c_void_p(s.cuda_stream))
+        self._check(rc, "mam4x_compute_tangent")
+
     def validate(self, scene, stream=None):
         inp = self.make_inputs(scene)
         v = ctypes.c_longlong(0)

@@ -11,6 +11,8 @@
 // chain_cell_fused runs the three phases back to back in one lane.
 #pragma once
 
+#include <type_traits>
+
 #include "chain_device.hpp"
 
 namespace mam4x {
@@ -25,6 +27,11 @@ struct ChainArgs {
   double *tend[NSTATE];
   double *dgi[4], *dgc[4];
   double *sout[NSTATE];
+  // tangent mode: directions of the inputs (NULL = 0) and the derivative of
+  // the tendencies along them
+  const double *dstate[NSTATE];
+  const double *dT, *dp, *dqv;
+  double *dtend[NSTATE];
   const ChainConsts *K;
   unsigned long long *stats;  // SOA sub-step counter (NULL: off)
   int ncol, nlev, ldc;
@@ -36,53 +43,83 @@ struct ChainArgs {
 
 namespace dev {
 
+// an input value in the wide type: with its direction in tangent mode
+template <class P>
+__device__ __forceinline__ typename P::W ldw(const double *p, const double *dp, long off) {
+  using W = typename P::W;
+  if constexpr (std::is_same<W, num::Dual>::value)
+    return num::Dual(p[off], dp ? dp[off] : 0.0);
+  else
+    return W(p[off]);
+}
+
 // get_partition_factors (mam4_amicphys.hpp:492-546)
-__device__ __forceinline__ void partition_factors(double qi, double qc, double fcldy,
-                                                  double fclea, double &fa, double &fc) {
-  const double qc_cldy = qc / fcldy;
-  const double qi_cldy = kmax(0.0, ((qi + qc) - qc_cldy));
-  const double qi_clea = (qi - fcldy * qi_cldy) / fclea;
-  double r = kmax(1.e-35, qi_clea * fclea) / kmax(1.e-35, qi);
+template <class R>
+__device__ __forceinline__ void partition_factors(R qi, R qc, double fcldy, double fclea, R &fa,
+                                                  R &fc) {
+  const R qc_cldy = qc / fcldy;
+  const R qi_cldy = kmax(0.0, ((qi + qc) - qc_cldy));
+  const R qi_clea = (qi - fcldy * qi_cldy) / fclea;
+  R r = kmax(1.e-35, qi_clea * fclea) / kmax(1.e-35, qi);
   r = kmax(0.0, kmin(1.0, r));
   fa = r / fclea;
   fc = (1 - r) / fcldy;
 }
 
-// grid-cell means in chain units, clipped at 0 (mam4_amicphys.hpp:2258-2267)
-__device__ __forceinline__ double ld3(const ChainArgs &a, long off, int f, double c) {
-  return kmax(0.0, a.state[f][off] * c);
+// grid-cell means in chain units, clipped at 0 (mam4_amicphys.hpp:2258-2267);
+// the product is formed in the wide type
+template <class P>
+__device__ __forceinline__ typename P::W ld3w(const ChainArgs &a, long off, int f, double c) {
+  return kmax(0.0, ldw<P>(a.state[f], a.dstate[f], off) * c);
 }
-template <bool PRE>
-__device__ __forceinline__ double ld2(const ChainArgs &a, long off, int f, double c) {
-  return PRE ? kmax(0.0, (a.precld[f] ? a.precld[f][off] : a.state[f][off]) * c)
-             : ld3(a, off, f, c);
+template <class P>
+__device__ __forceinline__ typename P::R ld3(const ChainArgs &a, long off, int f, double c) {
+  return num::to<typename P::R>(ld3w<P>(a, off, f, c));
+}
+template <bool PRE, class P>
+__device__ __forceinline__ typename P::W ld2w(const ChainArgs &a, long off, int f, double c) {
+  if (PRE && a.precld[f]) return kmax(0.0, ldw<P>(a.precld[f], nullptr, off) * c);
+  return ld3w<P>(a, off, f, c);
+}
+template <bool PRE, class P>
+__device__ __forceinline__ typename P::R ld2(const ChainArgs &a, long off, int f, double c) {
+  return num::to<typename P::R>(ld2w<PRE, P>(a, off, f, c));
 }
 
 // Per-cell context shared by the phases.
+template <class P>
 struct CellCtx {
-  double T, p, z, pblh;
+  using R = typename P::R;
+  using W = typename P::W;
+  R T, p, z, pblh;
   int cat;  // setup_subareas (mam4_amicphys.hpp:241-317): 0 clear, 1 overcast, 2 partly
-  double fcldy, fclea, rh_clear;
-  double dga[4], dgw[4], wd[4];
-  double h1, s1, h2, s2, h3, s3;  // gases before gas chem / before cloud chem / now
-  double fa_n[4], fc_n[4], fa_q[4], fc_q[4];
+  double fcldy, fclea;
+  R rh_clear;
+  R dga[4], dgw[4], wd[4];
+  W h1, s1, h2, s2, h3, s3;  // gases before gas chem / before cloud chem / now
+  R fa_n[4], fc_n[4], fa_q[4], fc_q[4];
   bool map_gas;
 };
 
-template <bool PRE>
-__device__ __forceinline__ void cell_setup(const ChainArgs &a, long off, int icol, CellCtx &X) {
-  X.T = a.T[off];
-  X.p = a.p[off];
+template <bool PRE, class P>
+__device__ __forceinline__ void cell_setup(const ChainArgs &a, long off, int icol,
+                                           CellCtx<P> &X) {
+  using R = typename P::R;
+  using W = typename P::W;
+  const W Tw = ldw<P>(a.T, a.dT, off), pw = ldw<P>(a.p, a.dp, off);
+  X.T = num::to<R>(Tw);
+  X.p = num::to<R>(pw);
   X.z = a.z[off];
   X.pblh = a.pblh[icol];
-  const double qv = a.qv[off], cld = a.cld[off];
+  const R qv = num::to<R>(ldw<P>(a.qv, a.dqv, off));
+  const double cld = a.cld[off];
   // relative humidity from the improved Magnus formula (conversions.hpp:148-180)
   const double epsqs = MW_H2O / MW_DRY_AIR;
-  const double tc = X.T - 273.15;
-  const double ew = 610.94 * exp(17.625 * tc / (234.04 + tc));
-  const double ev = 1.00071 * exp(4.5e-8 * X.p) * ew;
-  const double qvs = epsqs * ev / (X.p - (1 - epsqs) * ev);
-  const double rhgcm = kmax(0.0, kmin(1.0, qv / qvs));
+  const R tc = X.T - 273.15;
+  const R ew = 610.94 * exp(17.625 * tc / (234.04 + tc));
+  const R ev = 1.00071 * exp(4.5e-8 * X.p) * ew;
+  const R qvs = epsqs * ev / (X.p - (1 - epsqs) * ev);
+  const R rhgcm = kmax(0.0, kmin(1.0, qv / qvs));
   if (cld < 1.0e-5) {
     X.cat = 0;
     X.fcldy = 0;
@@ -94,38 +131,40 @@ __device__ __forceinline__ void cell_setup(const ChainArgs &a, long off, int ico
     X.fcldy = cld;
   }
   X.fclea = 1.0 - X.fcldy;
-  X.rh_clear = (X.cat == 2) ? mmb(0.0, 1.0, (rhgcm - X.fcldy) / X.fclea) : rhgcm;
+  X.rh_clear = (X.cat == 2) ? R(mmb(0.0, 1.0, (rhgcm - X.fcldy) / X.fclea)) : rhgcm;
 #pragma unroll
   for (int m = 0; m < 4; ++m) {
     X.dga[m] = a.dga[m][off];
     X.dgw[m] = a.dgw[m][off];
-    X.wd[m] = kmax(1000.0, a.wd[m][off]);
+    X.wd[m] = kmax(1000.0, R(a.wd[m][off]));
   }
   if (PRE) {
-    X.h1 = kmax(0.0, (a.pre_h2so4 ? a.pre_h2so4[off] : a.state[F_H2SO4][off]) * TO_VMR_H2SO4);
-    X.s1 = kmax(0.0, (a.pre_soag ? a.pre_soag[off] : a.state[F_SOAG][off]) * TO_VMR_SOAG);
+    X.h1 = a.pre_h2so4 ? kmax(0.0, W(a.pre_h2so4[off]) * TO_VMR_H2SO4)
+                       : ld3w<P>(a, off, F_H2SO4, TO_VMR_H2SO4);
+    X.s1 = a.pre_soag ? kmax(0.0, W(a.pre_soag[off]) * TO_VMR_SOAG)
+                      : ld3w<P>(a, off, F_SOAG, TO_VMR_SOAG);
   } else {
-    X.h1 = ld3(a, off, F_H2SO4, TO_VMR_H2SO4);
-    X.s1 = ld3(a, off, F_SOAG, TO_VMR_SOAG);
+    X.h1 = ld3w<P>(a, off, F_H2SO4, TO_VMR_H2SO4);
+    X.s1 = ld3w<P>(a, off, F_SOAG, TO_VMR_SOAG);
   }
-  X.h3 = ld3(a, off, F_H2SO4, TO_VMR_H2SO4);
-  X.s3 = ld3(a, off, F_SOAG, TO_VMR_SOAG);
-  X.h2 = ld2<PRE>(a, off, F_H2SO4, TO_VMR_H2SO4);
-  X.s2 = ld2<PRE>(a, off, F_SOAG, TO_VMR_SOAG);
+  X.h3 = ld3w<P>(a, off, F_H2SO4, TO_VMR_H2SO4);
+  X.s3 = ld3w<P>(a, off, F_SOAG, TO_VMR_SOAG);
+  X.h2 = ld2w<PRE, P>(a, off, F_H2SO4, TO_VMR_H2SO4);
+  X.s2 = ld2w<PRE, P>(a, off, F_SOAG, TO_VMR_SOAG);
   // partition factors of the interstitial aerosol (partly cloudy cells)
 #pragma unroll
   for (int m = 0; m < 4; ++m) {
     X.fa_n[m] = X.fc_n[m] = X.fa_q[m] = X.fc_q[m] = 0.0;
     if (X.cat == 2) {
-      partition_factors(ld2<PRE>(a, off, f_ni(m), TO_VMR_NUM),
-                        ld2<PRE>(a, off, f_nc(m), TO_VMR_NUM), X.fcldy, X.fclea, X.fa_n[m],
+      partition_factors(ld2<PRE, P>(a, off, f_ni(m), TO_VMR_NUM),
+                        ld2<PRE, P>(a, off, f_nc(m), TO_VMR_NUM), X.fcldy, X.fclea, X.fa_n[m],
                         X.fc_n[m]);
-      double qi = 0.0, qc = 0.0;
+      R qi = 0.0, qc = 0.0;
 #pragma unroll
       for (int s = 0; s < 7; ++s) {
         if (s >= NSPEC_MODE[m]) continue;
-        qi = qi + ld2<PRE>(a, off, f_qi(m, s), to_vmr_mass(m, s));
-        qc = qc + ld2<PRE>(a, off, f_qc(m, s), to_vmr_mass(m, s));
+        qi = qi + ld2<PRE, P>(a, off, f_qi(m, s), to_vmr_mass(m, s));
+        qc = qc + ld2<PRE, P>(a, off, f_qc(m, s), to_vmr_mass(m, s));
       }
       partition_factors(qi, qc, X.fcldy, X.fclea, X.fa_q[m], X.fc_q[m]);
     }
@@ -135,23 +174,27 @@ __device__ __forceinline__ void cell_setup(const ChainArgs &a, long off, int ico
 
 // Clear-subarea state (set_subarea_gases_and_aerosols, mam4_amicphys.hpp:
 // 678-911, and the mapping of mam_amicphys_1gridcell, :1784-1851).
+template <class W>
 struct ClearGas {
-  double g1[2], g[2];  // {SOAG, H2SO4} before gas chem / current, chain units
-  double hcl, scl;     // clear-subarea gases before mapping
-  double hcd, scd;     // cloudy-subarea gases (partly cloudy cells)
+  W g1[2], g[2];  // {SOAG, H2SO4} before gas chem / current, chain units
+  W hcl, scl;     // clear-subarea gases before mapping
+  W hcd, scd;     // cloudy-subarea gases (partly cloudy cells)
 };
 
-__device__ __forceinline__ void clear_build(const ChainArgs &a, long off, const CellCtx &X,
-                                            Aer &S, ClearGas &G) {
+template <class P>
+__device__ __forceinline__ void clear_build(const ChainArgs &a, long off, const CellCtx<P> &X,
+                                            Aer<typename P::R> &S, ClearGas<typename P::W> &G) {
+  using R = typename P::R;
+  using W = typename P::W;
 #pragma unroll
   for (int m = 0; m < 4; ++m) {
-    const double nn = ld3(a, off, f_ni(m), TO_VMR_NUM);
+    const R nn = ld3<P>(a, off, f_ni(m), TO_VMR_NUM);
     S.n[m] = (X.cat == 2 ? nn * X.fa_n[m] : nn) * 1.0;
 #pragma unroll
     for (int s = 0; s < 7; ++s) {
       if (s >= NSPEC_MODE[m]) continue;
       const int sp = mode_spec(m, s);
-      const double qq = ld3(a, off, f_qi(m, s), to_vmr_mass(m, s));
+      const R qq = ld3<P>(a, off, f_qi(m, s), to_vmr_mass(m, s));
       S.q[sp][m] = (X.cat == 2 ? qq * X.fa_q[m] : qq) * fcvt_aer(sp);
     }
   }
@@ -174,24 +217,28 @@ __device__ __forceinline__ void clear_build(const ChainArgs &a, long off, const 
       G.scl = X.s3 / X.fclea;
     }
   }
-  G.g1[0] = X.map_gas ? X.s1 * FCVT_SOAG : 0.0;
-  G.g1[1] = X.map_gas ? X.h1 * 1.0 : 0.0;
-  G.g[0] = X.map_gas ? G.scl * FCVT_SOAG : 0.0;
-  G.g[1] = X.map_gas ? G.hcl * 1.0 : 0.0;
+  G.g1[0] = X.map_gas ? X.s1 * FCVT_SOAG : W(0.0);
+  G.g1[1] = X.map_gas ? X.h1 * 1.0 : W(0.0);
+  G.g[0] = X.map_gas ? G.scl * FCVT_SOAG : W(0.0);
+  G.g[1] = X.map_gas ? G.hcl * 1.0 : W(0.0);
 }
 
 // Cloudy subarea (renaming only), recombination, calcsize and the writes.
-// S/G: the clear subarea after clear_post (unused for overcast cells).
-template <bool PRE>
-__device__ __forceinline__ void cell_finish(const ChainArgs &a, long off, const CellCtx &X,
-                                            const Aer &S, const ClearGas &G) {
+// S/G: the clear subarea after clear_post (unused for overcast cells).  The
+// grid-cell sums, the calcsize step and the tendencies are in the wide type.
+template <bool PRE, class P>
+__device__ __forceinline__ void cell_finish(const ChainArgs &a, long off, const CellCtx<P> &X,
+                                            const Aer<typename P::R> &S,
+                                            const ClearGas<typename P::W> &G) {
+  using R = typename P::R;
+  using W = typename P::W;
   const ChainConsts &K = *a.K;
   const double dt = a.dt;
-  double gi_out[NSTATE];  // grid-cell mean after the chain, chain units
+  W gi_out[NSTATE];  // grid-cell mean after the chain, chain units
   if (X.cat != 1) {
     // map back (mam4_amicphys.hpp:1940-1969) and weight by the clear fraction
-    const double hout = X.map_gas ? G.g[1] / 1.0 : G.hcl;
-    const double sout = X.map_gas ? G.g[0] / FCVT_SOAG : G.scl;
+    const W hout = X.map_gas ? G.g[1] / 1.0 : G.hcl;
+    const W sout = X.map_gas ? G.g[0] / FCVT_SOAG : G.scl;
     gi_out[F_H2SO4] = hout * X.fclea;
     gi_out[F_SOAG] = sout * X.fclea;
     if (X.cat == 2) {
@@ -200,12 +247,12 @@ __device__ __forceinline__ void cell_finish(const ChainArgs &a, long off, const 
     }
 #pragma unroll
     for (int m = 0; m < 4; ++m) {
-      gi_out[f_ni(m)] = (S.n[m] / 1.0) * X.fclea;
+      gi_out[f_ni(m)] = (num::to<W>(S.n[m]) / 1.0) * X.fclea;
 #pragma unroll
       for (int s = 0; s < 7; ++s) {
         if (s >= NSPEC_MODE[m]) continue;
         const int sp = mode_spec(m, s);
-        gi_out[f_qi(m, s)] = (S.q[sp][m] / fcvt_aer(sp)) * X.fclea;
+        gi_out[f_qi(m, s)] = (num::to<W>(S.q[sp][m]) / fcvt_aer(sp)) * X.fclea;
       }
     }
   } else {
@@ -215,64 +262,64 @@ __device__ __forceinline__ void cell_finish(const ChainArgs &a, long off, const 
 
   if (X.cat != 0) {
     // ---- cloudy subarea: renaming of interstitial + cloud-borne only ----
-    Aer C, Cc;
-    double gi[7] = {0, 0, 0, 0, 0, 0, 0}, gc[7] = {0, 0, 0, 0, 0, 0, 0};
+    Aer<R> C, Cc;
+    R gi[7] = {0, 0, 0, 0, 0, 0, 0}, gc[7] = {0, 0, 0, 0, 0, 0, 0};
 #pragma unroll
     for (int m = 0; m < 4; ++m) {
-      const double n3 = ld3(a, off, f_ni(m), TO_VMR_NUM);
-      const double nc3 = ld3(a, off, f_nc(m), TO_VMR_NUM);
+      const R n3 = ld3<P>(a, off, f_ni(m), TO_VMR_NUM);
+      const R nc3 = ld3<P>(a, off, f_nc(m), TO_VMR_NUM);
       C.n[m] = (X.cat == 2 ? n3 * X.fc_n[m] : n3) * 1.0;
       Cc.n[m] = (X.cat == 2 ? nc3 / X.fcldy : nc3) * 1.0;
 #pragma unroll
       for (int s = 0; s < 7; ++s) {
         if (s >= NSPEC_MODE[m]) continue;
         const int sp = mode_spec(m, s);
-        const double q3 = ld3(a, off, f_qi(m, s), to_vmr_mass(m, s));
-        const double qc3 = ld3(a, off, f_qc(m, s), to_vmr_mass(m, s));
+        const R q3 = ld3<P>(a, off, f_qi(m, s), to_vmr_mass(m, s));
+        const R qc3 = ld3<P>(a, off, f_qc(m, s), to_vmr_mass(m, s));
         C.q[sp][m] = (X.cat == 2 ? q3 * X.fc_q[m] : q3) * fcvt_aer(sp);
         Cc.q[sp][m] = (X.cat == 2 ? qc3 / X.fcldy : qc3) * fcvt_aer(sp);
         if (m == NAIT) {
-          const double q2 = ld2<PRE>(a, off, f_qi(m, s), to_vmr_mass(m, s));
-          const double qc2 = ld2<PRE>(a, off, f_qc(m, s), to_vmr_mass(m, s));
-          const double a2 = (X.cat == 2 ? q2 * X.fc_q[m] : q2) * fcvt_aer(sp);
-          const double c2 = (X.cat == 2 ? qc2 / X.fcldy : qc2) * fcvt_aer(sp);
+          const R q2 = ld2<PRE, P>(a, off, f_qi(m, s), to_vmr_mass(m, s));
+          const R qc2 = ld2<PRE, P>(a, off, f_qc(m, s), to_vmr_mass(m, s));
+          const R a2 = (X.cat == 2 ? q2 * X.fc_q[m] : q2) * fcvt_aer(sp);
+          const R c2 = (X.cat == 2 ? qc2 / X.fcldy : qc2) * fcvt_aer(sp);
           gi[sp] = (C.q[sp][m] - a2) / 1;
           gc[sp] = (Cc.q[sp][m] - c2) / 1;
         }
       }
     }
-    if (a.cfg.do_rename) rename_ait_to_acc(K, true, C, gi, Cc, gc);
+    if (a.cfg.do_rename) rename_ait_to_acc<P>(K, true, C, gi, Cc, gc);
     const double w = X.fcldy;
 #pragma unroll
     for (int m = 0; m < 4; ++m) {
-      const double ni = (C.n[m] / 1.0) * w;
+      const W ni = (num::to<W>(C.n[m]) / 1.0) * w;
       gi_out[f_ni(m)] = (X.cat == 2) ? gi_out[f_ni(m)] + ni : ni;
-      gi_out[f_nc(m)] = (Cc.n[m] / 1.0) * w;
+      gi_out[f_nc(m)] = (num::to<W>(Cc.n[m]) / 1.0) * w;
 #pragma unroll
       for (int s = 0; s < 7; ++s) {
         if (s >= NSPEC_MODE[m]) continue;
         const int sp = mode_spec(m, s);
-        const double qi = (C.q[sp][m] / fcvt_aer(sp)) * w;
+        const W qi = (num::to<W>(C.q[sp][m]) / fcvt_aer(sp)) * w;
         gi_out[f_qi(m, s)] = (X.cat == 2) ? gi_out[f_qi(m, s)] + qi : qi;
-        gi_out[f_qc(m, s)] = (Cc.q[sp][m] / fcvt_aer(sp)) * w;
+        gi_out[f_qc(m, s)] = (num::to<W>(Cc.q[sp][m]) / fcvt_aer(sp)) * w;
       }
     }
   } else {
     // no cloudy subarea: cloud-borne aerosol keeps its (clipped) value
 #pragma unroll
     for (int m = 0; m < 4; ++m) {
-      gi_out[f_nc(m)] = ld3(a, off, f_nc(m), TO_VMR_NUM);
+      gi_out[f_nc(m)] = ld3w<P>(a, off, f_nc(m), TO_VMR_NUM);
 #pragma unroll
       for (int s = 0; s < 7; ++s) {
         if (s >= NSPEC_MODE[m]) continue;
-        gi_out[f_qc(m, s)] = ld3(a, off, f_qc(m, s), to_vmr_mass(m, s));
+        gi_out[f_qc(m, s)] = ld3w<P>(a, off, f_qc(m, s), to_vmr_mass(m, s));
       }
     }
   }
 
   // back to prognostic units; interstitial and gases are clipped at 0
   // (form_gcm_of_gases_and_aerosols_from_subareas, :2045-2047)
-  double post[NSTATE];
+  W post[NSTATE];
 #pragma unroll
   for (int m = 0; m < 4; ++m) {
     post[f_ni(m)] = kmax(0.0, gi_out[f_ni(m)]) * TO_MMR_NUM;
@@ -289,25 +336,25 @@ __device__ __forceinline__ void cell_finish(const ChainArgs &a, long off, const 
 
   // calcsize on the post-chain state, then the tendencies: (post - state)/dt
   // of the chain plus calcsize's own (zero where calcsize does not act)
-  CalcOut o;
+  CalcOut<W> o;
   if (a.do_calcsize) {
-    double n_i[4], n_c[4], q_i[4][7], q_c[4][7];
+    W n_i[4], n_c[4], q_i[4][7], q_c[4][7];
 #pragma unroll
     for (int m = 0; m < 4; ++m) {
       n_i[m] = post[f_ni(m)];
       n_c[m] = post[f_nc(m)];
 #pragma unroll
       for (int s = 0; s < 7; ++s) {
-        q_i[m][s] = s < NSPEC_MODE[m] ? post[f_qi(m, s)] : 0.0;
-        q_c[m][s] = s < NSPEC_MODE[m] ? post[f_qc(m, s)] : 0.0;
+        q_i[m][s] = s < NSPEC_MODE[m] ? post[f_qi(m, s)] : W(0.0);
+        q_c[m][s] = s < NSPEC_MODE[m] ? post[f_qc(m, s)] : W(0.0);
       }
     }
     calcsize(K, a.do_xfer != 0, a.do_adjust != 0, dt, n_i, n_c, q_i, q_c, o);
   }
 #pragma unroll
   for (int f = 0; f < NSTATE; ++f) {
-    const double st = a.state[f][off];
-    double t = (post[f] - st) / dt;
+    const W st = ldw<P>(a.state[f], a.dstate[f], off);
+    W t = (post[f] - st) / dt;
     if (a.do_calcsize) {
       if (f < F_Q_I) {
         t += f < F_NUM_C ? o.dn_i[f] : o.dn_c[f - F_NUM_C];
@@ -319,14 +366,15 @@ __device__ __forceinline__ void cell_finish(const ChainArgs &a, long off, const 
         t += cw ? o.dq_c[m][sp] : o.dq_i[m][sp];
       }
     }
-    a.tend[f][off] = t;
-    if (a.has_sout) a.sout[f][off] = st + dt * t;
+    a.tend[f][off] = num::val(t);
+    if constexpr (std::is_same<W, num::Dual>::value) a.dtend[f][off] = num::tan(t);
+    if (a.has_sout) a.sout[f][off] = num::val(st) + dt * num::val(t);
   }
   if (a.has_dgn) {
 #pragma unroll
     for (int m = 0; m < 4; ++m) {
-      const double di = a.do_calcsize ? o.dgn_i[m] : X.dga[m];
-      const double dc = a.do_calcsize ? o.dgn_c[m] : X.dga[m];
+      const double di = a.do_calcsize ? num::val(o.dgn_i[m]) : num::val(X.dga[m]);
+      const double dc = a.do_calcsize ? num::val(o.dgn_c[m]) : num::val(X.dga[m]);
       if (a.dgi[m]) a.dgi[m][off] = di;
       if (a.dgc[m]) a.dgc[m][off] = dc;
     }
@@ -334,21 +382,22 @@ __device__ __forceinline__ void cell_finish(const ChainArgs &a, long off, const 
 }
 
 // The whole step for one cell in one lane.  Returns the SOA sub-step count.
-template <bool PRE>
+template <bool PRE, class P>
 __device__ __forceinline__ int chain_cell_fused(const ChainArgs &a, long off, int icol) {
+  using W = typename P::W;
   const ChainConsts &K = *a.K;
-  CellCtx X;
-  cell_setup<PRE>(a, off, icol, X);
-  Aer S;
-  ClearGas G;
+  CellCtx<P> X;
+  cell_setup<PRE, P>(a, off, icol, X);
+  Aer<typename P::R> S;
+  ClearGas<W> G;
   int niter = 0;
   if (X.cat != 1) {
-    clear_build(a, off, X, S, G);
-    SoaIn in = {};
-    double sg = 0.0, sgavg = 0.0, sa[4] = {0, 0, 0, 0};
+    clear_build<P>(a, off, X, S, G);
+    SoaIn<W> in = {};
+    W sg = 0.0, sgavg = 0.0, sa[4] = {0, 0, 0, 0};
     if (a.cfg.do_cond) {
-      clear_pre(K, X.T, X.p, X.dgw, S, G.g[0], in);
-      SoaState st;
+      clear_pre<P>(K, X.T, X.p, X.dgw, S, G.g[0], in);
+      SoaState<W> st;
       soa_init(st, in);
       const double dtfull = (double)(int)(a.dt / 1);
       while (soa_tick(st, dtfull)) {
@@ -359,10 +408,10 @@ __device__ __forceinline__ int chain_cell_fused(const ChainArgs &a, long off, in
 #pragma unroll
       for (int n = 0; n < 4; ++n) sa[n] = st.a[n];
     }
-    clear_post(K, a.cfg, a.dt, X.T, X.p, X.z, X.pblh, X.rh_clear, X.dga, X.dgw, X.wd, G.g1,
-               G.g, S, in.uptk, sg, sgavg, sa);
+    clear_post<P>(K, a.cfg, a.dt, X.T, X.p, X.z, X.pblh, X.rh_clear, X.dga, X.dgw, X.wd, G.g1,
+                  G.g, S, in.uptk, sg, sgavg, sa);
   }
-  cell_finish<PRE>(a, off, X, S, G);
+  cell_finish<PRE, P>(a, off, X, S, G);
   return niter;
 }
 

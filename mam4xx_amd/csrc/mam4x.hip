@@ -60,7 +60,7 @@ constexpr unsigned char kNoSoa = 0xff;
 // and the time-averaged SOAG.
 constexpr int kRec = 16;
 
-__device__ __forceinline__ void rec_store_in(double *r, const dev::SoaIn &in) {
+__device__ __forceinline__ void rec_store_in(double *r, const dev::SoaIn<double> &in) {
   double2 *v = reinterpret_cast<double2 *>(r);
   v[0] = make_double2(in.g, in.a[0]);
   v[1] = make_double2(in.a[1], in.a[2]);
@@ -71,7 +71,7 @@ __device__ __forceinline__ void rec_store_in(double *r, const dev::SoaIn &in) {
   v[7] = make_double2(in.aopoa[2], in.aopoa[3]);
 }
 
-__device__ __forceinline__ void rec_load_in(const double *r, dev::SoaIn &in) {
+__device__ __forceinline__ void rec_load_in(const double *r, dev::SoaIn<double> &in) {
   const double2 *v = reinterpret_cast<const double2 *>(r);
   const double2 v0 = v[0], v1 = v[1], v2 = v[2], v4 = v[4], v5 = v[5], v6 = v[6], v7 = v[7];
   in.g = v0.x;
@@ -125,7 +125,7 @@ __device__ __forceinline__ void add_stats(unsigned long long *stats, unsigned lo
   if ((threadIdx.x & 63) == 0 && v) atomicAdd(stats, v);
 }
 
-template <bool PRE>
+template <bool PRE, class P>
 __global__ void __launch_bounds__(kBlock) chain_fused_kernel(ChainArgs a) {
   const long ncell = (long)a.ncol * a.nlev;
   const long c = (long)blockIdx.x * kBlock + threadIdx.x;
@@ -134,7 +134,7 @@ __global__ void __launch_bounds__(kBlock) chain_fused_kernel(ChainArgs a) {
     int icol;
     long off;
     cell_index(a, c, icol, off);
-    niter = (unsigned long long)dev::chain_cell_fused<PRE>(a, off, icol);
+    niter = (unsigned long long)dev::chain_cell_fused<PRE, P>(a, off, icol);
   }
   add_stats(a.stats, niter);
 }
@@ -149,7 +149,7 @@ __global__ void queue_reset_kernel(QueueCtl *q) {
 }
 
 // Phase 1.
-template <bool PRE>
+template <bool PRE, class P>
 __global__ void __launch_bounds__(kBlock) chain_pre_kernel(ChainArgs a, Work w) {
   __shared__ unsigned lh[kNBucket];
   if (threadIdx.x < kNBucket) lh[threadIdx.x] = 0;
@@ -159,17 +159,17 @@ __global__ void __launch_bounds__(kBlock) chain_pre_kernel(ChainArgs a, Work w) 
     int icol;
     long off;
     cell_index(a, c, icol, off);
-    dev::CellCtx X;
-    dev::cell_setup<PRE>(a, off, icol, X);
+    dev::CellCtx<P> X;
+    dev::cell_setup<PRE, P>(a, off, icol, X);
     unsigned char b = kNoSoa;
     if (X.cat != 1) {
-      dev::Aer S;
-      dev::ClearGas G;
-      dev::clear_build(a, off, X, S, G);
-      dev::SoaIn in;
-      dev::clear_pre(*a.K, X.T, X.p, X.dgw, S, G.g[0], in);
+      dev::Aer<typename P::R> S;
+      dev::ClearGas<double> G;
+      dev::clear_build<P>(a, off, X, S, G);
+      dev::SoaIn<double> in;
+      dev::clear_pre<P>(*a.K, X.T, X.p, X.dgw, S, G.g[0], in);
       rec_store_in(w.ws + c * kRec, in);
-      dev::SoaState st;
+      dev::SoaState<double> st;
       dev::soa_init(st, in);
       const float pred = (float)dev::soa_predict(st, w.dtfull);
       int ib = kNBucket - 1;
@@ -230,8 +230,8 @@ __global__ void __launch_bounds__(kBlock) soa_queue_kernel(Work w, unsigned long
   bool exhausted = false;
   bool active = false, have_next = false;
   int cell = 0, next_cell = 0;
-  dev::SoaIn nx;
-  dev::SoaState s;
+  dev::SoaIn<double> nx;
+  dev::SoaState<double> s;
   unsigned long long niter = 0;
   while (true) {
     bool done = false;
@@ -294,19 +294,19 @@ __global__ void __launch_bounds__(kBlock) soa_queue_kernel(Work w, unsigned long
 }
 
 // Phase 4.
-template <bool PRE>
+template <bool PRE, class P>
 __global__ void __launch_bounds__(kBlock) chain_post_kernel(ChainArgs a, Work w) {
   const long c = (long)blockIdx.x * kBlock + threadIdx.x;
   if (c >= w.ncell) return;
   int icol;
   long off;
   cell_index(a, c, icol, off);
-  dev::CellCtx X;
-  dev::cell_setup<PRE>(a, off, icol, X);
-  dev::Aer S;
-  dev::ClearGas G;
+  dev::CellCtx<P> X;
+  dev::cell_setup<PRE, P>(a, off, icol, X);
+  dev::Aer<typename P::R> S;
+  dev::ClearGas<double> G;
   if (X.cat != 1) {
-    dev::clear_build(a, off, X, S, G);
+    dev::clear_build<P>(a, off, X, S, G);
     double uptk[4] = {0, 0, 0, 0}, sa[4] = {0, 0, 0, 0}, sg = 0.0, sgavg = 0.0;
     if (a.cfg.do_cond) {
       const double2 *v = reinterpret_cast<const double2 *>(w.ws + c * kRec);
@@ -322,10 +322,10 @@ __global__ void __launch_bounds__(kBlock) chain_post_kernel(ChainArgs a, Work w)
       uptk[2] = v5.x;
       uptk[3] = v5.y;
     }
-    dev::clear_post(*a.K, a.cfg, a.dt, X.T, X.p, X.z, X.pblh, X.rh_clear, X.dga, X.dgw, X.wd,
+    dev::clear_post<P>(*a.K, a.cfg, a.dt, X.T, X.p, X.z, X.pblh, X.rh_clear, X.dga, X.dgw, X.wd,
                     G.g1, G.g, S, uptk, sg, sgavg, sa);
   }
-  dev::cell_finish<PRE>(a, off, X, S, G);
+  dev::cell_finish<PRE, P>(a, off, X, S, G);
 }
 
 // AeroProcess::validate checks (gasaerexch.hpp:96-120): negative T, p, qv or
@@ -348,6 +348,35 @@ __global__ void __launch_bounds__(kBlock) validate_kernel(ChainArgs a,
 }
 
 }  // namespace
+
+enum { kPre = 0, kPost = 1, kFused = 2 };
+
+template <class P>
+static void launch_fused(bool pre, unsigned grid, hipStream_t s, const ChainArgs &a) {
+  if (pre)
+    hipLaunchKernelGGL((chain_fused_kernel<true, P>), dim3(grid), dim3(kBlock), 0, s, a);
+  else
+    hipLaunchKernelGGL((chain_fused_kernel<false, P>), dim3(grid), dim3(kBlock), 0, s, a);
+}
+
+// launches one of the per-cell kernels for policy P (FP64 wide type)
+template <class P>
+static void launch_cells(int kind, bool pre, unsigned grid, hipStream_t s, const ChainArgs &a,
+                         const Work &w) {
+  if (kind == kPre) {
+    if (pre)
+      hipLaunchKernelGGL((chain_pre_kernel<true, P>), dim3(grid), dim3(kBlock), 0, s, a, w);
+    else
+      hipLaunchKernelGGL((chain_pre_kernel<false, P>), dim3(grid), dim3(kBlock), 0, s, a, w);
+  } else if (kind == kPost) {
+    if (pre)
+      hipLaunchKernelGGL((chain_post_kernel<true, P>), dim3(grid), dim3(kBlock), 0, s, a, w);
+    else
+      hipLaunchKernelGGL((chain_post_kernel<false, P>), dim3(grid), dim3(kBlock), 0, s, a, w);
+  } else {
+    launch_fused<P>(pre, grid, s, a);
+  }
+}
 
 struct mam4x_handle_s {
   mam4x_config cfg;
@@ -638,18 +667,9 @@ int mam4x_validate(mam4x_handle h, const mam4x_inputs *in, long long *violations
   return MAM4X_OK;
 }
 
-int mam4x_compute_tendencies(mam4x_handle h, double t, double dt, const mam4x_inputs *in,
-                             const mam4x_outputs *out, void *stream) {
-  (void)t;
-  if (!h) return MAM4X_ERR_ARG;
-  int rc = check_inputs(h, in);
-  if (rc) return rc;
+// outputs of a step into the argument block
+static int fill_outputs(mam4x_handle h, const mam4x_outputs *out, ChainArgs &a) {
   if (!out) return fail(h, MAM4X_ERR_ARG, "outputs is NULL");
-  if (!(dt > 0.0)) return fail(h, MAM4X_ERR_ARG, "dt must be > 0");
-  if (h->cfg.precision != MAM4X_PREC_FP64)
-    return fail(h, MAM4X_ERR_UNSUPPORTED, "mixed precision not built yet");
-  ChainArgs a;
-  fill_args(h, in, a);
   for (int f = 0; f < NSTATE; ++f) {
     if (!out->tendencies[f])
       return fail(h, MAM4X_ERR_ARG, "tendency plane " + std::to_string(f) + " is NULL");
@@ -667,11 +687,63 @@ int mam4x_compute_tendencies(mam4x_handle h, double t, double dt, const mam4x_in
     a.dgc[m] = out->dgncur_c[m];
     a.has_dgn |= (a.dgi[m] != nullptr) || (a.dgc[m] != nullptr);
   }
+  return MAM4X_OK;
+}
+
+static bool has_pre(const mam4x_inputs *in) {
+  bool pre = in->pregaschem_h2so4 || in->pregaschem_soag;
+  for (int f = 0; f < NSTATE; ++f) pre = pre || in->precldchem[f];
+  return pre;
+}
+
+int mam4x_compute_tangent(mam4x_handle h, double t, double dt, const mam4x_inputs *in,
+                          const mam4x_tangent *dir, const mam4x_outputs *out, void *stream) {
+  (void)t;
+  if (!h) return MAM4X_ERR_ARG;
+  int rc = check_inputs(h, in);
+  if (rc) return rc;
+  if (!dir) return fail(h, MAM4X_ERR_ARG, "tangent is NULL");
+  if (!(dt > 0.0)) return fail(h, MAM4X_ERR_ARG, "dt must be > 0");
+  ChainArgs a;
+  fill_args(h, in, a);
+  rc = fill_outputs(h, out, a);
+  if (rc) return rc;
+  for (int f = 0; f < NSTATE; ++f) {
+    if (!dir->tendencies[f])
+      return fail(h, MAM4X_ERR_ARG, "tangent tendency plane " + std::to_string(f) + " is NULL");
+    a.dstate[f] = dir->state[f];
+    a.dtend[f] = dir->tendencies[f];
+  }
+  a.dT = dir->temperature;
+  a.dp = dir->pressure;
+  a.dqv = dir->vapor_mixing_ratio;
   a.dt = dt;
   hipStream_t s = (hipStream_t)stream;
   if (h->stats) a.stats = h->d_stats;
-  bool pre = in->pregaschem_h2so4 || in->pregaschem_soag;
-  for (int f = 0; f < NSTATE; ++f) pre = pre || in->precldchem[f];
+  const long ncell = (long)in->ncol * in->nlev;
+  if (ncell == 0) return MAM4X_OK;
+  const unsigned grid = (unsigned)((ncell + kBlock - 1) / kBlock);
+  launch_fused<Tangent>(has_pre(in), grid, s, a);
+  HIP_TRY(h, hipGetLastError());
+  return MAM4X_OK;
+}
+
+int mam4x_compute_tendencies(mam4x_handle h, double t, double dt, const mam4x_inputs *in,
+                             const mam4x_outputs *out, void *stream) {
+  (void)t;
+  if (!h) return MAM4X_ERR_ARG;
+  int rc = check_inputs(h, in);
+  if (rc) return rc;
+  if (!(dt > 0.0)) return fail(h, MAM4X_ERR_ARG, "dt must be > 0");
+  const bool mixed = h->cfg.precision == MAM4X_PREC_MIXED;
+  ChainArgs a;
+  fill_args(h, in, a);
+  rc = fill_outputs(h, out, a);
+  if (rc) return rc;
+  a.dt = dt;
+  hipStream_t s = (hipStream_t)stream;
+  if (h->stats) a.stats = h->d_stats;
+  const bool pre = has_pre(in);
   const long ncell = (long)in->ncol * in->nlev;
   if (ncell == 0) return MAM4X_OK;
   const unsigned grid = (unsigned)((ncell + kBlock - 1) / kBlock);
@@ -704,27 +776,21 @@ int mam4x_compute_tendencies(mam4x_handle h, double t, double dt, const mam4x_in
   }
   if (queue) {
     hipLaunchKernelGGL(queue_reset_kernel, dim3(1), dim3(64), 0, s, h->ctl);
-    if (pre)
-      hipLaunchKernelGGL(chain_pre_kernel<true>, dim3(grid), dim3(kBlock), 0, s, a, w);
+    if (mixed)
+      launch_cells<Mixed>(kPre, pre, grid, s, a, w);
     else
-      hipLaunchKernelGGL(chain_pre_kernel<false>, dim3(grid), dim3(kBlock), 0, s, a, w);
+      launch_cells<Fp64>(kPre, pre, grid, s, a, w);
     hipLaunchKernelGGL(queue_scatter_kernel, dim3(grid), dim3(kBlock), 0, s, w);
   }
   if (ev) HIP_TRY(h, hipEventRecord(ev[1], s));
   if (queue)
     hipLaunchKernelGGL(soa_queue_kernel, dim3(h->soa_grid), dim3(kBlock), 0, s, w, a.stats);
   if (ev) HIP_TRY(h, hipEventRecord(ev[2], s));
-  if (fused) {
-    if (pre)
-      hipLaunchKernelGGL(chain_fused_kernel<true>, dim3(grid), dim3(kBlock), 0, s, a);
-    else
-      hipLaunchKernelGGL(chain_fused_kernel<false>, dim3(grid), dim3(kBlock), 0, s, a);
-  } else {
-    if (pre)
-      hipLaunchKernelGGL(chain_post_kernel<true>, dim3(grid), dim3(kBlock), 0, s, a, w);
-    else
-      hipLaunchKernelGGL(chain_post_kernel<false>, dim3(grid), dim3(kBlock), 0, s, a, w);
-  }
+  const int kind = fused ? kFused : kPost;
+  if (mixed)
+    launch_cells<Mixed>(kind, pre, grid, s, a, w);
+  else
+    launch_cells<Fp64>(kind, pre, grid, s, a, w);
   HIP_TRY(h, hipGetLastError());
   if (ev) HIP_TRY(h, hipEventRecord(ev[3], s));
   return MAM4X_OK;

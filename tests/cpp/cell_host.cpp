@@ -1,7 +1,13 @@
 // CPU build of the HIP kernel's per-cell code (mam4xx_amd/csrc/chain_cell.hpp)
-// for the test suite.  It separates logic errors from libm differences: run
-// on the CPU with glibc, the kernel's restructured chain must agree with the
-// oracle to round-off.  Test infrastructure only; the library never uses it.
+// for the test suite, in the FP64 (prec 0), mixed (prec 1) or tangent
+// (prec 2) policy.  It separates logic errors from libm differences: run on
+// the CPU with glibc, the kernel's restructured FP64 chain must agree with
+// the oracle to round-off.  Test infrastructure only; the library never uses
+// it.
+//
+// Tangent mode: dstate ([52][nlev][ncol]), dT, dp, dqv are the input
+// direction (NULL = 0); dtend receives the directional derivative of the
+// tendencies.
 #include <cmath>
 #include <cstring>
 
@@ -9,11 +15,13 @@
 
 using namespace mam4x;
 
-extern "C" long cell_host_run(const int *cfgv, int ncol, int nlev, double dt,
-                              const double *state, const double *T, const double *p,
-                              const double *z, const double *qv, const double *cld,
-                              const double *pblh, const double *dga, const double *dgw,
-                              const double *wd, double *tend, double *dgi, double *dgc) {
+extern "C" long cell_host_run2(const int *cfgv, int prec, int ncol, int nlev, double dt,
+                               const double *state, const double *T, const double *p,
+                               const double *z, const double *qv, const double *cld,
+                               const double *pblh, const double *dga, const double *dgw,
+                               const double *wd, double *tend, double *dgi, double *dgc,
+                               const double *dstate, const double *dT, const double *dp,
+                               const double *dqv, double *dtend) {
   static ChainConsts K;
   chain_consts_init(&K);
   const long n = (long)ncol * nlev;
@@ -22,8 +30,11 @@ extern "C" long cell_host_run(const int *cfgv, int ncol, int nlev, double dt,
   for (int f = 0; f < NSTATE; ++f) {
     a.state[f] = state + f * n;
     a.tend[f] = tend + f * n;
+    if (dstate) a.dstate[f] = dstate + f * n;
+    if (dtend) a.dtend[f] = dtend + f * n;
   }
   a.T = T; a.p = p; a.z = z; a.qv = qv; a.cld = cld; a.pblh = pblh;
+  a.dT = dT; a.dp = dp; a.dqv = dqv;
   for (int m = 0; m < 4; ++m) {
     a.dga[m] = dga + m * n;
     a.dgw[m] = dgw + m * n;
@@ -40,6 +51,20 @@ extern "C" long cell_host_run(const int *cfgv, int ncol, int nlev, double dt,
   a.has_dgn = 1; a.has_sout = 0; a.dt = dt;
   long iters = 0;
   for (int k = 0; k < nlev; ++k)
-    for (int i = 0; i < ncol; ++i) iters += dev::chain_cell_fused<false>(a, (long)k * ncol + i, i);
+    for (int i = 0; i < ncol; ++i) {
+      const long off = (long)k * ncol + i;
+      iters += prec == 2   ? dev::chain_cell_fused<false, Tangent>(a, off, i)
+               : prec == 1 ? dev::chain_cell_fused<false, Mixed>(a, off, i)
+                           : dev::chain_cell_fused<false, Fp64>(a, off, i);
+    }
   return iters;
+}
+
+extern "C" long cell_host_run(const int *cfgv, int prec, int ncol, int nlev, double dt,
+                              const double *state, const double *T, const double *p,
+                              const double *z, const double *qv, const double *cld,
+                              const double *pblh, const double *dga, const double *dgw,
+                              const double *wd, double *tend, double *dgi, double *dgc) {
+  return cell_host_run2(cfgv, prec, ncol, nlev, dt, state, T, p, z, qv, cld, pblh, dga, dgw, wd,
+                        tend, dgi, dgc, nullptr, nullptr, nullptr, nullptr, nullptr);
 }

@@ -33,7 +33,7 @@ def cellhost():
     return lib
 
 
-def run_host(lib, sc, dt, cfg):
+def run_host(lib, sc, dt, cfg, prec=0):
     dp = ctypes.POINTER(ctypes.c_double)
     p = lambda a: a.ctypes.data_as(dp)
     st = np.ascontiguousarray(sc["state"])
@@ -45,7 +45,7 @@ def run_host(lib, sc, dt, cfg):
                                                   "cloud_fraction", "pblh", "dgncur_a",
                                                   "dgncur_awet", "wetdens")]
     c = (ctypes.c_int * 9)(*cfg)
-    it = lib.cell_host_run(c, ncol, nlev, ctypes.c_double(dt), p(st), *[p(a) for a in arr],
+    it = lib.cell_host_run(c, prec, ncol, nlev, ctypes.c_double(dt), p(st), *[p(a) for a in arr],
                            p(tend), p(dgi), p(dgc))
     return tend, dgi, dgc, it
 
