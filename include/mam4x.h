@@ -189,6 +189,22 @@ int mam4x_compute_tendencies(mam4x_handle h, double t, double dt,
 int mam4x_compute_tangent(mam4x_handle h, double t, double dt, const mam4x_inputs *in,
                           const mam4x_tangent *dir, const mam4x_outputs *out, void *stream);
 
+/* Batches: many scenes (domains of any column count, each with its own
+ * inputs, outputs and layout) stepped by one set of launches.  The scenes'
+ * cells form one work queue, so uneven scenes and uneven per-cell cost share
+ * the GPU.  A batch is planned once for fixed pointers and dt; runs only
+ * launch kernels -- no allocation, copy, memset or synchronisation -- so a
+ * run can be captured in a hipGraph and replayed (with stats disabled).
+ *   mam4x_batch_create: validates and uploads the argument blocks and sizes
+ *     the handle's workspace (allocates and synchronizes).
+ *   mam4x_batch_run: one step of every scene; dt must equal the planned dt.
+ * No counterpart in the reference, which steps one column per Kokkos team. */
+typedef struct mam4x_batch_s *mam4x_batch;
+int mam4x_batch_create(mam4x_handle h, int nscene, const mam4x_inputs *in,
+                       const mam4x_outputs *out, double dt, mam4x_batch *batch);
+int mam4x_batch_run(mam4x_batch batch, double t, double dt, void *stream);
+int mam4x_batch_destroy(mam4x_batch batch);
+
 /* Cost diagnostic: with stats enabled, every compute_tendencies call adds
  * the SOA sub-steps (gasaerexch_soaexch.hpp:170-335) its cells took to a
  * device counter.  mam4x_last_stats returns the sum since stats were enabled

@@ -109,6 +109,11 @@ SIGNATURES = {
     "mam4x_compute_tangent": (ctypes.c_int, [_vp, ctypes.c_double, ctypes.c_double,
                                              ctypes.POINTER(Inputs), ctypes.POINTER(Tangent),
                                              ctypes.POINTER(Outputs), _vp]),
+    "mam4x_batch_create": (ctypes.c_int, [_vp, ctypes.c_int, ctypes.POINTER(Inputs),
+                                          ctypes.POINTER(Outputs), ctypes.c_double,
+                                          ctypes.POINTER(_vp)]),
+    "mam4x_batch_run": (ctypes.c_int, [_vp, ctypes.c_double, ctypes.c_double, _vp]),
+    "mam4x_batch_destroy": (ctypes.c_int, [_vp]),
     "mam4x_enable_stats": (ctypes.c_int, [_vp, ctypes.c_int]),
     "mam4x_last_stats": (ctypes.c_int, [_vp, ctypes.POINTER(ctypes.c_longlong), _vp]),
     "mam4x_last_phase_times": (ctypes.c_int, [_vp, ctypes.POINTER(ctypes.c_double),

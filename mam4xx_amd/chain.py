@@ -124,6 +124,27 @@ class Chain:
                                                ctypes.byref(out), ctypes.c_void_p(s.cuda_stream))
         self._check(rc, "mam4x_compute_tendencies")
 
+    def make_outputs(self, tend, dgn_i=None, dgn_c=None, state_out=None):
+        out = _abi.Outputs()
+        for i, p in enumerate(_plane_ptrs(tend, L.NSTATE)):
+            out.tendencies[i] = p
+        if dgn_i is not None:
+            for m, p in enumerate(_plane_ptrs(dgn_i, 4)):
+                out.dgncur_i[m] = p
+        if dgn_c is not None:
+            for m, p in enumerate(_plane_ptrs(dgn_c, 4)):
+                out.dgncur_c[m] = p
+        if state_out is not None:
+            for i, p in enumerate(_plane_ptrs(state_out, L.NSTATE)):
+                out.state_out[i] = p
+        return out
+
+    def batch(self, scenes, dt, tends, dgn_i=None, dgn_c=None, state_out=None,
+              layout=_abi.LAYOUT_LEV_COL):
+        """Plan one step of many scenes (lists of scene dicts and output
+        tensors, None entries allowed in the optional lists); see Batch."""
+        return Batch(self, scenes, dt, tends, dgn_i, dgn_c, state_out, layout)
+
     def compute_tangent(self, scene, dt, tend, dtend, dstate=None, dtemperature=None,
                         dpressure=None, dqv=None, dgn_i=None, dgn_c=None, t=0.0, stream=None,
                         layout=_abi.LAYOUT_LEV_COL):
@@ -185,6 +206,45 @@ class Chain:
                                                     ctypes.c_void_p(s.cuda_stream)),
                     "mam4x_last_phase_times")
         return dict(zip(_abi.PHASE_NAMES, list(ms))), int(n.value)
+
+
+class Batch:
+    """A planned multi-scene step (mam4x_batch_*).  The tensors' storage is
+    bound at creation; run() launches kernels only and can be captured in a
+    CUDA graph (torch.cuda.graph) and replayed."""
+
+    def __init__(self, chain, scenes, dt, tends, dgn_i, dgn_c, state_out, layout):
+        n = len(scenes)
+        pick = lambda lst, i: None if lst is None else lst[i]
+        ins = (_abi.Inputs * n)()
+        outs = (_abi.Outputs * n)()
+        for i, sc in enumerate(scenes):
+            ins[i] = chain.make_inputs(sc, None, None, layout)
+            outs[i] = chain.make_outputs(tends[i], pick(dgn_i, i), pick(dgn_c, i),
+                                         pick(state_out, i))
+        self._keep = (scenes, tends, dgn_i, dgn_c, state_out)
+        self.chain, self.dt, self.nscene = chain, dt, n
+        b = ctypes.c_void_p()
+        chain._check(chain.lib.mam4x_batch_create(chain.h, n, ins, outs, dt, ctypes.byref(b)),
+                     "mam4x_batch_create")
+        self.b = b
+
+    def run(self, t=0.0, stream=None):
+        s = stream if stream is not None else torch.cuda.current_stream()
+        self.chain._check(self.chain.lib.mam4x_batch_run(self.b, t, self.dt,
+                                                          ctypes.c_void_p(s.cuda_stream)),
+                          "mam4x_batch_run")
+
+    def close(self):
+        if getattr(self, "b", None):
+            self.chain.lib.mam4x_batch_destroy(self.b)
+            self.b = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
 
 
 def scene_to_device(scene, device="cuda"):

@@ -33,7 +33,6 @@ struct ChainArgs {
   const double *dT, *dp, *dqv;
   double *dtend[NSTATE];
   const ChainConsts *K;
-  unsigned long long *stats;  // SOA sub-step counter (NULL: off)
   int ncol, nlev, ldc;
   int layout;  // 0: [nlev][ldc] column fastest; 1: [ncol][ldc] level fastest
   int do_calcsize, do_xfer, do_adjust, has_dgn, has_sout;

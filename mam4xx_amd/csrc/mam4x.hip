@@ -102,8 +102,37 @@ struct Work {
   unsigned char *bucket;   // per cell, kNoSoa when the cell has no SOA loop
   int *queue;              // cell indices, longest first
   QueueCtl *ctl;
-  long ncell;
+  long ncell;              // cells of the launch (batches: padded to whole blocks)
   double dtfull;           // SOA loop length: the integer sub-step (:1319)
+  unsigned long long *stats;  // SOA sub-step counter (NULL: off)
+};
+
+// Where a block's cells come from.  A launch covers either one domain, or a
+// batch of scenes each padded to whole blocks, so a block never straddles
+// two scenes.  scene() maps the launch-wide cell index (which also indexes
+// the workspace) to the scene's argument block and its scene-local index.
+struct OneDomain {
+  ChainArgs a;
+  __device__ __forceinline__ const ChainArgs &scene(long &c) const { return a; }
+};
+
+struct SceneBatch {
+  const ChainArgs *__restrict__ args;  // one per scene
+  const unsigned *__restrict__ blk0;   // first block of each scene, [nscene + 1]
+  int nscene;
+  __device__ __forceinline__ const ChainArgs &scene(long &c) const {
+    const unsigned b = blockIdx.x;
+    int lo = 0, hi = nscene;  // blk0[lo] <= b < blk0[hi]
+    while (hi - lo > 1) {
+      const int mid = (lo + hi) >> 1;
+      if (blk0[mid] <= b)
+        lo = mid;
+      else
+        hi = mid;
+    }
+    c = (long)(b - blk0[lo]) * kBlock + threadIdx.x;
+    return args[lo];
+  }
 };
 
 // cell c of the memory-order enumeration -> column and plane offset
@@ -125,10 +154,11 @@ __device__ __forceinline__ void add_stats(unsigned long long *stats, unsigned lo
   if ((threadIdx.x & 63) == 0 && v) atomicAdd(stats, v);
 }
 
-template <bool PRE, class P>
-__global__ void __launch_bounds__(kBlock) chain_fused_kernel(ChainArgs a) {
+template <bool PRE, class P, class Src>
+__global__ void __launch_bounds__(kBlock) chain_fused_kernel(Src src, Work w) {
+  long c = (long)blockIdx.x * kBlock + threadIdx.x;
+  const ChainArgs &a = src.scene(c);
   const long ncell = (long)a.ncol * a.nlev;
-  const long c = (long)blockIdx.x * kBlock + threadIdx.x;
   unsigned long long niter = 0;
   if (c < ncell) {
     int icol;
@@ -136,7 +166,7 @@ __global__ void __launch_bounds__(kBlock) chain_fused_kernel(ChainArgs a) {
     cell_index(a, c, icol, off);
     niter = (unsigned long long)dev::chain_cell_fused<PRE, P>(a, off, icol);
   }
-  add_stats(a.stats, niter);
+  add_stats(w.stats, niter);
 }
 
 __global__ void queue_reset_kernel(QueueCtl *q) {
@@ -149,13 +179,16 @@ __global__ void queue_reset_kernel(QueueCtl *q) {
 }
 
 // Phase 1.
-template <bool PRE, class P>
-__global__ void __launch_bounds__(kBlock) chain_pre_kernel(ChainArgs a, Work w) {
+template <bool PRE, class P, class Src>
+__global__ void __launch_bounds__(kBlock) chain_pre_kernel(Src src, Work w) {
   __shared__ unsigned lh[kNBucket];
   if (threadIdx.x < kNBucket) lh[threadIdx.x] = 0;
   __syncthreads();
-  const long c = (long)blockIdx.x * kBlock + threadIdx.x;
-  if (c < w.ncell) {
+  const long g = (long)blockIdx.x * kBlock + threadIdx.x;  // workspace index
+  long c = g;
+  const ChainArgs &a = src.scene(c);
+  if (g < w.ncell && c >= (long)a.ncol * a.nlev) w.bucket[g] = kNoSoa;  // batch padding
+  if (g < w.ncell && c < (long)a.ncol * a.nlev) {
     int icol;
     long off;
     cell_index(a, c, icol, off);
@@ -168,7 +201,7 @@ __global__ void __launch_bounds__(kBlock) chain_pre_kernel(ChainArgs a, Work w) 
       dev::clear_build<P>(a, off, X, S, G);
       dev::SoaIn<double> in;
       dev::clear_pre<P>(*a.K, X.T, X.p, X.dgw, S, G.g[0], in);
-      rec_store_in(w.ws + c * kRec, in);
+      rec_store_in(w.ws + g * kRec, in);
       dev::SoaState<double> st;
       dev::soa_init(st, in);
       const float pred = (float)dev::soa_predict(st, w.dtfull);
@@ -177,7 +210,7 @@ __global__ void __launch_bounds__(kBlock) chain_pre_kernel(ChainArgs a, Work w) 
       b = (unsigned char)ib;
       atomicAdd(&lh[ib], 1u);
     }
-    w.bucket[c] = b;
+    w.bucket[g] = b;
   }
   __syncthreads();
   if (threadIdx.x < kNBucket && lh[threadIdx.x]) atomicAdd(&w.ctl->hist[threadIdx.x], lh[threadIdx.x]);
@@ -294,10 +327,12 @@ __global__ void __launch_bounds__(kBlock) soa_queue_kernel(Work w, unsigned long
 }
 
 // Phase 4.
-template <bool PRE, class P>
-__global__ void __launch_bounds__(kBlock) chain_post_kernel(ChainArgs a, Work w) {
-  const long c = (long)blockIdx.x * kBlock + threadIdx.x;
-  if (c >= w.ncell) return;
+template <bool PRE, class P, class Src>
+__global__ void __launch_bounds__(kBlock) chain_post_kernel(Src src, Work w) {
+  const long g = (long)blockIdx.x * kBlock + threadIdx.x;  // workspace index
+  long c = g;
+  const ChainArgs &a = src.scene(c);
+  if (g >= w.ncell || c >= (long)a.ncol * a.nlev) return;
   int icol;
   long off;
   cell_index(a, c, icol, off);
@@ -309,7 +344,7 @@ __global__ void __launch_bounds__(kBlock) chain_post_kernel(ChainArgs a, Work w)
     dev::clear_build<P>(a, off, X, S, G);
     double uptk[4] = {0, 0, 0, 0}, sa[4] = {0, 0, 0, 0}, sg = 0.0, sgavg = 0.0;
     if (a.cfg.do_cond) {
-      const double2 *v = reinterpret_cast<const double2 *>(w.ws + c * kRec);
+      const double2 *v = reinterpret_cast<const double2 *>(w.ws + g * kRec);
       const double2 v0 = v[0], v1 = v[1], v2 = v[2], v4 = v[4], v5 = v[5];
       sg = v0.x;
       sa[0] = v0.y;
@@ -351,30 +386,30 @@ __global__ void __launch_bounds__(kBlock) validate_kernel(ChainArgs a,
 
 enum { kPre = 0, kPost = 1, kFused = 2 };
 
-template <class P>
-static void launch_fused(bool pre, unsigned grid, hipStream_t s, const ChainArgs &a) {
+template <class P, class Src>
+static void launch_fused(bool pre, unsigned grid, hipStream_t s, const Src &src, const Work &w) {
   if (pre)
-    hipLaunchKernelGGL((chain_fused_kernel<true, P>), dim3(grid), dim3(kBlock), 0, s, a);
+    hipLaunchKernelGGL((chain_fused_kernel<true, P, Src>), dim3(grid), dim3(kBlock), 0, s, src, w);
   else
-    hipLaunchKernelGGL((chain_fused_kernel<false, P>), dim3(grid), dim3(kBlock), 0, s, a);
+    hipLaunchKernelGGL((chain_fused_kernel<false, P, Src>), dim3(grid), dim3(kBlock), 0, s, src, w);
 }
 
 // launches one of the per-cell kernels for policy P (FP64 wide type)
-template <class P>
-static void launch_cells(int kind, bool pre, unsigned grid, hipStream_t s, const ChainArgs &a,
+template <class P, class Src>
+static void launch_cells(int kind, bool pre, unsigned grid, hipStream_t s, const Src &src,
                          const Work &w) {
   if (kind == kPre) {
     if (pre)
-      hipLaunchKernelGGL((chain_pre_kernel<true, P>), dim3(grid), dim3(kBlock), 0, s, a, w);
+      hipLaunchKernelGGL((chain_pre_kernel<true, P, Src>), dim3(grid), dim3(kBlock), 0, s, src, w);
     else
-      hipLaunchKernelGGL((chain_pre_kernel<false, P>), dim3(grid), dim3(kBlock), 0, s, a, w);
+      hipLaunchKernelGGL((chain_pre_kernel<false, P, Src>), dim3(grid), dim3(kBlock), 0, s, src, w);
   } else if (kind == kPost) {
     if (pre)
-      hipLaunchKernelGGL((chain_post_kernel<true, P>), dim3(grid), dim3(kBlock), 0, s, a, w);
+      hipLaunchKernelGGL((chain_post_kernel<true, P, Src>), dim3(grid), dim3(kBlock), 0, s, src, w);
     else
-      hipLaunchKernelGGL((chain_post_kernel<false, P>), dim3(grid), dim3(kBlock), 0, s, a, w);
+      hipLaunchKernelGGL((chain_post_kernel<false, P, Src>), dim3(grid), dim3(kBlock), 0, s, src, w);
   } else {
-    launch_fused<P>(pre, grid, s, a);
+    launch_fused<P>(pre, grid, s, src, w);
   }
 }
 
@@ -719,11 +754,14 @@ int mam4x_compute_tangent(mam4x_handle h, double t, double dt, const mam4x_input
   a.dqv = dir->vapor_mixing_ratio;
   a.dt = dt;
   hipStream_t s = (hipStream_t)stream;
-  if (h->stats) a.stats = h->d_stats;
   const long ncell = (long)in->ncol * in->nlev;
   if (ncell == 0) return MAM4X_OK;
   const unsigned grid = (unsigned)((ncell + kBlock - 1) / kBlock);
-  launch_fused<Tangent>(has_pre(in), grid, s, a);
+  Work w;
+  std::memset(&w, 0, sizeof(w));
+  w.ncell = ncell;
+  if (h->stats) w.stats = h->d_stats;
+  launch_fused<Tangent>(has_pre(in), grid, s, OneDomain{a}, w);
   HIP_TRY(h, hipGetLastError());
   return MAM4X_OK;
 }
@@ -742,7 +780,6 @@ int mam4x_compute_tendencies(mam4x_handle h, double t, double dt, const mam4x_in
   if (rc) return rc;
   a.dt = dt;
   hipStream_t s = (hipStream_t)stream;
-  if (h->stats) a.stats = h->d_stats;
   const bool pre = has_pre(in);
   const long ncell = (long)in->ncol * in->nlev;
   if (ncell == 0) return MAM4X_OK;
@@ -754,6 +791,8 @@ int mam4x_compute_tendencies(mam4x_handle h, double t, double dt, const mam4x_in
   w.ncell = ncell;
   w.ctl = h->ctl;
   w.dtfull = (double)(int)(dt / 1);  // QUIRK (mam4_amicphys.hpp:1319): int sub-step
+  if (h->stats) w.stats = h->d_stats;
+  const OneDomain src{a};
   const bool queue = !fused && a.cfg.do_cond;
   if (queue) {
     rc = reserve_work(h, ncell, s);
@@ -777,22 +816,142 @@ int mam4x_compute_tendencies(mam4x_handle h, double t, double dt, const mam4x_in
   if (queue) {
     hipLaunchKernelGGL(queue_reset_kernel, dim3(1), dim3(64), 0, s, h->ctl);
     if (mixed)
-      launch_cells<Mixed>(kPre, pre, grid, s, a, w);
+      launch_cells<Mixed>(kPre, pre, grid, s, src, w);
     else
-      launch_cells<Fp64>(kPre, pre, grid, s, a, w);
+      launch_cells<Fp64>(kPre, pre, grid, s, src, w);
     hipLaunchKernelGGL(queue_scatter_kernel, dim3(grid), dim3(kBlock), 0, s, w);
   }
   if (ev) HIP_TRY(h, hipEventRecord(ev[1], s));
   if (queue)
-    hipLaunchKernelGGL(soa_queue_kernel, dim3(h->soa_grid), dim3(kBlock), 0, s, w, a.stats);
+    hipLaunchKernelGGL(soa_queue_kernel, dim3(h->soa_grid), dim3(kBlock), 0, s, w, w.stats);
   if (ev) HIP_TRY(h, hipEventRecord(ev[2], s));
   const int kind = fused ? kFused : kPost;
   if (mixed)
-    launch_cells<Mixed>(kind, pre, grid, s, a, w);
+    launch_cells<Mixed>(kind, pre, grid, s, src, w);
   else
-    launch_cells<Fp64>(kind, pre, grid, s, a, w);
+    launch_cells<Fp64>(kind, pre, grid, s, src, w);
   HIP_TRY(h, hipGetLastError());
   if (ev) HIP_TRY(h, hipEventRecord(ev[3], s));
+  return MAM4X_OK;
+}
+
+struct mam4x_batch_s {
+  mam4x_handle h = nullptr;
+  int nscene = 0;
+  double dt = 0.0;
+  bool pre = false, do_cond = false;
+  ChainArgs *d_args = nullptr;
+  unsigned *d_blk0 = nullptr;
+  unsigned nblock = 0;
+  long ncell = 0;  // padded to whole blocks per scene
+};
+
+int mam4x_batch_destroy(mam4x_batch b) {
+  if (!b) return MAM4X_ERR_ARG;
+  if (b->d_args) (void)hipFree(b->d_args);
+  if (b->d_blk0) (void)hipFree(b->d_blk0);
+  delete b;
+  return MAM4X_OK;
+}
+
+int mam4x_batch_create(mam4x_handle h, int nscene, const mam4x_inputs *in,
+                       const mam4x_outputs *out, double dt, mam4x_batch *batch) {
+  if (!h || !batch) return MAM4X_ERR_ARG;
+  *batch = nullptr;
+  if (nscene <= 0 || !in || !out) return fail(h, MAM4X_ERR_ARG, "empty batch");
+  if (!(dt > 0.0)) return fail(h, MAM4X_ERR_ARG, "dt must be > 0");
+  std::vector<ChainArgs> args(nscene);
+  std::vector<unsigned> blk0(nscene + 1, 0);
+  bool pre = false;
+  for (int i = 0; i < nscene; ++i) {
+    int rc = check_inputs(h, &in[i]);
+    if (rc) return rc;
+    fill_args(h, &in[i], args[i]);
+    rc = fill_outputs(h, &out[i], args[i]);
+    if (rc) return rc;
+    args[i].dt = dt;
+    pre = pre || has_pre(&in[i]);
+    const long nc = (long)in[i].ncol * in[i].nlev;
+    const long nb = (nc + kBlock - 1) / kBlock;
+    if ((long)blk0[i] + nb > 0x7fffffffL / kBlock)
+      return fail(h, MAM4X_ERR_ARG, "batch too large (cells must fit in int32)");
+    blk0[i + 1] = blk0[i] + (unsigned)nb;
+  }
+  mam4x_batch b = new (std::nothrow) mam4x_batch_s();
+  if (!b) return MAM4X_ERR_NOMEM;
+  b->h = h;
+  b->nscene = nscene;
+  b->dt = dt;
+  b->pre = pre;
+  b->do_cond = args[0].cfg.do_cond != 0;
+  b->nblock = blk0[nscene];
+  b->ncell = (long)b->nblock * kBlock;
+  if (hipMalloc(&b->d_args, sizeof(ChainArgs) * nscene) != hipSuccess ||
+      hipMalloc(&b->d_blk0, sizeof(unsigned) * (nscene + 1)) != hipSuccess) {
+    (void)hipGetLastError();
+    mam4x_batch_destroy(b);
+    return fail(h, MAM4X_ERR_NOMEM, "batch argument blocks");
+  }
+  if (hipMemcpy(b->d_args, args.data(), sizeof(ChainArgs) * nscene, hipMemcpyHostToDevice) !=
+          hipSuccess ||
+      hipMemcpy(b->d_blk0, blk0.data(), sizeof(unsigned) * (nscene + 1),
+                hipMemcpyHostToDevice) != hipSuccess) {
+    mam4x_batch_destroy(b);
+    return fail(h, MAM4X_ERR_HIP, "batch upload failed");
+  }
+  if (b->do_cond && b->ncell > 0) {
+    int rc = reserve_work(h, b->ncell, nullptr);
+    if (rc) {
+      mam4x_batch_destroy(b);
+      return rc;
+    }
+    if (hipDeviceSynchronize() != hipSuccess) {
+      mam4x_batch_destroy(b);
+      return fail(h, MAM4X_ERR_HIP, "batch set-up");
+    }
+  }
+  *batch = b;
+  return MAM4X_OK;
+}
+
+int mam4x_batch_run(mam4x_batch b, double t, double dt, void *stream) {
+  (void)t;
+  if (!b) return MAM4X_ERR_ARG;
+  mam4x_handle h = b->h;
+  if (dt != b->dt) return fail(h, MAM4X_ERR_ARG, "batch planned for another dt");
+  if (b->ncell == 0) return MAM4X_OK;
+  if (b->do_cond && h->cap < b->ncell)
+    return fail(h, MAM4X_ERR_ARG, "workspace shrank below the batch (handle reused?)");
+  hipStream_t s = (hipStream_t)stream;
+  const bool mixed = h->cfg.precision == MAM4X_PREC_MIXED;
+  const bool fused = h->cfg.schedule == MAM4X_SCHED_FUSED ||
+                     (h->cfg.schedule == MAM4X_SCHED_AUTO && b->ncell < kFusedBelow);
+  const SceneBatch src{b->d_args, b->d_blk0, b->nscene};
+  Work w;
+  std::memset(&w, 0, sizeof(w));
+  w.ncell = b->ncell;
+  w.ctl = h->ctl;
+  w.dtfull = (double)(int)(dt / 1);  // QUIRK (mam4_amicphys.hpp:1319): int sub-step
+  if (h->stats) w.stats = h->d_stats;
+  const bool queue = !fused && b->do_cond;
+  if (queue) {
+    w.ws = h->ws;
+    w.bucket = h->bucket;
+    w.queue = h->queue;
+    hipLaunchKernelGGL(queue_reset_kernel, dim3(1), dim3(64), 0, s, h->ctl);
+    if (mixed)
+      launch_cells<Mixed>(kPre, b->pre, b->nblock, s, src, w);
+    else
+      launch_cells<Fp64>(kPre, b->pre, b->nblock, s, src, w);
+    hipLaunchKernelGGL(queue_scatter_kernel, dim3(b->nblock), dim3(kBlock), 0, s, w);
+    hipLaunchKernelGGL(soa_queue_kernel, dim3(h->soa_grid), dim3(kBlock), 0, s, w, w.stats);
+  }
+  const int kind = fused ? kFused : kPost;
+  if (mixed)
+    launch_cells<Mixed>(kind, b->pre, b->nblock, s, src, w);
+  else
+    launch_cells<Fp64>(kind, b->pre, b->nblock, s, src, w);
+  HIP_TRY(h, hipGetLastError());
   return MAM4X_OK;
 }
 
