@@ -102,6 +102,8 @@ typedef struct mam4x_config {
   int calcsize_do_adjust;
   int precision; /* MAM4X_PREC_*                                          */
   int schedule;  /* MAM4X_SCHED_*                                         */
+  int soa_blocks_per_cu; /* persistent SOA kernel: resident blocks per CU,
+                            0 = as many as fit (performance only)        */
 } mam4x_config;
 
 /* Atmosphere (atmosphere.hpp:16-137) and modal diagnostics the chain reads. */

@@ -48,6 +48,7 @@ class Config(ctypes.Structure):
         ("calcsize_do_adjust", ctypes.c_int),
         ("precision", ctypes.c_int),
         ("schedule", ctypes.c_int),
+        ("soa_blocks_per_cu", ctypes.c_int),
     ]
 
 

@@ -564,6 +564,7 @@ int mam4x_default_config(mam4x_config *out) {
   out->calcsize_do_adjust = 1;
   out->precision = MAM4X_PREC_FP64;
   out->schedule = MAM4X_SCHED_AUTO;
+  out->soa_blocks_per_cu = 0;
   return MAM4X_OK;
 }
 
@@ -577,7 +578,8 @@ int mam4x_create(const mam4x_config *config, mam4x_handle *out) {
   else
     mam4x_default_config(&h->cfg);
   if ((h->cfg.precision != MAM4X_PREC_FP64 && h->cfg.precision != MAM4X_PREC_MIXED) ||
-      h->cfg.schedule < MAM4X_SCHED_AUTO || h->cfg.schedule > MAM4X_SCHED_QUEUE) {
+      h->cfg.schedule < MAM4X_SCHED_AUTO || h->cfg.schedule > MAM4X_SCHED_QUEUE ||
+      h->cfg.soa_blocks_per_cu < 0) {
     delete h;
     return MAM4X_ERR_ARG;
   }
@@ -590,7 +592,10 @@ int mam4x_create(const mam4x_config *config, mam4x_handle *out) {
     delete h;
     return MAM4X_ERR_HIP;
   }
-  h->soa_grid = (unsigned)(ncu * (per_cu > 0 ? per_cu : 1));
+  if (per_cu < 1) per_cu = 1;
+  if (h->cfg.soa_blocks_per_cu > 0 && h->cfg.soa_blocks_per_cu < per_cu)
+    per_cu = h->cfg.soa_blocks_per_cu;
+  h->soa_grid = (unsigned)(ncu * per_cu);
   ChainConsts k;
   chain_consts_init(&k);
   if (hipMalloc(&h->d_consts, sizeof(ChainConsts)) != hipSuccess ||

@@ -91,3 +91,17 @@ def test_batch_graph_replays_match_eager(torch_cuda):
             assert torch.equal(da["dgncur_a"], db["dgncur_a"]), step
     for b in batches:
         b.close()
+
+
+def test_autotuner_picks_a_candidate(torch_cuda, tmp_path):
+    from mam4xx_amd import tune
+    from mam4xx_amd.chain import scene_to_device
+    ds = [scene_to_device(s) for s in scenes.uneven_batch(nscenes=3, seed=5, total_cols=300)]
+    before = [d["state"].clone() for d in ds]
+    cache = str(tmp_path / "tune.json")
+    best, ms = tune.tune(ds, DT, steps=2, cache=cache)
+    assert best in tune.CANDIDATES and len(ms) == len(tune.CANDIDATES)
+    assert all(v > 0 for v in ms.values())
+    for d, s0 in zip(ds, before):  # tuning ran on copies
+        assert torch_cuda.equal(d["state"], s0)
+    assert tune.tune(ds, DT, cache=cache)[0] == best  # cached
