@@ -170,7 +170,7 @@ def main():
                 "traffic": _traffic("traffic_soa_queue_kernel_%dx%d.txt" % (NCOL, NLEV))}
     else:
         kern = {"name": "chain_fused_kernel", "ms": ph["finish"], "flops": step_flops,
-                "traffic": _traffic("traffic_chain_fp64_%dx%d.txt" % (NCOL, NLEV))}
+                "traffic": _traffic("traffic_chain_fused_kernel_%dx%d.txt" % (NCOL, NLEV))}
     achieved = kern["flops"] / (kern["ms"] * 1e-3) / 1e12
     out = {
         "metric": "microphysics column-steps/sec at 72 lev: mixed vs FP64 GPU at <=1e-5 rel err",

@@ -132,10 +132,15 @@ class LibraryMissing(RuntimeError):
 
 
 def load(path=None):
-    """Load libmam4x.so (never falls back to anything else)."""
+    """Load libmam4x.so (never falls back to anything else).
+
+    torch's GPU runtime is initialised first: a process that starts a HIP
+    runtime of its own before torch's can leave torch without a device."""
     global _lib
     if _lib is not None and path is None:
         return _lib
+    import torch
+    torch.cuda.is_available()
     p = path or LIB_PATH
     if not os.path.exists(p):
         raise LibraryMissing(

@@ -5,8 +5,9 @@
 differentiable in the prognostic state, temperature, pressure and water
 vapour.
 
-* Forward mode (``torch.func.jvp``, ``torch.autograd.forward_ad``): one call of
-  the tangent kernel (mam4x_compute_tangent) per direction.
+* Forward mode (``torch.autograd.forward_ad``): one call of the tangent
+  kernel (mam4x_compute_tangent) per direction.  The functorch transforms
+  (``torch.func``) are not supported: the kernels need plain device tensors.
 * Reverse mode (``backward``): cells are independent, so each cell's Jacobian
   is a 52 x 55 block.  The vector-Jacobian product is assembled from one
   tangent call per differentiable input field, each along the unit direction
@@ -54,23 +55,30 @@ def _tangent(d, dt, dstate, dT, dp, dqv):
 
 class _Tendencies(torch.autograd.Function):
     @staticmethod
-    def forward(ctx, state, temperature, pressure, qv, scene, dt):
+    def forward(state, temperature, pressure, qv, scene, dt):
         d = _scene(scene, state, temperature, pressure, qv)
         nlev, ncol = d["temperature"].shape
         tend = torch.empty((L.NSTATE, nlev, ncol), dtype=torch.float64, device="cuda")
         _chain().compute_tendencies(d, dt, tend)
-        ctx.scene, ctx.dt = d, dt
         return tend
+
+    @staticmethod
+    def setup_context(ctx, inputs, output):
+        state, temperature, pressure, qv, scene, dt = inputs
+        ctx.save_for_forward(state, temperature, pressure, qv)
+        ctx.save_for_backward(state, temperature, pressure, qv)
+        ctx.rest, ctx.dt = scene, dt
 
     @staticmethod
     def jvp(ctx, dstate, dT, dp, dqv, _dscene, _ddt):
         c = lambda x: None if x is None else x.contiguous()
-        _, dtend = _tangent(ctx.scene, ctx.dt, c(dstate), c(dT), c(dp), c(dqv))
+        d = _scene(ctx.rest, *ctx.saved_tensors)
+        _, dtend = _tangent(d, ctx.dt, c(dstate), c(dT), c(dp), c(dqv))
         return dtend
 
     @staticmethod
     def backward(ctx, g):
-        d, dt = ctx.scene, ctx.dt
+        d, dt = _scene(ctx.rest, *ctx.saved_tensors), ctx.dt
         g = g.contiguous()
         nlev, ncol = d["temperature"].shape
         one = torch.ones((nlev, ncol), dtype=torch.float64, device="cuda")

@@ -39,8 +39,9 @@ namespace {
 
 constexpr int kBlock = 256;
 constexpr int kNBucket = 24;  // half-octave cost buckets: 2 log2(1 + predicted sub-steps)
-// kChunk must stay 64: a chunk is staged one entry per lane
+#ifndef MAM4X_QUEUE_CHUNK
 #define MAM4X_QUEUE_CHUNK 64
+#endif
 constexpr int kChunk = MAM4X_QUEUE_CHUNK;  // queue entries a wave takes per refill
 // Waves whose first chunk comes from buckets >= kLongBucket (predicted >= ~180
 // sub-steps) raise their issue priority: a cell's sub-steps are a serial
@@ -52,42 +53,38 @@ constexpr int kChunk = MAM4X_QUEUE_CHUNK;  // queue entries a wave takes per ref
 constexpr int kLongBucket = 15;
 constexpr unsigned char kNoSoa = 0xff;
 
-// Workspace: one 128-byte record (one L2 line) per cell, so the queue
-// kernel's gather of a cell touches one line, not one line per field.
-//   [0] SOAG  [1..4] SOA per mode  [5] equilibrium SOAG g0  [6..7] unused
-//   [8..11] H2SO4 uptake rates     [12..15] oxygenated POM per mode
-// The queue kernel overwrites [0..5] with its results: SOAG, SOA per mode
-// and the time-averaged SOAG.
-constexpr int kRec = 16;
+// Workspace: 14 planes of one double per cell.  The set-up kernel writes
+// them in cell order (coalesced); the queue kernel's chunks hold cells of a
+// few set-up blocks, so its gathers stay on few lines.
+//   plane 0 SOAG, 1..4 SOA per mode, 5 equilibrium SOAG g0,
+//   6..9 H2SO4 uptake rates, 10..13 oxygenated POM per mode.
+// The queue kernel overwrites planes 0..5 with its results: SOAG, SOA per
+// mode and the time-averaged SOAG.
+enum { W_G = 0, W_A = 1, W_G0 = 5, W_UPTK = 6, W_AOPOA = 10, W_NPLANE = 14 };
+constexpr int W_GAVG = W_G0;
 
-__device__ __forceinline__ void rec_store_in(double *r, const dev::SoaIn<double> &in) {
-  double2 *v = reinterpret_cast<double2 *>(r);
-  v[0] = make_double2(in.g, in.a[0]);
-  v[1] = make_double2(in.a[1], in.a[2]);
-  v[2] = make_double2(in.a[3], in.g0);
-  v[4] = make_double2(in.uptk[0], in.uptk[1]);
-  v[5] = make_double2(in.uptk[2], in.uptk[3]);
-  v[6] = make_double2(in.aopoa[0], in.aopoa[1]);
-  v[7] = make_double2(in.aopoa[2], in.aopoa[3]);
+__device__ __forceinline__ void ws_store_in(double *ws, long n, long c,
+                                            const dev::SoaIn<double> &in) {
+  ws[W_G * n + c] = in.g;
+#pragma unroll
+  for (int m = 0; m < 4; ++m) {
+    ws[(W_A + m) * n + c] = in.a[m];
+    ws[(W_UPTK + m) * n + c] = in.uptk[m];
+    ws[(W_AOPOA + m) * n + c] = in.aopoa[m];
+  }
+  ws[W_G0 * n + c] = in.g0;
 }
 
-__device__ __forceinline__ void rec_load_in(const double *r, dev::SoaIn<double> &in) {
-  const double2 *v = reinterpret_cast<const double2 *>(r);
-  const double2 v0 = v[0], v1 = v[1], v2 = v[2], v4 = v[4], v5 = v[5], v6 = v[6], v7 = v[7];
-  in.g = v0.x;
-  in.a[0] = v0.y;
-  in.a[1] = v1.x;
-  in.a[2] = v1.y;
-  in.a[3] = v2.x;
-  in.g0 = v2.y;
-  in.uptk[0] = v4.x;
-  in.uptk[1] = v4.y;
-  in.uptk[2] = v5.x;
-  in.uptk[3] = v5.y;
-  in.aopoa[0] = v6.x;
-  in.aopoa[1] = v6.y;
-  in.aopoa[2] = v7.x;
-  in.aopoa[3] = v7.y;
+__device__ __forceinline__ void ws_load_in(const double *ws, long n, long c,
+                                           dev::SoaIn<double> &in) {
+  in.g = ws[W_G * n + c];
+#pragma unroll
+  for (int m = 0; m < 4; ++m) {
+    in.a[m] = ws[(W_A + m) * n + c];
+    in.uptk[m] = ws[(W_UPTK + m) * n + c];
+    in.aopoa[m] = ws[(W_AOPOA + m) * n + c];
+  }
+  in.g0 = ws[W_G0 * n + c];
 }
 
 struct QueueCtl {
@@ -98,7 +95,7 @@ struct QueueCtl {
 };
 
 struct Work {
-  double *ws;              // kRec doubles per cell
+  double *ws;              // W_NPLANE planes of ncell doubles
   unsigned char *bucket;   // per cell, kNoSoa when the cell has no SOA loop
   int *queue;              // cell indices, longest first
   QueueCtl *ctl;
@@ -201,7 +198,7 @@ __global__ void __launch_bounds__(kBlock) chain_pre_kernel(Src src, Work w) {
       dev::clear_build<P>(a, off, X, S, G);
       dev::SoaIn<double> in;
       dev::clear_pre<P>(*a.K, X.T, X.p, X.dgw, S, G.g[0], in);
-      rec_store_in(w.ws + g * kRec, in);
+      ws_store_in(w.ws, w.ncell, g, in);
       dev::SoaState<double> st;
       dev::soa_init(st, in);
       const float pred = (float)dev::soa_predict(st, w.dtfull);
@@ -241,14 +238,10 @@ __global__ void __launch_bounds__(kBlock) queue_scatter_kernel(Work w) {
 }
 
 // Phase 3: persistent SOA loop.  Each wave takes kChunk queue entries at a
-// time.  Every lane holds the record of its next cell in registers, fetched
-// while its current cell is still sub-stepping.  A pass is: one sub-step of
-// every active lane; lanes whose cell converged start their next cell (its
-// record arrived at least one pass earlier, so the wave does not wait on
-// memory) and store the finished cell's results; lanes without a next record
-// fetch one.  A wave exits once the queue is exhausted and no lane holds a
-// cell; the queue is finite and every cell ends after at most 1000
-// sub-steps, so every wave exits.
+// time; a lane whose cell has converged takes the next entry of the wave's
+// chunk before the following sub-step.  A wave exits once the queue is
+// exhausted and all its lanes are idle; the queue is finite and every cell
+// ends after at most 1000 sub-steps, so every wave exits.
 __global__ void __launch_bounds__(kBlock) soa_queue_kernel(Work w, unsigned long long *stats) {
   unsigned nq = 0, nlong = 0;
   for (int j = 0; j < kNBucket; ++j) {
@@ -256,45 +249,15 @@ __global__ void __launch_bounds__(kBlock) soa_queue_kernel(Work w, unsigned long
     if (j >= kLongBucket) nlong += w.ctl->hist[j];
   }
   const int lane = threadIdx.x & 63;
+  const long n = w.ncell;
   unsigned cbeg = 0, cend = 0;  // the wave's current chunk (wave-uniform)
-  unsigned chunk0 = 0;          // first queue slot of the chunk
-  __shared__ int qlds[kBlock / 64][kChunk];  // the chunk's entries, per wave
-  const int wave = threadIdx.x >> 6;
   bool exhausted = false;
-  bool active = false, have_next = false;
-  int cell = 0, next_cell = 0;
-  dev::SoaIn<double> nx;
+  bool active = false;
+  int cell = 0;
   dev::SoaState<double> s;
   unsigned long long niter = 0;
   while (true) {
-    bool done = false;
-    int done_cell = 0;
-    double r0 = 0, r1 = 0, r2 = 0, r3 = 0, r4 = 0, r5 = 0;
-    if (active && !dev::soa_tick(s, w.dtfull)) {
-      done = true;
-      done_cell = cell;
-      r0 = s.g;
-      r1 = s.a[0];
-      r2 = s.a[1];
-      r3 = s.a[2];
-      r4 = s.a[3];
-      r5 = dev::soa_gavg(s);
-      niter += (unsigned long long)s.niter;
-      active = false;
-    }
-    if (!active && have_next) {
-      cell = next_cell;
-      dev::soa_init(s, nx);
-      active = true;
-      have_next = false;
-    }
-    if (done) {
-      double2 *v = reinterpret_cast<double2 *>(w.ws + (long)done_cell * kRec);
-      v[0] = make_double2(r0, r1);
-      v[1] = make_double2(r2, r3);
-      v[2] = make_double2(r4, r5);
-    }
-    const unsigned long long need = __ballot(!have_next);
+    const unsigned long long need = __ballot(!active);
     if (need && cbeg == cend && !exhausted) {
       unsigned b0 = 0;
       if (lane == 0) b0 = atomicAdd(&w.ctl->head, (unsigned)kChunk);
@@ -305,23 +268,29 @@ __global__ void __launch_bounds__(kBlock) soa_queue_kernel(Work w, unsigned long
       } else {
         cbeg = b0;
         cend = min(b0 + (unsigned)kChunk, nq);
-        chunk0 = b0;
-        // one coalesced load of the chunk's entries, staged in LDS so later
-        // passes read them without waiting on the vector-memory counter
-        qlds[wave][lane] = (b0 + lane < cend) ? w.queue[b0 + lane] : 0;
       }
     }
     if (need && cbeg < cend) {
       const unsigned rank = (unsigned)__popcll(need & ((1ull << lane) - 1ull));
       const unsigned avail = cend - cbeg;
-      if (!have_next && rank < avail) {
-        next_cell = qlds[wave][cbeg - chunk0 + rank];
-        rec_load_in(w.ws + (long)next_cell * kRec, nx);
-        have_next = true;
+      if (!active && rank < avail) {
+        cell = w.queue[cbeg + rank];
+        dev::SoaIn<double> in;
+        ws_load_in(w.ws, n, cell, in);
+        dev::soa_init(s, in);
+        active = true;
       }
       cbeg += min((unsigned)__popcll(need), avail);
     }
-    if (!__any(active || have_next)) break;
+    if (!__any(active)) break;
+    if (active && !dev::soa_tick(s, w.dtfull)) {
+      w.ws[W_G * n + cell] = s.g;
+#pragma unroll
+      for (int m = 0; m < 4; ++m) w.ws[(W_A + m) * n + cell] = s.a[m];
+      w.ws[W_GAVG * n + cell] = dev::soa_gavg(s);
+      niter += (unsigned long long)s.niter;
+      active = false;
+    }
   }
   add_stats(stats, niter);
 }
@@ -344,18 +313,14 @@ __global__ void __launch_bounds__(kBlock) chain_post_kernel(Src src, Work w) {
     dev::clear_build<P>(a, off, X, S, G);
     double uptk[4] = {0, 0, 0, 0}, sa[4] = {0, 0, 0, 0}, sg = 0.0, sgavg = 0.0;
     if (a.cfg.do_cond) {
-      const double2 *v = reinterpret_cast<const double2 *>(w.ws + g * kRec);
-      const double2 v0 = v[0], v1 = v[1], v2 = v[2], v4 = v[4], v5 = v[5];
-      sg = v0.x;
-      sa[0] = v0.y;
-      sa[1] = v1.x;
-      sa[2] = v1.y;
-      sa[3] = v2.x;
-      sgavg = v2.y;
-      uptk[0] = v4.x;
-      uptk[1] = v4.y;
-      uptk[2] = v5.x;
-      uptk[3] = v5.y;
+      const long n = w.ncell;
+      sg = w.ws[W_G * n + g];
+      sgavg = w.ws[W_GAVG * n + g];
+#pragma unroll
+      for (int m = 0; m < 4; ++m) {
+        sa[m] = w.ws[(W_A + m) * n + g];
+        uptk[m] = w.ws[(W_UPTK + m) * n + g];
+      }
     }
     dev::clear_post<P>(*a.K, a.cfg, a.dt, X.T, X.p, X.z, X.pblh, X.rh_clear, X.dga, X.dgw, X.wd,
                     G.g1, G.g, S, uptk, sg, sgavg, sa);
@@ -473,7 +438,7 @@ static int reserve_work(mam4x_handle h, long ncell, hipStream_t s) {
   if (ncell <= h->cap) return MAM4X_OK;
   if (h->cap) HIP_TRY(h, hipStreamSynchronize(s));
   free_work(h);
-  if (hipMalloc(&h->ws, sizeof(double) * kRec * ncell) != hipSuccess ||
+  if (hipMalloc(&h->ws, sizeof(double) * W_NPLANE * ncell) != hipSuccess ||
       hipMalloc(&h->bucket, (size_t)ncell) != hipSuccess ||
       hipMalloc(&h->queue, sizeof(int) * ncell) != hipSuccess) {
     (void)hipGetLastError();

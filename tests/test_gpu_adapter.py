@@ -23,6 +23,17 @@ HERE = os.path.dirname(os.path.abspath(__file__))
 DRIVER = os.path.join(HERE, "_build", "libadapter_driver.so")
 
 
+@pytest.fixture(scope="module", autouse=True)
+def hip_runtime_from_torch():
+    """Initialise the GPU through torch before the driver library loads, so
+    the whole test process shares torch's HIP runtime (the driver and
+    libmam4x.so then bind to the runtime already loaded)."""
+    import torch
+    if not torch.cuda.is_available():
+        pytest.fail("GPU test run without a GPU")
+    torch.cuda.init()
+
+
 def run_driver(sc, dt):
     if not os.path.exists(DRIVER):
         pytest.fail("%s not built: run __graft_entry__.build()" % DRIVER)

@@ -4,8 +4,8 @@
   for bit (the dual numbers' value part performs the same operations);
 * its derivative matches central finite differences of the CPU oracle on the
   smooth elements (see test_tangent_host.py for the criterion);
-* autograd: reverse mode is the adjoint of forward mode, <g, J v> = <J^T g, v>,
-  and torch.func.jvp through the Function equals the tangent entry point.
+* autograd: reverse mode is the adjoint of forward mode, <g, J v> = <J^T g, v>
+  (forward mode through torch.autograd.forward_ad).
 """
 import numpy as np
 import pytest
@@ -64,7 +64,7 @@ def test_tangent_primal_and_derivative(orc, torch_cuda):
 
 
 def test_autograd_reverse_is_adjoint_of_forward(torch_cuda):
-    import torch.func
+    import torch.autograd.forward_ad as fwAD
     from mam4xx_amd.autograd import tendencies
     from mam4xx_amd.chain import scene_to_device
     torch = torch_cuda
@@ -74,7 +74,9 @@ def test_autograd_reverse_is_adjoint_of_forward(torch_cuda):
     v_p = torch.zeros_like(d["pressure"])
     x = [d["state"], d["temperature"], d["pressure"], d["qv"]]
     f = lambda s, T, p, q: tendencies(s, T, p, q, d, DT)
-    y, jv = torch.func.jvp(f, tuple(x), (v_state, v_T, v_p, v_qv))
+    with fwAD.dual_level():
+        duals = [fwAD.make_dual(a, b) for a, b in zip(x, (v_state, v_T, v_p, v_qv))]
+        y, jv = fwAD.unpack_dual(f(*duals))
     g = torch.randn_like(y)
     xs = [t.clone().requires_grad_(True) for t in x]
     y2 = f(*xs)
