@@ -190,15 +190,15 @@ __device__ __forceinline__ bool soa_tick(SoaState<W> &s, double dtfull) {
   W tmpb = 0.0;
 #pragma unroll
   for (int n = 0; n < 4; ++n) {
-    sat[n] = s.g0 / kmax(s.aopoa[n] + a_soa[n], a_min1);
+    sat[n] = num::div(s.g0, kmax(s.aopoa[n] + a_soa[n], a_min1));
     gstar[n] = sat[n] * a_soa[n];
-    const W phi = (g_soa - gstar[n]) / kmax(g_soa, kmax(gstar[n], g_min1));
+    const W phi = num::div(g_soa - gstar[n], kmax(g_soa, kmax(gstar[n], g_min1)));
     tmpb = s.skip[n] ? tmpb : tmpb + s.uk[n] * fabs(phi);
   }
   const W tmpa = kmax(0.0, tmpb);
   const W dtmax = dtfull - s.tcur;
   const bool last = dtmax * tmpa <= alpha_astem;
-  const W dtcur = last ? dtmax : alpha_astem / tmpa;
+  const W dtcur = last ? dtmax : num::div(W(alpha_astem), tmpa);
   s.tcur = last ? W(dtfull) : s.tcur + dtcur;
   // step 1: explicit estimate of condensing modes' a_soa and sat
   W beta[4];
@@ -207,7 +207,7 @@ __device__ __forceinline__ bool soa_tick(SoaState<W> &s, double dtfull) {
     beta[n] = dtcur * s.uk[n];
     const W del = g_soa - gstar[n];
     const W at = a_soa[n] + beta[n] * del;
-    const W sat1 = s.g0 / kmax(s.aopoa[n] + at, a_min1);
+    const W sat1 = num::div(s.g0, kmax(s.aopoa[n] + at, a_min1));
     sat[n] = del > 0.0 ? sat1 : sat[n];
   }
   // step 2: implicit in g, semi-implicit in a
@@ -215,16 +215,16 @@ __device__ __forceinline__ bool soa_tick(SoaState<W> &s, double dtfull) {
 #pragma unroll
   for (int n = 0; n < 4; ++n) {
     den[n] = 1.0 + beta[n] * sat[n];
-    sa = s.skip[n] ? sa : sa + a_soa[n] / den[n];
-    sb = s.skip[n] ? sb : sb + beta[n] / den[n];
+    sa = s.skip[n] ? sa : sa + num::div(a_soa[n], den[n]);
+    sb = s.skip[n] ? sb : sb + num::div(beta[n], den[n]);
   }
-  g_soa = (tot - sa) / (1.0 + sb);
+  g_soa = num::div(tot - sa, 1.0 + sb);
   g_soa = kmax(0.0, g_soa);
 #pragma unroll
   for (int n = 0; n < 4; ++n) {
     // skipped modes: a_soa is never set, so their SOA becomes 0 (quirk of
     // gasaerexch_soaexch.hpp:327-331)
-    s.a[n] = s.skip[n] ? W(0.0) : (a_soa[n] + beta[n] * g_soa) / den[n];
+    s.a[n] = s.skip[n] ? W(0.0) : num::div(a_soa[n] + beta[n] * g_soa, den[n]);
   }
   s.g = g_soa;
   s.avg += dtcur * (gprv + 0.5 * (g_soa - gprv));

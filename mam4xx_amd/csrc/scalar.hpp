@@ -126,6 +126,33 @@ __host__ __device__ __forceinline__ Dual erfc(Dual a) {
 __host__ __device__ __forceinline__ Dual fabs(Dual a) { return a.v < 0 ? -a : a; }
 __host__ __device__ __forceinline__ Dual round(Dual a) { return Dual(::round(a.v), 0.0); }
 
+// ------------------------------------------------------------------ division
+// Correctly rounded FP64 division without the range scaling and special-case
+// fix-up of the general IEEE sequence: reciprocal, two Newton steps, then
+// q + r (a - b q) (Markstein).  These are the same operations the general
+// sequence performs when its scaling is the identity, which holds whenever a,
+// b and a/b are normal numbers away from the ends of the exponent range
+// (|a| > 2^-969, 2^-1021 < |b| < 2^1021); a +0 numerator gives +0, a -0
+// numerator +0 as well (IEEE: -0).  For other operands the result may differ
+// from a / b (b = 0 or inf gives NaN), so callers use it only where the
+// operands are bounded and numerators are never -0 (the SOA sub-step).
+__host__ __device__ __forceinline__ double div(double a, double b) {
+#if defined(__HIP_DEVICE_COMPILE__)
+  double r = __builtin_amdgcn_rcp(b);
+  double e = __builtin_fma(-b, r, 1.0);
+  r = __builtin_fma(r, e, r);
+  e = __builtin_fma(-b, r, 1.0);
+  r = __builtin_fma(r, e, r);
+  const double q = a * r;
+  const double res = __builtin_fma(-b, q, a);
+  return __builtin_fma(res, r, q);
+#else
+  return a / b;
+#endif
+}
+__host__ __device__ __forceinline__ F32 div(F32 a, F32 b) { return a / b; }
+__host__ __device__ __forceinline__ Dual div(Dual a, Dual b) { return a / b; }
+
 // ------------------------------------------------------------------ helpers
 // primal value as a double (branch decisions, integer indices, outputs)
 __host__ __device__ __forceinline__ double val(double x) { return x; }

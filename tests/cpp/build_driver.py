@@ -1,9 +1,9 @@
-"""Build recipe of the C++ host-interface test driver (tests/cpp/adapter_driver.cpp).
-
-Produces tests/_build/libadapter_driver.so, linked against
-mam4xx_amd/lib/libmam4x.so (build that first).  Called by
-__graft_entry__.build(); the GPU test loads the result and fails when it is
-missing.
+"""Build recipe of the GPU test drivers: the C++ host-interface driver
+(tests/cpp/adapter_driver.cpp -> tests/_build/libadapter_driver.so, linked
+against mam4xx_amd/lib/libmam4x.so, build that first) and the division
+check kernel (tests/cpp/div_check.hip -> tests/_build/libdivcheck.so).
+Called by __graft_entry__.build(); the GPU tests load the results and fail
+when they are missing.
 """
 import os
 import subprocess
@@ -25,6 +25,14 @@ def build(verbose=False):
         print(" ".join(cmd), flush=True)
     subprocess.run(cmd, check=True)
     os.replace(OUT + ".tmp", OUT)
+    div = os.path.join(os.path.dirname(OUT), "libdivcheck.so")
+    cmd = [HIPCC, "-O3", "-std=c++17", "-fPIC", "-shared", "--offload-arch=gfx950",
+           "-Xclang", "-target-feature", "-Xclang", "-packed-fp32-ops", "-ffp-contract=off",
+           "-o", div + ".tmp", os.path.join(HERE, "div_check.hip")]
+    if verbose:
+        print(" ".join(cmd), flush=True)
+    subprocess.run(cmd, check=True)
+    os.replace(div + ".tmp", div)
     return OUT
 
 
