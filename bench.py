@@ -99,6 +99,121 @@ def _traffic(name):
     return None
 
 
+def bench_batch(args):
+    """32 uneven scenes (scenes.uneven_batch, seed 0) stepped by one launch
+    sequence per step: eager, replayed from a CUDA graph, and autotuned."""
+    import torch
+    from mam4xx_amd import _abi
+    from mam4xx_amd import layout as L
+    from mam4xx_amd import scenes, tune
+    from mam4xx_amd.chain import Chain, scene_to_device
+    scs = scenes.uneven_batch(nscenes=32, seed=0)
+    ncols = sum(s["temperature"].shape[1] for s in scs)
+    ds = [scene_to_device(s) for s in scs]
+    for d in ds:
+        d["dgncur_awet"] = d["dgncur_a"]
+    out = {"metric": "batched uneven scenes: column-steps/sec at 72 lev", "unit": "column-steps/s",
+           "n_gpus": 1, "higher_is_better": True, "dtype": "f64", "data": "synthetic",
+           "config": {"workload": "32 scenes, %d columns x 72 levels, column counts log-uniform "
+                                  "in [96, 21600], chained 1800 s steps" % ncols,
+                      "scenes": 32, "columns": ncols}}
+    for name, sched in (("queue", _abi.SCHED_QUEUE), ("fused", _abi.SCHED_FUSED)):
+        ch = Chain(schedule=sched)
+        tends = [torch.empty((L.NSTATE,) + tuple(d["temperature"].shape), dtype=torch.float64,
+                             device="cuda") for d in ds]
+        b = ch.batch(ds, DT, tends, dgn_i=[d["dgncur_a"] for d in ds],
+                     state_out=[d["state"] for d in ds])
+        for _ in range(args.warmup):
+            b.run()
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        for _ in range(args.steps):
+            b.run()
+        torch.cuda.synchronize()
+        eager = (time.perf_counter() - t0) / args.steps
+        g = torch.cuda.CUDAGraph()
+        with torch.cuda.graph(g):
+            b.run()
+        g.replay()
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        for _ in range(args.steps):
+            g.replay()
+        torch.cuda.synchronize()
+        graph = (time.perf_counter() - t0) / args.steps
+        out[name] = {"eager_ms_per_step": eager * 1e3, "graph_ms_per_step": graph * 1e3,
+                     "graph_column_steps_per_s": ncols / graph}
+        b.close()
+        ch.close()
+    best, ms = tune.tune(ds, DT, steps=5)
+    out["tuned"] = {"best": best, "ms_per_step": {str(tune.CANDIDATES[i]): v for i, v in ms.items()}}
+    out["value"] = out["queue"]["graph_column_steps_per_s"]
+    print(json.dumps(out), flush=True)
+
+
+def bench_large(args):
+    """345,600 columns x 128 levels (top 10 Pa): a seeded 21,600 x 128 scene
+    tiled 16 times across columns on the device (cells are independent, so
+    the value distributions are kept).  FP64 and mixed forward, the tangent,
+    and the device memory in use."""
+    import numpy as np
+    import torch
+    from mam4xx_amd import _abi
+    from mam4xx_amd import layout as L
+    from mam4xx_amd import scenes
+    from mam4xx_amd.chain import Chain, scene_to_device
+    ncol0, nlev, rep = 21600, 128, 16
+    free0, _ = torch.cuda.mem_get_info()
+    d0 = scene_to_device(scenes.generate(ncol0, nlev, seed=0, ptop=10.0))
+    tile = lambda v: v.repeat(rep) if v.dim() == 1 else v.repeat(*([1] * (v.dim() - 1) + [rep]))
+    d = {k: tile(v).contiguous() for k, v in d0.items()}
+    del d0
+    ncol = ncol0 * rep
+    tend = torch.empty((L.NSTATE, nlev, ncol), dtype=torch.float64, device="cuda")
+    dgc = torch.empty((4, nlev, ncol), dtype=torch.float64, device="cuda")
+    out = {"metric": "large scene: column-steps/sec at 128 lev", "unit": "column-steps/s",
+           "n_gpus": 1, "higher_is_better": True, "dtype": "f64", "data": "synthetic",
+           "config": {"workload": "345,600 columns x 128 levels (21,600-column seeded scene "
+                                  "tiled 16x), chained 1800 s steps",
+                      "columns": ncol, "levels": nlev}}
+    for name, prec in (("fp64", _abi.PREC_FP64), ("mixed", _abi.PREC_MIXED)):
+        ch = Chain(precision=prec)
+        st, dg = d["state"].clone(), d["dgncur_a"].clone()
+        dd = dict(d, state=st, dgncur_a=dg, dgncur_awet=dg)
+        ts = []
+        for i in range(1 + args.steps):
+            e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+            e0.record()
+            ch.compute_tendencies(dd, DT, tend, dgn_i=dg, dgn_c=dgc, state_out=st)
+            e1.record()
+            torch.cuda.synchronize()
+            if i:
+                ts.append(e0.elapsed_time(e1))
+        free1, _ = torch.cuda.mem_get_info()
+        out[name] = {"ms_per_step": float(np.mean(ts)), "column_steps_per_s": ncol / (np.mean(ts) * 1e-3),
+                     "device_memory_in_use_gb": (free0 - free1) / 1e9}
+        del st, dg, dd
+        ch.close()
+    ch = Chain()
+    dtend = torch.empty_like(tend)
+    v = d["state"] * torch.randn_like(d["state"])
+    ch.compute_tangent(d, DT, tend, dtend, dstate=v)
+    times = {}
+    for name, fn in (("tangent", lambda: ch.compute_tangent(d, DT, tend, dtend, dstate=v)),
+                     ("forward", lambda: ch.compute_tendencies(d, DT, tend))):
+        e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+        e0.record()
+        fn()
+        e1.record()
+        torch.cuda.synchronize()
+        times[name] = e0.elapsed_time(e1)
+    out["tangent"] = {"ms": times["tangent"], "forward_ms_same_input": times["forward"],
+                      "ratio": times["tangent"] / times["forward"]}
+    out["peak_torch_allocated_gb"] = torch.cuda.max_memory_allocated() / 1e9
+    out["value"] = out["fp64"]["column_steps_per_s"]
+    print(json.dumps(out), flush=True)
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -107,6 +222,9 @@ def main():
     ap.add_argument("--precision", choices=["fp64", "mixed"], default="fp64")
     ap.add_argument("--schedule", choices=["auto", "fused", "queue"], default="auto")
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline")
+    ap.add_argument("--workload", choices=["headline", "batch", "large"], default="headline",
+                    help="headline: the metric's 21,600 x 72 step (default); batch and large "
+                         "print their own line for the other configurations")
     args = ap.parse_args()
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) > 1:
         print("bench.py measures one GPU: the chain has no cross-GPU step and this build "
@@ -120,6 +238,10 @@ def main():
     from mam4xx_amd.chain import Chain, scene_to_device
 
     assert torch.cuda.is_available(), "bench.py needs a GPU"
+    if args.workload == "batch":
+        return bench_batch(args)
+    if args.workload == "large":
+        return bench_large(args)
     scene = scenes.generate(NCOL, NLEV, seed=0)
     d = scene_to_device(scene)
     # chained steps: calcsize diameters feed the next step's dry and wet
