@@ -165,13 +165,25 @@ __device__ __forceinline__ double soa_predict(const SoaState<W> &s, double dtful
   return dtfull * tmpb / 0.05;
 }
 
+// A skipped mode's term of a sum.  In the plain types a skipped mode's term is
+// already exactly +0 (see soa_tick), so it is added unconditionally; the Dual
+// type selects, so that a derivative of a skipped mode's discarded quotient
+// (0 * inf) cannot reach the sums.
+__device__ __forceinline__ double soa_term(bool, double x) { return x; }
+__device__ __forceinline__ num::Dual soa_term(bool skip, num::Dual x) {
+  return skip ? num::Dual(0.0) : x;
+}
+
 // One pass of the while loop (gasaerexch_soaexch.hpp:170-335).  Returns false
 // once the loop has ended (time reached or 1000 sub-steps taken).
 //
-// Written without branches on the mode: a skipped mode (uptake <= 1e-15) has
-// uk = 0 and a_soa = 0, so its terms add exactly +0 to every sum and its
-// denominators are exactly 1 -- the same values the reference's skip produces
-// -- and the divisions of all four modes issue as straight-line code.
+// Written without branches on the mode.  A skipped mode (uptake <= 1e-15) has
+// uk = 0 and a_soa = 0, so beta = 0, den = 1 and each of its terms
+// (a_soa, uk |phi|, a_soa / den, beta / den) is exactly +0: adding it leaves
+// a sum bit-for-bit unchanged, and its new SOA, (0 + 0 g) / 1, is +0 -- the
+// values the reference's skip produces.  The four modes' divisions are then
+// four independent chains the scheduler interleaves, instead of a branch per
+// mode that serialises them.
 template <class W>
 __device__ __forceinline__ bool soa_tick(SoaState<W> &s, double dtfull) {
   const double alpha_astem = 0.05, a_min1 = 1.0e-20, g_min1 = 1.0e-20;
@@ -180,20 +192,21 @@ __device__ __forceinline__ bool soa_tick(SoaState<W> &s, double dtfull) {
   if (s.niter > 1000) return false;
   const W gprv = s.g;
   W g_soa = kmax(gprv, 0.0);
-  W tot = g_soa;
-  W a_soa[4], sat[4], gstar[4];
+  W a_soa[4], sat[4], term[4];
 #pragma unroll
   for (int n = 0; n < 4; ++n) {
     a_soa[n] = s.skip[n] ? W(0.0) : kmax(s.a[n], 0.0);
-    tot = s.skip[n] ? tot : tot + a_soa[n];
+    sat[n] = num::div(s.g0, kmax(s.aopoa[n] + a_soa[n], a_min1));
+    const W gstar = sat[n] * a_soa[n];
+    const W phi = num::div(g_soa - gstar, kmax(g_soa, kmax(gstar, g_min1)));
+    term[n] = soa_term(s.skip[n], s.uk[n] * fabs(phi));
   }
-  W tmpb = 0.0;
+  // the sums in the reference's mode order
+  W tot = g_soa, tmpb = 0.0;
 #pragma unroll
   for (int n = 0; n < 4; ++n) {
-    sat[n] = num::div(s.g0, kmax(s.aopoa[n] + a_soa[n], a_min1));
-    gstar[n] = sat[n] * a_soa[n];
-    const W phi = num::div(g_soa - gstar[n], kmax(g_soa, kmax(gstar[n], g_min1)));
-    tmpb = s.skip[n] ? tmpb : tmpb + s.uk[n] * fabs(phi);
+    tot = tot + soa_term(s.skip[n], a_soa[n]);
+    tmpb = tmpb + term[n];
   }
   const W tmpa = kmax(0.0, tmpb);
   const W dtmax = dtfull - s.tcur;
@@ -205,7 +218,8 @@ __device__ __forceinline__ bool soa_tick(SoaState<W> &s, double dtfull) {
 #pragma unroll
   for (int n = 0; n < 4; ++n) {
     beta[n] = dtcur * s.uk[n];
-    const W del = g_soa - gstar[n];
+    const W gstar = sat[n] * a_soa[n];
+    const W del = g_soa - gstar;
     const W at = a_soa[n] + beta[n] * del;
     const W sat1 = num::div(s.g0, kmax(s.aopoa[n] + at, a_min1));
     sat[n] = del > 0.0 ? sat1 : sat[n];
@@ -215,8 +229,8 @@ __device__ __forceinline__ bool soa_tick(SoaState<W> &s, double dtfull) {
 #pragma unroll
   for (int n = 0; n < 4; ++n) {
     den[n] = 1.0 + beta[n] * sat[n];
-    sa = s.skip[n] ? sa : sa + num::div(a_soa[n], den[n]);
-    sb = s.skip[n] ? sb : sb + num::div(beta[n], den[n]);
+    sa = sa + soa_term(s.skip[n], num::div(a_soa[n], den[n]));
+    sb = sb + soa_term(s.skip[n], num::div(beta[n], den[n]));
   }
   g_soa = num::div(tot - sa, 1.0 + sb);
   g_soa = kmax(0.0, g_soa);
@@ -224,7 +238,7 @@ __device__ __forceinline__ bool soa_tick(SoaState<W> &s, double dtfull) {
   for (int n = 0; n < 4; ++n) {
     // skipped modes: a_soa is never set, so their SOA becomes 0 (quirk of
     // gasaerexch_soaexch.hpp:327-331)
-    s.a[n] = s.skip[n] ? W(0.0) : num::div(a_soa[n] + beta[n] * g_soa, den[n]);
+    s.a[n] = soa_term(s.skip[n], num::div(a_soa[n] + beta[n] * g_soa, den[n]));
   }
   s.g = g_soa;
   s.avg += dtcur * (gprv + 0.5 * (g_soa - gprv));
