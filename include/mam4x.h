@@ -40,7 +40,7 @@
 extern "C" {
 #endif
 
-#define MAM4X_ABI_VERSION 2
+#define MAM4X_ABI_VERSION 3
 
 /* status codes */
 #define MAM4X_OK 0
@@ -155,6 +155,19 @@ typedef struct mam4x_tangent {
   double *tendencies[MAM4X_NSTATE];   /* out: d(tendency) along the direction */
 } mam4x_tangent;
 
+/* Vector-Jacobian product (mam4x_compute_adjoint): a cotangent of the
+ * tendencies in, the gradients of <cotangent, tendencies> with respect to
+ * the inputs out, cell by cell (cells are independent, so the Jacobian of a
+ * step is block diagonal: one 52 x 55 block per cell).  NULL cotangent
+ * planes are zero; NULL gradient planes are not formed. */
+typedef struct mam4x_adjoint {
+  const double *tendencies[MAM4X_NSTATE];  /* cotangent of each tendency     */
+  double *state[MAM4X_NSTATE];             /* out: gradient w.r.t. the state */
+  double *temperature;                     /* out: gradient w.r.t. T         */
+  double *pressure;                        /* out: gradient w.r.t. p         */
+  double *vapor_mixing_ratio;              /* out: gradient w.r.t. qv        */
+} mam4x_adjoint;
+
 typedef struct mam4x_handle_s *mam4x_handle;
 
 /* Library / handle lifetime.  mam4x_create replaces AeroProcess<Impl>'s
@@ -190,6 +203,18 @@ int mam4x_compute_tendencies(mam4x_handle h, double t, double dt,
  * choice).  FP64 only; one lane runs one cell's whole step. */
 int mam4x_compute_tangent(mam4x_handle h, double t, double dt, const mam4x_inputs *in,
                           const mam4x_tangent *dir, const mam4x_outputs *out, void *stream);
+
+/* Vector-Jacobian product of mam4x_compute_tendencies (no counterpart in the
+ * reference): for every gradient plane requested in *adj, one pass of the
+ * tangent step seeded with the unit direction of that input, whose
+ * derivative is contracted with adj->tendencies in the same lane (so no
+ * Jacobian is stored).  The result is exact to round-off for the branch each
+ * cell takes, like the tangent; its cost is one tangent step per requested
+ * plane (up to 55), not a reverse sweep.  Uses a workspace plane of the
+ * input extent held by the handle (allocated on first use); calls on one
+ * handle must be stream-ordered. */
+int mam4x_compute_adjoint(mam4x_handle h, double t, double dt, const mam4x_inputs *in,
+                          const mam4x_adjoint *adj, void *stream);
 
 /* Batches: many scenes (domains of any column count, each with its own
  * inputs, outputs and layout) stepped by one set of launches.  The scenes'

@@ -6,7 +6,7 @@ The structures below must match the C declarations field for field; the
 import ctypes
 import os
 
-ABI_VERSION = 2
+ABI_VERSION = 3
 
 NSTATE = 52
 NMODE = 4
@@ -84,6 +84,16 @@ class Tangent(ctypes.Structure):
     ]
 
 
+class Adjoint(ctypes.Structure):
+    _fields_ = [
+        ("tendencies", _vp * NSTATE),
+        ("state", _vp * NSTATE),
+        ("temperature", _vp),
+        ("pressure", _vp),
+        ("vapor_mixing_ratio", _vp),
+    ]
+
+
 class Outputs(ctypes.Structure):
     _fields_ = [
         ("tendencies", _vp * NSTATE),
@@ -110,6 +120,9 @@ SIGNATURES = {
     "mam4x_compute_tangent": (ctypes.c_int, [_vp, ctypes.c_double, ctypes.c_double,
                                              ctypes.POINTER(Inputs), ctypes.POINTER(Tangent),
                                              ctypes.POINTER(Outputs), _vp]),
+    "mam4x_compute_adjoint": (ctypes.c_int, [_vp, ctypes.c_double, ctypes.c_double,
+                                             ctypes.POINTER(Inputs), ctypes.POINTER(Adjoint),
+                                             _vp]),
     "mam4x_batch_create": (ctypes.c_int, [_vp, ctypes.c_int, ctypes.POINTER(Inputs),
                                           ctypes.POINTER(Outputs), ctypes.c_double,
                                           ctypes.POINTER(_vp)]),

@@ -9,13 +9,13 @@ vapour.
   kernel (mam4x_compute_tangent) per direction.  The functorch transforms
   (``torch.func``) are not supported: the kernels need plain device tensors.
 * Reverse mode (``backward``): cells are independent, so each cell's Jacobian
-  is a 52 x 55 block.  The vector-Jacobian product is assembled from one
-  tangent call per differentiable input field, each along the unit direction
-  of that field in every cell at once:
-  grad_x_j(cell) = sum_f g_f(cell) * d tend_f(cell) / d x_j(cell).
-  This is exact (the same derivative as forward mode) and costs one tangent
-  pass per input field that requires a gradient -- up to 55 -- not the few
-  forward passes a hand-written reverse sweep would take.
+  is a 52 x 55 block.  ``backward`` is one call of mam4x_compute_adjoint,
+  which forms grad_x_j(cell) = sum_f g_f(cell) * d tend_f(cell) / d x_j(cell)
+  by one tangent pass per input field that requires a gradient (up to 55),
+  each seeded with the unit direction of that field in every cell and
+  contracted with g in the same lane.  This is exact (the same derivative as
+  forward mode); its cost is up to 55 tangent passes, not the few forward
+  passes a hand-written reverse sweep would take.
 
 Derivatives are those of the branch each cell takes (include/mam4x.h,
 mam4x_compute_tangent).
@@ -81,23 +81,15 @@ class _Tendencies(torch.autograd.Function):
         d, dt = _scene(ctx.rest, *ctx.saved_tensors), ctx.dt
         g = g.contiguous()
         nlev, ncol = d["temperature"].shape
-        one = torch.ones((nlev, ncol), dtype=torch.float64, device="cuda")
+        new = lambda *shape: torch.empty(shape, dtype=torch.float64, device="cuda")
         grads = [None, None, None, None]
         if ctx.needs_input_grad[0]:
-            gs = torch.empty((L.NSTATE, nlev, ncol), dtype=torch.float64, device="cuda")
-            unit = torch.zeros((L.NSTATE, nlev, ncol), dtype=torch.float64, device="cuda")
-            for j in range(L.NSTATE):
-                unit.zero_()
-                unit[j] = 1.0
-                _, dtend = _tangent(d, dt, unit, None, None, None)
-                gs[j] = (g * dtend).sum(dim=0)
-            grads[0] = gs
-        for k, key in ((1, "T"), (2, "p"), (3, "qv")):
+            grads[0] = new(L.NSTATE, nlev, ncol)
+        for k in (1, 2, 3):
             if ctx.needs_input_grad[k]:
-                args = [None, None, None]
-                args[k - 1] = one
-                _, dtend = _tangent(d, dt, None, *args)
-                grads[k] = (g * dtend).sum(dim=0)
+                grads[k] = new(nlev, ncol)
+        _chain().compute_adjoint(d, dt, g, state_bar=grads[0], temperature_bar=grads[1],
+                                 pressure_bar=grads[2], qv_bar=grads[3])
         return grads[0], grads[1], grads[2], grads[3], None, None
 
 

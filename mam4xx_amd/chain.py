@@ -176,6 +176,38 @@ class Chain:
                                             ctypes.byref(out), ctypes.c_void_p(s.cuda_stream))
         self._check(rc, "mam4x_compute_tangent")
 
+    def compute_adjoint(self, scene, dt, tbar, state_bar=None, temperature_bar=None,
+                        pressure_bar=None, qv_bar=None, t=0.0, stream=None,
+                        layout=_abi.LAYOUT_LEV_COL):
+        """Per-cell vector-Jacobian product through mam4x_compute_adjoint:
+        tbar ([52, ...] cotangent of the tendencies, or a list of 52 planes
+        with None for zero planes) in; the gradients of
+        sum(tbar * tendencies) with respect to the state, T, p and qv written
+        to the given tensors (None = not formed)."""
+        inp = self.make_inputs(scene, None, None, layout)
+        adj = _abi.Adjoint()
+        if isinstance(tbar, (list, tuple)):  # one plane or None per field
+            assert len(tbar) == L.NSTATE
+            for i, pl in enumerate(tbar):
+                if pl is not None:
+                    assert pl.dtype == torch.float64 and pl.is_cuda and pl.is_contiguous()
+                    adj.tendencies[i] = pl.data_ptr()
+        else:
+            for i, p in enumerate(_plane_ptrs(tbar, L.NSTATE)):
+                adj.tendencies[i] = p
+        if state_bar is not None:
+            for i, p in enumerate(_plane_ptrs(state_bar, L.NSTATE)):
+                adj.state[i] = p
+        for name, tt in (("temperature", temperature_bar), ("pressure", pressure_bar),
+                         ("vapor_mixing_ratio", qv_bar)):
+            if tt is not None:
+                assert tt.dtype == torch.float64 and tt.is_cuda and tt.is_contiguous()
+                setattr(adj, name, tt.data_ptr())
+        s = stream if stream is not None else torch.cuda.current_stream()
+        rc = self.lib.mam4x_compute_adjoint(self.h, t, dt, ctypes.byref(inp), ctypes.byref(adj),
+                                            ctypes.c_void_p(s.cuda_stream))
+        self._check(rc, "mam4x_compute_adjoint")
+
     def validate(self, scene, stream=None):
         inp = self.make_inputs(scene)
         v = ctypes.c_longlong(0)

@@ -32,6 +32,11 @@ struct ChainArgs {
   const double *dstate[NSTATE];
   const double *dT, *dp, *dqv;
   double *dtend[NSTATE];
+  // vector-Jacobian product by a seeded tangent pass: when xbar is set, the
+  // lane writes sum_f tbar[f] * d(tendency f) to xbar instead of dtend
+  // (tbar[f] NULL = 0)
+  const double *tbar[NSTATE];
+  double *xbar;
   const ChainConsts *K;
   int ncol, nlev, ldc;
   int layout;  // 0: [nlev][ldc] column fastest; 1: [ncol][ldc] level fastest
@@ -350,6 +355,7 @@ __device__ __forceinline__ void cell_finish(const ChainArgs &a, long off, const 
     }
     calcsize(K, a.do_xfer != 0, a.do_adjust != 0, dt, n_i, n_c, q_i, q_c, o);
   }
+  double xbar = 0.0;  // tangent mode: the contraction with tbar
 #pragma unroll
   for (int f = 0; f < NSTATE; ++f) {
     const W st = ldw<P>(a.state[f], a.dstate[f], off);
@@ -365,10 +371,18 @@ __device__ __forceinline__ void cell_finish(const ChainArgs &a, long off, const 
         t += cw ? o.dq_c[m][sp] : o.dq_i[m][sp];
       }
     }
-    a.tend[f][off] = num::val(t);
-    if constexpr (std::is_same<W, num::Dual>::value) a.dtend[f][off] = num::tan(t);
+    if (a.tend[f]) a.tend[f][off] = num::val(t);
+    if constexpr (std::is_same<W, num::Dual>::value) {
+      if (a.xbar) {
+        if (a.tbar[f]) xbar += a.tbar[f][off] * num::tan(t);
+      } else {
+        a.dtend[f][off] = num::tan(t);
+      }
+    }
     if (a.has_sout) a.sout[f][off] = num::val(st) + dt * num::val(t);
   }
+  if constexpr (std::is_same<W, num::Dual>::value)
+    if (a.xbar) a.xbar[off] = xbar;
   if (a.has_dgn) {
 #pragma unroll
     for (int m = 0; m < 4; ++m) {

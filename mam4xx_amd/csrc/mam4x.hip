@@ -347,6 +347,11 @@ __global__ void __launch_bounds__(kBlock) validate_kernel(ChainArgs a,
   if ((threadIdx.x & 63) == 0 && bad) atomicAdd(count, bad);
 }
 
+__global__ void __launch_bounds__(kBlock) fill_kernel(double *p, long n, double v) {
+  const long i = (long)blockIdx.x * kBlock + threadIdx.x;
+  if (i < n) p[i] = v;
+}
+
 }  // namespace
 
 enum { kPre = 0, kPost = 1, kFused = 2 };
@@ -392,6 +397,9 @@ struct mam4x_handle_s {
   int *queue = nullptr;
   QueueCtl *ctl = nullptr;
   unsigned soa_grid = 0;  // resident blocks of soa_queue_kernel on the device
+  // adjoint: a plane of ones (the unit seed direction), grown on demand
+  double *ones = nullptr;
+  long ones_cap = 0;
   // phase timing (stats enabled): 4 events per call, recycled through a pool
   std::vector<std::array<hipEvent_t, 4>> pending, pool;
   std::string err;
@@ -586,6 +594,7 @@ int mam4x_destroy(mam4x_handle h) {
   if (h->d_count) (void)hipFree(h->d_count);
   if (h->d_stats) (void)hipFree(h->d_stats);
   if (h->ctl) (void)hipFree(h->ctl);
+  if (h->ones) (void)hipFree(h->ones);
   free_work(h);
   destroy_events(h);
   delete h;
@@ -733,6 +742,64 @@ int mam4x_compute_tangent(mam4x_handle h, double t, double dt, const mam4x_input
   if (h->stats) w.stats = h->d_stats;
   launch_fused<Tangent>(has_pre(in), grid, s, OneDomain{a}, w);
   HIP_TRY(h, hipGetLastError());
+  return MAM4X_OK;
+}
+
+int mam4x_compute_adjoint(mam4x_handle h, double t, double dt, const mam4x_inputs *in,
+                          const mam4x_adjoint *adj, void *stream) {
+  (void)t;
+  if (!h) return MAM4X_ERR_ARG;
+  int rc = check_inputs(h, in);
+  if (rc) return rc;
+  if (!adj) return fail(h, MAM4X_ERR_ARG, "adjoint is NULL");
+  if (!(dt > 0.0)) return fail(h, MAM4X_ERR_ARG, "dt must be > 0");
+  ChainArgs a;
+  fill_args(h, in, a);  // no primal outputs: tend/sout/dgn stay NULL
+  for (int f = 0; f < NSTATE; ++f) a.tbar[f] = adj->tendencies[f];
+  a.dt = dt;
+  hipStream_t s = (hipStream_t)stream;
+  const long ncell = (long)in->ncol * in->nlev;
+  if (ncell == 0) return MAM4X_OK;
+  // extent of the plane offsets: slow extent x ldc
+  const long extent = (long)(in->layout == MAM4X_LAYOUT_COL_LEV ? in->ncol : in->nlev) * a.ldc;
+  if (extent > h->ones_cap) {
+    if (h->ones) {
+      HIP_TRY(h, hipStreamSynchronize(s));
+      (void)hipFree(h->ones);
+      h->ones = nullptr;
+      h->ones_cap = 0;
+    }
+    if (hipMalloc(&h->ones, sizeof(double) * extent) != hipSuccess) {
+      (void)hipGetLastError();
+      h->ones = nullptr;
+      return fail(h, MAM4X_ERR_NOMEM, "adjoint seed plane allocation failed");
+    }
+    h->ones_cap = extent;
+    hipLaunchKernelGGL(fill_kernel, dim3((unsigned)((extent + kBlock - 1) / kBlock)),
+                       dim3(kBlock), 0, s, h->ones, extent, 1.0);
+    HIP_TRY(h, hipGetLastError());
+  }
+  const unsigned grid = (unsigned)((ncell + kBlock - 1) / kBlock);
+  Work w;
+  std::memset(&w, 0, sizeof(w));
+  w.ncell = ncell;
+  if (h->stats) w.stats = h->d_stats;
+  const bool pre = has_pre(in);
+  // one seeded tangent pass per requested gradient plane: state fields, T, p, qv
+  for (int i = 0; i < NSTATE + 3; ++i) {
+    double *xbar = i < NSTATE        ? adj->state[i]
+                   : i == NSTATE     ? adj->temperature
+                   : i == NSTATE + 1 ? adj->pressure
+                                     : adj->vapor_mixing_ratio;
+    if (!xbar) continue;
+    for (int f = 0; f < NSTATE; ++f) a.dstate[f] = f == i ? h->ones : nullptr;
+    a.dT = i == NSTATE ? h->ones : nullptr;
+    a.dp = i == NSTATE + 1 ? h->ones : nullptr;
+    a.dqv = i == NSTATE + 2 ? h->ones : nullptr;
+    a.xbar = xbar;
+    launch_fused<Tangent>(pre, grid, s, OneDomain{a}, w);
+    HIP_TRY(h, hipGetLastError());
+  }
   return MAM4X_OK;
 }
 

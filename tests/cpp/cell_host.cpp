@@ -4,10 +4,6 @@
 // the CPU with glibc, the kernel's restructured FP64 chain must agree with
 // the oracle to round-off.  Test infrastructure only; the library never uses
 // it.
-//
-// Tangent mode: dstate ([52][nlev][ncol]), dT, dp, dqv are the input
-// direction (NULL = 0); dtend receives the directional derivative of the
-// tendencies.
 #include <cmath>
 #include <cstring>
 
@@ -15,13 +11,14 @@
 
 using namespace mam4x;
 
-extern "C" long cell_host_run2(const int *cfgv, int prec, int ncol, int nlev, double dt,
+extern "C" long cell_host_run3(const int *cfgv, int prec, int ncol, int nlev, double dt,
                                const double *state, const double *T, const double *p,
                                const double *z, const double *qv, const double *cld,
                                const double *pblh, const double *dga, const double *dgw,
                                const double *wd, double *tend, double *dgi, double *dgc,
                                const double *dstate, const double *dT, const double *dp,
-                               const double *dqv, double *dtend) {
+                               const double *dqv, double *dtend, const double *tbar,
+                               double *xbar) {
   static ChainConsts K;
   chain_consts_init(&K);
   const long n = (long)ncol * nlev;
@@ -29,18 +26,20 @@ extern "C" long cell_host_run2(const int *cfgv, int prec, int ncol, int nlev, do
   std::memset(&a, 0, sizeof(a));
   for (int f = 0; f < NSTATE; ++f) {
     a.state[f] = state + f * n;
-    a.tend[f] = tend + f * n;
+    if (tend) a.tend[f] = tend + f * n;
     if (dstate) a.dstate[f] = dstate + f * n;
     if (dtend) a.dtend[f] = dtend + f * n;
+    if (tbar) a.tbar[f] = tbar + f * n;
   }
+  a.xbar = xbar;
   a.T = T; a.p = p; a.z = z; a.qv = qv; a.cld = cld; a.pblh = pblh;
   a.dT = dT; a.dp = dp; a.dqv = dqv;
   for (int m = 0; m < 4; ++m) {
     a.dga[m] = dga + m * n;
     a.dgw[m] = dgw + m * n;
     a.wd[m] = wd + m * n;
-    a.dgi[m] = dgi + m * n;
-    a.dgc[m] = dgc + m * n;
+    if (dgi) a.dgi[m] = dgi + m * n;
+    if (dgc) a.dgc[m] = dgc + m * n;
   }
   a.K = &K;
   a.ncol = ncol; a.nlev = nlev; a.ldc = ncol; a.layout = 0;
@@ -48,7 +47,7 @@ extern "C" long cell_host_run2(const int *cfgv, int prec, int ncol, int nlev, do
   a.cfg.do_coag = cfgv[3]; a.cfg.gaexch_h2so4_uptake_optaa = cfgv[4];
   a.cfg.nmono = (unsigned)cfgv[5];
   a.do_calcsize = cfgv[6]; a.do_xfer = cfgv[7]; a.do_adjust = cfgv[8];
-  a.has_dgn = 1; a.has_sout = 0; a.dt = dt;
+  a.has_dgn = dgi || dgc; a.has_sout = 0; a.dt = dt;
   long iters = 0;
   for (int k = 0; k < nlev; ++k)
     for (int i = 0; i < ncol; ++i) {
@@ -58,6 +57,21 @@ extern "C" long cell_host_run2(const int *cfgv, int prec, int ncol, int nlev, do
                            : dev::chain_cell_fused<false, Fp64>(a, off, i);
     }
   return iters;
+}
+
+// Tangent mode: dstate ([52][nlev][ncol]), dT, dp, dqv are the input
+// direction (NULL = 0); dtend receives the directional derivative of the
+// tendencies.  With xbar set, the lane writes sum_f tbar[f] * d(tendency f)
+// to xbar instead (tbar: [52][nlev][ncol], NULL = 0).
+extern "C" long cell_host_run2(const int *cfgv, int prec, int ncol, int nlev, double dt,
+                               const double *state, const double *T, const double *p,
+                               const double *z, const double *qv, const double *cld,
+                               const double *pblh, const double *dga, const double *dgw,
+                               const double *wd, double *tend, double *dgi, double *dgc,
+                               const double *dstate, const double *dT, const double *dp,
+                               const double *dqv, double *dtend) {
+  return cell_host_run3(cfgv, prec, ncol, nlev, dt, state, T, p, z, qv, cld, pblh, dga, dgw, wd,
+                        tend, dgi, dgc, dstate, dT, dp, dqv, dtend, nullptr, nullptr);
 }
 
 extern "C" long cell_host_run(const int *cfgv, int prec, int ncol, int nlev, double dt,

@@ -72,3 +72,48 @@ def test_tangent_matches_central_differences(orc, cellhost, seed):
     assert smooth.mean() > 0.999
     assert np.median(err[smooth]) < 1e-8
     assert (err[smooth] > 1e-4).mean() < 5e-4, np.sort(err[smooth])[-5:]
+
+
+def test_seeded_contraction_matches_tangent(cellhost):
+    """The vector-Jacobian product path of the per-cell code (xbar = sum_f
+    tbar_f d tend_f, the building block of mam4x_compute_adjoint) against the
+    tangent's derivative planes contracted on the host, for unit seeds of
+    single inputs; zero cotangent planes contribute nothing."""
+    import oracle.pyoracle as P
+    sc = scenes.generate(12, 72, seed=31)
+    oc = P.default_config()
+    cfg = (oc.do_cond, oc.do_rename, oc.do_newnuc, oc.do_coag, oc.newnuc_h2so4_conc_optaa,
+           oc.n_so4_monolayers_pcage, oc.do_calcsize, oc.calcsize_do_aitacc_transfer,
+           oc.calcsize_do_adjust)
+    dp = ctypes.POINTER(ctypes.c_double)
+    p = lambda a: a.ctypes.data_as(dp) if a is not None else None
+    st = np.ascontiguousarray(sc["state"])
+    _, nlev, ncol = st.shape
+    arr = [np.ascontiguousarray(sc[k]) for k in ("temperature", "pressure", "height", "qv",
+                                                  "cloud_fraction", "pblh", "dgncur_a",
+                                                  "dgncur_awet", "wetdens")]
+    rng = np.random.default_rng(32)
+    tbar = rng.standard_normal(st.shape)
+    tbar[3] = 0.0
+    ones = np.ones((nlev, ncol))
+    lib = cellhost
+    lib.cell_host_run3.restype = ctypes.c_long
+    for j in (1, 12, 44, "T", "qv"):
+        unit = np.zeros_like(st)
+        v_T = v_qv = None
+        if j == "T":
+            v_T = ones
+        elif j == "qv":
+            v_qv = ones
+        else:
+            unit[j] = 1.0
+        tend, dtend = run_tangent(lib, sc, cfg, unit, v_T, v_qv)
+        xbar = np.full((nlev, ncol), np.nan)
+        lib.cell_host_run3((ctypes.c_int * 9)(*cfg), 2, ncol, nlev, ctypes.c_double(DT), p(st),
+                           *[p(a) for a in arr], None, None, None, p(unit), p(v_T), None,
+                           p(v_qv), None, p(tbar), p(xbar))
+        # the same products summed in the same field order
+        ref = np.zeros((nlev, ncol))
+        for f in range(st.shape[0]):
+            ref = ref + tbar[f] * dtend[f]
+        assert np.array_equal(xbar, ref), j
