@@ -51,6 +51,24 @@ constexpr int kChunk = MAM4X_QUEUE_CHUNK;  // queue entries a wave takes per ref
 #define MAM4X_SOA_PRIO 1
 #endif
 constexpr int kLongBucket = 15;
+// Refills cost a device-scope atomic on a queue counter, and those atomics
+// serialise across the device (tens of ns each), so the kernel keeps their
+// number down: every wave's first chunk is assigned statically (chunk = wave
+// index: the longest cells go to the first-dispatched, oldest waves), and the
+// cells of buckets < kShortBucket (predicted <= ~30 sub-steps) form a second
+// part of the queue taken kShortChunk at a time -- holding a few short cells
+// back in a busy wave costs little.
+#ifndef MAM4X_SHORT_BUCKET
+#define MAM4X_SHORT_BUCKET 10
+#endif
+#ifndef MAM4X_SHORT_CHUNK
+#define MAM4X_SHORT_CHUNK 256
+#endif
+#ifndef MAM4X_STATIC_FIRST
+#define MAM4X_STATIC_FIRST 1
+#endif
+constexpr int kShortBucket = MAM4X_SHORT_BUCKET;
+constexpr int kShortChunk = MAM4X_SHORT_CHUNK;
 constexpr unsigned char kNoSoa = 0xff;
 
 // Workspace: 14 planes of one double per cell.  The set-up kernel writes
@@ -90,8 +108,9 @@ __device__ __forceinline__ void ws_load_in(const double *ws, long n, long c,
 struct QueueCtl {
   unsigned hist[kNBucket];  // cells per bucket
   unsigned fill[kNBucket];  // scatter cursor per bucket
-  unsigned head;            // next free queue slot
-  unsigned pad;
+  unsigned head;            // next free slot of the long part of the queue
+  unsigned pad[31];         // the two counters on separate cache lines
+  unsigned shead;           // next free slot of the short part
 };
 
 struct Work {
@@ -172,7 +191,10 @@ __global__ void queue_reset_kernel(QueueCtl *q) {
     q->hist[t] = 0;
     q->fill[t] = 0;
   }
-  if (t == 0) q->head = 0;
+  if (t == 0) {
+    q->head = 0;
+    q->shead = 0;
+  }
 }
 
 // Phase 1.
@@ -237,21 +259,26 @@ __global__ void __launch_bounds__(kBlock) queue_scatter_kernel(Work w) {
   if (b != kNoSoa) w.queue[base[b] + r] = (int)c;
 }
 
-// Phase 3: persistent SOA loop.  Each wave takes kChunk queue entries at a
-// time; a lane whose cell has converged takes the next entry of the wave's
-// chunk before the following sub-step.  A wave exits once the queue is
-// exhausted and all its lanes are idle; the queue is finite and every cell
-// ends after at most 1000 sub-steps, so every wave exits.
+// Phase 3: persistent SOA loop.  Each wave takes a chunk of queue entries at
+// a time (kChunk from the long part, kShortChunk from the short part); a lane
+// whose cell has converged takes the next entry of the wave's chunk before
+// the following sub-step.  A wave exits once the queue is exhausted and all
+// its lanes are idle; the queue is finite and every cell ends after at most
+// 1000 sub-steps, so every wave exits.
 __global__ void __launch_bounds__(kBlock) soa_queue_kernel(Work w, unsigned long long *stats) {
-  unsigned nq = 0, nlong = 0;
+  unsigned nq = 0, nlong = 0, nsplit = 0;
   for (int j = 0; j < kNBucket; ++j) {
     nq += w.ctl->hist[j];
     if (j >= kLongBucket) nlong += w.ctl->hist[j];
+    if (j >= kShortBucket) nsplit += w.ctl->hist[j];
   }
+  const unsigned nwave = gridDim.x * (kBlock / 64);
+  const unsigned wid = blockIdx.x * (kBlock / 64) + (threadIdx.x >> 6);
+  bool first = MAM4X_STATIC_FIRST != 0;
   const int lane = threadIdx.x & 63;
   const long n = w.ncell;
   unsigned cbeg = 0, cend = 0;  // the wave's current chunk (wave-uniform)
-  bool exhausted = false;
+  bool lexhausted = false, exhausted = false;
   bool active = false;
   int cell = 0;
   dev::SoaState<double> s;
@@ -259,15 +286,31 @@ __global__ void __launch_bounds__(kBlock) soa_queue_kernel(Work w, unsigned long
   while (true) {
     const unsigned long long need = __ballot(!active);
     if (need && cbeg == cend && !exhausted) {
-      unsigned b0 = 0;
-      if (lane == 0) b0 = atomicAdd(&w.ctl->head, (unsigned)kChunk);
-      b0 = __builtin_amdgcn_readfirstlane(b0);
+      // long part: the first chunk of every wave is assigned statically
+      // (chunk wid), later ones through the counter; then the short part
+      unsigned b0 = nsplit, e0 = nsplit;
+      if (first) {
+        first = false;
+        b0 = wid * (unsigned)kChunk;
+      } else if (!lexhausted) {
+        if (lane == 0) b0 = atomicAdd(&w.ctl->head, (unsigned)kChunk);
+        b0 = (MAM4X_STATIC_FIRST ? nwave * (unsigned)kChunk : 0u) +
+             __builtin_amdgcn_readfirstlane(b0);
+      }
+      if (b0 < nsplit) {
+        e0 = min(b0 + (unsigned)kChunk, nsplit);
+      } else {
+        lexhausted = true;
+        if (lane == 0) b0 = atomicAdd(&w.ctl->shead, (unsigned)kShortChunk);
+        b0 = nsplit + __builtin_amdgcn_readfirstlane(b0);
+        e0 = min(b0 + (unsigned)kShortChunk, nq);
+      }
       if (MAM4X_SOA_PRIO && b0 < nlong) __builtin_amdgcn_s_setprio(2);
       if (b0 >= nq) {
         exhausted = true;
       } else {
         cbeg = b0;
-        cend = min(b0 + (unsigned)kChunk, nq);
+        cend = e0;
       }
     }
     if (need && cbeg < cend) {
