@@ -40,7 +40,7 @@
 extern "C" {
 #endif
 
-#define MAM4X_ABI_VERSION 3
+#define MAM4X_ABI_VERSION 4
 
 /* status codes */
 #define MAM4X_OK 0
@@ -168,6 +168,14 @@ typedef struct mam4x_adjoint {
   double *vapor_mixing_ratio;              /* out: gradient w.r.t. qv        */
 } mam4x_adjoint;
 
+/* Outputs of the aerosol water uptake (mam4x_compute_water_uptake), one
+ * plane per mode (mode order of the state). */
+typedef struct mam4x_water_outputs {
+  double *dgncur_awet[4]; /* wet geometric mean diameter [m]           */
+  double *qaerwat[4];     /* aerosol water mass mixing ratio [kg/kg]   */
+  double *wetdens[4];     /* wet density [kg/m3] (NULL: not formed)    */
+} mam4x_water_outputs;
+
 typedef struct mam4x_handle_s *mam4x_handle;
 
 /* Library / handle lifetime.  mam4x_create replaces AeroProcess<Impl>'s
@@ -215,6 +223,20 @@ int mam4x_compute_tangent(mam4x_handle h, double t, double dt, const mam4x_input
  * handle must be stream-ordered. */
 int mam4x_compute_adjoint(mam4x_handle h, double t, double dt, const mam4x_inputs *in,
                           const mam4x_adjoint *adj, void *stream);
+
+/* Aerosol water uptake, the producer of the step's dgncur_awet / wetdens
+ * inputs: modal_aero_water_uptake_dr with the wet density
+ * (water_uptake.hpp:481-505; its callers are compute_calcsize_and_water_
+ * uptake_dr, modal_aero_opt.hpp:510-533, and wet deposition) for every cell.
+ * Reads the interstitial mass planes of in->state (kg/kg), temperature,
+ * pressure, vapor_mixing_ratio (kg/kg), cloud_fraction and the dry
+ * diameters in->dgncur_a; the other input planes may be NULL.  Hygroscopic
+ * growth follows Kohler theory (closed-form quartic, as the reference), with
+ * the clear-air RH from the Goff & Gratch saturation humidity and the
+ * crystallisation / deliquescence hysteresis.  FP64; stream-ordered; no
+ * workspace. */
+int mam4x_compute_water_uptake(mam4x_handle h, const mam4x_inputs *in,
+                               const mam4x_water_outputs *out, void *stream);
 
 /* Batches: many scenes (domains of any column count, each with its own
  * inputs, outputs and layout) stepped by one set of launches.  The scenes'

@@ -6,7 +6,7 @@ The structures below must match the C declarations field for field; the
 import ctypes
 import os
 
-ABI_VERSION = 3
+ABI_VERSION = 4
 
 NSTATE = 52
 NMODE = 4
@@ -94,6 +94,14 @@ class Adjoint(ctypes.Structure):
     ]
 
 
+class WaterOutputs(ctypes.Structure):
+    _fields_ = [
+        ("dgncur_awet", _vp * 4),
+        ("qaerwat", _vp * 4),
+        ("wetdens", _vp * 4),
+    ]
+
+
 class Outputs(ctypes.Structure):
     _fields_ = [
         ("tendencies", _vp * NSTATE),
@@ -123,6 +131,8 @@ SIGNATURES = {
     "mam4x_compute_adjoint": (ctypes.c_int, [_vp, ctypes.c_double, ctypes.c_double,
                                              ctypes.POINTER(Inputs), ctypes.POINTER(Adjoint),
                                              _vp]),
+    "mam4x_compute_water_uptake": (ctypes.c_int, [_vp, ctypes.POINTER(Inputs),
+                                                  ctypes.POINTER(WaterOutputs), _vp]),
     "mam4x_batch_create": (ctypes.c_int, [_vp, ctypes.c_int, ctypes.POINTER(Inputs),
                                           ctypes.POINTER(Outputs), ctypes.c_double,
                                           ctypes.POINTER(_vp)]),

@@ -208,6 +208,29 @@ class Chain:
                                             ctypes.c_void_p(s.cuda_stream))
         self._check(rc, "mam4x_compute_adjoint")
 
+    def compute_water_uptake(self, scene, dgncur_awet, qaerwat, wetdens=None, dgncur_a=None,
+                             stream=None, layout=_abi.LAYOUT_LEV_COL):
+        """Aerosol water uptake through mam4x_compute_water_uptake: writes the
+        wet diameters, aerosol water and (optionally) wet density, [4, ...]
+        each, from the scene's interstitial masses, T, p, qv, cloud fraction
+        and dry diameters (dgncur_a overrides the scene's)."""
+        if dgncur_a is not None:
+            scene = dict(scene)
+            scene["dgncur_a"] = dgncur_a
+        inp = self.make_inputs(scene, None, None, layout)
+        out = _abi.WaterOutputs()
+        for m, p in enumerate(_plane_ptrs(dgncur_awet, 4)):
+            out.dgncur_awet[m] = p
+        for m, p in enumerate(_plane_ptrs(qaerwat, 4)):
+            out.qaerwat[m] = p
+        if wetdens is not None:
+            for m, p in enumerate(_plane_ptrs(wetdens, 4)):
+                out.wetdens[m] = p
+        s = stream if stream is not None else torch.cuda.current_stream()
+        rc = self.lib.mam4x_compute_water_uptake(self.h, ctypes.byref(inp), ctypes.byref(out),
+                                                 ctypes.c_void_p(s.cuda_stream))
+        self._check(rc, "mam4x_compute_water_uptake")
+
     def validate(self, scene, stream=None):
         inp = self.make_inputs(scene)
         v = ctypes.c_longlong(0)

@@ -32,6 +32,7 @@
 
 #include "../../include/mam4x.h"
 #include "chain_cell.hpp"
+#include "water_uptake.hpp"
 
 using namespace mam4x;
 
@@ -393,6 +394,40 @@ __global__ void __launch_bounds__(kBlock) validate_kernel(ChainArgs a,
 __global__ void __launch_bounds__(kBlock) fill_kernel(double *p, long n, double v) {
   const long i = (long)blockIdx.x * kBlock + threadIdx.x;
   if (i < n) p[i] = v;
+}
+
+// Aerosol water uptake (water_uptake.hpp): one lane per cell.
+struct WaterArgs {
+  const double *qi[21];  // interstitial mass planes, state order
+  const double *T, *p, *qv, *cld, *dga[4];
+  double *dgw[4], *qw[4], *wd[4];
+  const ChainConsts *K;
+  int ncol, nlev, ldc, layout;
+};
+
+__global__ void __launch_bounds__(kBlock) water_uptake_kernel(WaterArgs a) {
+  const long c = (long)blockIdx.x * kBlock + threadIdx.x;
+  if (c >= (long)a.ncol * a.nlev) return;
+  long off;
+  if (a.layout == MAM4X_LAYOUT_COL_LEV) {
+    const int icol = (int)(c / a.nlev);
+    off = (long)icol * a.ldc + (c - (long)icol * a.nlev);
+  } else {
+    const int k = (int)(c / a.ncol);
+    off = (long)k * a.ldc + (c - (long)k * a.ncol);
+  }
+  double dga[4];
+#pragma unroll
+  for (int m = 0; m < 4; ++m) dga[m] = a.dga[m][off];
+  wu::WaterOut o;
+  wu::water_uptake_cell([&](int m, int s) { return a.qi[MODE_OFF[m] + s][off]; }, NSPEC_MODE,
+                        a.K->vol_from_dgn, a.T[off], a.p[off], a.qv[off], a.cld[off], dga, o);
+#pragma unroll
+  for (int m = 0; m < 4; ++m) {
+    a.dgw[m][off] = o.dgncur_awet[m];
+    a.qw[m][off] = o.qaerwat[m];
+    if (a.wd[m]) a.wd[m][off] = o.wetdens[m];
+  }
 }
 
 }  // namespace
@@ -843,6 +878,52 @@ int mam4x_compute_adjoint(mam4x_handle h, double t, double dt, const mam4x_input
     launch_fused<Tangent>(pre, grid, s, OneDomain{a}, w);
     HIP_TRY(h, hipGetLastError());
   }
+  return MAM4X_OK;
+}
+
+int mam4x_compute_water_uptake(mam4x_handle h, const mam4x_inputs *in,
+                               const mam4x_water_outputs *out, void *stream) {
+  if (!h) return MAM4X_ERR_ARG;
+  if (!in || !out) return fail(h, MAM4X_ERR_ARG, "inputs or outputs is NULL");
+  if (in->ncol < 0 || in->nlev < 0) return fail(h, MAM4X_ERR_ARG, "negative ncol or nlev");
+  if (in->layout != MAM4X_LAYOUT_LEV_COL && in->layout != MAM4X_LAYOUT_COL_LEV)
+    return fail(h, MAM4X_ERR_ARG, "unknown layout");
+  const bool lc = in->layout == MAM4X_LAYOUT_LEV_COL;
+  const int fast = lc ? in->ncol : in->nlev, slow = lc ? in->nlev : in->ncol;
+  const int ld = in->ldc ? in->ldc : fast;
+  if (ld < fast) return fail(h, MAM4X_ERR_ARG, "ldc smaller than the fastest extent");
+  if ((long long)ld * slow > 0x7fffffffLL)
+    return fail(h, MAM4X_ERR_ARG, "domain too large (ldc x slow extent must fit in int32)");
+  WaterArgs a;
+  std::memset(&a, 0, sizeof(a));
+  for (int r = 0; r < 21; ++r) {
+    a.qi[r] = in->state[8 + r];
+    if (!a.qi[r]) return fail(h, MAM4X_ERR_ARG, "interstitial mass plane " + std::to_string(8 + r) + " is NULL");
+  }
+  a.T = in->temperature;
+  a.p = in->pressure;
+  a.qv = in->vapor_mixing_ratio;
+  a.cld = in->cloud_fraction;
+  if (!a.T || !a.p || !a.qv || !a.cld)
+    return fail(h, MAM4X_ERR_ARG, "temperature, pressure, vapour or cloud fraction plane is NULL");
+  for (int m = 0; m < 4; ++m) {
+    a.dga[m] = in->dgncur_a[m];
+    a.dgw[m] = out->dgncur_awet[m];
+    a.qw[m] = out->qaerwat[m];
+    a.wd[m] = out->wetdens[m];
+    if (!a.dga[m] || !a.dgw[m] || !a.qw[m])
+      return fail(h, MAM4X_ERR_ARG, "a dgncur_a, dgncur_awet or qaerwat plane is NULL");
+  }
+  a.K = h->d_consts;
+  a.ncol = in->ncol;
+  a.nlev = in->nlev;
+  a.ldc = ld;
+  a.layout = in->layout;
+  const long ncell = (long)in->ncol * in->nlev;
+  if (ncell == 0) return MAM4X_OK;
+  hipLaunchKernelGGL(water_uptake_kernel, dim3((unsigned)((ncell + kBlock - 1) / kBlock)),
+                     dim3(kBlock), 0, (hipStream_t)stream, a);
+  HIP_TRY(h, hipGetLastError());
   return MAM4X_OK;
 }
 

@@ -2428,3 +2428,268 @@ long orc_chain_step(const orc_config *cfg, int ncol, int nlev, double dt,
   }
   return total_iter;
 }
+
+/* ======================= water uptake ===================================== */
+/* modal_aero_water_uptake_dr (water_uptake.hpp:481-505) and the functions it
+ * calls, per cell.  The reference's complex arithmetic is Kokkos::complex
+ * (Kokkos, a dependency not vendored in the reference tree); the helpers
+ * below restate its published algorithms:
+ *   z1 * z2   (a c - b d, a d + b c)
+ *   z1 / z2   both scaled by s = |re(z2)| + |im(z2)|, then
+ *             (z1/s) * conj(z2/s) / |z2/s|^2
+ *   sqrt(z)   re == 0: t = sqrt(|im|/2), (t, sign(im) t); else
+ *             t = sqrt(2 (|z| + |re|)), u = t/2:
+ *             re > 0 ? (u, im/t) : (|im|/t, sign(im) u)
+ *   pow(z, y) polar(|z|^y, y atan2(im, re)),  |z| = hypot(re, im)
+ * The restatement is pinned by the quartic residual and the reference's own
+ * Mathematica roots of the Kohler polynomial (tests/test_water_uptake.py). */
+/* Kokkos::max/min and utils::min_max_bound (mam4_math.hpp:33-34,
+ * utils.hpp:24-26) */
+static double wu_max(double x, double y) { return x < y ? y : x; }
+static double wu_min(double x, double y) { return y < x ? y : x; }
+static double wu_mmb(double lo, double hi, double v) { return wu_max(lo, wu_min(hi, v)); }
+
+typedef struct {
+  double re, im;
+} cplx;
+static cplx cx(double re, double im) {
+  cplx z = {re, im};
+  return z;
+}
+static cplx c_add(cplx a, cplx b) { return cx(a.re + b.re, a.im + b.im); }
+static cplx c_sub(cplx a, cplx b) { return cx(a.re - b.re, a.im - b.im); }
+static cplx c_mul(cplx a, cplx b) {
+  return cx(a.re * b.re - a.im * b.im, a.re * b.im + a.im * b.re);
+}
+static cplx c_divr(cplx a, double r) { return cx(a.re / r, a.im / r); }
+static cplx c_div(cplx x, cplx y) {
+  const double s = fabs(y.re) + fabs(y.im);
+  const cplx xs = cx(x.re / s, x.im / s);
+  const cplx ys = cx(y.re / s, -y.im / s);
+  const double a = ys.re * ys.re + ys.im * ys.im;
+  return c_divr(c_mul(xs, ys), a);
+}
+static double c_abs(cplx z) { return hypot(z.re, z.im); }
+static cplx c_sqrt(cplx z) {
+  const double r = z.re, i = z.im;
+  if (r == 0.0) {
+    const double t = M_sqrt(fabs(i) / 2);
+    return cx(t, i < 0.0 ? -t : t);
+  }
+  const double t = M_sqrt(2 * (c_abs(z) + fabs(r)));
+  const double u = t / 2;
+  return r > 0.0 ? cx(u, i / t) : cx(fabs(i) / t, i < 0.0 ? -u : u);
+}
+static cplx c_pow(cplx z, double y) {
+  const double rho = M_pow(c_abs(z), y), th = y * atan2(z.im, z.re);
+  return cx(rho * cos(th), rho * sin(th));
+}
+
+/* makoh_quartic (water_uptake.hpp:124-169): roots of
+ * x^4 + p3 x^3 + p2 x^2 + p1 x + p0 */
+void orc_makoh_quartic(double p3, double p2, double p1, double p0, double re[4],
+                       double im[4]) {
+  cplx r[4];
+  const double qq = -p2 * p2 / 36.0 + (p3 * p1 - 4.0 * p0) / 12.0;
+  const double rr = -cube(p2 / 6.0) + p2 * (p3 * p1 - 4.0 * p0) / 48.0 +
+                    (4.0 * p0 * p2 - p0 * p3 * p3 - p1 * p1) / 16.0;
+  cplx crad = c_sqrt(cx(rr * rr + qq * qq * qq, 0.0));
+  cplx cb = c_sub(cx(rr, 0.0), crad);
+  if (cb.re == 0.0 && cb.im == 0.0) { /* insoluble particle */
+    const double c = M_cbrt(-p1);
+    for (int n = 0; n < 4; ++n) r[n] = cx(c, 0.0);
+  } else {
+    cb = c_pow(cb, 1.0 / 3.0);
+    const cplx cy = c_add(c_add(cx(-cb.re, -cb.im), c_div(cx(qq, 0.0), cb)), cx(p2 / 6.0, 0.0));
+    const cplx cb0 = c_sqrt(c_sub(c_mul(cy, cy), cx(p0, 0.0)));
+    const cplx cb1 = c_div(c_sub(c_mul(cx(p3, 0.0), cy), cx(p1, 0.0)), c_mul(cx(2.0, 0.0), cb0));
+    cb = c_add(cx(p3 / 2.0, 0.0), cb1);
+    crad = c_sqrt(c_sub(c_mul(cb, cb), c_mul(cx(4.0, 0.0), c_add(cy, cb0))));
+    r[0] = c_divr(c_add(cx(-cb.re, -cb.im), crad), 2.0);
+    r[1] = c_divr(c_sub(cx(-cb.re, -cb.im), crad), 2.0);
+    cb = c_sub(cx(p3 / 2.0, 0.0), cb1);
+    crad = c_sqrt(c_sub(c_mul(cb, cb), c_mul(cx(4.0, 0.0), c_sub(cy, cb0))));
+    r[2] = c_divr(c_add(cx(-cb.re, -cb.im), crad), 2.0);
+    r[3] = c_divr(c_sub(cx(-cb.re, -cb.im), crad), 2.0);
+  }
+  for (int n = 0; n < 4; ++n) {
+    re[n] = r[n].re;
+    im[n] = r[n].im;
+  }
+}
+
+#define WU_EPS 1e-4 /* Water_Uptake::eps (water_uptake.hpp:38) */
+
+/* find_real_solution (water_uptake.hpp:93-117): the smallest real root in
+ * [rdry (1 - eps), 1000 rdry] */
+static double find_real_solution(double rdry, const double re[4], const double im[4]) {
+  double rwet = 1000.0 * rdry;
+  for (int n = 0; n < 4; ++n) {
+    const double xr = re[n], xi = im[n];
+    if (fabs(xi) > fabs(xr) * WU_EPS) continue;
+    if (xr > rwet) continue;
+    if (xr < rdry * (1.0 - WU_EPS)) continue;
+    if (isnan(xr)) continue;
+    rwet = xr;
+  }
+  return rwet;
+}
+
+/* Kohler quartic solve for a given Kelvin coefficient aa [micron]:
+ * the body of modal_aero_kohler (water_uptake.hpp:178-229) from the
+ * coefficients on; rdry [micron], returns rwet [micron] before the bound. */
+double orc_kohler_root(double aa, double rdry, double hygro, double rh) {
+  const double vol = cube(rdry);
+  const double bb = vol * hygro;
+  const double ss = wu_mmb(1.e-10, 1.0 - WU_EPS, rh);
+  const double slog = M_log(ss);
+  const double p43 = -aa / slog;
+  const double p42 = 0.0;
+  const double p41 = bb / slog - vol;
+  const double p40 = aa * vol / slog;
+  const double pp = fabs(-bb / aa) / (rdry * rdry);
+  if (pp < WU_EPS) /* approximate solution for small particles */
+    return rdry * (1.0 + pp * (1.0 / 3.0) / (1.0 - slog * rdry / aa));
+  double re[4], im[4];
+  orc_makoh_quartic(p43, p42, p41, p40, re, im);
+  return find_real_solution(rdry, re, im);
+}
+
+/* modal_aero_kohler (water_uptake.hpp:178-229): rdry_in [m] -> rwet [m] */
+static double modal_aero_kohler(double rdry_in, double hygro, double rh) {
+  const double rhow = 1.0, surften = 76.0, mw = 18.0, tair = 273.0, ugascon = 8.3e7;
+  const double aa = 2.0e4 * mw * surften / (ugascon * tair * rhow);
+  double rwet = orc_kohler_root(aa, rdry_in * 1.e6, hygro, rh);
+  rwet = wu_min(rwet, 30.0); /* rmax: upper bound based on 1 day lifetime */
+  return rwet * 1.e-6;
+}
+
+/* get_e3sm_parameters (water_uptake.hpp:312-360): E3SM species type (1-based)
+ * of the ispec-th species of each mode, and density / hygroscopicity by type.
+ * The ispec-th E3SM species of a mode is the ispec-th species of the mode in
+ * mode_aero_species order (state_q layout, utils.hpp:224-258). */
+static const int WU_LSPECTYPE[4][7] = {{1, 4, 5, 6, 8, 7, 9},
+                                       {1, 5, 7, 9, 0, 0, 0},
+                                       {8, 7, 1, 6, 4, 5, 9},
+                                       {4, 6, 9, 0, 0, 0, 0}};
+/* types 2 and 3 are unused by the four modes; their entries are written
+ * "0.1797693135 + 309" in the reference */
+static const double WU_SPECDENS[9] = {1770.0, 0.1797693135 + 309, 0.1797693135 + 309,
+                                      1000.0, 1000.0, 1700.0, 1900.0, 2600.0, 1601.0};
+static const double WU_SPECHYGRO[9] = {0.507, 0.1797693135 + 309, 0.1797693135 + 309,
+                                       0.1000000083e-09, 0.14, 0.1000000013e-09,
+                                       1.16, 0.068, 0.1000000015};
+#define WU_RHCRYSTAL 0.35 /* aero_modes.hpp:76-77 */
+#define WU_RHDELIQUES 0.8
+
+/* wv_sat_qsat_water (wv_sat_methods.hpp:13-133): Goff & Gratch SVP over
+ * water, then saturation specific humidity */
+static void qsat_water(double t, double p, double *es, double *qs) {
+  const double tboil = 373.16, ten = 10, one = 1;
+  const double e = M_pow(ten, -7.90298 * (tboil / t - one) + 5.02808 * log10(tboil / t) -
+                                 1.3816e-7 * (M_pow(ten, 11.344 * (one - t / tboil)) - one) +
+                                 8.1328e-3 * (M_pow(ten, -3.49149 * (tboil / t - one)) - one) +
+                                 log10(1013.246)) *
+                   ten * ten;
+  const double epsilo = MW_H2O / MW_DRY_AIR;
+  const double omeps = one - epsilo;
+  *qs = (p - e) <= 0.0 ? one : epsilo * e / (p - omeps * e);
+  *es = wu_min(e, p);
+}
+
+void orc_qsat_water(double t, double p, double *es, double *qs) { qsat_water(t, p, es, qs); }
+
+/* One cell of modal_aero_water_uptake_dr with wet density
+ * (water_uptake.hpp:362-505).  qaer[m][s]: interstitial mass mixing ratios
+ * [kg/kg] in mode_aero_species order; qnum is not used by the reference (the
+ * number comes from the dry volume and dgncur_a).  qv [kg/kg]. */
+void orc_water_uptake_cell(const double qaer[4][7], double temperature, double pmid,
+                           double qv, double cldn, const double dgncur_a[4],
+                           double dgncur_awet[4], double qaerwat[4], double wetdens[4]) {
+  double hygro[4], naer[4], dryrad[4], dryvol[4], drymass[4], specdens_1[4];
+  /* modal_aero_water_uptake_dryaer (:362-437) */
+  for (int m = 0; m < 4; ++m) {
+    double dryvolmr = 0.0, maer = 0.0;
+    hygro[m] = 0.0;
+    const double alnsg = M_log(MODE_SIGMA[m]);
+    specdens_1[m] = WU_SPECDENS[WU_LSPECTYPE[m][0] - 1];
+    const double spechygro_1 = WU_SPECHYGRO[WU_LSPECTYPE[m][0] - 1];
+    for (int s = 0; s < NSPEC_MODE[m]; ++s) {
+      const int t = WU_LSPECTYPE[m][s] - 1;
+      const double raer = qaer[m][s];
+      const double vol_tmp = raer / WU_SPECDENS[t];
+      maer = maer + raer;
+      dryvolmr += vol_tmp;
+      hygro[m] += vol_tmp * WU_SPECHYGRO[t];
+    }
+    hygro[m] = dryvolmr > 1e-30 ? hygro[m] / dryvolmr : spechygro_1;
+    const double v2ncur_a = 1.0 / ((PI / 6.0) * cube(dgncur_a[m]) * M_exp(4.5 * sq(alnsg)));
+    naer[m] = dryvolmr * v2ncur_a;
+    const double drydens = maer > 1e-31 ? maer / dryvolmr : 1.0;
+    dryvol[m] = 1.0 / v2ncur_a;
+    drymass[m] = drydens * dryvol[m];
+    dryrad[m] = M_cbrt(dryvol[m] / (PI * 4.0 / 3.0));
+  }
+  /* modal_aero_water_uptake_rh_clearair (:259-285) */
+  double es, qs, rh;
+  qsat_water(temperature, pmid, &es, &qs);
+  rh = qs > qv ? qv / qs : 0.98;
+  rh = wu_mmb(0.0, 0.98, rh);
+  if (cldn < 1.0) rh = (rh - cldn) / (1.0 - cldn);
+  rh = wu_max(rh, 0.0);
+  /* modal_aero_water_uptake_wetaer (:241-257) */
+  double wetvol[4], wtrvol[4];
+  for (int m = 0; m < 4; ++m) {
+    const double hystfac = 1.0 / wu_max(1.0e-5, WU_RHDELIQUES - WU_RHCRYSTAL);
+    double wetrad = modal_aero_kohler(dryrad[m], hygro[m], rh);
+    wetrad = wu_max(wetrad, dryrad[m]);
+    wetvol[m] = (PI * 4.0 / 3.0) * cube(wetrad);
+    wetvol[m] = wu_max(wetvol[m], dryvol[m]);
+    wtrvol[m] = wetvol[m] - dryvol[m];
+    wtrvol[m] = wu_max(wtrvol[m], 0.0);
+    if (rh < WU_RHCRYSTAL) {
+      wetrad = dryrad[m];
+      wetvol[m] = dryvol[m];
+      wtrvol[m] = 0.0;
+    } else if (rh < WU_RHDELIQUES) {
+      wtrvol[m] = wtrvol[m] * hystfac * (rh - WU_RHCRYSTAL);
+      wtrvol[m] = wu_max(wtrvol[m], 0.0);
+      wetvol[m] = dryvol[m] + wtrvol[m];
+      wetrad = M_cbrt(wetvol[m] / (4.0 / 3.0 * PI));
+    }
+    dgncur_awet[m] = dgncur_a[m] * (wetrad / dryrad[m]);
+    qaerwat[m] = 1.0e3 * naer[m] * wtrvol[m];
+  }
+  /* modal_aero_water_uptake_wetdens (:67-90) */
+  for (int m = 0; m < 4; ++m)
+    wetdens[m] = wetvol[m] > 1.0e-30 ? (drymass[m] + 1.0e3 * wtrvol[m]) / wetvol[m]
+                                     : specdens_1[m];
+}
+
+/* Water uptake over [field][level][column] buffers (column fastest): state
+ * as for orc_chain_step (only the interstitial mass fields are read),
+ * dgncur_a [4][nlev][ncol]; outputs [4][nlev][ncol] each (wetdens may be
+ * NULL). */
+void orc_water_uptake(int ncol, int nlev, const double *state, const double *temperature,
+                      const double *pressure, const double *qv, const double *cloud_fraction,
+                      const double *dgncur_a, double *dgncur_awet, double *qaerwat,
+                      double *wetdens) {
+  const long ncell = (long)ncol * nlev;
+#ifdef _OPENMP
+#pragma omp parallel for schedule(static)
+#endif
+  for (long c = 0; c < ncell; ++c) {
+    double qaer[4][7] = {{0}}, dga[4], dgw[4], qw[4], wd[4];
+    for (int m = 0; m < 4; ++m) {
+      dga[m] = dgncur_a[m * ncell + c];
+      for (int s = 0; s < NSPEC_MODE[m]; ++s) qaer[m][s] = state[(long)F_QI(m, s) * ncell + c];
+    }
+    orc_water_uptake_cell((const double(*)[7])qaer, temperature[c], pressure[c], qv[c],
+                          cloud_fraction[c], dga, dgw, qw, wd);
+    for (int m = 0; m < 4; ++m) {
+      dgncur_awet[m * ncell + c] = dgw[m];
+      qaerwat[m * ncell + c] = qw[m];
+      if (wetdens) wetdens[m * ncell + c] = wd[m];
+    }
+  }
+}

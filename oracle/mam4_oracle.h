@@ -141,6 +141,40 @@ void orc_pcarbon_aging_1subarea(unsigned nmono, const double dgn_a[4],
 double orc_mean_particle_diameter_from_volume(double vol, double sigma);
 double orc_mean_particle_volume_from_diameter(double dgn, double sigma);
 
+/* ---- water uptake (water_uptake.hpp) ---- */
+
+/* makoh_quartic (water_uptake.hpp:124-169): the four complex roots of
+ * x^4 + p3 x^3 + p2 x^2 + p1 x + p0. */
+void orc_makoh_quartic(double p3, double p2, double p1, double p0, double re[4],
+                       double im[4]);
+
+/* The Kohler solve of modal_aero_kohler (water_uptake.hpp:178-229) for a
+ * given Kelvin coefficient aa [micron]: dry radius [micron] -> wet radius
+ * [micron] (small-particle approximation or quartic + find_real_solution),
+ * before the 30-micron bound. */
+double orc_kohler_root(double aa, double rdry, double hygro, double rh);
+
+/* wv_sat_qsat_water (wv_sat_methods.hpp:114-133): Goff & Gratch saturation
+ * vapour pressure over water es [Pa] and saturation specific humidity qs. */
+void orc_qsat_water(double t, double p, double *es, double *qs);
+
+/* One cell of modal_aero_water_uptake_dr with wet density
+ * (water_uptake.hpp:362-505): interstitial mass mixing ratios [kg/kg] per
+ * mode in mode_aero_species order, T [K], p [Pa], qv [kg/kg], cloud
+ * fraction, dry diameters [m] -> wet diameters [m], aerosol water [kg/kg],
+ * wet density [kg/m3]. */
+void orc_water_uptake_cell(const double qaer[4][7], double temperature, double pmid,
+                           double qv, double cldn, const double dgncur_a[4],
+                           double dgncur_awet[4], double qaerwat[4], double wetdens[4]);
+
+/* orc_water_uptake_cell over [field][level][column] buffers (state as for
+ * orc_chain_step; dgncur_a and the outputs [4][nlev][ncol]; wetdens may be
+ * NULL). */
+void orc_water_uptake(int ncol, int nlev, const double *state, const double *temperature,
+                      const double *pressure, const double *qv, const double *cloud_fraction,
+                      const double *dgncur_a, double *dgncur_awet, double *qaerwat,
+                      double *wetdens);
+
 #ifdef __cplusplus
 }
 #endif

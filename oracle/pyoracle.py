@@ -93,6 +93,16 @@ class Oracle:
         L.orc_soaexch.argtypes = [ctypes.c_double] * 3 + [_dp, _dp, _dp, _dp, _dp]
         L.orc_set_libm_noise.argtypes = [ctypes.c_ulonglong]
         L.orc_set_libm_noise.restype = None
+        L.orc_makoh_quartic.argtypes = [ctypes.c_double] * 4 + [_dp, _dp]
+        L.orc_makoh_quartic.restype = None
+        L.orc_kohler_root.argtypes = [ctypes.c_double] * 4
+        L.orc_kohler_root.restype = ctypes.c_double
+        L.orc_water_uptake_cell.argtypes = [_dp] + [ctypes.c_double] * 4 + [_dp] * 4
+        L.orc_water_uptake_cell.restype = None
+        L.orc_water_uptake.argtypes = [ctypes.c_int, ctypes.c_int] + [_dp] * 9
+        L.orc_water_uptake.restype = None
+        L.orc_qsat_water.argtypes = [ctypes.c_double, ctypes.c_double, _dp, _dp]
+        L.orc_qsat_water.restype = None
 
     def set_libm_noise(self, seed):
         """seed != 0: randomly move each exp/log/pow/cbrt/erfc result by one
@@ -146,6 +156,43 @@ class Oracle:
         return tend, dgn_i, dgn_c, int(iters)
 
     # ---- process-level helpers for the known-answer tests ----
+    def makoh_quartic(self, p3, p2, p1, p0):
+        """The four complex roots of x^4 + p3 x^3 + p2 x^2 + p1 x + p0."""
+        re, im = np.zeros(4), np.zeros(4)
+        self.lib.orc_makoh_quartic(p3, p2, p1, p0, _p(re), _p(im))
+        return re + 1j * im
+
+    def kohler_root(self, aa, rdry, hygro, rh):
+        return self.lib.orc_kohler_root(aa, rdry, hygro, rh)
+
+    def qsat_water(self, T, p):
+        """(es [Pa], qs) of Goff & Gratch over water."""
+        es, qs = np.zeros(1), np.zeros(1)
+        self.lib.orc_qsat_water(T, p, _p(es), _p(qs))
+        return es[0], qs[0]
+
+    def water_uptake_cell(self, qaer, T, p, qv, cld, dgncur_a):
+        """One cell: qaer [4][7] kg/kg -> (dgncur_awet[4], qaerwat[4], wetdens[4])."""
+        qa = np.ascontiguousarray(qaer, dtype=np.float64).reshape(4, 7)
+        dga = np.ascontiguousarray(dgncur_a, dtype=np.float64)
+        out = [np.zeros(4) for _ in range(3)]
+        self.lib.orc_water_uptake_cell(_p(qa), T, p, qv, cld, _p(dga), *[_p(o) for o in out])
+        return tuple(out)
+
+    def water_uptake(self, scene, dgncur_a=None):
+        """Water uptake over a scene dict ([field][level][column] arrays);
+        dgncur_a defaults to the scene's.  Returns (dgncur_awet, qaerwat,
+        wetdens), each [4, nlev, ncol]."""
+        st = np.ascontiguousarray(scene["state"])
+        _, nlev, ncol = st.shape
+        dga = np.ascontiguousarray(scene["dgncur_a"] if dgncur_a is None else dgncur_a)
+        out = [np.empty((4, nlev, ncol)) for _ in range(3)]
+        c = lambda k: np.ascontiguousarray(scene[k])
+        self.lib.orc_water_uptake(ncol, nlev, _p(st), _p(c("temperature")), _p(c("pressure")),
+                                  _p(c("qv")), _p(c("cloud_fraction")), _p(dga),
+                                  *[_p(o) for o in out])
+        return tuple(out)
+
     def soaexch(self, dt, temp, pmid, q_soag, soa, pom, uptk_soag):
         """SOA exchange of one subarea; returns (q_soag, q_soag_avg, soa[4], niter)."""
         g = np.array([q_soag], dtype=np.float64)
