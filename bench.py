@@ -215,9 +215,12 @@ def bench_large(args):
 
 
 def bench_water(args):
-    """Aerosol water uptake (mam4x_compute_water_uptake) over the 21,600 x 72
-    scene: kernel time from HIP events on the launch stream, the HBM
-    roofline of its compulsory traffic, and the oracle on the host cores."""
+    """Aerosol water uptake (mam4x_compute_water_uptake; --workload water) or
+    the wet-size producer, E3SM calcsize then water uptake
+    (mam4x_compute_wet_size; --workload wetsize), over the 21,600 x 72 scene:
+    kernel time from HIP events on the launch stream, the HBM roofline of its
+    compulsory traffic, and the oracle on the host cores."""
+    wet = args.workload == "wetsize"
     import numpy as np
     import torch
     import ctypes
@@ -230,17 +233,27 @@ def bench_water(args):
     shape = (4, NLEV, NCOL)
     out = [torch.empty(shape, dtype=torch.float64, device="cuda") for _ in range(3)]
     ch = Chain()
-    ch.compute_water_uptake(d, out[0], out[1], out[2])  # checks the arguments once
-    # the argument blocks are built once, so the timed loop is one C call per
-    # launch and the stream stays busy
     s = torch.cuda.current_stream()
     inp = ch.make_inputs(d)
-    wo = _abi.WaterOutputs()
-    for m in range(4):
-        wo.dgncur_awet[m], wo.qaerwat[m], wo.wetdens[m] = (o[m].data_ptr() for o in out)
-    fn, h, sp = ch.lib.mam4x_compute_water_uptake, ch.h, ctypes.c_void_p(s.cuda_stream)
-    ri, ro = ctypes.byref(inp), ctypes.byref(wo)
-    run = lambda: fn(h, ri, ro, sp)
+    # the argument blocks are built once, so the timed loop is one C call per
+    # launch and the stream stays busy
+    if wet:
+        out.append(torch.empty(shape, dtype=torch.float64, device="cuda"))  # dry diameters
+        wo = _abi.WetSizeOutputs()
+        for m in range(4):
+            wo.dgncur_awet[m], wo.qaerwat[m], wo.wetdens[m], wo.dgncur_i[m] = (
+                o[m].data_ptr() for o in out)
+        fn, h, sp = ch.lib.mam4x_compute_wet_size, ch.h, ctypes.c_void_p(s.cuda_stream)
+        ri, ro = ctypes.byref(inp), ctypes.byref(wo)
+        run = lambda: fn(h, DT, ri, ro, sp)
+    else:
+        wo = _abi.WaterOutputs()
+        for m in range(4):
+            wo.dgncur_awet[m], wo.qaerwat[m], wo.wetdens[m] = (o[m].data_ptr() for o in out)
+        fn, h, sp = ch.lib.mam4x_compute_water_uptake, ch.h, ctypes.c_void_p(s.cuda_stream)
+        ri, ro = ctypes.byref(inp), ctypes.byref(wo)
+        run = lambda: fn(h, ri, ro, sp)
+    assert run() == 0
     for _ in range(args.warmup):
         assert run() == 0
     torch.cuda.synchronize()
@@ -254,28 +267,36 @@ def bench_water(args):
     wall_ms = (time.perf_counter() - t0) * 1e3 / args.steps
     ms = e0.elapsed_time(e1) / args.steps
     ncell = NCOL * NLEV
-    # compulsory traffic per cell: 21 interstitial masses, T, p, qv, cloud
-    # fraction and 4 dry diameters read; 4 x 3 outputs written
-    bytes_cell = (21 + 4 + 4) * 8 + 12 * 8
+    # compulsory traffic per cell -- water: 21 interstitial masses, T, p, qv,
+    # cloud fraction and 4 dry diameters read, 4 x 3 outputs written;
+    # wetsize: 50 numbers and masses, T, p, qv, cloud fraction read, 4 x 4
+    # outputs written
+    bytes_cell = ((50 + 4) * 8 + 16 * 8) if wet else ((21 + 4 + 4) * 8 + 12 * 8)
     achieved = ncell * bytes_cell / (ms * 1e-3) / 1e9
-    res = {"metric": "aerosol water uptake: column-steps/sec at 72 lev", "unit": "column-steps/s",
+    what = ("wet-size producer (E3SM calcsize + water uptake)" if wet else
+            "aerosol water uptake")
+    res = {"metric": "%s: column-steps/sec at 72 lev" % what, "unit": "column-steps/s",
            "value": NCOL / (ms * 1e-3), "n_gpus": 1, "higher_is_better": True, "dtype": "f64",
            "data": "synthetic", "steps": args.steps, "warmup": args.warmup,
-           "config": {"workload": "21600 columns x 72 levels, modal_aero_water_uptake_dr with "
-                                  "wet density per cell", "columns": NCOL, "levels": NLEV},
+           "config": {"workload": "21600 columns x 72 levels, %s per cell" % (
+                          "compute_calcsize_and_water_uptake_dr" if wet else
+                          "modal_aero_water_uptake_dr with wet density"),
+                      "columns": NCOL, "levels": NLEV},
            "kernel_ms": ms, "ms_per_call_host_clock": wall_ms,
            "roofline": {"bound": "hbm", "achieved": achieved, "peak": 8000.0, "unit": "GB/s",
                         "frac": achieved / 8000.0,
-                        "traffic": _traffic("traffic_water_uptake_kernel_%dx%d.txt" % (NCOL, NLEV)),
+                        "traffic": _traffic("%s_kernel_%dx%d.txt" % (
+                            "wet_size" if wet else "water_uptake", NCOL, NLEV)),
                         "algorithmic_bytes_per_cell": bytes_cell}}
     if not args.no_cpu:
         orc = oracle.Oracle()
         nth = min(16, os.cpu_count() or 1)
-        orc.water_uptake(scene)
+        cpu = (lambda: orc.wet_size(scene, DT)) if wet else (lambda: orc.water_uptake(scene))
+        cpu()
         best = 1e30
         for _ in range(5):
             t0 = time.perf_counter()
-            orc.water_uptake(scene)
+            cpu()
             best = min(best, time.perf_counter() - t0)
         res["cpu_baseline"] = {"value": NCOL / best, "unit": "column-steps/s", "cores": nth,
                                "kind": "port", "sample": "the full 21600 x 72 scene, best of 5 "
@@ -292,10 +313,11 @@ def main():
     ap.add_argument("--precision", choices=["fp64", "mixed"], default="fp64")
     ap.add_argument("--schedule", choices=["auto", "fused", "queue"], default="auto")
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline")
-    ap.add_argument("--workload", choices=["headline", "batch", "large", "water"],
+    ap.add_argument("--workload", choices=["headline", "batch", "large", "water", "wetsize"],
                     default="headline",
-                    help="headline: the metric's 21,600 x 72 step (default); batch, large and "
-                         "water (aerosol water uptake) print their own line")
+                    help="headline: the metric's 21,600 x 72 step (default); batch, large, "
+                         "water (aerosol water uptake) and wetsize (calcsize + water uptake) "
+                         "print their own line")
     args = ap.parse_args()
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) > 1:
         print("bench.py measures one GPU: the chain has no cross-GPU step and this build "
@@ -313,7 +335,7 @@ def main():
         return bench_batch(args)
     if args.workload == "large":
         return bench_large(args)
-    if args.workload == "water":
+    if args.workload in ("water", "wetsize"):
         return bench_water(args)
     scene = scenes.generate(NCOL, NLEV, seed=0)
     d = scene_to_device(scene)

@@ -40,7 +40,7 @@
 extern "C" {
 #endif
 
-#define MAM4X_ABI_VERSION 4
+#define MAM4X_ABI_VERSION 5
 
 /* status codes */
 #define MAM4X_OK 0
@@ -176,6 +176,18 @@ typedef struct mam4x_water_outputs {
   double *wetdens[4];     /* wet density [kg/m3] (NULL: not formed)    */
 } mam4x_water_outputs;
 
+/* Outputs of the wet-size producer (mam4x_compute_wet_size). */
+typedef struct mam4x_wetsize_outputs {
+  double *dgncur_i[4];    /* dry diameter, interstitial [m] (calcsize)         */
+  double *dgncur_c[4];    /* dry diameter, cloud-borne [m] (NULL: not written) */
+  double *dgncur_awet[4]; /* wet diameter [m] (water uptake)                   */
+  double *qaerwat[4];     /* aerosol water [kg/kg]                             */
+  double *wetdens[4];     /* wet density [kg/m3] (NULL: not formed)            */
+  /* calcsize tendencies [1/s] (ptend / dqqcwdt) in state order, gases 0;
+   * all NULL: not written */
+  double *tendencies[MAM4X_NSTATE];
+} mam4x_wetsize_outputs;
+
 typedef struct mam4x_handle_s *mam4x_handle;
 
 /* Library / handle lifetime.  mam4x_create replaces AeroProcess<Impl>'s
@@ -237,6 +249,18 @@ int mam4x_compute_adjoint(mam4x_handle h, double t, double dt, const mam4x_input
  * workspace. */
 int mam4x_compute_water_uptake(mam4x_handle h, const mam4x_inputs *in,
                                const mam4x_water_outputs *out, void *stream);
+
+/* The wet-size producer, compute_calcsize_and_water_uptake_dr
+ * (modal_aero_opt.hpp:510-533; also wet_dep.hpp:2064): the E3SM-layout
+ * calcsize modal_aero_calcsize_sub (modal_aero_calcsize.hpp:748-928) on the
+ * state -- dry diameters and its number / mass tendencies, with
+ * do_adjust and do_aitacc_transfer on, over the adjustment time dt -- then
+ * water uptake (as mam4x_compute_water_uptake) on the dry diameters it
+ * returns.  Reads all number and aerosol-mass planes of in->state,
+ * temperature, pressure, vapor_mixing_ratio and cloud_fraction.  FP64;
+ * stream-ordered; no workspace. */
+int mam4x_compute_wet_size(mam4x_handle h, double dt, const mam4x_inputs *in,
+                           const mam4x_wetsize_outputs *out, void *stream);
 
 /* Batches: many scenes (domains of any column count, each with its own
  * inputs, outputs and layout) stepped by one set of launches.  The scenes'

@@ -6,7 +6,7 @@ The structures below must match the C declarations field for field; the
 import ctypes
 import os
 
-ABI_VERSION = 4
+ABI_VERSION = 5
 
 NSTATE = 52
 NMODE = 4
@@ -102,6 +102,17 @@ class WaterOutputs(ctypes.Structure):
     ]
 
 
+class WetSizeOutputs(ctypes.Structure):
+    _fields_ = [
+        ("dgncur_i", _vp * 4),
+        ("dgncur_c", _vp * 4),
+        ("dgncur_awet", _vp * 4),
+        ("qaerwat", _vp * 4),
+        ("wetdens", _vp * 4),
+        ("tendencies", _vp * NSTATE),
+    ]
+
+
 class Outputs(ctypes.Structure):
     _fields_ = [
         ("tendencies", _vp * NSTATE),
@@ -133,6 +144,8 @@ SIGNATURES = {
                                              _vp]),
     "mam4x_compute_water_uptake": (ctypes.c_int, [_vp, ctypes.POINTER(Inputs),
                                                   ctypes.POINTER(WaterOutputs), _vp]),
+    "mam4x_compute_wet_size": (ctypes.c_int, [_vp, ctypes.c_double, ctypes.POINTER(Inputs),
+                                              ctypes.POINTER(WetSizeOutputs), _vp]),
     "mam4x_batch_create": (ctypes.c_int, [_vp, ctypes.c_int, ctypes.POINTER(Inputs),
                                           ctypes.POINTER(Outputs), ctypes.c_double,
                                           ctypes.POINTER(_vp)]),

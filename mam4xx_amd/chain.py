@@ -231,6 +231,27 @@ class Chain:
                                                  ctypes.c_void_p(s.cuda_stream))
         self._check(rc, "mam4x_compute_water_uptake")
 
+    def compute_wet_size(self, scene, dt, dgncur_i, dgncur_awet, qaerwat, dgncur_c=None,
+                         wetdens=None, tend=None, stream=None, layout=_abi.LAYOUT_LEV_COL):
+        """The wet-size producer through mam4x_compute_wet_size: E3SM-layout
+        calcsize (dry diameters, optional tendencies [52, ...]) then water
+        uptake on those diameters; outputs [4, ...] each."""
+        inp = self.make_inputs(scene, None, None, layout)
+        out = _abi.WetSizeOutputs()
+        for key, t in (("dgncur_i", dgncur_i), ("dgncur_c", dgncur_c),
+                       ("dgncur_awet", dgncur_awet), ("qaerwat", qaerwat), ("wetdens", wetdens)):
+            if t is not None:
+                arr = getattr(out, key)
+                for m, p in enumerate(_plane_ptrs(t, 4)):
+                    arr[m] = p
+        if tend is not None:
+            for f, p in enumerate(_plane_ptrs(tend, L.NSTATE)):
+                out.tendencies[f] = p
+        s = stream if stream is not None else torch.cuda.current_stream()
+        rc = self.lib.mam4x_compute_wet_size(self.h, dt, ctypes.byref(inp), ctypes.byref(out),
+                                             ctypes.c_void_p(s.cuda_stream))
+        self._check(rc, "mam4x_compute_wet_size")
+
     def validate(self, scene, stream=None):
         inp = self.make_inputs(scene)
         v = ctypes.c_longlong(0)

@@ -901,11 +901,11 @@ __device__ __forceinline__ W dgn_from_v2n(const ChainConsts &K, int m, W v2n) {
 
 template <class W>
 __device__ __forceinline__ void update_diameter(const ChainConsts &K, int m, W drv, W num,
-                                                W &dgn) {
+                                                W &dgn, double v2nmin, double v2nmax) {
   if (!(drv > 0.0)) return;
-  if (num <= drv * K.v2n_min[m])
+  if (num <= drv * v2nmin)
     dgn = MODE_DMAX[m];
-  else if (num >= drv * K.v2n_max[m])
+  else if (num >= drv * v2nmax)
     dgn = MODE_DMIN[m];
   else
     dgn = dgn_from_v2n(K, m, num / drv);
@@ -995,10 +995,18 @@ __device__ __forceinline__ void new_size_after_transfer(const ChainConsts &K, in
   }
 }
 
+// e3sm: modal_aero_calcsize_sub (modal_aero_calcsize.hpp:748-928), the
+// E3SM-layout calcsize, instead of CalcSize::compute_tendencies: the same
+// algorithm on the same quantities except that the diameter update divides
+// the Aitken mode's num2vol_ratio_min and multiplies the accumulation mode's
+// num2vol_ratio_max by szadj_block_fac = 1e6 (:373-382), and the
+// accumulation -> Aitken transfer counts the non-transferable species'
+// number with num2vol_ratio_max (:219-222).
 template <class W>
 __device__ __forceinline__ void calcsize(const ChainConsts &K, bool do_xfer, bool do_adjust,
                                          double dt, const W n_i[4], const W n_c[4],
-                                         const W q_i[4][7], const W q_c[4][7], CalcOut<W> &o) {
+                                         const W q_i[4][7], const W q_c[4][7], CalcOut<W> &o,
+                                         bool e3sm = false) {
   const double tsinv = safe_den(kmax(86400.0, dt));
   W dv_i[4], dv_c[4], nk_i[4], nk_c[4];
 #pragma unroll
@@ -1027,8 +1035,13 @@ __device__ __forceinline__ void calcsize(const ChainConsts &K, bool do_xfer, boo
     if (do_adjust && !(aa && do_xfer))
       adjust_num_sizes(vi, vc, n_i[m], n_c[m], dt, K.v2n_min[m], K.v2n_max[m], tsinv, ni, nc,
                        o.dn_i[m], o.dn_c[m]);
-    update_diameter(K, m, vi, ni, o.dgn_i[m]);
-    update_diameter(K, m, vc, nc, o.dgn_c[m]);
+    double v2nmin = K.v2n_min[m], v2nmax = K.v2n_max[m];
+    if (e3sm) {
+      if (m == NAIT) v2nmin /= 1.0e6;
+      if (m == NACC) v2nmax *= 1.0e6;
+    }
+    update_diameter(K, m, vi, ni, o.dgn_i[m], v2nmin, v2nmax);
+    update_diameter(K, m, vc, nc, o.dgn_c[m], v2nmin, v2nmax);
     nk_i[m] = ni;
     nk_c[m] = nc;
   }
@@ -1079,7 +1092,9 @@ __device__ __forceinline__ void calcsize(const ChainConsts &K, bool do_xfer, boo
         drv_c_noxf += kmax(0.0, q_c[NACC][s]) * K.inv_density[NACC][s];
       }
       const W drv_t_noxf = drv_i_noxf + drv_c_noxf;
-      const W num_t_noxf = drv_t_noxf * K.v2n_min[NACC];  // QUIRK (calcsize.hpp:438)
+      // QUIRK (calcsize.hpp:438): num2vol_ratio_min; modal_aero_calcsize_sub
+      // uses num2vol_ratio_max
+      const W num_t_noxf = drv_t_noxf * (e3sm ? K.v2n_max[NACC] : K.v2n_min[NACC]);
       num_t0 = num_t;
       num_t = kmax(0.0, num_t - num_t_noxf);
       drv_t = kmax(0.0, drv_t - drv_t_noxf);

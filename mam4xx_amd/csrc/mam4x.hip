@@ -430,6 +430,72 @@ __global__ void __launch_bounds__(kBlock) water_uptake_kernel(WaterArgs a) {
   }
 }
 
+// The wet-size producer (compute_calcsize_and_water_uptake_dr,
+// modal_aero_opt.hpp:510-533): the E3SM-layout calcsize on the state, then
+// water uptake on the dry diameters it returns.  One lane per cell.
+struct WetSizeArgs {
+  const double *state[NSTATE];
+  const double *T, *p, *qv, *cld;
+  double *dgi[4], *dgc[4], *dgw[4], *qw[4], *wd[4];
+  double *tend[NSTATE];  // calcsize tendencies (NULL: not written)
+  const ChainConsts *K;
+  double dt;
+  int ncol, nlev, ldc, layout;
+};
+
+__global__ void __launch_bounds__(kBlock) wet_size_kernel(WetSizeArgs a) {
+  const long c = (long)blockIdx.x * kBlock + threadIdx.x;
+  if (c >= (long)a.ncol * a.nlev) return;
+  long off;
+  if (a.layout == MAM4X_LAYOUT_COL_LEV) {
+    const int icol = (int)(c / a.nlev);
+    off = (long)icol * a.ldc + (c - (long)icol * a.nlev);
+  } else {
+    const int k = (int)(c / a.ncol);
+    off = (long)k * a.ldc + (c - (long)k * a.ncol);
+  }
+  const ChainConsts &K = *a.K;
+  double n_i[4], n_c[4], q_i[4][7], q_c[4][7];
+#pragma unroll
+  for (int m = 0; m < 4; ++m) {
+    n_i[m] = a.state[f_ni(m)][off];
+    n_c[m] = a.state[f_nc(m)][off];
+#pragma unroll
+    for (int s = 0; s < 7; ++s) {
+      q_i[m][s] = s < NSPEC_MODE[m] ? a.state[f_qi(m, s)][off] : 0.0;
+      q_c[m][s] = s < NSPEC_MODE[m] ? a.state[f_qc(m, s)][off] : 0.0;
+    }
+  }
+  dev::CalcOut<double> o;
+  dev::calcsize(K, true, true, a.dt, n_i, n_c, q_i, q_c, o, true);
+  wu::WaterOut w;
+  wu::water_uptake_cell([&](int m, int s) { return q_i[m][s]; }, NSPEC_MODE, K.vol_from_dgn,
+                        a.T[off], a.p[off], a.qv[off], a.cld[off], o.dgn_i, w);
+#pragma unroll
+  for (int m = 0; m < 4; ++m) {
+    a.dgi[m][off] = o.dgn_i[m];
+    if (a.dgc[m]) a.dgc[m][off] = o.dgn_c[m];
+    a.dgw[m][off] = w.dgncur_awet[m];
+    a.qw[m][off] = w.qaerwat[m];
+    if (a.wd[m]) a.wd[m][off] = w.wetdens[m];
+  }
+  if (a.tend[0]) {
+#pragma unroll
+    for (int m = 0; m < 4; ++m) {
+      a.tend[f_ni(m)][off] = o.dn_i[m];
+      a.tend[f_nc(m)][off] = o.dn_c[m];
+#pragma unroll
+      for (int s = 0; s < 7; ++s) {
+        if (s >= NSPEC_MODE[m]) continue;
+        a.tend[f_qi(m, s)][off] = o.dq_i[m][s];
+        a.tend[f_qc(m, s)][off] = o.dq_c[m][s];
+      }
+    }
+    a.tend[F_H2SO4][off] = 0.0;
+    a.tend[F_SOAG][off] = 0.0;
+  }
+}
+
 }  // namespace
 
 enum { kPre = 0, kPost = 1, kFused = 2 };
@@ -922,6 +988,62 @@ int mam4x_compute_water_uptake(mam4x_handle h, const mam4x_inputs *in,
   const long ncell = (long)in->ncol * in->nlev;
   if (ncell == 0) return MAM4X_OK;
   hipLaunchKernelGGL(water_uptake_kernel, dim3((unsigned)((ncell + kBlock - 1) / kBlock)),
+                     dim3(kBlock), 0, (hipStream_t)stream, a);
+  HIP_TRY(h, hipGetLastError());
+  return MAM4X_OK;
+}
+
+int mam4x_compute_wet_size(mam4x_handle h, double dt, const mam4x_inputs *in,
+                           const mam4x_wetsize_outputs *out, void *stream) {
+  if (!h) return MAM4X_ERR_ARG;
+  if (!in || !out) return fail(h, MAM4X_ERR_ARG, "inputs or outputs is NULL");
+  if (!(dt > 0.0)) return fail(h, MAM4X_ERR_ARG, "dt must be > 0");
+  if (in->ncol < 0 || in->nlev < 0) return fail(h, MAM4X_ERR_ARG, "negative ncol or nlev");
+  if (in->layout != MAM4X_LAYOUT_LEV_COL && in->layout != MAM4X_LAYOUT_COL_LEV)
+    return fail(h, MAM4X_ERR_ARG, "unknown layout");
+  const bool lc = in->layout == MAM4X_LAYOUT_LEV_COL;
+  const int fast = lc ? in->ncol : in->nlev, slow = lc ? in->nlev : in->ncol;
+  const int ld = in->ldc ? in->ldc : fast;
+  if (ld < fast) return fail(h, MAM4X_ERR_ARG, "ldc smaller than the fastest extent");
+  if ((long long)ld * slow > 0x7fffffffLL)
+    return fail(h, MAM4X_ERR_ARG, "domain too large (ldc x slow extent must fit in int32)");
+  WetSizeArgs a;
+  std::memset(&a, 0, sizeof(a));
+  for (int f = 0; f < F_H2SO4; ++f) {
+    a.state[f] = in->state[f];
+    if (!a.state[f]) return fail(h, MAM4X_ERR_ARG, "state plane " + std::to_string(f) + " is NULL");
+  }
+  a.T = in->temperature;
+  a.p = in->pressure;
+  a.qv = in->vapor_mixing_ratio;
+  a.cld = in->cloud_fraction;
+  if (!a.T || !a.p || !a.qv || !a.cld)
+    return fail(h, MAM4X_ERR_ARG, "temperature, pressure, vapour or cloud fraction plane is NULL");
+  for (int m = 0; m < 4; ++m) {
+    a.dgi[m] = out->dgncur_i[m];
+    a.dgc[m] = out->dgncur_c[m];
+    a.dgw[m] = out->dgncur_awet[m];
+    a.qw[m] = out->qaerwat[m];
+    a.wd[m] = out->wetdens[m];
+    if (!a.dgi[m] || !a.dgw[m] || !a.qw[m])
+      return fail(h, MAM4X_ERR_ARG, "a dgncur_i, dgncur_awet or qaerwat plane is NULL");
+  }
+  int ntend = 0;
+  for (int f = 0; f < NSTATE; ++f) {
+    a.tend[f] = out->tendencies[f];
+    ntend += a.tend[f] != nullptr;
+  }
+  if (ntend != 0 && ntend != NSTATE)
+    return fail(h, MAM4X_ERR_ARG, "tendencies must be all NULL or all set");
+  a.K = h->d_consts;
+  a.dt = dt;
+  a.ncol = in->ncol;
+  a.nlev = in->nlev;
+  a.ldc = ld;
+  a.layout = in->layout;
+  const long ncell = (long)in->ncol * in->nlev;
+  if (ncell == 0) return MAM4X_OK;
+  hipLaunchKernelGGL(wet_size_kernel, dim3((unsigned)((ncell + kBlock - 1) / kBlock)),
                      dim3(kBlock), 0, (hipStream_t)stream, a);
   HIP_TRY(h, hipGetLastError());
   return MAM4X_OK;

@@ -2087,7 +2087,7 @@ static void aitken_accum_exchange(const calcsize_consts *c, double adj_tscale_in
                                   double num_i_acc, double drv_c_acc,
                                   double num_c_acc, double dgn_i[4],
                                   double dgn_c[4], double dn_i[4], double dn_c[4],
-                                  double dq_i[4][7], double dq_c[4][7]) {
+                                  double dq_i[4][7], double dq_c[4][7], int e3sm) {
   const double zero = 0, one = 1;
   double v2ncur;
   const double voltonum_ait = c->v2n_nom[NAIT];
@@ -2141,7 +2141,10 @@ static void aitken_accum_exchange(const calcsize_consts *c, double adj_tscale_in
           }
         drv_t_noxf = drv_i_noxf + drv_c_noxf;
         /* QUIRK (calcsize.hpp:438): the *min* num2vol ratio. */
-        num_t_noxf = drv_t_noxf * c->v2n_min[NACC];
+        /* the number that cannot move: num2vol_ratio_min (calcsize.hpp:436-438);
+         * modal_aero_calcsize_sub uses num2vol_ratio_max
+         * (modal_aero_calcsize.hpp:219-222) */
+        num_t_noxf = drv_t_noxf * (e3sm ? c->v2n_max[NACC] : c->v2n_min[NACC]);
         num_t0 = num_t;
         num_t = kmax(zero, num_t - num_t_noxf);
         drv_t = kmax(zero, drv_t - drv_t_noxf);
@@ -2212,23 +2215,61 @@ static calcsize_consts g_cs;
 static int g_cs_ready = 0;
 
 static void ensure_calcsize_consts(void) {
+  if (__atomic_load_n(&g_cs_ready, __ATOMIC_ACQUIRE)) return;
 #ifdef _OPENMP
 #pragma omp critical(orc_cs_init)
 #endif
   {
     if (!g_cs_ready) {
       calcsize_init(&g_cs);
-      g_cs_ready = 1;
+      __atomic_store_n(&g_cs_ready, 1, __ATOMIC_RELEASE);
     }
   }
 }
 
 /* CalcSize::compute_tendencies body for one level (calcsize.hpp:999-1216) */
+static void calcsize_level(const orc_config *cfg, double dt, const double n_i[4],
+                           const double n_c[4], const double q_i[4][7],
+                           const double q_c[4][7], double dn_i[4], double dn_c[4],
+                           double dq_i[4][7], double dq_c[4][7], double dgn_i[4],
+                           double dgn_c[4], int e3sm);
+
 void orc_calcsize_level(const orc_config *cfg, double dt, const double n_i[4],
                         const double n_c[4], const double q_i[4][7],
                         const double q_c[4][7], double dn_i[4], double dn_c[4],
                         double dq_i[4][7], double dq_c[4][7], double dgn_i[4],
                         double dgn_c[4]) {
+  calcsize_level(cfg, dt, n_i, n_c, q_i, q_c, dn_i, dn_c, dq_i, dq_c, dgn_i, dgn_c, 0);
+}
+
+/* modal_aero_calcsize_sub (modal_aero_calcsize.hpp:748-928) for one level:
+ * the E3SM-layout calcsize.  Same algorithm as CalcSize::compute_tendencies
+ * on the same quantities (its state_q / qqcw slots are the AeroProcess
+ * fields in mode_aero_species order, ndrop.hpp:55-90; its ptend / dqqcwdt
+ * are dn_i/dq_i and dn_c/dq_c), with two differences:
+ *   - size_adjustment divides the Aitken mode's num2vol_ratio_min and
+ *     multiplies the accumulation mode's num2vol_ratio_max by
+ *     szadj_block_fac = 1e6 before updating the diameters (:373-382);
+ *   - the accumulation -> Aitken transfer counts the non-transferable
+ *     species' number with num2vol_ratio_max (:219-222).
+ * do_adjust and do_aitacc_transfer are always on (CalcsizeData, :138-139). */
+void orc_calcsize_sub_level(double dt, const double n_i[4], const double n_c[4],
+                            const double q_i[4][7], const double q_c[4][7], double dn_i[4],
+                            double dn_c[4], double dq_i[4][7], double dq_c[4][7],
+                            double dgn_i[4], double dgn_c[4]) {
+  orc_config cfg;
+  memset(&cfg, 0, sizeof(cfg));
+  cfg.do_calcsize = 1;
+  cfg.calcsize_do_aitacc_transfer = 1;
+  cfg.calcsize_do_adjust = 1;
+  calcsize_level(&cfg, dt, n_i, n_c, q_i, q_c, dn_i, dn_c, dq_i, dq_c, dgn_i, dgn_c, 1);
+}
+
+static void calcsize_level(const orc_config *cfg, double dt, const double n_i[4],
+                           const double n_c[4], const double q_i[4][7],
+                           const double q_c[4][7], double dn_i[4], double dn_c[4],
+                           double dq_i[4][7], double dq_c[4][7], double dgn_i[4],
+                           double dgn_c[4], int e3sm) {
   ensure_calcsize_consts();
   const calcsize_consts *c = &g_cs;
   const int do_xfer = cfg->calcsize_do_aitacc_transfer;
@@ -2257,12 +2298,15 @@ void orc_calcsize_level(const orc_config *cfg, double dt, const double n_i[4],
       adjust_num_sizes(dv_i, dv_c, init_num_i, init_num_c, dt, c->v2n_min[m],
                        c->v2n_max[m], adj_tscale_inv, &num_i, &num_c, &dn_i[m],
                        &dn_c[m]);
-    update_diameter_and_vol2num(dv_i, num_i, c->v2n_min[m], c->v2n_max[m],
-                                c->dgnmin[m], c->dgnmax[m], c->sigma[m],
-                                &dgn_i[m], &v2n_i);
-    update_diameter_and_vol2num(dv_c, num_c, c->v2n_min[m], c->v2n_max[m],
-                                c->dgnmin[m], c->dgnmax[m], c->sigma[m],
-                                &dgn_c[m], &v2n_c);
+    double v2nmin = c->v2n_min[m], v2nmax = c->v2n_max[m];
+    if (e3sm) { /* szadj_block_fac (modal_aero_calcsize.hpp:373-382) */
+      if (m == NAIT) v2nmin /= 1.0e6;
+      if (m == NACC) v2nmax *= 1.0e6;
+    }
+    update_diameter_and_vol2num(dv_i, num_i, v2nmin, v2nmax, c->dgnmin[m], c->dgnmax[m],
+                                c->sigma[m], &dgn_i[m], &v2n_i);
+    update_diameter_and_vol2num(dv_c, num_c, v2nmin, v2nmax, c->dgnmin[m], c->dgnmax[m],
+                                c->sigma[m], &dgn_c[m], &v2n_c);
     if (do_xfer) {
       if (m == NAIT) {
         dvi_ait = dv_i; ni_ait = num_i; dvc_ait = dv_c; nc_ait = num_c;
@@ -2274,7 +2318,7 @@ void orc_calcsize_level(const orc_config *cfg, double dt, const double n_i[4],
   if (do_xfer)
     aitken_accum_exchange(c, adj_tscale_inv, dt, q_i, q_c, dvi_ait, ni_ait,
                           dvc_ait, nc_ait, dvi_acc, ni_acc, dvc_acc, nc_acc,
-                          dgn_i, dgn_c, dn_i, dn_c, dq_i, dq_c);
+                          dgn_i, dgn_c, dn_i, dn_c, dq_i, dq_c, e3sm);
 }
 
 /* ======================= composed step ==================================== */
@@ -2690,6 +2734,56 @@ void orc_water_uptake(int ncol, int nlev, const double *state, const double *tem
       dgncur_awet[m * ncell + c] = dgw[m];
       qaerwat[m * ncell + c] = qw[m];
       if (wetdens) wetdens[m * ncell + c] = wd[m];
+    }
+  }
+}
+
+/* compute_calcsize_and_water_uptake_dr (modal_aero_opt.hpp:510-533) over
+ * [field][level][column] buffers: modal_aero_calcsize_sub on the state, then
+ * water uptake on its interstitial dry diameters.  Outputs [4][nlev][ncol]
+ * (dgn_c, wetdens may be NULL) and the calcsize tendencies [ORC_NSTATE]
+ * (may be NULL; gas fields 0). */
+void orc_wet_size(int ncol, int nlev, double dt, const double *state,
+                  const double *temperature, const double *pressure, const double *qv,
+                  const double *cloud_fraction, double *dgn_i, double *dgn_c,
+                  double *dgncur_awet, double *qaerwat, double *wetdens, double *tend) {
+  const long ncell = (long)ncol * nlev;
+#ifdef _OPENMP
+#pragma omp parallel for schedule(static)
+#endif
+  for (long c = 0; c < ncell; ++c) {
+    double n_i[4], n_c[4], q_i[4][7] = {{0}}, q_c[4][7] = {{0}};
+    double dn_i[4] = {0}, dn_c[4] = {0}, dq_i[4][7] = {{0}}, dq_c[4][7] = {{0}};
+    double di[4], dc[4], dgw[4], qw[4], wd[4];
+    for (int m = 0; m < 4; ++m) {
+      n_i[m] = state[(long)F_NI(m) * ncell + c];
+      n_c[m] = state[(long)F_NC(m) * ncell + c];
+      for (int s = 0; s < NSPEC_MODE[m]; ++s) {
+        q_i[m][s] = state[(long)F_QI(m, s) * ncell + c];
+        q_c[m][s] = state[(long)F_QC(m, s) * ncell + c];
+      }
+    }
+    orc_calcsize_sub_level(dt, n_i, n_c, (const double(*)[7])q_i, (const double(*)[7])q_c, dn_i,
+                           dn_c, dq_i, dq_c, di, dc);
+    orc_water_uptake_cell((const double(*)[7])q_i, temperature[c], pressure[c], qv[c],
+                          cloud_fraction[c], di, dgw, qw, wd);
+    for (int m = 0; m < 4; ++m) {
+      dgn_i[m * ncell + c] = di[m];
+      if (dgn_c) dgn_c[m * ncell + c] = dc[m];
+      dgncur_awet[m * ncell + c] = dgw[m];
+      qaerwat[m * ncell + c] = qw[m];
+      if (wetdens) wetdens[m * ncell + c] = wd[m];
+    }
+    if (tend) {
+      for (int f = 0; f < ORC_NSTATE; ++f) tend[(long)f * ncell + c] = 0.0;
+      for (int m = 0; m < 4; ++m) {
+        tend[(long)F_NI(m) * ncell + c] = dn_i[m];
+        tend[(long)F_NC(m) * ncell + c] = dn_c[m];
+        for (int s = 0; s < NSPEC_MODE[m]; ++s) {
+          tend[(long)F_QI(m, s) * ncell + c] = dq_i[m][s];
+          tend[(long)F_QC(m, s) * ncell + c] = dq_c[m][s];
+        }
+      }
     }
   }
 }

@@ -175,6 +175,24 @@ void orc_water_uptake(int ncol, int nlev, const double *state, const double *tem
                       const double *dgncur_a, double *dgncur_awet, double *qaerwat,
                       double *wetdens);
 
+/* ---- the wet-size producer ---- */
+
+/* modal_aero_calcsize_sub (modal_aero_calcsize.hpp:748-928) for one level:
+ * the E3SM-layout calcsize (arguments as orc_calcsize_level; do_adjust and
+ * do_aitacc_transfer on).  Differs from CalcSize in the size adjustment's
+ * szadj_block_fac and the non-transferable number's num2vol ratio. */
+void orc_calcsize_sub_level(double dt, const double n_i[4], const double n_c[4],
+                            const double q_i[4][7], const double q_c[4][7], double dn_i[4],
+                            double dn_c[4], double dq_i[4][7], double dq_c[4][7],
+                            double dgn_i[4], double dgn_c[4]);
+
+/* compute_calcsize_and_water_uptake_dr (modal_aero_opt.hpp:510-533) over
+ * [field][level][column] buffers; dgn_c, wetdens, tend may be NULL. */
+void orc_wet_size(int ncol, int nlev, double dt, const double *state,
+                  const double *temperature, const double *pressure, const double *qv,
+                  const double *cloud_fraction, double *dgn_i, double *dgn_c,
+                  double *dgncur_awet, double *qaerwat, double *wetdens, double *tend);
+
 #ifdef __cplusplus
 }
 #endif

@@ -103,6 +103,10 @@ class Oracle:
         L.orc_water_uptake.restype = None
         L.orc_qsat_water.argtypes = [ctypes.c_double, ctypes.c_double, _dp, _dp]
         L.orc_qsat_water.restype = None
+        L.orc_calcsize_sub_level.argtypes = [ctypes.c_double] + [_dp] * 10
+        L.orc_calcsize_sub_level.restype = None
+        L.orc_wet_size.argtypes = [ctypes.c_int, ctypes.c_int, ctypes.c_double] + [_dp] * 11
+        L.orc_wet_size.restype = None
 
     def set_libm_noise(self, seed):
         """seed != 0: randomly move each exp/log/pow/cbrt/erfc result by one
@@ -192,6 +196,35 @@ class Oracle:
                                   _p(c("qv")), _p(c("cloud_fraction")), _p(dga),
                                   *[_p(o) for o in out])
         return tuple(out)
+
+    def calcsize_level(self, dt, n_i, n_c, q_i, q_c, e3sm=False, config=None):
+        """One level of calcsize: CalcSize::compute_tendencies (e3sm=False)
+        or modal_aero_calcsize_sub (e3sm=True).  Returns (dn_i, dn_c, dq_i,
+        dq_c, dgn_i, dgn_c)."""
+        a = [np.ascontiguousarray(x, dtype=np.float64) for x in (n_i, n_c)]
+        q = [np.ascontiguousarray(x, dtype=np.float64).reshape(4, 7) for x in (q_i, q_c)]
+        dn_i, dn_c = np.zeros(4), np.zeros(4)
+        dq_i, dq_c = np.zeros((4, 7)), np.zeros((4, 7))
+        di, dc = np.zeros(4), np.zeros(4)
+        outs = [_p(x) for x in (dn_i, dn_c, dq_i, dq_c, di, dc)]
+        if e3sm:
+            self.lib.orc_calcsize_sub_level(dt, _p(a[0]), _p(a[1]), _p(q[0]), _p(q[1]), *outs)
+        else:
+            cfg = config or default_config()
+            self.lib.orc_calcsize_level(ctypes.byref(cfg), dt, _p(a[0]), _p(a[1]), _p(q[0]),
+                                        _p(q[1]), *outs)
+        return dn_i, dn_c, dq_i, dq_c, di, dc
+
+    def wet_size(self, scene, dt):
+        """The wet-size producer over a scene: (dgn_i, dgn_c, dgncur_awet,
+        qaerwat, wetdens, tend)."""
+        st = np.ascontiguousarray(scene["state"])
+        nf, nlev, ncol = st.shape
+        outs = [np.empty((4, nlev, ncol)) for _ in range(5)] + [np.empty_like(st)]
+        c = lambda k: np.ascontiguousarray(scene[k])
+        self.lib.orc_wet_size(ncol, nlev, dt, _p(st), _p(c("temperature")), _p(c("pressure")),
+                              _p(c("qv")), _p(c("cloud_fraction")), *[_p(o) for o in outs])
+        return tuple(outs)
 
     def soaexch(self, dt, temp, pmid, q_soag, soa, pom, uptk_soag):
         """SOA exchange of one subarea; returns (q_soag, q_soag_avg, soa[4], niter)."""

@@ -74,3 +74,42 @@ def test_water_uptake_layouts_and_optional_output(torch_cuda):
     c = _run(torch_cuda, sc, wetdens=False)
     assert np.array_equal(a[0], c[0]) and np.array_equal(a[1], c[1])
     assert np.isnan(c[2]).all()  # not requested: untouched
+
+
+def test_wet_size_matches_oracle(orc, torch_cuda):
+    """mam4x_compute_wet_size (E3SM calcsize, then water uptake on its dry
+    diameters) against orc_wet_size: diameters to 1e-12 relative, the
+    calcsize tendencies to 1e-12 of the state over the day-long adjustment
+    time, the water outputs to the bar above."""
+    from mam4xx_amd import layout as L
+    from mam4xx_amd.chain import Chain, scene_to_device
+    torch = torch_cuda
+    sc = scenes.generate(64, 72, seed=44)
+    rng = np.random.default_rng(44)
+    sc["qv"] = sc["qv"] * rng.uniform(0.2, 1.1, sc["qv"].shape)
+    # spread the numbers so the size adjustment and both transfers occur
+    sc["state"][:8] *= 10 ** rng.uniform(-1.5, 1.5, sc["state"][:8].shape)
+    d = scene_to_device(sc)
+    shape = (4, 72, 64)
+    new = lambda *s: torch.full(s, np.nan, dtype=torch.float64, device="cuda")
+    dgi, dgc, dgw, qw, wd = (new(*shape) for _ in range(5))
+    tend = new(L.NSTATE, 72, 64)
+    ch = Chain()
+    ch.compute_wet_size(d, 1800.0, dgi, dgw, qw, dgncur_c=dgc, wetdens=wd, tend=tend)
+    torch.cuda.synchronize()
+    ch.close()
+    ref = orc.wet_size(sc, 1800.0)
+    got = [x.cpu().numpy() for x in (dgi, dgc, dgw, qw, wd, tend)]
+    for name, g, r in zip(("dgn_i", "dgn_c"), got[:2], ref[:2]):
+        err = np.abs(g - r) / np.abs(r)
+        assert err.max() < 1e-12, (name, err.max())
+    scale = np.abs(sc["state"]) / 86400.0 + np.abs(ref[5])
+    terr = np.abs(got[5] - ref[5]) / np.maximum(scale, 1e-300)
+    assert terr.max() < 1e-12, terr.max()
+    kappa = 1.0 / np.maximum(1.0 - (ref[0] / ref[2]) ** 3, 1e-300)
+    for name, g, r in zip(("dgncur_awet", "qaerwat", "wetdens"), got[2:5], ref[2:5]):
+        den = np.maximum(np.abs(r), 1e-300) * (kappa if name == "qaerwat" else 1.0)
+        err = np.where(r == g, 0.0, np.abs(g - r) / den)
+        assert np.median(err) < 1e-15, (name, np.median(err))
+        assert (err > 1e-12).mean() < 1e-3, (name, np.sort(err.ravel())[-5:])
+        assert err.max() < 1e-6, (name, err.max())
