@@ -40,7 +40,16 @@
 extern "C" {
 #endif
 
-#define MAM4X_ABI_VERSION 5
+#define MAM4X_ABI_VERSION 6
+
+/* per-process tendency diagnostics (mam4x_outputs.process_tendencies) */
+#define MAM4X_GAS_PCNST 31
+#define MAM4X_NPROCESS 5
+#define MAM4X_PROC_CONDENSATION 0        /* gas_aero_exchange_condensation   */
+#define MAM4X_PROC_RENAMING 1            /* gas_aero_exchange_renaming       */
+#define MAM4X_PROC_NUCLEATION 2          /* gas_aero_exchange_nucleation     */
+#define MAM4X_PROC_COAGULATION 3         /* gas_aero_exchange_coagulation    */
+#define MAM4X_PROC_RENAMING_CLOUD_BORNE 4 /* ..._renaming_cloud_borne        */
 
 /* status codes */
 #define MAM4X_OK 0
@@ -142,6 +151,20 @@ typedef struct mam4x_outputs {
   /* Optional: state after the step, state + dt * tendencies.  May alias the
    * input state planes (in-place update). */
   double *state_out[MAM4X_NSTATE];
+  /* Optional per-process tendency diagnostics: the gas_aero_exchange_*
+   * arrays of MicrophysDiagnosticArrays (diagnostic_arrays.hpp:13-18) that
+   * modal_aero_amicphys_intr fills (mam4_amicphys.hpp:2343-2374).  Block
+   * process_tendencies[k] (MAM4X_PROC_*; NULL = not formed) holds
+   * MAM4X_GAS_PCNST planes, tracer i at process_tendencies[k] + i *
+   * process_stride (0 = the plane extent: slow extent x ld), indexed by the
+   * reference's gas_pcnst tracer order (mam4_amicphys.hpp:35-69; slots 0, 1,
+   * 3, 4 are gases outside MAM4 and stay 0).  Units: the reference's tracer
+   * mixing-ratio tendencies, mol/mol/s (#/kmol/s for numbers), area-weighted
+   * over the clear and cloudy subareas.  The condensation block includes its
+   * share of primary-carbon aging; the coagulation block is taken before
+   * aging.  FP64 precision, single-domain calls only. */
+  double *process_tendencies[MAM4X_NPROCESS];
+  long long process_stride;
 } mam4x_outputs;
 
 /* Forward-mode tangent: the direction of the inputs and the derivative of

@@ -6,7 +6,7 @@ The structures below must match the C declarations field for field; the
 import ctypes
 import os
 
-ABI_VERSION = 5
+ABI_VERSION = 6
 
 NSTATE = 52
 NMODE = 4
@@ -29,6 +29,14 @@ LAYOUT_COL_LEV = 1
 
 NPHASE = 3
 PHASE_NAMES = ("setup", "soa", "finish")
+
+GAS_PCNST = 31
+NPROCESS = 5
+PROC_CONDENSATION = 0
+PROC_RENAMING = 1
+PROC_NUCLEATION = 2
+PROC_COAGULATION = 3
+PROC_RENAMING_CLOUD_BORNE = 4
 
 _dp = ctypes.POINTER(ctypes.c_double)
 _vp = ctypes.c_void_p
@@ -119,6 +127,8 @@ class Outputs(ctypes.Structure):
         ("dgncur_i", _vp * NMODE),
         ("dgncur_c", _vp * NMODE),
         ("state_out", _vp * NSTATE),
+        ("process_tendencies", _vp * NPROCESS),
+        ("process_stride", ctypes.c_longlong),
     ]
 
 

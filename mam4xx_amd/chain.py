@@ -31,6 +31,27 @@ def _plane_ptrs(t, n):
     return [base + i * step for i in range(n)]
 
 
+def _process_ptrs(process):
+    """Block pointers and tracer-plane stride of the per-process diagnostics:
+    a [5][31][nlev][ncol] tensor, or a list of five [31][nlev][ncol] tensors
+    (None = block not formed) sharing one tracer-plane stride."""
+    blocks = list(process) if isinstance(process, (list, tuple)) else \
+        [process[k] for k in range(_abi.NPROCESS)]
+    assert len(blocks) == _abi.NPROCESS
+    ptrs, stride = [], None
+    for b in blocks:
+        if b is None:
+            ptrs.append(None)
+            continue
+        assert b.dim() == 3 and b.shape[0] == _abi.GAS_PCNST, b.shape
+        assert b.dtype == torch.float64 and b.is_cuda
+        assert b.stride(2) == 1 and b.stride(1) == b.shape[2]
+        assert stride in (None, b.stride(0)), "blocks must share one tracer-plane stride"
+        stride = b.stride(0)
+        ptrs.append(b.data_ptr())
+    return ptrs, stride or 0
+
+
 def default_config(**kw):
     cfg = _abi.Config()
     _abi.load().mam4x_default_config(ctypes.byref(cfg))
@@ -105,9 +126,15 @@ class Chain:
 
     def compute_tendencies(self, scene, dt, tend, dgn_i=None, dgn_c=None, state_out=None,
                            precldchem=None, pregaschem=None, t=0.0, stream=None,
-                           layout=_abi.LAYOUT_LEV_COL):
+                           layout=_abi.LAYOUT_LEV_COL, process=None):
+        """One step (mam4x_compute_tendencies).  process: optional per-process
+        tendency diagnostics, [5][31][nlev][ncol] (see _process_ptrs)."""
         inp = self.make_inputs(scene, precldchem, pregaschem, layout)
         out = _abi.Outputs()
+        if process is not None:
+            ptrs, out.process_stride = _process_ptrs(process)
+            for k, p in enumerate(ptrs):
+                out.process_tendencies[k] = p
         for i, p in enumerate(_plane_ptrs(tend, L.NSTATE)):
             out.tendencies[i] = p
         if dgn_i is not None:

@@ -37,6 +37,10 @@ struct ChainArgs {
   // (tbar[f] NULL = 0)
   const double *tbar[NSTATE];
   double *xbar;
+  // per-process tendency diagnostics (dev::ProcSink; P::diag only): block k
+  // of 31 gas_pcnst planes, tracer i at proc[k] + i * pstride + off
+  double *proc[5];
+  long pstride;
   const ChainConsts *K;
   int ncol, nlev, ldc;
   int layout;  // 0: [nlev][ldc] column fastest; 1: [ncol][ldc] level fastest
@@ -46,6 +50,17 @@ struct ChainArgs {
 };
 
 namespace dev {
+
+__device__ __forceinline__ ProcSink proc_sink(const ChainArgs &a, long off, double afrac) {
+  ProcSink D;
+#pragma unroll
+  for (int k = 0; k < 5; ++k) D.blk[k] = a.proc[k];
+  D.stride = a.pstride;
+  D.off = off;
+  D.deltat = a.dt;
+  D.afrac = afrac;
+  return D;
+}
 
 // an input value in the wide type: with its direction in tangent mode
 template <class P>
@@ -292,7 +307,17 @@ __device__ __forceinline__ void cell_finish(const ChainArgs &a, long off, const 
         }
       }
     }
-    if (a.cfg.do_rename) rename_ait_to_acc<P>(K, true, C, gi, Cc, gc);
+    if (a.cfg.do_rename) {
+      if constexpr (P::diag) {
+        const Aer<R> C0 = C, Cc0 = Cc;
+        rename_ait_to_acc<P>(K, true, C, gi, Cc, gc);
+        const ProcSink D = proc_sink(a, off, X.fcldy);
+        D.add_aer(1, C, C0);
+        D.add_aer(4, Cc, Cc0);
+      } else {
+        rename_ait_to_acc<P>(K, true, C, gi, Cc, gc);
+      }
+    }
     const double w = X.fcldy;
 #pragma unroll
     for (int m = 0; m < 4; ++m) {
@@ -404,6 +429,11 @@ __device__ __forceinline__ int chain_cell_fused(const ChainArgs &a, long off, in
   Aer<typename P::R> S;
   ClearGas<W> G;
   int niter = 0;
+  [[maybe_unused]] ProcSink D;
+  if constexpr (P::diag) {
+    D = proc_sink(a, off, X.fclea);
+    D.zero();
+  }
   if (X.cat != 1) {
     clear_build<P>(a, off, X, S, G);
     SoaIn<W> in = {};
@@ -422,7 +452,7 @@ __device__ __forceinline__ int chain_cell_fused(const ChainArgs &a, long off, in
       for (int n = 0; n < 4; ++n) sa[n] = st.a[n];
     }
     clear_post<P>(K, a.cfg, a.dt, X.T, X.p, X.z, X.pblh, X.rh_clear, X.dga, X.dgw, X.wd, G.g1,
-                  G.g, S, in.uptk, sg, sgavg, sa);
+                  G.g, S, in.uptk, sg, sgavg, sa, &D);
   }
   cell_finish<PRE, P>(a, off, X, S, G);
   return niter;

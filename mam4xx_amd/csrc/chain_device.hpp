@@ -7,9 +7,10 @@
 // condensation/coagulation can put into the primary-carbon mode before aging
 // empties it (chain_consts.hpp: live()) -- so the whole chain stays in
 // registers.  Per-process increment bookkeeping the reference keeps only for
-// its optional history diagnostics is dropped; the few increments that feed
-// back into the physics (Aitken growth for renaming, primary-carbon
-// condensation/coagulation increments for aging) are kept.
+// its optional history diagnostics is kept only by the instantiation that
+// forms them (P::diag, ProcSink below); the few increments that feed back
+// into the physics (Aitken growth for renaming, primary-carbon
+// condensation/coagulation increments for aging) are always kept.
 //
 // Every routine is a template over a precision policy P (scalar.hpp): R is
 // the bulk type, W the type of the FP64-kept parts.  With P = Fp64 both are
@@ -54,6 +55,47 @@ struct Cfg {
   int do_cond, do_rename, do_newnuc, do_coag;
   int gaexch_h2so4_uptake_optaa;
   unsigned nmono;
+};
+
+// Per-process tendency diagnostics of one cell: the gas_aero_exchange_*
+// arrays of modal_aero_amicphys_intr (mam4_amicphys.hpp:1940-1986,
+// 2069-2113, 2365-2374).  Block k (0 condensation with its share of aging,
+// 1 renaming, 2 nucleation, 3 coagulation, 4 cloud-borne renaming) holds
+// TMR tendencies [mol/mol/s, #/kmol/s] of the 31 gas_pcnst tracers.  A
+// subarea's change over a process, divided by fcvt * deltat and weighted by
+// the subarea's area fraction, is added to the zeroed planes in subarea
+// order (clear, then cloudy) -- the reference's order of operations.
+struct ProcSink {
+  double *blk[5];  // NULL: block not formed
+  long stride;     // between the tracer planes of a block
+  long off;        // the cell's plane offset
+  double deltat;
+  double afrac;    // area fraction of the subarea being added
+  __host__ __device__ __forceinline__ void zero() const {
+#pragma unroll
+    for (int k = 0; k < 5; ++k)
+      if (blk[k])
+        for (int i = 0; i < 31; ++i) blk[k][i * stride + off] = 0.0;
+  }
+  __host__ __device__ __forceinline__ void add(int k, int slot, double dq, double fcvt) const {
+    if (!blk[k]) return;
+    double *p = blk[k] + slot * stride + off;
+    *p = *p + (dq / (fcvt * deltat)) * afrac;
+  }
+  // the change of a subarea's aerosol between two snapshots
+  __host__ __device__ __forceinline__ void add_aer(int k, const Aer<double> &now,
+                                                   const Aer<double> &before) const {
+#pragma unroll
+    for (int m = 0; m < 4; ++m) {
+      add(k, NUMPTR[m], now.n[m] - before.n[m], 1.0);
+#pragma unroll
+      for (int s = 0; s < 7; ++s) {
+        if (s >= NSPEC_MODE[m]) continue;
+        const int sp = mode_spec(m, s);
+        add(k, mass_slot(m, s), now.q[sp][m] - before.q[sp][m], fcvt_aer(sp));
+      }
+    }
+  }
 };
 
 // ------------------------------------------------------------------ uptake
@@ -750,16 +792,19 @@ __device__ __forceinline__ void coagulate(const ChainConsts &K, double deltat,
 // mam_pcarbon_aging_1subarea (aging.hpp:69-358): the aged fraction of
 // POM/BC/MOM and of the number, and all SO4/SOA/NaCl/DST, move from the
 // primary-carbon mode to accumulation.
+// xf_out / frac_cond_out (optional): the transferred fraction and the share
+// of the transfer the diagnostics attribute to condensation.
 template <class R>
 __device__ __forceinline__ void age_pcarbon(const ChainConsts &K, unsigned nmono, R dgn_a_pca,
                                             Aer<R> &S, R dcond_so4, R dcond_soa, R coag_so4,
-                                            R coag_soa) {
+                                            R coag_soa, R *xf_out = nullptr,
+                                            R *frac_cond_out = nullptr) {
   const R vol_shell = S.q[A_SO4][NPCA] * K.so4_vol + S.q[A_SOA][NPCA] * K.fac_m2v_eqvhyg;
   R del_cond = dcond_so4 * K.so4_vol + dcond_soa * K.fac_m2v_eqvhyg;
   const R del_coag = coag_so4 * K.so4_vol + coag_soa * K.fac_m2v_eqvhyg;
   del_cond = kmax(del_cond, 1e-35);
   const R frac_cond = del_cond / (del_cond + kmax(del_coag, 0.0));
-  (void)frac_cond;  // only splits the increment diagnostics
+  if (frac_cond_out) *frac_cond_out = frac_cond;  // only splits the diagnostics
   R vol_core = 0.0;
 #pragma unroll
   for (int a = 0; a < 7; ++a)
@@ -771,6 +816,7 @@ __device__ __forceinline__ void age_pcarbon(const ChainConsts &K, unsigned nmono
     xf = K.xferfrac_max;
   else
     xf = kmin(t1 / t2, K.xferfrac_max);
+  if (xf_out) *xf_out = xf;
   // POM, BC, MOM: aged fraction (aging.hpp:166-194)
 #pragma unroll
   for (int a = 0; a < 7; ++a) {
@@ -826,6 +872,7 @@ __device__ __forceinline__ void clear_pre(const ChainConsts &K, typename P::R te
 // nucleation, coagulation and aging.  uptk: H2SO4 uptake rates from
 // clear_pre; soa_g/soa_gavg/soa_a: SOA loop results (ignored without
 // condensation).  g1/g: gases {SOAG, H2SO4} before gas chemistry / current.
+// D: the diagnostics sink (P::diag only), with the clear area fraction set.
 template <class P>
 __device__ __forceinline__ void clear_post(
     const ChainConsts &K, const Cfg &cfg, double deltat, typename P::R temp, typename P::R pmid,
@@ -833,7 +880,7 @@ __device__ __forceinline__ void clear_post(
     const typename P::R dgn_a[4], const typename P::R dgn_awet[4],
     const typename P::R wetdens[4], const typename P::W g1[2], typename P::W g[2],
     Aer<typename P::R> &S, const typename P::W uptk[4], typename P::W soa_g,
-    typename P::W soa_gavg, const typename P::W soa_a[4]) {
+    typename P::W soa_gavg, const typename P::W soa_a[4], const ProcSink *D = nullptr) {
   using R = typename P::R;
   using W = typename P::W;
   const R aircon = pmid / (8314.46759100000 * temp);
@@ -842,6 +889,7 @@ __device__ __forceinline__ void clear_post(
     netprod = (g[1] - g1[1]) / deltat;
     g[1] = (netprod >= 0) ? g1[1] : W(0);  // QUIRK (:1283)
   }
+  [[maybe_unused]] const W gsv[2] = {g[0], g[1]};  // the gases condensation starts from
   const double dtsub = (double)(int)(deltat / 1);  // QUIRK (:1319): int sub-step
   W uptkrate_h2so4 = 0;
   W gavg[2];
@@ -862,6 +910,10 @@ __device__ __forceinline__ void clear_post(
       S.q[A_SOA][n] = num::to<R>(soa_a[n]);
       S.q[A_SO4][n] = num::to<R>(so4[n]);
     }
+    if constexpr (P::diag) {  // gas_pcnst 5 SOAG, 2 H2SO4 (mam4_amicphys.hpp:66-69)
+      D->add(0, 5, g[0] - (gsv[0] + 0.0 * dtsub), FCVT_SOAG);
+      D->add(0, 2, g[1] - (gsv[1] + netprod * dtsub), 1.0);
+    }
   } else {
     gavg[0] = g[0];
     gavg[1] = g[1];
@@ -871,15 +923,73 @@ __device__ __forceinline__ void clear_post(
     gi[A_SO4] = dc_so4[NAIT];
     gi[A_SOA] = dc_soa[NAIT];
     Aer<R> none;
-    rename_ait_to_acc<P>(K, false, S, gi, none, gc);
+    if constexpr (P::diag) {
+      const Aer<R> S0 = S;
+      rename_ait_to_acc<P>(K, false, S, gi, none, gc);
+      D->add_aer(1, S, S0);
+    } else {
+      rename_ait_to_acc<P>(K, false, S, gi, none, gc);
+    }
   }
-  if (cfg.do_newnuc)
+  if (cfg.do_newnuc) {
+    [[maybe_unused]] const W h0 = g[1];
+    [[maybe_unused]] const R n0 = S.n[NAIT], q0 = S.q[A_SO4][NAIT];
     newnuc<P>(K, dtsub, temp, pmid, zmid, pblh, relhum, num::to<R>(uptkrate_h2so4), g[1],
               gavg[1], S.n[NAIT], S.q[A_SO4][NAIT]);
+    if constexpr (P::diag) {
+      D->add(2, 2, g[1] - h0, 1.0);
+      D->add(2, NUMPTR[NAIT], S.n[NAIT] - n0, 1.0);
+      D->add(2, mass_slot(NAIT, spec_in_mode(A_SO4, NAIT)), S.q[A_SO4][NAIT] - q0,
+             fcvt_aer(A_SO4));
+    }
+  }
   R cin_so4 = 0.0, cin_soa = 0.0;
-  if (cfg.do_coag)
-    coagulate<P>(K, dtsub, temp, pmid, aircon, dgn_awet, wetdens, S, cin_so4, cin_soa);
-  age_pcarbon(K, cfg.nmono, dgn_a[NPCA], S, dc_so4[NPCA], dc_soa[NPCA], cin_so4, cin_soa);
+  if (cfg.do_coag) {
+    if constexpr (P::diag) {
+      const Aer<R> S0 = S;
+      coagulate<P>(K, dtsub, temp, pmid, aircon, dgn_awet, wetdens, S, cin_so4, cin_soa);
+      D->add_aer(3, S, S0);
+    } else {
+      coagulate<P>(K, dtsub, temp, pmid, aircon, dgn_awet, wetdens, S, cin_so4, cin_soa);
+    }
+  }
+  if constexpr (P::diag) {
+    const Aer<R> S0 = S;
+    R xf, fc;
+    age_pcarbon(K, cfg.nmono, dgn_a[NPCA], S, dc_so4[NPCA], dc_soa[NPCA], cin_so4, cin_soa,
+                &xf, &fc);
+    if (cfg.do_cond) {
+      // the condensation increments, with aging's transfer re-attributed:
+      // the condensation share of the aged POM/BC/MOM and number, and all
+      // of the condensed primary-carbon SO4/SOA (aging.hpp:166-231)
+      Aer<R> d, z;
+#pragma unroll
+      for (int m = 0; m < 4; ++m) {
+        d.n[m] = z.n[m] = 0.0;
+#pragma unroll
+        for (int a = 0; a < 7; ++a) d.q[a][m] = z.q[a][m] = 0.0;
+        d.q[A_SO4][m] = dc_so4[m];
+        d.q[A_SOA][m] = dc_soa[m];
+      }
+#pragma unroll
+      for (int a = 0; a < 7; ++a) {
+        if (a == A_POM || a == A_BC || a == A_MOM) {
+          const R qt = S0.q[a][NPCA] * xf;
+          d.q[a][NPCA] -= qt * fc;
+          d.q[a][NACC] += qt * fc;
+        } else {
+          d.q[a][NACC] += d.q[a][NPCA];
+          d.q[a][NPCA] = 0.0;
+        }
+      }
+      const R nt = S0.n[NPCA] * xf;
+      d.n[NPCA] -= nt * fc;
+      d.n[NACC] += nt * fc;
+      D->add_aer(0, d, z);
+    }
+  } else {
+    age_pcarbon(K, cfg.nmono, dgn_a[NPCA], S, dc_so4[NPCA], dc_soa[NPCA], cin_so4, cin_soa);
+  }
 }
 
 // ------------------------------------------------------------------ calcsize

@@ -353,6 +353,11 @@ __global__ void __launch_bounds__(kBlock) chain_post_kernel(Src src, Work w) {
   dev::cell_setup<PRE, P>(a, off, icol, X);
   dev::Aer<typename P::R> S;
   dev::ClearGas<double> G;
+  [[maybe_unused]] dev::ProcSink D;
+  if constexpr (P::diag) {
+    D = dev::proc_sink(a, off, X.fclea);
+    D.zero();
+  }
   if (X.cat != 1) {
     dev::clear_build<P>(a, off, X, S, G);
     double uptk[4] = {0, 0, 0, 0}, sa[4] = {0, 0, 0, 0}, sg = 0.0, sgavg = 0.0;
@@ -367,7 +372,7 @@ __global__ void __launch_bounds__(kBlock) chain_post_kernel(Src src, Work w) {
       }
     }
     dev::clear_post<P>(*a.K, a.cfg, a.dt, X.T, X.p, X.z, X.pblh, X.rh_clear, X.dga, X.dgw, X.wd,
-                    G.g1, G.g, S, uptk, sg, sgavg, sa);
+                    G.g1, G.g, S, uptk, sg, sgavg, sa, &D);
   }
   dev::cell_finish<PRE, P>(a, off, X, S, G);
 }
@@ -845,7 +850,26 @@ static int fill_outputs(mam4x_handle h, const mam4x_outputs *out, ChainArgs &a) 
     a.dgc[m] = out->dgncur_c[m];
     a.has_dgn |= (a.dgi[m] != nullptr) || (a.dgc[m] != nullptr);
   }
+  bool proc = false;
+  for (int k = 0; k < MAM4X_NPROCESS; ++k) {
+    a.proc[k] = out->process_tendencies[k];
+    proc = proc || a.proc[k];
+  }
+  a.pstride = 0;
+  if (proc) {
+    // plane extent: slow extent x ldc
+    const long extent = (long)(a.layout == MAM4X_LAYOUT_COL_LEV ? a.ncol : a.nlev) * a.ldc;
+    if (out->process_stride < 0 || (out->process_stride && out->process_stride < extent))
+      return fail(h, MAM4X_ERR_ARG, "process_stride smaller than the plane extent");
+    a.pstride = out->process_stride ? (long)out->process_stride : extent;
+  }
   return MAM4X_OK;
+}
+
+static bool has_proc(const ChainArgs &a) {
+  for (int k = 0; k < MAM4X_NPROCESS; ++k)
+    if (a.proc[k]) return true;
+  return false;
 }
 
 static bool has_pre(const mam4x_inputs *in) {
@@ -866,6 +890,8 @@ int mam4x_compute_tangent(mam4x_handle h, double t, double dt, const mam4x_input
   fill_args(h, in, a);
   rc = fill_outputs(h, out, a);
   if (rc) return rc;
+  if (has_proc(a))
+    return fail(h, MAM4X_ERR_UNSUPPORTED, "process tendencies are not formed by the tangent step");
   for (int f = 0; f < NSTATE; ++f) {
     if (!dir->tendencies[f])
       return fail(h, MAM4X_ERR_ARG, "tangent tendency plane " + std::to_string(f) + " is NULL");
@@ -1061,6 +1087,9 @@ int mam4x_compute_tendencies(mam4x_handle h, double t, double dt, const mam4x_in
   fill_args(h, in, a);
   rc = fill_outputs(h, out, a);
   if (rc) return rc;
+  const bool proc = has_proc(a);
+  if (proc && mixed)
+    return fail(h, MAM4X_ERR_UNSUPPORTED, "process tendencies need FP64 precision");
   a.dt = dt;
   hipStream_t s = (hipStream_t)stream;
   const bool pre = has_pre(in);
@@ -1111,6 +1140,8 @@ int mam4x_compute_tendencies(mam4x_handle h, double t, double dt, const mam4x_in
   const int kind = fused ? kFused : kPost;
   if (mixed)
     launch_cells<Mixed>(kind, pre, grid, s, src, w);
+  else if (proc)  // the same step, also forming the per-process diagnostics
+    launch_cells<Fp64Diag>(kind, pre, grid, s, src, w);
   else
     launch_cells<Fp64>(kind, pre, grid, s, src, w);
   HIP_TRY(h, hipGetLastError());
@@ -1152,6 +1183,8 @@ int mam4x_batch_create(mam4x_handle h, int nscene, const mam4x_inputs *in,
     fill_args(h, &in[i], args[i]);
     rc = fill_outputs(h, &out[i], args[i]);
     if (rc) return rc;
+    if (has_proc(args[i]))
+      return fail(h, MAM4X_ERR_UNSUPPORTED, "process tendencies need a single-domain call");
     args[i].dt = dt;
     pre = pre || has_pre(&in[i]);
     const long nc = (long)in[i].ncol * in[i].nlev;

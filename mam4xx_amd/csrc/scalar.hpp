@@ -175,18 +175,28 @@ __host__ __device__ __forceinline__ double to<double>(Dual x) { return x.v; }
 
 }  // namespace num
 
-// precision policies
+// precision policies; diag: the step also forms the per-process tendency
+// diagnostics (a separate instantiation, so the plain step carries none of
+// their bookkeeping)
 struct Fp64 {
   using R = double;
   using W = double;
+  static constexpr bool diag = false;
+};
+struct Fp64Diag {
+  using R = double;
+  using W = double;
+  static constexpr bool diag = true;
 };
 struct Mixed {
   using R = num::F32;
   using W = double;
+  static constexpr bool diag = false;
 };
 struct Tangent {
   using R = num::Dual;
   using W = num::Dual;
+  static constexpr bool diag = false;
 };
 
 }  // namespace mam4x

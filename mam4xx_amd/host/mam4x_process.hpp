@@ -63,6 +63,20 @@ struct Diagnostics {  // aero_config.hpp:170-260 (fields the chain reads / write
   double *wet_density[num_modes] = {};                    // in
 };
 
+// The fields of MicrophysDiagnosticArrays (diagnostic_arrays.hpp:5-48) the
+// microphysics fills (mam4_amicphys.hpp:2343-2374): per process, a device
+// block of MAM4X_GAS_PCNST tracer planes in the reference's gas_pcnst order,
+// tracer i at pointer + i * tracer_stride (0 = one plane of the domain).
+// Units mol/mol/s (#/kmol/s for numbers).  NULL members are not formed.
+struct MicrophysDiagnosticArrays {
+  double *gas_aero_exchange_condensation = nullptr;
+  double *gas_aero_exchange_renaming = nullptr;
+  double *gas_aero_exchange_nucleation = nullptr;
+  double *gas_aero_exchange_coagulation = nullptr;
+  double *gas_aero_exchange_renaming_cloud_borne = nullptr;
+  long long tracer_stride = 0;
+};
+
 // species of each mode, in state order (aero_modes.hpp:214-243)
 constexpr int kModeSpecies[num_modes][num_aerosol_ids] = {
     {1, 2, 0, 3, 5, 4, 6}, {1, 0, 4, 6, -1, -1, -1}, {5, 4, 1, 3, 2, 0, 6}, {2, 3, 6, -1, -1, -1, -1}};
@@ -109,8 +123,17 @@ class MicrophysicsProcess {
   // tendencies and diags.dry_geometric_mean_diameter_{i,c}; enqueued on
   // `stream`, no host synchronisation.
   void compute_tendencies(const Domain &d, double t, double dt, const Atmosphere &atm,
-                          const Surface &, const Prognostics &progs, const Diagnostics &diags,
+                          const Surface &sfc, const Prognostics &progs, const Diagnostics &diags,
                           const Tendencies &tends, void *stream = nullptr) const {
+    compute_tendencies(d, t, dt, atm, sfc, progs, diags, tends, MicrophysDiagnosticArrays{},
+                       stream);
+  }
+  // the same step, also filling the per-process diagnostic arrays (FP64
+  // precision only)
+  void compute_tendencies(const Domain &d, double t, double dt, const Atmosphere &atm,
+                          const Surface &, const Prognostics &progs, const Diagnostics &diags,
+                          const Tendencies &tends, const MicrophysDiagnosticArrays &diag_arrays,
+                          void *stream = nullptr) const {
     mam4x_inputs in = inputs(d, atm, progs, diags);
     mam4x_outputs out = {};
     state_table(tends, out.tendencies);
@@ -118,6 +141,13 @@ class MicrophysicsProcess {
       out.dgncur_i[m] = diags.dry_geometric_mean_diameter_i[m];
       out.dgncur_c[m] = diags.dry_geometric_mean_diameter_c[m];
     }
+    out.process_tendencies[MAM4X_PROC_CONDENSATION] = diag_arrays.gas_aero_exchange_condensation;
+    out.process_tendencies[MAM4X_PROC_RENAMING] = diag_arrays.gas_aero_exchange_renaming;
+    out.process_tendencies[MAM4X_PROC_NUCLEATION] = diag_arrays.gas_aero_exchange_nucleation;
+    out.process_tendencies[MAM4X_PROC_COAGULATION] = diag_arrays.gas_aero_exchange_coagulation;
+    out.process_tendencies[MAM4X_PROC_RENAMING_CLOUD_BORNE] =
+        diag_arrays.gas_aero_exchange_renaming_cloud_borne;
+    out.process_stride = diag_arrays.tracer_stride;
     check(mam4x_compute_tendencies(h_, t, dt, &in, &out, stream), "mam4x_compute_tendencies");
   }
 

@@ -76,3 +76,45 @@ def is_mass(f):
 
 def is_gas(f):
     return f >= 50
+
+
+def is_cloud_borne(f):
+    return 4 <= f < 8 or 29 <= f < 50
+
+
+# ---- per-process tendency diagnostics (include/mam4x.h, process_tendencies)
+# Blocks of the gas_aero_exchange_* arrays (diagnostic_arrays.hpp:13-18),
+# each over the 31 gas_pcnst tracers, in mol/mol/s (#/kmol/s for numbers).
+GAS_PCNST = 31
+PROCESS_NAMES = ("condensation", "renaming", "nucleation", "coagulation",
+                 "renaming_cloud_borne")
+NPROCESS = 5
+# E3SM adv_mass of the gas_pcnst tracers [g/mol] (gas_chem_mechanism.hpp:31-37)
+ADV_MASS = (47.998200, 34.013600, 98.078400, 64.064800, 62.132400, 12.011000,
+            115.107340, 12.011000, 12.011000, 12.011000, 135.064039, 58.442468,
+            250092.672000, 1.007400, 115.107340, 12.011000, 58.442468, 250092.672000,
+            1.007400, 135.064039, 58.442468, 115.107340, 12.011000, 12.011000,
+            12.011000, 250092.672000, 1.007400, 12.011000, 12.011000, 250092.672000,
+            1.007400)
+MWAIR = 28.966  # g/mol (mam4_constants.hpp:60)
+_NUMPTR = (13, 18, 26, 30)          # mam4_amicphys.hpp:35-63
+_MASS_SLOT0 = (6, 14, 19, 27)
+
+
+def pcnst_slot(f):
+    """gas_pcnst index of state field f (interstitial and cloud-borne
+    copies of a species share it)"""
+    if f < 8:
+        return _NUMPTR[f % 4]
+    if f == F_H2SO4:
+        return 2
+    if f == F_SOAG:
+        return 5
+    r = (f - 8) % 21
+    m = 3 if r >= MODE_OFFSET[3] else 2 if r >= MODE_OFFSET[2] else 1 if r >= MODE_OFFSET[1] else 0
+    return _MASS_SLOT0[m] + r - MODE_OFFSET[m]
+
+
+def to_vmr(f):
+    """kg/kg -> mol/mol (#/kg -> #/kmol for numbers) for state field f"""
+    return MWAIR if f < 8 else MWAIR / ADV_MASS[pcnst_slot(f)]

@@ -2366,6 +2366,21 @@ long orc_chain_step(const orc_config *cfg, int ncol, int nlev, double dt,
                     const double *dgncur_awet, const double *wetdens,
                     const double *pregaschem_gas, const double *precldchem,
                     double *tend, double *dgn_i, double *dgn_c, int nthreads) {
+  return orc_chain_step_proc(cfg, ncol, nlev, dt, state, temperature, pressure,
+                             height, qv, cloud_fraction, pblh, dgncur_a,
+                             dgncur_awet, wetdens, pregaschem_gas, precldchem,
+                             tend, dgn_i, dgn_c, NULL, nthreads);
+}
+
+long orc_chain_step_proc(const orc_config *cfg, int ncol, int nlev, double dt,
+                         const double *state, const double *temperature,
+                         const double *pressure, const double *height,
+                         const double *qv, const double *cloud_fraction,
+                         const double *pblh, const double *dgncur_a,
+                         const double *dgncur_awet, const double *wetdens,
+                         const double *pregaschem_gas, const double *precldchem,
+                         double *tend, double *dgn_i, double *dgn_c,
+                         double *proc, int nthreads) {
   int pcnst[ORC_NSTATE];
   double to_vmr[ORC_NSTATE], to_mmr[ORC_NSTATE];
   field_maps(pcnst, to_vmr, to_mmr);
@@ -2419,10 +2434,18 @@ long orc_chain_step(const orc_config *cfg, int ncol, int nlev, double dt,
         wd[m] = AT(wetdens, m);
       }
       int stats[2];
+      double qgcm_tendaa[31 * 4], qqcwgcm_tendaa[31];
       orc_amicphys_intr(cfg, dt, temperature[c], pressure[c], 0.0, height[c],
                         pblh[icol], qv[c], cloud_fraction[c], qq, qqcw, q1, q2,
-                        qcw2, dga, dgw, wd, NULL, NULL, stats);
+                        qcw2, dga, dgw, wd, proc ? qgcm_tendaa : NULL,
+                        proc ? qqcwgcm_tendaa : NULL, stats);
       total_iter += stats[0];
+      if (proc) /* the five diagnostic blocks (mam4_amicphys.hpp:2365-2374) */
+        for (int i = 0; i < 31; ++i) {
+          for (int iq = 0; iq < 4; ++iq)
+            proc[((long)iq * 31 + i) * ncell + c] = qgcm_tendaa[i * 4 + iq];
+          proc[((long)4 * 31 + i) * ncell + c] = qqcwgcm_tendaa[i];
+        }
       /* back to prognostic units; amicphys tendency */
       double post[ORC_NSTATE];
       for (int f = 0; f < ORC_NSTATE; ++f) {

@@ -79,6 +79,21 @@ long orc_chain_step(const orc_config *cfg, int ncol, int nlev, double dt,
                     const double *pregaschem_gas, const double *precldchem,
                     double *tend, double *dgn_i, double *dgn_c, int nthreads);
 
+/* orc_chain_step plus the per-process tendency diagnostics of
+ * modal_aero_amicphys_intr (mam4_amicphys.hpp:2343-2374): proc holds 5
+ * blocks (condensation with aging, renaming, nucleation, coagulation,
+ * cloud-borne renaming) of 31 gas_pcnst planes, [5][31][nlev][ncol], in
+ * mol/mol/s (#/kmol/s for numbers); NULL skips them. */
+long orc_chain_step_proc(const orc_config *cfg, int ncol, int nlev, double dt,
+                         const double *state, const double *temperature,
+                         const double *pressure, const double *height,
+                         const double *qv, const double *cloud_fraction,
+                         const double *pblh, const double *dgncur_a,
+                         const double *dgncur_awet, const double *wetdens,
+                         const double *pregaschem_gas, const double *precldchem,
+                         double *tend, double *dgn_i, double *dgn_c,
+                         double *proc, int nthreads);
+
 /* Conditioning probe: seed != 0 perturbs every transcendental result by a
  * random -1/0/+1 ulp (per-cell reproducible); 0 switches it off. */
 void orc_set_libm_noise(unsigned long long seed);

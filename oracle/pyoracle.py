@@ -69,6 +69,9 @@ class Oracle:
         L.orc_chain_step.restype = ctypes.c_long
         L.orc_chain_step.argtypes = [ctypes.POINTER(OrcConfig), ctypes.c_int, ctypes.c_int,
                                      ctypes.c_double] + [_dp] * 15 + [ctypes.c_int]
+        L.orc_chain_step_proc.restype = ctypes.c_long
+        L.orc_chain_step_proc.argtypes = [ctypes.POINTER(OrcConfig), ctypes.c_int, ctypes.c_int,
+                                          ctypes.c_double] + [_dp] * 16 + [ctypes.c_int]
         L.orc_nonvolatile_update.argtypes = [ctypes.c_double, ctypes.c_double, _dp, _dp, _dp, _dp]
         L.orc_getcoags_wrapper.argtypes = [ctypes.c_double] * 10 + [_dp]
         for fn in ("orc_bm0ij", "orc_bm3i"):
@@ -141,8 +144,11 @@ class Oracle:
         return base, env
 
     def chain_step(self, scene, dt=1800.0, config=None, nthreads=0, pregaschem_gas=None,
-                   precldchem=None):
-        """Run the composed step on a scene dict; returns (tend, dgn_i, dgn_c, soa_iters)."""
+                   precldchem=None, proc=False):
+        """Run the composed step on a scene dict; returns (tend, dgn_i, dgn_c, soa_iters)
+        and, with proc, the per-process tendency diagnostics [5, 31, nlev, ncol]
+        (condensation, renaming, nucleation, coagulation, cloud-borne renaming;
+        gas_pcnst tracers, mol/mol/s or #/kmol/s) as a fifth element."""
         cfg = config or default_config()
         st = np.ascontiguousarray(scene["state"], dtype=np.float64)
         nf, nlev, ncol = st.shape
@@ -154,9 +160,12 @@ class Oracle:
                  "dgncur_a", "dgncur_awet", "wetdens")]
         pg = None if pregaschem_gas is None else np.ascontiguousarray(pregaschem_gas)
         pc = None if precldchem is None else np.ascontiguousarray(precldchem)
-        iters = self.lib.orc_chain_step(ctypes.byref(cfg), ncol, nlev, dt, _p(st),
-                                        *[_p(a) for a in arrs], _p(pg), _p(pc), _p(tend),
-                                        _p(dgn_i), _p(dgn_c), int(nthreads))
+        pr = np.empty((5, 31, nlev, ncol)) if proc else None
+        iters = self.lib.orc_chain_step_proc(ctypes.byref(cfg), ncol, nlev, dt, _p(st),
+                                             *[_p(a) for a in arrs], _p(pg), _p(pc), _p(tend),
+                                             _p(dgn_i), _p(dgn_c), _p(pr), int(nthreads))
+        if proc:
+            return tend, dgn_i, dgn_c, int(iters), pr
         return tend, dgn_i, dgn_c, int(iters)
 
     # ---- process-level helpers for the known-answer tests ----

@@ -4,7 +4,9 @@
 // EAMxx layout (ncol x nlev, level fastest), fills the AeroProcess-shaped
 // containers species by species, calls validate and compute_tendencies
 // through MicrophysicsProcess, and returns tendencies and diameters in the
-// scene's layout.  Test infrastructure only.
+// scene's layout; with proc set, also the per-process diagnostic arrays
+// ([5][31][nlev][ncol]), each block one allocation of 31 EAMxx-layout planes.
+// Test infrastructure only.
 #include <hip/hip_runtime.h>
 
 #include <cstring>
@@ -36,9 +38,10 @@ bool upload(DevField &f, const double *plane, int ncol, int nlev) {
          hipSuccess;
 }
 
-bool download(double *plane, const DevField &f, int ncol, int nlev) {
+// [ncol][nlev] device field -> [nlev][ncol] host plane
+bool download(double *plane, const double *dev, int ncol, int nlev) {
   std::vector<double> t((size_t)ncol * nlev);
-  if (hipMemcpy(t.data(), f.p, t.size() * sizeof(double), hipMemcpyDeviceToHost) != hipSuccess)
+  if (hipMemcpy(t.data(), dev, t.size() * sizeof(double), hipMemcpyDeviceToHost) != hipSuccess)
     return false;
   for (int k = 0; k < nlev; ++k)
     for (int i = 0; i < ncol; ++i) plane[(size_t)k * ncol + i] = t[(size_t)i * nlev + k];
@@ -53,7 +56,7 @@ extern "C" int adapter_run(int ncol, int nlev, double dt, const double *state, c
                            const double *p, const double *z, const double *qv,
                            const double *cld, const double *pblh, const double *dga,
                            const double *dgw, const double *wd, double *tend, double *dgi,
-                           double *dgc, char *err, int errlen) {
+                           double *dgc, double *proc, char *err, int errlen) {
   const size_t n = (size_t)ncol * nlev;
   try {
     DevField st[MAM4X_NSTATE], td[MAM4X_NSTATE], met[5], dga_d[4], dgw_d[4], wd_d[4], dgc_d[4];
@@ -112,14 +115,32 @@ extern "C" int adapter_run(int ncol, int nlev, double dt, const double *state, c
       diags.wet_geometric_mean_diameter_i[m] = dgw_d[m].p;
       diags.wet_density[m] = wd_d[m].p;
     }
-    MicrophysicsProcess proc;
-    if (!proc.validate(d, atm, Surface{}, progs, diags)) return 2;
-    proc.compute_tendencies(d, 0.0, dt, atm, Surface{}, progs, diags, tends);
+    DevField blk[5];
+    MicrophysDiagnosticArrays da;
+    if (proc) {
+      for (int k = 0; k < 5; ++k)
+        if (hipMalloc(&blk[k].p, 31 * n * sizeof(double)) != hipSuccess) return 1;
+      da.gas_aero_exchange_condensation = blk[0].p;
+      da.gas_aero_exchange_renaming = blk[1].p;
+      da.gas_aero_exchange_nucleation = blk[2].p;
+      da.gas_aero_exchange_coagulation = blk[3].p;
+      da.gas_aero_exchange_renaming_cloud_borne = blk[4].p;
+    }
+    MicrophysicsProcess process;
+    if (!process.validate(d, atm, Surface{}, progs, diags)) return 2;
+    if (proc)
+      process.compute_tendencies(d, 0.0, dt, atm, Surface{}, progs, diags, tends, da);
+    else
+      process.compute_tendencies(d, 0.0, dt, atm, Surface{}, progs, diags, tends);
     if (hipDeviceSynchronize() != hipSuccess) return 1;
+    if (proc)
+      for (int k = 0; k < 5; ++k)
+        for (int i = 0; i < 31; ++i)
+          if (!download(proc + (k * 31 + i) * n, blk[k].p + i * n, ncol, nlev)) return 1;
     for (int f = 0; f < MAM4X_NSTATE; ++f)
-      if (!download(tend + f * n, td[f], ncol, nlev)) return 1;
+      if (!download(tend + f * n, td[f].p, ncol, nlev)) return 1;
     for (int m = 0; m < 4; ++m)
-      if (!download(dgi + m * n, dga_d[m], ncol, nlev) || !download(dgc + m * n, dgc_d[m], ncol, nlev))
+      if (!download(dgi + m * n, dga_d[m].p, ncol, nlev) || !download(dgc + m * n, dgc_d[m].p, ncol, nlev))
         return 1;
     return 0;
   } catch (const std::exception &e) {

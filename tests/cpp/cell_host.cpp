@@ -11,6 +11,50 @@
 
 using namespace mam4x;
 
+// Per-process tendency diagnostics (the Fp64Diag policy): proc receives
+// [5][31][nlev][ncol]; precld (NULL = none) is the optional pre-cloud-
+// chemistry state [52][nlev][ncol].
+extern "C" long cell_host_proc(const int *cfgv, int ncol, int nlev, double dt,
+                               const double *state, const double *T, const double *p,
+                               const double *z, const double *qv, const double *cld,
+                               const double *pblh, const double *dga, const double *dgw,
+                               const double *wd, const double *precld, double *tend,
+                               double *proc) {
+  static ChainConsts K;
+  chain_consts_init(&K);
+  const long n = (long)ncol * nlev;
+  ChainArgs a;
+  std::memset(&a, 0, sizeof(a));
+  for (int f = 0; f < NSTATE; ++f) {
+    a.state[f] = state + f * n;
+    a.tend[f] = tend + f * n;
+    if (precld) a.precld[f] = precld + f * n;
+  }
+  for (int k = 0; k < 5; ++k) a.proc[k] = proc + (long)k * 31 * n;
+  a.pstride = n;
+  a.T = T; a.p = p; a.z = z; a.qv = qv; a.cld = cld; a.pblh = pblh;
+  for (int m = 0; m < 4; ++m) {
+    a.dga[m] = dga + m * n;
+    a.dgw[m] = dgw + m * n;
+    a.wd[m] = wd + m * n;
+  }
+  a.K = &K;
+  a.ncol = ncol; a.nlev = nlev; a.ldc = ncol; a.layout = 0;
+  a.cfg.do_cond = cfgv[0]; a.cfg.do_rename = cfgv[1]; a.cfg.do_newnuc = cfgv[2];
+  a.cfg.do_coag = cfgv[3]; a.cfg.gaexch_h2so4_uptake_optaa = cfgv[4];
+  a.cfg.nmono = (unsigned)cfgv[5];
+  a.do_calcsize = cfgv[6]; a.do_xfer = cfgv[7]; a.do_adjust = cfgv[8];
+  a.dt = dt;
+  long iters = 0;
+  for (int k = 0; k < nlev; ++k)
+    for (int i = 0; i < ncol; ++i) {
+      const long off = (long)k * ncol + i;
+      iters += precld ? dev::chain_cell_fused<true, Fp64Diag>(a, off, i)
+                      : dev::chain_cell_fused<false, Fp64Diag>(a, off, i);
+    }
+  return iters;
+}
+
 extern "C" long cell_host_run3(const int *cfgv, int prec, int ncol, int nlev, double dt,
                                const double *state, const double *T, const double *p,
                                const double *z, const double *qv, const double *cld,

@@ -38,9 +38,11 @@ def test_library_exports_every_declared_symbol():
 def test_struct_layouts_match_c(tmp_path):
     prog = tmp_path / "sz.c"
     prog.write_text('#include <stdio.h>\n#include <stddef.h>\n#include "mam4x.h"\n'
-                    'int main(void){printf("%zu %zu %zu %zu %zu\\n", sizeof(mam4x_config),'
+                    'int main(void){printf("%zu %zu %zu %zu %zu %zu %zu\\n", sizeof(mam4x_config),'
                     ' sizeof(mam4x_inputs), sizeof(mam4x_outputs),'
-                    ' offsetof(mam4x_inputs, precldchem), offsetof(mam4x_outputs, state_out));'
+                    ' offsetof(mam4x_inputs, precldchem), offsetof(mam4x_outputs, state_out),'
+                    ' offsetof(mam4x_outputs, process_tendencies),'
+                    ' offsetof(mam4x_outputs, process_stride));'
                     'return 0;}\n')
     exe = tmp_path / "sz"
     subprocess.run(["gcc", "-I", os.path.join(ROOT, "include"), "-o", str(exe), str(prog)],
@@ -48,13 +50,14 @@ def test_struct_layouts_match_c(tmp_path):
     c = [int(v) for v in subprocess.run([str(exe)], capture_output=True, text=True,
                                         check=True).stdout.split()]
     py = [ctypes.sizeof(_abi.Config), ctypes.sizeof(_abi.Inputs), ctypes.sizeof(_abi.Outputs),
-          _abi.Inputs.precldchem.offset, _abi.Outputs.state_out.offset]
+          _abi.Inputs.precldchem.offset, _abi.Outputs.state_out.offset,
+          _abi.Outputs.process_tendencies.offset, _abi.Outputs.process_stride.offset]
     assert c == py
 
 
 def test_abi_version_and_default_config():
     lib = _abi.load()
-    assert lib.mam4x_abi_version() == _abi.ABI_VERSION == 5
+    assert lib.mam4x_abi_version() == _abi.ABI_VERSION == 6
     cfg = _abi.Config()
     assert lib.mam4x_default_config(ctypes.byref(cfg)) == _abi.OK
     assert (cfg.do_cond, cfg.do_rename, cfg.do_newnuc, cfg.do_coag) == (1, 1, 1, 1)

@@ -84,3 +84,45 @@ def test_kernel_code_bitwise_equals_oracle(orc, cellhost, name, cfgd, dt, seed):
     assert ith == ito
     assert np.array_equal(th, to), np.argwhere(th != to)[:5]
     assert np.array_equal(dih, dio) and np.array_equal(dch, dco)
+
+
+PROC_CASES = [c for c in CASES if c[0] in ("full", "cond", "rename", "newnuc", "coag",
+                                           "dt60", "sequential", "swapped_optaa")]
+
+
+@pytest.mark.parametrize("name,cfgd,dt,seed", PROC_CASES, ids=[c[0] for c in PROC_CASES])
+@pytest.mark.parametrize("precld", [False, True], ids=["current", "precld"])
+def test_process_diagnostics_bitwise_equal_oracle(orc, cellhost, name, cfgd, dt, seed, precld):
+    """The diagnostics instantiation (Fp64Diag) of the kernel's cell code:
+    the per-process tendency blocks and the tendencies bit for bit against
+    the oracle's orc_chain_step_proc, every cloud category, with and without
+    a pre-cloud-chemistry state (cloud-borne renaming)."""
+    import oracle.pyoracle as P
+    sc = scenes.generate(48, 72, seed=200 + seed)
+    sc["cloud_fraction"][:, :8] = 0.9995
+    sc["cloud_fraction"][:, 8:16] = 0.5
+    sc["state"][:, :2, 16:20] *= -1.0
+    pre = None
+    if precld:
+        rng = np.random.default_rng(seed)
+        pre = sc["state"] * rng.uniform(0.5, 1.0, sc["state"].shape)
+    oc = P.default_config(**cfgd)
+    cfg = (oc.do_cond, oc.do_rename, oc.do_newnuc, oc.do_coag, oc.newnuc_h2so4_conc_optaa,
+           oc.n_so4_monolayers_pcage, oc.do_calcsize, oc.calcsize_do_aitacc_transfer,
+           oc.calcsize_do_adjust)
+    dp = ctypes.POINTER(ctypes.c_double)
+    p = lambda a: None if a is None else a.ctypes.data_as(dp)
+    st = np.ascontiguousarray(sc["state"])
+    nf, nlev, ncol = st.shape
+    tend = np.empty_like(st)
+    proc = np.full((5, 31, nlev, ncol), np.nan)
+    arr = [np.ascontiguousarray(sc[k]) for k in ("temperature", "pressure", "height", "qv",
+                                                  "cloud_fraction", "pblh", "dgncur_a",
+                                                  "dgncur_awet", "wetdens")]
+    cellhost.cell_host_proc.restype = ctypes.c_long
+    ith = cellhost.cell_host_proc((ctypes.c_int * 9)(*cfg), ncol, nlev, ctypes.c_double(dt),
+                                  p(st), *[p(a) for a in arr], p(pre), p(tend), p(proc))
+    to, _, _, ito, po = orc.chain_step(sc, dt=dt, config=oc, precldchem=pre, proc=True)
+    assert ith == ito
+    assert np.array_equal(tend, to)
+    assert np.array_equal(proc, po), np.argwhere(proc != po)[:5]

@@ -34,7 +34,7 @@ def hip_runtime_from_torch():
     torch.cuda.init()
 
 
-def run_driver(sc, dt):
+def run_driver(sc, dt, proc=False):
     if not os.path.exists(DRIVER):
         pytest.fail("%s not built: run __graft_entry__.build()" % DRIVER)
     lib = ctypes.CDLL(DRIVER)
@@ -48,10 +48,13 @@ def run_driver(sc, dt):
     tend = np.empty_like(st)
     dgi = np.empty((4, nlev, ncol))
     dgc = np.empty_like(dgi)
+    pr = np.full((5, 31, nlev, ncol), np.nan) if proc else None
     err = ctypes.create_string_buffer(512)
     lib.adapter_run.restype = ctypes.c_int
     rc = lib.adapter_run(ncol, nlev, ctypes.c_double(dt), p(st), *[p(a) for a in arr], p(tend),
-                         p(dgi), p(dgc), err, 512)
+                         p(dgi), p(dgc), None if pr is None else p(pr), err, 512)
+    if proc:
+        return rc, err.value.decode(), tend, dgi, dgc, pr
     return rc, err.value.decode(), tend, dgi, dgc
 
 
@@ -74,3 +77,25 @@ def test_host_interface_validate_rejects_negative_state():
     sc["state"][3, 4, 5] = -1.0
     rc, msg, _, _, _ = run_driver(sc, 1800.0)
     assert rc == 2, (rc, msg)
+
+
+def test_host_interface_fills_diagnostic_arrays(orc):
+    """MicrophysDiagnosticArrays through the host interface (separate
+    allocations, EAMxx layout) against the oracle's orc_chain_step_proc, to
+    the bar of test_gpu_process.py."""
+    from .test_gpu_process import _compare, _scales
+    sc = scenes.generate(96, 72, seed=19)
+    sc["cloud_fraction"][:, :16] = 0.5
+    dt = 1800.0
+    rc, msg, tg, _, _, pg = run_driver(sc, dt, proc=True)
+    assert rc == 0, (rc, msg)
+    to, _, _, _, po = orc.chain_step(sc, dt, proc=True)
+    env = np.zeros_like(po)
+    try:
+        for s in range(1, 7):
+            orc.set_libm_noise(s)
+            env = np.maximum(env, np.abs(orc.chain_step(sc, dt, proc=True)[4] - po))
+    finally:
+        orc.set_libm_noise(0)
+    _compare(pg, po, env, _scales(sc, to))
+    assert np.count_nonzero(pg[:4]) > 1000
