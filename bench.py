@@ -305,6 +305,95 @@ def bench_water(args):
     print(json.dumps(res), flush=True)
 
 
+def bench_tracers(args):
+    """The host model's wire format (--workload tracers): the E3SM tracer
+    arrays state_q / qqcw of the 21,600 x 72 domain, (ncol, nlev, 40) stacked
+    by column, to the 52 state planes in the EAMxx (ncol, nlev) layout
+    (mam4x_tracers_to_fields) and back (mam4x_fields_to_tracers, state mode):
+    the transfer around one step.  Kernel times from HIP events on the launch
+    stream; HBM roofline of the values moved; the oracle's numpy walk
+    (oracle/tracers.py) on one host core."""
+    import ctypes
+    import numpy as np
+    import torch
+    from mam4xx_amd import _abi
+    from mam4xx_amd.chain import Chain
+    import oracle.tracers as T
+    g = torch.Generator(device="cuda").manual_seed(0)
+    q = torch.rand((NCOL, NLEV, T.PCNST), dtype=torch.float64, device="cuda", generator=g)
+    c = torch.rand((NCOL, NLEV, T.PCNST), dtype=torch.float64, device="cuda", generator=g)
+    fields = torch.empty((52, NCOL, NLEV), dtype=torch.float64, device="cuda")
+    ch = Chain()
+    s = torch.cuda.current_stream()
+    tr = ch._tracers(q, c)
+    fl = ch._fields(fields, NCOL, NLEV, _abi.LAYOUT_COL_LEV)
+    h, sp, rt, rf = ch.h, ctypes.c_void_p(s.cuda_stream), ctypes.byref(tr), ctypes.byref(fl)
+    unpack = lambda: ch.lib.mam4x_tracers_to_fields(h, rt, rf, sp)
+    pack = lambda: ch.lib.mam4x_fields_to_tracers(h, rf, rt, _abi.PACK_STATE, sp)
+    assert unpack() == 0 and pack() == 0
+    for _ in range(args.warmup):
+        unpack()
+        pack()
+    torch.cuda.synchronize()
+    ev = [torch.cuda.Event(enable_timing=True) for _ in range(3)]
+    t0 = time.perf_counter()
+    ms_u = ms_p = 0.0
+    for _ in range(args.steps):
+        ev[0].record(s)
+        unpack()
+        ev[1].record(s)
+        pack()
+        ev[2].record(s)
+        ev[2].synchronize()
+        ms_u += ev[0].elapsed_time(ev[1])
+        ms_p += ev[1].elapsed_time(ev[2])
+    wall_ms = (time.perf_counter() - t0) * 1e3 / args.steps
+    ms_u /= args.steps
+    ms_p /= args.steps
+    ncell = NCOL * NLEV
+    # values moved per cell and direction: 27 state_q + 25 qqcw tracers and
+    # the 52 planes
+    bytes_cell = (52 + 52) * 8
+    roof = lambda ms, name: {
+        "bound": "hbm", "achieved": ncell * bytes_cell / (ms * 1e-3) / 1e9, "peak": HBM_PEAK_GBS,
+        "unit": "GB/s", "frac": ncell * bytes_cell / (ms * 1e-3) / 1e9 / HBM_PEAK_GBS,
+        "traffic": _traffic("%s_kernel_%dx%d.txt" % (name, NCOL, NLEV)),
+        "algorithmic_bytes_per_cell": bytes_cell}
+    res = {"metric": "tracer-array transfer (state_q/qqcw <-> state planes): column-steps/sec "
+                     "at 72 lev", "unit": "column-steps/s",
+           "value": NCOL / ((ms_u + ms_p) * 1e-3), "n_gpus": 1, "higher_is_better": True,
+           "dtype": "f64", "data": "synthetic", "steps": args.steps, "warmup": args.warmup,
+           "config": {"workload": "21600 columns x 72 levels: state_q and qqcw (ncol, nlev, 40) "
+                                  "to 52 (ncol, nlev) planes and back", "columns": NCOL,
+                      "levels": NLEV},
+           "ms_unpack": ms_u, "ms_pack": ms_p, "ms_per_round_host_clock": wall_ms,
+           "roofline": roof(ms_u, "tracers_to_fields"),
+           "roofline_pack": roof(ms_p, "fields_to_tracers")}
+    if not args.no_cpu:
+        # one host core: the reference's index walk over a 2,160-column sample
+        nc = 2160
+        qh = q[:nc].cpu().numpy().reshape(-1, T.PCNST)
+        chh = c[:nc].cpu().numpy().reshape(-1, T.PCNST)
+        best = 1e30
+        for _ in range(3):
+            t0 = time.perf_counter()
+            progs = T.allocated_progs(qh.shape[0])
+            T.inject_qqcw_to_prognostics(chh, progs)
+            T.inject_stateq_to_prognostics(qh, progs)
+            st = T.progs_to_state(progs)
+            qo, co = qh.copy(), chh.copy()
+            p2 = T.state_to_progs(st, [qh[:, 9], qh[:, 10], qh[:, 12], qh[:, 13]])
+            T.extract_stateq_from_prognostics(p2, qo)
+            T.extract_qqcw_from_prognostics(p2, co)
+            best = min(best, time.perf_counter() - t0)
+        res["cpu_baseline"] = {"value": nc / best, "unit": "column-steps/s", "cores": 1,
+                               "kind": "port", "sample": "2160 of the 21600 columns x 72 levels, "
+                               "inject + extract, best of 3 (numpy, one core)",
+                               "cpu": _cpu_model()}
+    ch.close()
+    print(json.dumps(res), flush=True)
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -313,11 +402,12 @@ def main():
     ap.add_argument("--precision", choices=["fp64", "mixed"], default="fp64")
     ap.add_argument("--schedule", choices=["auto", "fused", "queue"], default="auto")
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline")
-    ap.add_argument("--workload", choices=["headline", "batch", "large", "water", "wetsize"],
+    ap.add_argument("--workload", choices=["headline", "batch", "large", "water", "wetsize",
+                                           "tracers"],
                     default="headline",
                     help="headline: the metric's 21,600 x 72 step (default); batch, large, "
-                         "water (aerosol water uptake) and wetsize (calcsize + water uptake) "
-                         "print their own line")
+                         "water (aerosol water uptake), wetsize (calcsize + water uptake) and "
+                         "tracers (state_q/qqcw <-> state planes) print their own line")
     args = ap.parse_args()
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) > 1:
         print("bench.py measures one GPU: the chain has no cross-GPU step and this build "
@@ -337,6 +427,8 @@ def main():
         return bench_large(args)
     if args.workload in ("water", "wetsize"):
         return bench_water(args)
+    if args.workload == "tracers":
+        return bench_tracers(args)
     scene = scenes.generate(NCOL, NLEV, seed=0)
     d = scene_to_device(scene)
     # chained steps: calcsize diameters feed the next step's dry and wet

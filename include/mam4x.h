@@ -285,6 +285,54 @@ int mam4x_compute_water_uptake(mam4x_handle h, const mam4x_inputs *in,
 int mam4x_compute_wet_size(mam4x_handle h, double dt, const mam4x_inputs *in,
                            const mam4x_wetsize_outputs *out, void *stream);
 
+/* ---------------------------------------------------------------------- */
+/* E3SM tracer arrays (the host model's wire format, utils.hpp:183-426):   */
+/* state_q and qqcw hold MAM4X_PCNST tracers per cell:                      */
+/*   0-4 qv qc qi nc ni; 5-8 precipitation; 9-14 gases O3 H2O2 H2SO4 SO2    */
+/*   DMS SOAG; 15-39 aerosol: per mode (acc, ait, coarse, pca) its species  */
+/*   in mode_aero_species order, then its number.                           */
+/* qqcw is indexed like state_q and holds cloud-borne aerosol at 15-39.     */
+/* Tracer 9 + i is gas_pcnst tracer i (mam4_amicphys.hpp:35-63).            */
+#define MAM4X_PCNST 40
+#define MAM4X_PACK_STATE 0    /* extract_stateq_from_prognostics            */
+#define MAM4X_PACK_TENDENCY 1 /* extract_ptend_from_tendencies              */
+
+/* One domain's tracer arrays.  The tracers of cell (icol, k) start at
+ * state_q + icol * col_stride + k * lev_stride (the same offset in qqcw),
+ * tracer index fastest: the reference's per-column (nlev, pcnst) views,
+ * stacked.  Strides in elements; 0 = lev_stride MAM4X_PCNST and col_stride
+ * nlev * lev_stride.  Rows must not overlap. */
+typedef struct mam4x_tracers {
+  double *state_q;
+  double *qqcw;  /* NULL: cloud-borne fields not transferred */
+  long long col_stride, lev_stride;
+} mam4x_tracers;
+
+/* The 52 state fields as planes, extents and layout as in mam4x_inputs. */
+typedef struct mam4x_fields {
+  int ncol, nlev;
+  int ldc;
+  int layout;
+  double *field[MAM4X_NSTATE]; /* NULL: field not transferred */
+} mam4x_fields;
+
+/* inject_stateq_to_prognostics and inject_qqcw_to_prognostics
+ * (utils.hpp:297-334, 388-420) over a domain: state_q's H2SO4 (11), SOAG
+ * (14) and interstitial aerosol (15-39), and qqcw's cloud-borne aerosol
+ * (15-39), into the state planes.  Stream-ordered; no workspace. */
+int mam4x_tracers_to_fields(mam4x_handle h, const mam4x_tracers *src, const mam4x_fields *dst,
+                            void *stream);
+
+/* extract_stateq_from_prognostics and extract_qqcw_from_prognostics
+ * (utils.hpp:217-253, 364-386), mode MAM4X_PACK_STATE, or
+ * extract_ptend_from_tendencies (utils.hpp:255-295), mode
+ * MAM4X_PACK_TENDENCY: the planes into state_q 11, 14 and 15-39 and qqcw
+ * 15-39.  The host's other tracers are left as they are; with
+ * MAM4X_PACK_TENDENCY the gases the chain does not carry (O3, H2O2, SO2,
+ * DMS: 9, 10, 12, 13) receive its zero tendency.  Stream-ordered. */
+int mam4x_fields_to_tracers(mam4x_handle h, const mam4x_fields *src, const mam4x_tracers *dst,
+                            int mode, void *stream);
+
 /* Batches: many scenes (domains of any column count, each with its own
  * inputs, outputs and layout) stepped by one set of launches.  The scenes'
  * cells form one work queue, so uneven scenes and uneven per-cell cost share

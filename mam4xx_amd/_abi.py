@@ -30,6 +30,10 @@ LAYOUT_COL_LEV = 1
 NPHASE = 3
 PHASE_NAMES = ("setup", "soa", "finish")
 
+PCNST = 40
+PACK_STATE = 0
+PACK_TENDENCY = 1
+
 GAS_PCNST = 31
 NPROCESS = 5
 PROC_CONDENSATION = 0
@@ -132,6 +136,25 @@ class Outputs(ctypes.Structure):
     ]
 
 
+class Tracers(ctypes.Structure):
+    _fields_ = [
+        ("state_q", _vp),
+        ("qqcw", _vp),
+        ("col_stride", ctypes.c_longlong),
+        ("lev_stride", ctypes.c_longlong),
+    ]
+
+
+class Fields(ctypes.Structure):
+    _fields_ = [
+        ("ncol", ctypes.c_int),
+        ("nlev", ctypes.c_int),
+        ("ldc", ctypes.c_int),
+        ("layout", ctypes.c_int),
+        ("field", _vp * NSTATE),
+    ]
+
+
 # entry points declared in include/mam4x.h, with ctypes signatures
 SIGNATURES = {
     "mam4x_abi_version": (ctypes.c_int, []),
@@ -156,6 +179,10 @@ SIGNATURES = {
                                                   ctypes.POINTER(WaterOutputs), _vp]),
     "mam4x_compute_wet_size": (ctypes.c_int, [_vp, ctypes.c_double, ctypes.POINTER(Inputs),
                                               ctypes.POINTER(WetSizeOutputs), _vp]),
+    "mam4x_tracers_to_fields": (ctypes.c_int, [_vp, ctypes.POINTER(Tracers),
+                                               ctypes.POINTER(Fields), _vp]),
+    "mam4x_fields_to_tracers": (ctypes.c_int, [_vp, ctypes.POINTER(Fields),
+                                               ctypes.POINTER(Tracers), ctypes.c_int, _vp]),
     "mam4x_batch_create": (ctypes.c_int, [_vp, ctypes.c_int, ctypes.POINTER(Inputs),
                                           ctypes.POINTER(Outputs), ctypes.c_double,
                                           ctypes.POINTER(_vp)]),

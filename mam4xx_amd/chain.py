@@ -151,6 +151,70 @@ class Chain:
                                                ctypes.byref(out), ctypes.c_void_p(s.cuda_stream))
         self._check(rc, "mam4x_compute_tendencies")
 
+    @staticmethod
+    def _tracers(state_q, qqcw):
+        """_abi.Tracers of (ncol, nlev, PCNST) tensors, tracer index fastest
+        (any column / level strides, e.g. a permuted level-major array)"""
+        for t in (state_q, qqcw):
+            if t is None:
+                continue
+            assert t.dim() == 3 and t.shape[2] == _abi.PCNST, t.shape
+            assert t.dtype == torch.float64 and t.is_cuda and t.stride(2) == 1
+        if qqcw is not None:
+            assert qqcw.shape == state_q.shape and qqcw.stride() == state_q.stride()
+        tr = _abi.Tracers()
+        tr.state_q = state_q.data_ptr()
+        tr.qqcw = qqcw.data_ptr() if qqcw is not None else None
+        tr.col_stride, tr.lev_stride = state_q.stride(0), state_q.stride(1)
+        return tr
+
+    @staticmethod
+    def _fields(fields, ncol, nlev, layout):
+        """_abi.Fields of a [52][nlev][ncol] ([52][ncol][nlev] for
+        LAYOUT_COL_LEV) tensor, or a list of 52 planes / None"""
+        fl = _abi.Fields()
+        fl.ncol, fl.nlev, fl.layout = ncol, nlev, layout
+        if isinstance(fields, (list, tuple)):
+            assert len(fields) == L.NSTATE
+            ld = None
+            for f, t in enumerate(fields):
+                if t is None:
+                    continue
+                assert t.dim() == 2 and t.dtype == torch.float64 and t.is_cuda
+                assert t.stride(1) == 1 and ld in (None, t.stride(0))
+                ld = t.stride(0)
+                fl.field[f] = t.data_ptr()
+            fl.ldc = ld or 0
+        else:  # a (possibly padded) [52][slow][fast] tensor
+            assert fields.dim() == 3 and fields.shape[0] == L.NSTATE, fields.shape
+            assert fields.dtype == torch.float64 and fields.is_cuda and fields.stride(2) == 1
+            for f in range(L.NSTATE):
+                fl.field[f] = fields.data_ptr() + f * fields.stride(0) * 8
+            fl.ldc = fields.stride(1)
+        return fl
+
+    def tracers_to_fields(self, state_q, qqcw, fields, layout=_abi.LAYOUT_LEV_COL, stream=None):
+        """state_q / qqcw (ncol, nlev, 40) -> the 52 state planes
+        (mam4x_tracers_to_fields; inject_stateq/qqcw_to_prognostics)"""
+        ncol, nlev = state_q.shape[:2]
+        tr, fl = self._tracers(state_q, qqcw), self._fields(fields, ncol, nlev, layout)
+        s = stream if stream is not None else torch.cuda.current_stream()
+        rc = self.lib.mam4x_tracers_to_fields(self.h, ctypes.byref(tr), ctypes.byref(fl),
+                                              ctypes.c_void_p(s.cuda_stream))
+        self._check(rc, "mam4x_tracers_to_fields")
+
+    def fields_to_tracers(self, fields, state_q, qqcw, mode=_abi.PACK_STATE,
+                          layout=_abi.LAYOUT_LEV_COL, stream=None):
+        """the 52 state (or tendency) planes -> state_q / qqcw
+        (mam4x_fields_to_tracers; extract_stateq/qqcw_from_prognostics, or
+        extract_ptend_from_tendencies with mode=PACK_TENDENCY)"""
+        ncol, nlev = state_q.shape[:2]
+        tr, fl = self._tracers(state_q, qqcw), self._fields(fields, ncol, nlev, layout)
+        s = stream if stream is not None else torch.cuda.current_stream()
+        rc = self.lib.mam4x_fields_to_tracers(self.h, ctypes.byref(fl), ctypes.byref(tr), mode,
+                                              ctypes.c_void_p(s.cuda_stream))
+        self._check(rc, "mam4x_fields_to_tracers")
+
     def make_outputs(self, tend, dgn_i=None, dgn_c=None, state_out=None):
         out = _abi.Outputs()
         for i, p in enumerate(_plane_ptrs(tend, L.NSTATE)):

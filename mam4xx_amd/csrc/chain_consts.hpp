@@ -127,6 +127,16 @@ constexpr int F_Q_I = 8;
 constexpr int F_Q_C = 29;
 constexpr int F_H2SO4 = 50;
 constexpr int F_SOAG = 51;
+// mode and mode-local species of a species-mass field (interstitial or
+// cloud-borne)
+__host__ __device__ constexpr int field_mode(int f) {
+  return (f - (f >= F_Q_C ? F_Q_C : F_Q_I)) < MODE_OFF[1]   ? 0
+       : (f - (f >= F_Q_C ? F_Q_C : F_Q_I)) < MODE_OFF[2] ? 1
+       : (f - (f >= F_Q_C ? F_Q_C : F_Q_I)) < MODE_OFF[3] ? 2 : 3;
+}
+__host__ __device__ constexpr int field_spec(int f) {
+  return f - (f >= F_Q_C ? F_Q_C : F_Q_I) - MODE_OFF[field_mode(f)];
+}
 
 // Derived constants, evaluated once on the host with the same libm calls the
 // reference makes per cell.

@@ -501,6 +501,169 @@ __global__ void __launch_bounds__(kBlock) wet_size_kernel(WetSizeArgs a) {
   }
 }
 
+// ------------------------------------------------------------------ tracers
+// E3SM tracer arrays <-> state planes (utils.hpp:183-426).  A block moves a
+// tile of kTile cells that are consecutive in the planes' memory order.  The
+// cells' tracer rows (tracer index fastest) pass through LDS, so the rows are
+// read or written as contiguous runs and every plane as 64 consecutive
+// doubles per wave.  Only tracers 11-39 of state_q (H2SO4 to the last
+// number; 12 and 13 ride along) and 15-39 of qqcw are moved.
+constexpr int kTile = 64;
+constexpr int kQ0 = 11, kNQ = MAM4X_PCNST - kQ0;  // state_q 11..39
+constexpr int kC0 = 15, kNC = MAM4X_PCNST - kC0;  // qqcw 15..39
+constexpr int kRow = kNQ + kNC + 1;               // LDS row, odd: no bank conflicts
+
+struct TracerArgs {
+  double *q, *qqcw;
+  long col_stride, lev_stride;
+  double *field[NSTATE];
+  int ncol, nlev, ldc, layout;
+  int mode;
+};
+
+// tracer index (state_q, or qqcw for cloud-borne fields) of state field f:
+// gas_pcnst slot + 9 (utils.hpp:36-48)
+__host__ __device__ constexpr int tracer_of(int f) {
+  return f < F_Q_I ? NUMPTR[f & 3] + 9
+       : f >= F_H2SO4 ? (f == F_H2SO4 ? 11 : 14)
+       : mass_slot(field_mode(f), field_spec(f)) + 9;
+}
+__host__ __device__ constexpr bool cloud_borne(int f) {
+  return (f >= F_NUM_C && f < F_Q_I) || (f >= F_Q_C && f < F_H2SO4);
+}
+// column of field f in an LDS row
+__host__ __device__ constexpr int lds_col(int f) {
+  return cloud_borne(f) ? kNQ + tracer_of(f) - kC0 : tracer_of(f) - kQ0;
+}
+
+static_assert(tracer_of(0) == 22 && tracer_of(1) == 27 && tracer_of(2) == 35 &&
+                  tracer_of(3) == 39 && tracer_of(F_NUM_C + 3) == 39,
+              "number tracers follow each mode's species");
+static_assert(tracer_of(F_Q_I) == 15 && tracer_of(F_Q_I + 7) == 23 &&
+                  tracer_of(F_Q_C + 11) == 28 && tracer_of(F_Q_C + 20) == 38,
+              "species tracers in mode_aero_species order from 15");
+
+// cell c of the planes' memory order: tracer-row offset and plane offset
+__device__ __forceinline__ void tracer_cell(const TracerArgs &a, long c, long &row, long &off) {
+  const int fast = a.layout == MAM4X_LAYOUT_COL_LEV ? a.nlev : a.ncol;
+  const long slow = c / fast;
+  const int i = (int)(c - slow * fast);
+  off = slow * a.ldc + i;
+  row = a.layout == MAM4X_LAYOUT_COL_LEV ? slow * a.col_stride + (long)i * a.lev_stride
+                                         : (long)i * a.col_stride + slow * a.lev_stride;
+}
+
+// Each thread issues all its loads of a phase before it consumes any, so a
+// block keeps ~15 loads per lane in flight instead of one.
+constexpr int kItQ = (kTile * kNQ + kBlock - 1) / kBlock;   // 8
+constexpr int kItC = (kTile * kNC + kBlock - 1) / kBlock;   // 7
+constexpr int kItF = NSTATE / (kBlock / 64);                // 13 fields per wave
+static_assert(kItF * (kBlock / 64) == NSTATE, "every wave takes the same number of fields");
+
+__global__ void __launch_bounds__(kBlock) tracers_to_fields_kernel(TracerArgs a) {
+  __shared__ double tile[kTile * kRow];
+  __shared__ long roff[kTile], poff[kTile];
+  const long ncell = (long)a.ncol * a.nlev;
+  const long c0 = (long)blockIdx.x * kTile;
+  const int n = (int)min((long)kTile, ncell - c0);
+  if (threadIdx.x < n) tracer_cell(a, c0 + threadIdx.x, roff[threadIdx.x], poff[threadIdx.x]);
+  __syncthreads();
+  // rows in
+  double vq[kItQ], vc[kItC];
+#pragma unroll
+  for (int j = 0; j < kItQ; ++j) {
+    const int e = threadIdx.x + j * kBlock, i = e / kNQ, t = e - i * kNQ;
+    if (e < n * kNQ) vq[j] = a.q[roff[i] + kQ0 + t];
+  }
+  if (a.qqcw) {
+#pragma unroll
+    for (int j = 0; j < kItC; ++j) {
+      const int e = threadIdx.x + j * kBlock, i = e / kNC, t = e - i * kNC;
+      if (e < n * kNC) vc[j] = a.qqcw[roff[i] + kC0 + t];
+    }
+  }
+#pragma unroll
+  for (int j = 0; j < kItQ; ++j) {
+    const int e = threadIdx.x + j * kBlock, i = e / kNQ, t = e - i * kNQ;
+    if (e < n * kNQ) tile[i * kRow + t] = vq[j];
+  }
+  if (a.qqcw) {
+#pragma unroll
+    for (int j = 0; j < kItC; ++j) {
+      const int e = threadIdx.x + j * kBlock, i = e / kNC, t = e - i * kNC;
+      if (e < n * kNC) tile[i * kRow + kNQ + t] = vc[j];
+    }
+  }
+  __syncthreads();
+  // planes out: one field per wave and pass
+  const int lane = threadIdx.x & 63;
+  const int wave = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
+#pragma unroll
+  for (int j = 0; j < kItF; ++j) {
+    const int f = wave + j * (kBlock / 64);
+    double *p = a.field[f];
+    if (!p || (cloud_borne(f) && !a.qqcw) || lane >= n) continue;
+    p[poff[lane]] = tile[lane * kRow + lds_col(f)];
+  }
+}
+
+__global__ void __launch_bounds__(kBlock) fields_to_tracers_kernel(TracerArgs a) {
+  __shared__ double tile[kTile * kRow];
+  __shared__ long roff[kTile], poff[kTile];
+  __shared__ unsigned char have[kRow];  // the row column has a source
+  const long ncell = (long)a.ncol * a.nlev;
+  const long c0 = (long)blockIdx.x * kTile;
+  const int n = (int)min((long)kTile, ncell - c0);
+  const bool tend = a.mode == MAM4X_PACK_TENDENCY;
+  if (threadIdx.x < n) tracer_cell(a, c0 + threadIdx.x, roff[threadIdx.x], poff[threadIdx.x]);
+  if (threadIdx.x < kRow) {
+    // SO2 and DMS: zero tendency in tendency mode, else left alone
+    const int t = threadIdx.x;
+    have[t] = tend && (t == 12 - kQ0 || t == 13 - kQ0);
+  }
+  __syncthreads();
+  if (threadIdx.x < NSTATE) {
+    const int f = threadIdx.x;
+    if (a.field[f] && !(cloud_borne(f) && !a.qqcw)) have[lds_col(f)] = 1;
+  }
+  // planes in: one field per wave and pass
+  const int lane = threadIdx.x & 63;
+  const int wave = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
+  double v[kItF];
+#pragma unroll
+  for (int j = 0; j < kItF; ++j) {
+    const int f = wave + j * (kBlock / 64);
+    const double *p = a.field[f];
+    if (p && !(cloud_borne(f) && !a.qqcw) && lane < n) v[j] = p[poff[lane]];
+  }
+#pragma unroll
+  for (int j = 0; j < kItF; ++j) {
+    const int f = wave + j * (kBlock / 64);
+    if (a.field[f] && !(cloud_borne(f) && !a.qqcw) && lane < n)
+      tile[lane * kRow + lds_col(f)] = v[j];
+  }
+  if (tend && threadIdx.x < n) {
+    tile[threadIdx.x * kRow + 12 - kQ0] = 0.0;
+    tile[threadIdx.x * kRow + 13 - kQ0] = 0.0;
+  }
+  __syncthreads();
+  // rows out
+#pragma unroll
+  for (int j = 0; j < kItQ; ++j) {
+    const int e = threadIdx.x + j * kBlock, i = e / kNQ, t = e - i * kNQ;
+    if (e < n * kNQ && have[t]) a.q[roff[i] + kQ0 + t] = tile[i * kRow + t];
+  }
+  if (a.qqcw) {
+#pragma unroll
+    for (int j = 0; j < kItC; ++j) {
+      const int e = threadIdx.x + j * kBlock, i = e / kNC, t = e - i * kNC;
+      if (e < n * kNC && have[kNQ + t]) a.qqcw[roff[i] + kC0 + t] = tile[i * kRow + kNQ + t];
+    }
+  }
+  if (tend)  // O3 and H2O2
+    for (int e = threadIdx.x; e < n * 2; e += kBlock) a.q[roff[e >> 1] + 9 + (e & 1)] = 0.0;
+}
+
 }  // namespace
 
 enum { kPre = 0, kPost = 1, kFused = 2 };
@@ -1146,6 +1309,69 @@ int mam4x_compute_tendencies(mam4x_handle h, double t, double dt, const mam4x_in
     launch_cells<Fp64>(kind, pre, grid, s, src, w);
   HIP_TRY(h, hipGetLastError());
   if (ev) HIP_TRY(h, hipEventRecord(ev[3], s));
+  return MAM4X_OK;
+}
+
+static int tracer_args(mam4x_handle h, const mam4x_tracers *tr, const mam4x_fields *fl,
+                       TracerArgs &a) {
+  if (!tr || !fl) return fail(h, MAM4X_ERR_ARG, "tracers or fields is NULL");
+  if (!tr->state_q) return fail(h, MAM4X_ERR_ARG, "state_q is NULL");
+  if (fl->ncol < 0 || fl->nlev < 0) return fail(h, MAM4X_ERR_ARG, "negative ncol or nlev");
+  if (fl->layout != MAM4X_LAYOUT_LEV_COL && fl->layout != MAM4X_LAYOUT_COL_LEV)
+    return fail(h, MAM4X_ERR_ARG, "unknown layout");
+  const bool lc = fl->layout == MAM4X_LAYOUT_LEV_COL;
+  const int fast = lc ? fl->ncol : fl->nlev, slow = lc ? fl->nlev : fl->ncol;
+  const int ld = fl->ldc ? fl->ldc : fast;
+  if (ld < fast) return fail(h, MAM4X_ERR_ARG, "ldc smaller than the fastest extent");
+  const long long ls = tr->lev_stride ? tr->lev_stride : MAM4X_PCNST;
+  const long long cs = tr->col_stride ? tr->col_stride : (long long)fl->nlev * ls;
+  // rows of MAM4X_PCNST tracers, stacked level-major within a column or
+  // column-major within a level, must not overlap
+  const bool by_col = ls >= MAM4X_PCNST && cs >= (long long)fl->nlev * ls;
+  const bool by_lev = cs >= MAM4X_PCNST && ls >= (long long)fl->ncol * cs;
+  if (!by_col && !by_lev)
+    return fail(h, MAM4X_ERR_ARG, "tracer rows overlap (col_stride / lev_stride too small)");
+  std::memset(&a, 0, sizeof(a));
+  a.q = tr->state_q;
+  a.qqcw = tr->qqcw;
+  a.col_stride = (long)cs;
+  a.lev_stride = (long)ls;
+  for (int f = 0; f < NSTATE; ++f) a.field[f] = fl->field[f];
+  a.ncol = fl->ncol;
+  a.nlev = fl->nlev;
+  a.ldc = ld;
+  a.layout = fl->layout;
+  return MAM4X_OK;
+}
+
+int mam4x_tracers_to_fields(mam4x_handle h, const mam4x_tracers *src, const mam4x_fields *dst,
+                            void *stream) {
+  if (!h) return MAM4X_ERR_ARG;
+  TracerArgs a;
+  int rc = tracer_args(h, src, dst, a);
+  if (rc) return rc;
+  const long ncell = (long)a.ncol * a.nlev;
+  if (ncell == 0) return MAM4X_OK;
+  hipLaunchKernelGGL(tracers_to_fields_kernel, dim3((unsigned)((ncell + kTile - 1) / kTile)),
+                     dim3(kBlock), 0, (hipStream_t)stream, a);
+  HIP_TRY(h, hipGetLastError());
+  return MAM4X_OK;
+}
+
+int mam4x_fields_to_tracers(mam4x_handle h, const mam4x_fields *src, const mam4x_tracers *dst,
+                            int mode, void *stream) {
+  if (!h) return MAM4X_ERR_ARG;
+  if (mode != MAM4X_PACK_STATE && mode != MAM4X_PACK_TENDENCY)
+    return fail(h, MAM4X_ERR_ARG, "unknown pack mode");
+  TracerArgs a;
+  int rc = tracer_args(h, dst, src, a);
+  if (rc) return rc;
+  a.mode = mode;
+  const long ncell = (long)a.ncol * a.nlev;
+  if (ncell == 0) return MAM4X_OK;
+  hipLaunchKernelGGL(fields_to_tracers_kernel, dim3((unsigned)((ncell + kTile - 1) / kTile)),
+                     dim3(kBlock), 0, (hipStream_t)stream, a);
+  HIP_TRY(h, hipGetLastError());
   return MAM4X_OK;
 }
 

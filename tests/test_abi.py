@@ -43,6 +43,8 @@ def test_struct_layouts_match_c(tmp_path):
                     ' offsetof(mam4x_inputs, precldchem), offsetof(mam4x_outputs, state_out),'
                     ' offsetof(mam4x_outputs, process_tendencies),'
                     ' offsetof(mam4x_outputs, process_stride));'
+                    'printf("%zu %zu %zu\\n", sizeof(mam4x_tracers), sizeof(mam4x_fields),'
+                    ' offsetof(mam4x_fields, field));'
                     'return 0;}\n')
     exe = tmp_path / "sz"
     subprocess.run(["gcc", "-I", os.path.join(ROOT, "include"), "-o", str(exe), str(prog)],
@@ -51,7 +53,8 @@ def test_struct_layouts_match_c(tmp_path):
                                         check=True).stdout.split()]
     py = [ctypes.sizeof(_abi.Config), ctypes.sizeof(_abi.Inputs), ctypes.sizeof(_abi.Outputs),
           _abi.Inputs.precldchem.offset, _abi.Outputs.state_out.offset,
-          _abi.Outputs.process_tendencies.offset, _abi.Outputs.process_stride.offset]
+          _abi.Outputs.process_tendencies.offset, _abi.Outputs.process_stride.offset,
+          ctypes.sizeof(_abi.Tracers), ctypes.sizeof(_abi.Fields), _abi.Fields.field.offset]
     assert c == py
 
 
