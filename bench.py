@@ -394,6 +394,108 @@ def bench_tracers(args):
     print(json.dumps(res), flush=True)
 
 
+def _setsox_domain(n, seed=0):
+    """synthetic in-cloud cells for the aqueous sulfur chemistry: met planes
+    and the gas_pcnst tracer arrays qcw / qin [31, n]"""
+    import numpy as np
+    rng = np.random.default_rng(seed)
+    T = rng.uniform(255, 300, n)
+    p = rng.uniform(5e4, 1.01e5, n)
+    met = dict(press=p, tfld=T, lwc=10 ** rng.uniform(-5, -3, n),
+               cldfrc=rng.uniform(0.05, 1.0, n), cldnum=10 ** rng.uniform(7, 9, n),
+               xhnm=p / (1.38e-23 * T) * 1e-6)
+    met["cldfrc"][rng.uniform(size=n) < 0.3] = 0.0  # clear cells
+    qcw, qin = np.zeros((31, n)), np.zeros((31, n))
+    for i in (6, 14, 21):
+        qcw[i] = 10 ** rng.uniform(-11, -9, n)
+    for i in (13, 18, 26, 30):
+        qcw[i] = 10 ** rng.uniform(9, 12, n)
+    qin[0] = 10 ** rng.uniform(-8, -7.3, n)
+    qin[1] = 10 ** rng.uniform(-10, -8.5, n)
+    qin[2] = 10 ** rng.uniform(-13, -11, n)
+    qin[3] = 10 ** rng.uniform(-11, -9, n)
+    return met, qcw, qin
+
+
+def bench_setsox(args):
+    """Aqueous sulfur chemistry (mam4x_compute_setsox; --workload setsox)
+    over 21,600 x 72 synthetic cells, 30 % cloud-free: kernel time from HIP
+    events on the launch stream, HBM roofline of the values moved, the oracle
+    on the host cores."""
+    import ctypes
+    import numpy as np
+    import torch
+    from mam4xx_amd import _abi
+    from mam4xx_amd.chain import Chain
+    import oracle
+    n = NCOL * NLEV
+    met, qcw, qin = _setsox_domain(n)
+    keys = {"pressure": "press", "temperature": "tfld", "liquid_water": "lwc",
+            "cloud_fraction": "cldfrc", "droplet_number": "cldnum", "air_number_density": "xhnm"}
+    dm = {k: torch.as_tensor(met[v].reshape(NLEV, NCOL)).cuda() for k, v in keys.items()}
+    qc0 = torch.as_tensor(qcw.reshape(31, NLEV, NCOL)).cuda()
+    qi0 = torch.as_tensor(qin.reshape(31, NLEV, NCOL)).cuda()
+    qc, qi = qc0.clone(), qi0.clone()
+    ch = Chain()
+    s = torch.cuda.current_stream()
+    a = _abi.SetsoxArgs()
+    a.nlev, a.ncol, a.ldc, a.layout = NLEV, NCOL, NCOL, _abi.LAYOUT_LEV_COL
+    for k in keys:
+        setattr(a, k, dm[k].data_ptr())
+    for i in range(31):
+        a.qcw[i] = qc[i].data_ptr()
+        a.qin[i] = qi[i].data_ptr()
+    fn, h, sp, ra = ch.lib.mam4x_compute_setsox, ch.h, ctypes.c_void_p(s.cuda_stream), ctypes.byref(a)
+    # each launch restarts from the same tracers (a device-to-device copy
+    # outside the timed kernel interval), so every launch does the same work
+    ev = [torch.cuda.Event(enable_timing=True) for _ in range(2)]
+    for _ in range(args.warmup + 1):
+        qc.copy_(qc0)
+        qi.copy_(qi0)
+        assert fn(h, DT, ra, sp) == 0
+    torch.cuda.synchronize()
+    ms = 0.0
+    for _ in range(args.steps):
+        qc.copy_(qc0)
+        qi.copy_(qi0)
+        ev[0].record(s)
+        fn(h, DT, ra, sp)
+        ev[1].record(s)
+        ev[1].synchronize()
+        ms += ev[0].elapsed_time(ev[1])
+    ms /= args.steps
+    # values per cell: 6 met + 4 gases + 3 cloud-borne sulfate + 3 numbers
+    # read, 3 gases + 3 cloud-borne sulfate written
+    bytes_cell = (6 + 4 + 6 + 3 + 3) * 8
+    achieved = n * bytes_cell / (ms * 1e-3) / 1e9
+    res = {"metric": "aqueous sulfur chemistry (setsox): column-steps/sec at 72 lev",
+           "unit": "column-steps/s", "value": NCOL / (ms * 1e-3), "n_gpus": 1,
+           "higher_is_better": True, "dtype": "f64", "data": "synthetic", "steps": args.steps,
+           "warmup": args.warmup,
+           "config": {"workload": "21600 columns x 72 levels of synthetic in-cloud cells (30 % "
+                                  "cloud-free), setsox_single_level per cell",
+                      "columns": NCOL, "levels": NLEV},
+           "kernel_ms": ms,
+           "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS,
+                        "unit": "GB/s", "frac": achieved / HBM_PEAK_GBS,
+                        "traffic": _traffic("setsox_kernel_%dx%d.txt" % (NCOL, NLEV)),
+                        "algorithmic_bytes_per_cell": bytes_cell}}
+    if not args.no_cpu:
+        orc = oracle.Oracle()
+        nth = min(16, os.cpu_count() or 1)
+        orc.setsox(DT, met, qcw, qin)
+        best = 1e30
+        for _ in range(5):
+            t0 = time.perf_counter()
+            orc.setsox(DT, met, qcw, qin)
+            best = min(best, time.perf_counter() - t0)
+        res["cpu_baseline"] = {"value": NCOL / best, "unit": "column-steps/s", "cores": nth,
+                               "kind": "port", "sample": "the full 21600 x 72 domain, best of 5 "
+                               "after a warm-up (OpenMP over cells)", "cpu": _cpu_model()}
+    ch.close()
+    print(json.dumps(res), flush=True)
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -403,11 +505,12 @@ def main():
     ap.add_argument("--schedule", choices=["auto", "fused", "queue"], default="auto")
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline")
     ap.add_argument("--workload", choices=["headline", "batch", "large", "water", "wetsize",
-                                           "tracers"],
+                                           "tracers", "setsox"],
                     default="headline",
                     help="headline: the metric's 21,600 x 72 step (default); batch, large, "
-                         "water (aerosol water uptake), wetsize (calcsize + water uptake) and "
-                         "tracers (state_q/qqcw <-> state planes) print their own line")
+                         "water (aerosol water uptake), wetsize (calcsize + water uptake), "
+                         "tracers (state_q/qqcw <-> state planes) and setsox (aqueous sulfur "
+                         "chemistry) print their own line")
     args = ap.parse_args()
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) > 1:
         print("bench.py measures one GPU: the chain has no cross-GPU step and this build "
@@ -429,6 +532,8 @@ def main():
         return bench_water(args)
     if args.workload == "tracers":
         return bench_tracers(args)
+    if args.workload == "setsox":
+        return bench_setsox(args)
     scene = scenes.generate(NCOL, NLEV, seed=0)
     d = scene_to_device(scene)
     # chained steps: calcsize diameters feed the next step's dry and wet

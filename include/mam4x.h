@@ -333,6 +333,48 @@ int mam4x_tracers_to_fields(mam4x_handle h, const mam4x_tracers *src, const mam4
 int mam4x_fields_to_tracers(mam4x_handle h, const mam4x_fields *src, const mam4x_tracers *dst,
                             int mode, void *stream);
 
+/* ---------------------------------------------------------------------- */
+/* Aqueous sulfur chemistry: mo_setsox::setsox_single_level               */
+/* (mo_setsox.hpp:943-1259) for every cell of a domain -- in-cloud pH,     */
+/* S(IV) oxidation by H2O2 and O3, H2SO4 uptake by cloud drops, cloud-     */
+/* borne sulfate gain split over the modes by activated number.  It is    */
+/* the cloud chemistry whose change of qqcw the microphysics reads as its */
+/* pre-cloud-chemistry input.  Tracer tables are indexed by gas_pcnst, in */
+/* the units setsox_single_level takes: mol/mol, cloud-borne number       */
+/* #/kmol.  Planes: extents and layout as in mam4x_inputs.                 */
+typedef struct mam4x_setsox_args {
+  int ncol, nlev;
+  int ldc;
+  int layout;
+  const double *pressure;           /* [Pa]                                */
+  const double *temperature;        /* [K]                                 */
+  const double *liquid_water;       /* cloud liquid water [kg/kg]          */
+  const double *cloud_fraction;     /* [-]                                 */
+  const double *droplet_number;     /* cloud droplet number [#/kg]         */
+  const double *air_number_density; /* xhnm [#/cm3]                        */
+  /* cloud-borne tracers: reads 6, 14, 21 (so4_c1-3) and 13, 18, 26
+   * (num_c1-3); updates 6, 14, 21.  Other entries are not used (the
+   * reference looks up num_c4, 30, but the primary-carbon mode has no
+   * sulfate slot, so it never enters the result). */
+  double *qcw[MAM4X_GAS_PCNST];
+  /* gas-phase tracers: reads 0 (O3); updates 1 (H2O2), 2 (H2SO4), 3 (SO2).
+   * Other entries are not used. */
+  double *qin[MAM4X_GAS_PCNST];
+  /* optional outputs [mol/mol/s] (NULL planes skipped): so4_c tendency from
+   * aqueous oxidation and from H2SO4 uptake, non-zero at 6, 14, 21; the
+   * reference zeroes the other entries, and so does this call for every
+   * plane given. */
+  double *dqdt_aqso4[MAM4X_GAS_PCNST];
+  double *dqdt_aqh2so4[MAM4X_GAS_PCNST];
+} mam4x_setsox_args;
+
+/* setsox_single_level over a domain with time step dt [s]: qcw / qin updated
+ * in place.  The reference's column driver mo_setsox::setsox
+ * (mo_setsox.hpp:1262-1300) passes per-level copies of qcw / qin and drops
+ * them; the single-level routine is what a host calls.  FP64;
+ * stream-ordered; no workspace. */
+int mam4x_compute_setsox(mam4x_handle h, double dt, const mam4x_setsox_args *args, void *stream);
+
 /* Batches: many scenes (domains of any column count, each with its own
  * inputs, outputs and layout) stepped by one set of launches.  The scenes'
  * cells form one work queue, so uneven scenes and uneven per-cell cost share

@@ -155,6 +155,25 @@ class Fields(ctypes.Structure):
     ]
 
 
+class SetsoxArgs(ctypes.Structure):
+    _fields_ = [
+        ("ncol", ctypes.c_int),
+        ("nlev", ctypes.c_int),
+        ("ldc", ctypes.c_int),
+        ("layout", ctypes.c_int),
+        ("pressure", _vp),
+        ("temperature", _vp),
+        ("liquid_water", _vp),
+        ("cloud_fraction", _vp),
+        ("droplet_number", _vp),
+        ("air_number_density", _vp),
+        ("qcw", _vp * GAS_PCNST),
+        ("qin", _vp * GAS_PCNST),
+        ("dqdt_aqso4", _vp * GAS_PCNST),
+        ("dqdt_aqh2so4", _vp * GAS_PCNST),
+    ]
+
+
 # entry points declared in include/mam4x.h, with ctypes signatures
 SIGNATURES = {
     "mam4x_abi_version": (ctypes.c_int, []),
@@ -183,6 +202,8 @@ SIGNATURES = {
                                                ctypes.POINTER(Fields), _vp]),
     "mam4x_fields_to_tracers": (ctypes.c_int, [_vp, ctypes.POINTER(Fields),
                                                ctypes.POINTER(Tracers), ctypes.c_int, _vp]),
+    "mam4x_compute_setsox": (ctypes.c_int, [_vp, ctypes.c_double, ctypes.POINTER(SetsoxArgs),
+                                            _vp]),
     "mam4x_batch_create": (ctypes.c_int, [_vp, ctypes.c_int, ctypes.POINTER(Inputs),
                                           ctypes.POINTER(Outputs), ctypes.c_double,
                                           ctypes.POINTER(_vp)]),

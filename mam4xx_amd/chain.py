@@ -215,6 +215,37 @@ class Chain:
                                               ctypes.c_void_p(s.cuda_stream))
         self._check(rc, "mam4x_fields_to_tracers")
 
+    def compute_setsox(self, dt, met, qcw, qin, dqdt_aqso4=None, dqdt_aqh2so4=None,
+                       layout=_abi.LAYOUT_LEV_COL, stream=None):
+        """Aqueous sulfur chemistry (mam4x_compute_setsox).  met: dict of
+        planes pressure, temperature, liquid_water, cloud_fraction,
+        droplet_number, air_number_density; qcw / qin: [31][nlev][ncol]
+        tensors (gas_pcnst order) updated in place; dqdt_*: optional
+        [31][nlev][ncol] outputs ([ncol][nlev] planes for LAYOUT_COL_LEV)."""
+        a = _abi.SetsoxArgs()
+        t = met["temperature"]
+        assert t.dim() == 2 and t.dtype == torch.float64 and t.is_cuda and t.is_contiguous()
+        if layout == _abi.LAYOUT_LEV_COL:
+            a.nlev, a.ncol = t.shape
+        else:
+            a.ncol, a.nlev = t.shape
+        a.ldc, a.layout = t.shape[1], layout
+        for k in ("pressure", "temperature", "liquid_water", "cloud_fraction", "droplet_number",
+                  "air_number_density"):
+            v = met[k]
+            assert v.shape == t.shape and v.dtype == torch.float64 and v.is_contiguous()
+            setattr(a, k, v.data_ptr())
+        for name, arr in (("qcw", qcw), ("qin", qin), ("dqdt_aqso4", dqdt_aqso4),
+                          ("dqdt_aqh2so4", dqdt_aqh2so4)):
+            if arr is None:
+                continue
+            for i, p in enumerate(_plane_ptrs(arr, _abi.GAS_PCNST)):
+                getattr(a, name)[i] = p
+        s = stream if stream is not None else torch.cuda.current_stream()
+        rc = self.lib.mam4x_compute_setsox(self.h, dt, ctypes.byref(a),
+                                           ctypes.c_void_p(s.cuda_stream))
+        self._check(rc, "mam4x_compute_setsox")
+
     def make_outputs(self, tend, dgn_i=None, dgn_c=None, state_out=None):
         out = _abi.Outputs()
         for i, p in enumerate(_plane_ptrs(tend, L.NSTATE)):

@@ -32,6 +32,7 @@
 
 #include "../../include/mam4x.h"
 #include "chain_cell.hpp"
+#include "setsox.hpp"
 #include "water_uptake.hpp"
 
 using namespace mam4x;
@@ -662,6 +663,52 @@ __global__ void __launch_bounds__(kBlock) fields_to_tracers_kernel(TracerArgs a)
   }
   if (tend)  // O3 and H2O2
     for (int e = threadIdx.x; e < n * 2; e += kBlock) a.q[roff[e >> 1] + 9 + (e & 1)] = 0.0;
+}
+
+// ------------------------------------------------------------------ setsox
+// Aqueous sulfur chemistry, one lane per cell (setsox.hpp).
+struct SoxArgs {
+  const double *met[6];  // pressure, temperature, liquid water, cloud fraction,
+                         // droplet number, air number density
+  double *qin[4];        // O3 (read only), H2O2, H2SO4, SO2
+  double *so4c[3];
+  const double *numc[3];
+  double *aq[MAM4X_GAS_PCNST], *h2[MAM4X_GAS_PCNST];
+  int ncol, nlev, ldc, layout;
+  double dt;
+};
+
+__global__ void __launch_bounds__(kBlock) setsox_kernel(SoxArgs a) {
+  const long c = (long)blockIdx.x * kBlock + threadIdx.x;
+  if (c >= (long)a.ncol * a.nlev) return;
+  const int fast = a.layout == MAM4X_LAYOUT_COL_LEV ? a.nlev : a.ncol;
+  const long slow = c / fast;
+  const long off = slow * a.ldc + (c - slow * fast);
+  sox::Met m{a.met[0][off], a.met[1][off], a.met[2][off],
+             a.met[3][off], a.met[4][off], a.met[5][off]};
+  sox::Tracers q;
+  q.o3 = a.qin[0][off];
+  q.h2o2 = a.qin[1][off];
+  q.h2so4 = a.qin[2][off];
+  q.so2 = a.qin[3][off];
+#pragma unroll
+  for (int k = 0; k < 3; ++k) q.so4c[k] = a.so4c[k][off];
+#pragma unroll
+  for (int k = 0; k < 3; ++k) q.numc[k] = a.numc[k][off];
+  q.numc[3] = 0.0;  // the primary-carbon mode has no cloud-borne sulfate slot
+  sox::Rates r;
+  sox::single_level(a.dt, m, q, r);
+  a.qin[1][off] = q.h2o2;
+  a.qin[2][off] = q.h2so4;
+  a.qin[3][off] = q.so2;
+#pragma unroll
+  for (int k = 0; k < 3; ++k) a.so4c[k][off] = q.so4c[k];
+  constexpr int so4[3] = {6, 14, 21};
+  for (int i = 0; i < MAM4X_GAS_PCNST; ++i) {
+    const int k = i == so4[0] ? 0 : i == so4[1] ? 1 : i == so4[2] ? 2 : -1;
+    if (a.aq[i]) a.aq[i][off] = k >= 0 ? r.aqso4[k] : 0.0;
+    if (a.h2[i]) a.h2[i][off] = k >= 0 ? r.aqh2so4[k] : 0.0;
+  }
 }
 
 }  // namespace
@@ -1371,6 +1418,57 @@ int mam4x_fields_to_tracers(mam4x_handle h, const mam4x_fields *src, const mam4x
   if (ncell == 0) return MAM4X_OK;
   hipLaunchKernelGGL(fields_to_tracers_kernel, dim3((unsigned)((ncell + kTile - 1) / kTile)),
                      dim3(kBlock), 0, (hipStream_t)stream, a);
+  HIP_TRY(h, hipGetLastError());
+  return MAM4X_OK;
+}
+
+int mam4x_compute_setsox(mam4x_handle h, double dt, const mam4x_setsox_args *in, void *stream) {
+  if (!h) return MAM4X_ERR_ARG;
+  if (!in) return fail(h, MAM4X_ERR_ARG, "args is NULL");
+  if (!(dt > 0.0)) return fail(h, MAM4X_ERR_ARG, "dt must be > 0");
+  if (in->ncol < 0 || in->nlev < 0) return fail(h, MAM4X_ERR_ARG, "negative ncol or nlev");
+  if (in->layout != MAM4X_LAYOUT_LEV_COL && in->layout != MAM4X_LAYOUT_COL_LEV)
+    return fail(h, MAM4X_ERR_ARG, "unknown layout");
+  const bool lc = in->layout == MAM4X_LAYOUT_LEV_COL;
+  const int fast = lc ? in->ncol : in->nlev, slow = lc ? in->nlev : in->ncol;
+  const int ld = in->ldc ? in->ldc : fast;
+  if (ld < fast) return fail(h, MAM4X_ERR_ARG, "ldc smaller than the fastest extent");
+  if ((long long)ld * slow > 0x7fffffffLL)
+    return fail(h, MAM4X_ERR_ARG, "domain too large (ldc x slow extent must fit in int32)");
+  SoxArgs a;
+  std::memset(&a, 0, sizeof(a));
+  const double *met[6] = {in->pressure, in->temperature, in->liquid_water, in->cloud_fraction,
+                          in->droplet_number, in->air_number_density};
+  for (int i = 0; i < 6; ++i) {
+    if (!met[i]) return fail(h, MAM4X_ERR_ARG, "an atmosphere plane is NULL");
+    a.met[i] = met[i];
+  }
+  for (int i = 0; i < 4; ++i) {
+    a.qin[i] = in->qin[i];
+    if (!a.qin[i]) return fail(h, MAM4X_ERR_ARG, "qin plane " + std::to_string(i) + " is NULL");
+  }
+  const int so4[3] = {6, 14, 21}, num[3] = {13, 18, 26};
+  for (int k = 0; k < 3; ++k) {
+    a.so4c[k] = in->qcw[so4[k]];
+    if (!a.so4c[k]) return fail(h, MAM4X_ERR_ARG, "qcw plane " + std::to_string(so4[k]) + " is NULL");
+  }
+  for (int k = 0; k < 3; ++k) {
+    a.numc[k] = in->qcw[num[k]];
+    if (!a.numc[k]) return fail(h, MAM4X_ERR_ARG, "qcw plane " + std::to_string(num[k]) + " is NULL");
+  }
+  for (int i = 0; i < MAM4X_GAS_PCNST; ++i) {
+    a.aq[i] = in->dqdt_aqso4[i];
+    a.h2[i] = in->dqdt_aqh2so4[i];
+  }
+  a.ncol = in->ncol;
+  a.nlev = in->nlev;
+  a.ldc = ld;
+  a.layout = in->layout;
+  a.dt = dt;
+  const long ncell = (long)in->ncol * in->nlev;
+  if (ncell == 0) return MAM4X_OK;
+  hipLaunchKernelGGL(setsox_kernel, dim3((unsigned)((ncell + kBlock - 1) / kBlock)), dim3(kBlock),
+                     0, (hipStream_t)stream, a);
   HIP_TRY(h, hipGetLastError());
   return MAM4X_OK;
 }

@@ -2810,3 +2810,288 @@ void orc_wet_size(int ncol, int nlev, double dt, const double *state,
     }
   }
 }
+
+/* ======================= aqueous sulfur chemistry ======================== */
+/* mo_setsox.hpp: setsox_single_level (:943-1259) and the routines it calls,
+ * per cell, on the gas_pcnst tracer arrays qin (gases, mol/mol) and qcw
+ * (cloud-borne aerosol, mol/mol and #/kmol) at the reference's indices
+ * (lptr_so4_cw_amode - loffset = 6, 14, 21; numptrcw_amode - loffset = 13,
+ * 18, 26, 30; O3 0, H2O2 1, H2SO4 2, SO2 3). */
+#define SOX_RA (8314.0 / 101325.0)
+#define SOX_XKW 1.0e-14
+#define SOX_CONST0 (1.0e3 / 6.023e23)
+#define SOX_CO2G 330.0e-6
+#define SOX_P0 101300.0
+static const int SOX_SO4_CW[3] = {6, 14, 21};
+static const int SOX_NUM_CW[4] = {13, 18, 26, 30};
+
+/* sox_cldaero_create_obj (:88-131) */
+void orc_sox_create_obj(double cldfrc, const double qcw[31], double lwc, double cfact,
+                        double out[3]) {
+  double xlwc = 0.0;
+  if (cldfrc > 0.0) {
+    xlwc = lwc * cfact;
+    xlwc = xlwc / cldfrc;
+  }
+  out[0] = qcw[SOX_SO4_CW[0]] + qcw[SOX_SO4_CW[1]] + qcw[SOX_SO4_CW[2]];
+  out[1] = xlwc;
+  out[2] = 1.0; /* so4_fact */
+}
+
+/* henry_factor_{so2,co2,h2o2,o3} (:133-186): species 0 SO2 (xk, xe, x2),
+ * 1 CO2, 2 H2O2 (xk, xe), 3 O3 (xk) */
+void orc_sox_henry(int species, double t_factor, double out[3]) {
+  out[0] = out[1] = out[2] = 0.0;
+  switch (species) {
+    case 0:
+      out[0] = 1.230 * M_exp(3120.0 * t_factor);
+      out[1] = 1.7e-2 * M_exp(2090.0 * t_factor);
+      out[2] = 6.0e-8 * M_exp(1120.0 * t_factor);
+      break;
+    case 1:
+      out[0] = 3.1e-2 * M_exp(2423.0 * t_factor);
+      out[1] = 4.3e-7 * M_exp(-913.0 * t_factor);
+      break;
+    case 2:
+      out[0] = 7.4e4 * M_exp(6621.0 * t_factor);
+      out[1] = 2.2e-12 * M_exp(-3730.0 * t_factor);
+      break;
+    default:
+      out[0] = 1.15e-2 * M_exp(2560.0 * t_factor);
+  }
+}
+
+/* calc_ynetpos (:189-223): [H+] of pH yph and the net positive charge */
+double orc_sox_ynetpos(double yph, const double f[4], double Eco2, double Eh2o, double Eso4,
+                       double so4_fact, double *xph) {
+  *xph = M_pow(10.0, -yph);
+  const double Eso2 = f[0] / (1.0 + f[1] * (1.0 + (f[2] / *xph) * (1.0 + f[3] / *xph)));
+  const double tmp_hso3 = Eso2 / *xph;
+  const double tmp_so3 = tmp_hso3 * 2.0 * f[3] / *xph;
+  const double tmp_hco3 = Eco2 / *xph;
+  const double tmp_oh = Eh2o / *xph;
+  const double tmp_so4 = so4_fact * Eso4;
+  return *xph - (tmp_oh + tmp_hco3 + tmp_hso3 + tmp_so3 + tmp_so4);
+}
+
+/* calc_ph_values (:225-413): bisection on pH in [2, 7].  f (out, optional):
+ * fact1-4_so2, then Eco2, Eh2o, Eso4.  Returns [H+]; *converged as the
+ * reference sets it. */
+double orc_sox_ph(double temperature, double patm, double xlwc, double t_factor,
+                  double xso2, double xso4, double xhnm, double so4_fact, int *converged,
+                  double *fout) {
+  double h[3];
+  orc_sox_henry(0, t_factor, h);
+  double f[4];
+  f[0] = h[0] * h[1] * patm * xso2;
+  f[1] = h[0] * SOX_RA * temperature * xlwc;
+  f[2] = h[1];
+  f[3] = h[2];
+  const double Eh2o = SOX_XKW;
+  orc_sox_henry(1, t_factor, h);
+  const double Eco2 = h[0] * h[1] * SOX_CO2G * patm;
+  const double Eso4 = xso4 * xhnm * SOX_CONST0 / xlwc;
+  if (fout) {
+    for (int i = 0; i < 4; ++i) fout[i] = f[i];
+    fout[4] = Eco2;
+    fout[5] = Eh2o;
+    fout[6] = Eso4;
+  }
+  double xph, ynetpos;
+  *converged = 0;
+  double yph_lo = 2.00, yph_hi = yph_lo, yph = yph_lo;
+  ynetpos = orc_sox_ynetpos(yph, f, Eco2, Eh2o, Eso4, so4_fact, &xph);
+  if (ynetpos <= 0.0) *converged = 1;
+  yph_hi = 7.0;
+  yph = yph_hi;
+  ynetpos = orc_sox_ynetpos(yph, f, Eco2, Eh2o, Eso4, so4_fact, &xph);
+  if (ynetpos >= 0.0) *converged = 1;
+  for (int i = 3; i < 20; ++i) {
+    yph = 0.5 * (yph_lo + yph_hi);
+    ynetpos = orc_sox_ynetpos(yph, f, Eco2, Eh2o, Eso4, so4_fact, &xph);
+    if (ynetpos >= 0.0)
+      yph_lo = yph;
+    else
+      yph_hi = yph;
+    if (fabs(yph_hi - yph_lo) <= 0.005) {
+      yph = 0.5 * (yph_hi + yph_lo);
+      xph = M_pow(10.0, -yph);
+      *converged = 1;
+      return xph;
+    }
+  }
+  return xph;
+}
+
+/* compute_aer_factor (:524-595) */
+void orc_sox_aer_factor(const double tmr[31], double faq[4]) {
+  double sumf = 0.0, qnum_c[4];
+  for (int m = 0; m < 4; ++m) qnum_c[m] = faq[m] = 0.0;
+  for (int m = 0; m < 4; ++m) {
+    qnum_c[m] = kmax(0.0, tmr[SOX_NUM_CW[m]]);
+    if (m == NACC) qnum_c[m] = kmax(1.0e-10, qnum_c[m]);
+    if (m < 3) { /* lptr_so4_cw_amode > 0 for acc, ait, coarse */
+      faq[m] = qnum_c[m];
+      sumf += faq[m];
+    }
+  }
+  if (sumf > 0.0)
+    for (int m = 0; m < 4; ++m) faq[m] /= sumf;
+}
+
+/* cldaero_uptakerate (:597-695) */
+double orc_sox_uptakerate(double xl, double cldnum, double cfact, double cldfrc, double tfld,
+                          double press) {
+  const double pi_x4 = 12.56637, one_third = 0.3333333, cm3_to_L = 1.0e-3;
+  double num_cd = cm3_to_L * cldnum * cfact / cldfrc;
+  num_cd = kmax(num_cd, 0.0);
+  const double volx34pi_cd = (xl * 0.75) / 3.14159265358979323846;
+  double radxnum_cd = M_pow(volx34pi_cd * sq(num_cd), one_third);
+  double rad_cd;
+  if (radxnum_cd <= volx34pi_cd * 4.0e4) {
+    radxnum_cd = volx34pi_cd * 4.0e4;
+    rad_cd = 50.0e-4;
+  } else if (radxnum_cd >= volx34pi_cd * 4.0e8) {
+    radxnum_cd = volx34pi_cd * 4.0e8;
+    rad_cd = 0.5e-4;
+  } else {
+    rad_cd = radxnum_cd / num_cd;
+  }
+  const double gasdiffus = 0.557 * M_pow(tfld, 1.75) / press;
+  const double gasspeed = 1.455e4 * M_sqrt(tfld / 98.0);
+  const double knudsen = 3.0 * gasdiffus / (gasspeed * rad_cd);
+  const double fs = (0.4875 * (1.0 + knudsen)) / (knudsen * (1.184 + knudsen) + 0.4875);
+  return pi_x4 * radxnum_cd * gasdiffus * fs;
+}
+
+/* setsox_single_level (:943-1259) with calc_sox_aqueous (:415-522) and
+ * sox_cldaero_update (:732-941) inline.  qcw/qin updated in place;
+ * dqdt_aqso4/dqdt_aqh2so4 [31] zeroed and filled; ph_out (optional): the
+ * [H+] used. */
+void orc_setsox_single_level(double dt, double press, double tfld, double lwc, double cldfrc,
+                             double cldnum, double xhnm, double qcw[31], double qin[31],
+                             double dqdt_aqso4[31], double dqdt_aqh2so4[31], double *ph_out) {
+  const double t298K = 298.0;
+  double xph = M_pow(10, -5.0); /* ph0 */
+  const double cfact = xhnm * 1.0e6 * 1.38e-23 / 287.0 * 1.0e-3;
+  double cc[3];
+  orc_sox_create_obj(cldfrc, qcw, lwc, cfact, cc);
+  double xso4 = 0.0;
+  double xso2 = qin[3], xh2o2 = qin[1], xo3 = qin[0], xh2so4 = qin[2];
+  const double xso4c = cc[0];
+  double xlwc = cc[1];
+  if (xlwc >= 1.0e-8) {
+    const double t_factor = (1.0 / tfld) - (1.0 / t298K);
+    const double patm = press / SOX_P0;
+    if (cldfrc > 0.0) xso4 = xso4c / cldfrc; /* cloud_borne */
+    int converged = 0;
+    xph = orc_sox_ph(tfld, patm, xlwc, t_factor, xso2, xso4, xhnm, cc[2], &converged, NULL);
+  } else {
+    xph = 1.0e-7;
+  }
+  if (ph_out) *ph_out = xph;
+  const double t_factor = (1.0 / tfld) - (1.0 / t298K);
+  xlwc = cc[1];
+  const double patm = press / SOX_P0;
+  double h[3];
+  orc_sox_henry(2, t_factor, h);
+  const double heh2o2 = h[0] * (1.0 + h[1] / xph);
+  orc_sox_henry(0, t_factor, h);
+  const double heso2 = h[0] * (1.0 + h[1] / xph * (1.0 + h[2] / xph));
+  orc_sox_henry(3, t_factor, h);
+  const double heo3 = h[0];
+  const double tz = tfld;
+  double px = heh2o2 * SOX_RA * tz * xlwc;
+  const double h2o2g = xh2o2 / (1.0 + px);
+  px = heso2 * SOX_RA * tz * xlwc;
+  const double so2g = xso2 / (1.0 + px);
+  px = heo3 * SOX_RA * tz * xlwc;
+  const double o3g = xo3 / (1.0 + px);
+  const double rah2o2 = 8.e4 * M_exp(-3650.0 * t_factor) / (0.1 + xph);
+  const double rao3 = 4.39e11 * M_exp(-4131.0 / tz) + 2.56e3 * M_exp(-996.0 / tz) / xph;
+  double xdelso4hp = 0.0, xso4_init = 0.0;
+  if (xlwc >= 1.0e-8) {
+    /* calc_sox_aqueous (:415-522) */
+    double pso4 = rah2o2 * 7.4e4 * M_exp(6621.0 * t_factor) * h2o2g * patm * 1.23 *
+                  M_exp(3120.0 * t_factor) * so2g * patm;
+    pso4 = pso4 * xlwc / SOX_CONST0 / xhnm;
+    double delta_s = kmax(pso4 * dt, 1.0e-30);
+    xso4_init = xso4;
+    if ((delta_s <= xso2) && (delta_s <= xh2o2)) {
+      xso4 = xso4 + delta_s;
+      xh2o2 = xh2o2 - delta_s;
+      xso2 = xso2 - delta_s;
+    } else if (xh2o2 > xso2) {
+      xso4 = xso4 + xso2;
+      xh2o2 = xh2o2 - xso2;
+      xso2 = 1.0e-20;
+    } else {
+      xso4 = xso4 + xh2o2;
+      xso2 = xso2 - xh2o2;
+      xh2o2 = 1.0e-20;
+    }
+    xdelso4hp = xso4 - xso4_init; /* modal_aerosols */
+    pso4 = rao3 * heo3 * o3g * patm * heso2 * so2g * patm;
+    pso4 = pso4 * xlwc / SOX_CONST0 / xhnm;
+    delta_s = kmax(pso4 * dt, 1.0e-30);
+    xso4_init = xso4;
+    if (delta_s > xso2) {
+      xso4 = xso4 + xso2;
+      xso2 = 1.0e-20;
+    } else {
+      xso4 = xso4 + delta_s;
+      xso2 = xso2 - delta_s;
+    }
+  }
+  /* sox_cldaero_update (:732-941) */
+  for (int i = 0; i < 31; ++i) dqdt_aqso4[i] = dqdt_aqh2so4[i] = 0.0;
+  if ((cldfrc >= 1.0e-5) && (xlwc >= 1.0e-8)) {
+    double faq[4];
+    orc_sox_aer_factor(qcw, faq);
+    double uptkrate = orc_sox_uptakerate(xlwc, cldnum, cfact, cldfrc, tfld, press);
+    uptkrate = (1.0 - M_exp(-1.0 * kmin(100.0, dt * uptkrate))) / dt;
+    const double dso4dt_gasuptk = xh2so4 * uptkrate;
+    const double delso4_o3rxn = xso4 - xso4_init;
+    const double dso4dt_aqrxn = (delso4_o3rxn + xdelso4hp) / dt;
+    const double dso4dt_hprxn = xdelso4hp / dt;
+    for (int m = 0; m < 3; ++m) {
+      const int l = SOX_SO4_CW[m];
+      dqdt_aqso4[l] = faq[m] * dso4dt_aqrxn * cldfrc;
+      dqdt_aqh2so4[l] = faq[m] * dso4dt_gasuptk * cldfrc;
+      const double dqdt_aq = dqdt_aqso4[l] + dqdt_aqh2so4[l];
+      qcw[l] += (dqdt_aq + 0.0) * dt;
+    }
+    qin[2] = qin[2] - dso4dt_gasuptk * dt * cldfrc;
+    qin[3] += (-dso4dt_aqrxn * cldfrc + 0.0) * dt;
+    qin[1] += (-dso4dt_hprxn * cldfrc + 0.0) * dt;
+  }
+  for (int m = 0; m < 3; ++m) qcw[SOX_SO4_CW[m]] = kmax(qcw[SOX_SO4_CW[m]], 1.0e-20);
+  qin[3] = kmax(qin[3], 1.0e-20);
+}
+
+/* setsox over a domain of planes [.][ncell]: met planes press, tfld, lwc,
+ * cldfrc, cldnum, xhnm; qcw/qin [31][ncell] updated in place; dqdt_*
+ * [31][ncell] (NULL to skip). */
+void orc_setsox(long ncell, double dt, const double *press, const double *tfld, const double *lwc,
+                const double *cldfrc, const double *cldnum, const double *xhnm, double *qcw,
+                double *qin, double *dqdt_aqso4, double *dqdt_aqh2so4) {
+#ifdef _OPENMP
+#pragma omp parallel for schedule(static)
+#endif
+  for (long c = 0; c < ncell; ++c) {
+    if (g_noise_seed) noise_cell(c);
+    double qc[31], qi[31], d1[31], d2[31];
+    for (int i = 0; i < 31; ++i) {
+      qc[i] = qcw[(long)i * ncell + c];
+      qi[i] = qin[(long)i * ncell + c];
+    }
+    orc_setsox_single_level(dt, press[c], tfld[c], lwc[c], cldfrc[c], cldnum[c], xhnm[c], qc, qi,
+                            d1, d2, NULL);
+    for (int i = 0; i < 31; ++i) {
+      qcw[(long)i * ncell + c] = qc[i];
+      qin[(long)i * ncell + c] = qi[i];
+      if (dqdt_aqso4) dqdt_aqso4[(long)i * ncell + c] = d1[i];
+      if (dqdt_aqh2so4) dqdt_aqh2so4[(long)i * ncell + c] = d2[i];
+    }
+  }
+}

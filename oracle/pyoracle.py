@@ -111,6 +111,97 @@ class Oracle:
         L.orc_wet_size.argtypes = [ctypes.c_int, ctypes.c_int, ctypes.c_double] + [_dp] * 11
         L.orc_wet_size.restype = None
 
+    # ---- aqueous sulfur chemistry (mo_setsox.hpp) ----
+    def _sox_sigs(self):
+        L = self.lib
+        if getattr(self, "_sox_ready", False):
+            return
+        L.orc_sox_create_obj.argtypes = [ctypes.c_double, _dp, ctypes.c_double, ctypes.c_double,
+                                         _dp]
+        L.orc_sox_create_obj.restype = None
+        L.orc_sox_henry.argtypes = [ctypes.c_int, ctypes.c_double, _dp]
+        L.orc_sox_henry.restype = None
+        L.orc_sox_ynetpos.argtypes = [ctypes.c_double, _dp] + [ctypes.c_double] * 4 + [_dp]
+        L.orc_sox_ynetpos.restype = ctypes.c_double
+        L.orc_sox_ph.argtypes = [ctypes.c_double] * 8 + [ctypes.POINTER(ctypes.c_int), _dp]
+        L.orc_sox_ph.restype = ctypes.c_double
+        L.orc_sox_aer_factor.argtypes = [_dp, _dp]
+        L.orc_sox_aer_factor.restype = None
+        L.orc_sox_uptakerate.argtypes = [ctypes.c_double] * 6
+        L.orc_sox_uptakerate.restype = ctypes.c_double
+        L.orc_setsox_single_level.argtypes = [ctypes.c_double] * 7 + [_dp] * 5
+        L.orc_setsox_single_level.restype = None
+        L.orc_setsox.argtypes = [ctypes.c_long, ctypes.c_double] + [_dp] * 10
+        L.orc_setsox.restype = None
+        self._sox_ready = True
+
+    def sox_create_obj(self, cldfrc, qcw, lwc, cfact):
+        self._sox_sigs()
+        out = np.zeros(3)
+        q = np.ascontiguousarray(qcw, dtype=np.float64)
+        self.lib.orc_sox_create_obj(cldfrc, _p(q), lwc, cfact, _p(out))
+        return out
+
+    def sox_henry(self, species, t_factor):
+        """0 SO2 (xk, xe, x2), 1 CO2 (xk, xe), 2 H2O2 (xk, xe), 3 O3 (xk)"""
+        self._sox_sigs()
+        out = np.zeros(3)
+        self.lib.orc_sox_henry(species, t_factor, _p(out))
+        return out
+
+    def sox_ynetpos(self, yph, f, Eco2, Eh2o, Eso4, so4_fact=1.0):
+        self._sox_sigs()
+        xph = np.zeros(1)
+        fa = np.ascontiguousarray(f, dtype=np.float64)
+        y = self.lib.orc_sox_ynetpos(yph, _p(fa), Eco2, Eh2o, Eso4, so4_fact, _p(xph))
+        return y, xph[0]
+
+    def sox_ph(self, T, patm, xlwc, t_factor, xso2, xso4, xhnm, so4_fact=1.0):
+        """([H+], converged, factors[7] = fact1-4_so2, Eco2, Eh2o, Eso4)"""
+        self._sox_sigs()
+        conv = ctypes.c_int(0)
+        f = np.zeros(7)
+        x = self.lib.orc_sox_ph(T, patm, xlwc, t_factor, xso2, xso4, xhnm, so4_fact,
+                                ctypes.byref(conv), _p(f))
+        return x, bool(conv.value), f
+
+    def sox_aer_factor(self, tmr):
+        self._sox_sigs()
+        t = np.ascontiguousarray(tmr, dtype=np.float64)
+        out = np.zeros(4)
+        self.lib.orc_sox_aer_factor(_p(t), _p(out))
+        return out
+
+    def sox_uptakerate(self, xl, cldnum, cfact, cldfrc, tfld, press):
+        self._sox_sigs()
+        return self.lib.orc_sox_uptakerate(xl, cldnum, cfact, cldfrc, tfld, press)
+
+    def setsox_single_level(self, dt, press, tfld, lwc, cldfrc, cldnum, xhnm, qcw, qin):
+        """one cell: returns (qcw, qin, dqdt_aqso4, dqdt_aqh2so4, [H+])"""
+        self._sox_sigs()
+        qc = np.array(qcw, dtype=np.float64)
+        qi = np.array(qin, dtype=np.float64)
+        d1, d2, ph = np.zeros(31), np.zeros(31), np.zeros(1)
+        self.lib.orc_setsox_single_level(dt, press, tfld, lwc, cldfrc, cldnum, xhnm, _p(qc),
+                                         _p(qi), _p(d1), _p(d2), _p(ph))
+        return qc, qi, d1, d2, ph[0]
+
+    def setsox(self, dt, met, qcw, qin):
+        """a domain: met dict of planes (press, tfld, lwc, cldfrc, cldnum,
+        xhnm; any common shape), qcw/qin [31, ...]; returns (qcw, qin,
+        dqdt_aqso4, dqdt_aqh2so4)"""
+        self._sox_sigs()
+        shape = np.shape(met["tfld"])
+        n = int(np.prod(shape))
+        m = [np.ascontiguousarray(met[k], dtype=np.float64).reshape(n)
+             for k in ("press", "tfld", "lwc", "cldfrc", "cldnum", "xhnm")]
+        qc = np.array(qcw, dtype=np.float64).reshape(31, n)
+        qi = np.array(qin, dtype=np.float64).reshape(31, n)
+        d1, d2 = np.zeros((31, n)), np.zeros((31, n))
+        self.lib.orc_setsox(n, dt, *[_p(a) for a in m], _p(qc), _p(qi), _p(d1), _p(d2))
+        sh = (31,) + tuple(shape)
+        return qc.reshape(sh), qi.reshape(sh), d1.reshape(sh), d2.reshape(sh)
+
     def set_libm_noise(self, seed):
         """seed != 0: randomly move each exp/log/pow/cbrt/erfc result by one
         ulp (conditioning probe); 0: exact libm."""

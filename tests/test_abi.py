@@ -45,6 +45,8 @@ def test_struct_layouts_match_c(tmp_path):
                     ' offsetof(mam4x_outputs, process_stride));'
                     'printf("%zu %zu %zu\\n", sizeof(mam4x_tracers), sizeof(mam4x_fields),'
                     ' offsetof(mam4x_fields, field));'
+                    'printf("%zu %zu\\n", sizeof(mam4x_setsox_args),'
+                    ' offsetof(mam4x_setsox_args, dqdt_aqh2so4));'
                     'return 0;}\n')
     exe = tmp_path / "sz"
     subprocess.run(["gcc", "-I", os.path.join(ROOT, "include"), "-o", str(exe), str(prog)],
@@ -54,7 +56,8 @@ def test_struct_layouts_match_c(tmp_path):
     py = [ctypes.sizeof(_abi.Config), ctypes.sizeof(_abi.Inputs), ctypes.sizeof(_abi.Outputs),
           _abi.Inputs.precldchem.offset, _abi.Outputs.state_out.offset,
           _abi.Outputs.process_tendencies.offset, _abi.Outputs.process_stride.offset,
-          ctypes.sizeof(_abi.Tracers), ctypes.sizeof(_abi.Fields), _abi.Fields.field.offset]
+          ctypes.sizeof(_abi.Tracers), ctypes.sizeof(_abi.Fields), _abi.Fields.field.offset,
+          ctypes.sizeof(_abi.SetsoxArgs), _abi.SetsoxArgs.dqdt_aqh2so4.offset]
     assert c == py
 
 
