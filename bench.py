@@ -496,6 +496,94 @@ def bench_setsox(args):
     print(json.dumps(res), flush=True)
 
 
+def bench_gas_chem(args):
+    """Gas-phase chemistry (mam4x_compute_gas_chemistry; --workload gaschem)
+    over 21,600 x 72 synthetic cells: kernel time from HIP events on the
+    launch stream, HBM roofline of the values moved, the oracle on the host
+    cores."""
+    import ctypes
+    import numpy as np
+    import torch
+    from mam4xx_amd import _abi
+    from mam4xx_amd.chain import Chain
+    import oracle
+    n = NCOL * NLEV
+    rng = np.random.default_rng(0)
+    temp = rng.uniform(200, 310, n)
+    photo = 10 ** rng.uniform(-8, -4, n)
+    M = 101325.0 / (1.380649e-23 * 298.0) * 1e-6 * rng.uniform(0.1, 1.0, n)
+    inv = np.stack([M, 0.78 * M, 0.21 * M, rng.uniform(0, 0.02, n) * M,
+                    10 ** rng.uniform(5, 7, n), 10 ** rng.uniform(5, 8, n),
+                    10 ** rng.uniform(7, 9, n), np.zeros(n)])
+    extfrc = 10 ** rng.uniform(0, 6, (9, n))
+    het = 10 ** rng.uniform(-8, -2, (31, n))
+    qq = 10 ** rng.uniform(-14, -7, (31, n))
+    sh = lambda a: torch.as_tensor(np.ascontiguousarray(a).reshape(a.shape[:-1] + (NLEV, NCOL)))
+    d = {k: sh(v).cuda() for k, v in (("temp", temp), ("photo", photo), ("inv", inv),
+                                       ("ext", extfrc), ("het", het))}
+    q0 = sh(qq).cuda()
+    q = q0.clone()
+    ch = Chain()
+    s = torch.cuda.current_stream()
+    a = _abi.GasChemArgs()
+    a.nlev, a.ncol, a.ldc, a.layout = NLEV, NCOL, NCOL, _abi.LAYOUT_LEV_COL
+    a.temperature, a.photolysis_rate = d["temp"].data_ptr(), d["photo"].data_ptr()
+    for i in range(9):
+        a.extfrc[i] = d["ext"][i].data_ptr()
+    for i in range(8):
+        a.invariants[i] = d["inv"][i].data_ptr()
+    for i in range(31):
+        a.het_rates[i] = d["het"][i].data_ptr()
+        a.qq[i] = q[i].data_ptr()
+    fn, h, sp, ra = (ch.lib.mam4x_compute_gas_chemistry, ch.h, ctypes.c_void_p(s.cuda_stream),
+                     ctypes.byref(a))
+    ev = [torch.cuda.Event(enable_timing=True) for _ in range(2)]
+    for _ in range(args.warmup + 1):
+        q.copy_(q0)
+        assert fn(h, DT, ra, sp) == 0
+    torch.cuda.synchronize()
+    ms = 0.0
+    for _ in range(args.steps):
+        q.copy_(q0)  # every launch advances the same mixing ratios
+        ev[0].record(s)
+        fn(h, DT, ra, sp)
+        ev[1].record(s)
+        ev[1].synchronize()
+        ms += ev[0].elapsed_time(ev[1])
+    ms /= args.steps
+    # values per cell: T, photolysis, 9 forcings, 5 invariants, 30 washout
+    # rates and 30 mixing ratios read, 30 mixing ratios written
+    bytes_cell = (2 + 9 + 5 + 30 + 30 + 30) * 8
+    achieved = n * bytes_cell / (ms * 1e-3) / 1e9
+    res = {"metric": "gas-phase chemistry (implicit solve, 30 species): column-steps/sec at 72 "
+                     "lev", "unit": "column-steps/s", "value": NCOL / (ms * 1e-3), "n_gpus": 1,
+           "higher_is_better": True, "dtype": "f64", "data": "synthetic", "steps": args.steps,
+           "warmup": args.warmup,
+           "config": {"workload": "21600 columns x 72 levels of synthetic cells, "
+                                  "gas_phase_chemistry per cell, dt 1800 s",
+                      "columns": NCOL, "levels": NLEV},
+           "kernel_ms": ms,
+           "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS,
+                        "unit": "GB/s", "frac": achieved / HBM_PEAK_GBS,
+                        "traffic": _traffic("gas_chem_kernel_%dx%d.txt" % (NCOL, NLEV)),
+                        "algorithmic_bytes_per_cell": bytes_cell}}
+    if not args.no_cpu:
+        orc = oracle.Oracle()
+        nth = min(16, os.cpu_count() or 1)
+        run = lambda: orc.gas_phase_chemistry(DT, temp, photo, extfrc, inv, het, qq)
+        run()
+        best = 1e30
+        for _ in range(5):
+            t0 = time.perf_counter()
+            run()
+            best = min(best, time.perf_counter() - t0)
+        res["cpu_baseline"] = {"value": NCOL / best, "unit": "column-steps/s", "cores": nth,
+                               "kind": "port", "sample": "the full 21600 x 72 domain, best of 5 "
+                               "after a warm-up (OpenMP over cells)", "cpu": _cpu_model()}
+    ch.close()
+    print(json.dumps(res), flush=True)
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -505,12 +593,12 @@ def main():
     ap.add_argument("--schedule", choices=["auto", "fused", "queue"], default="auto")
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline")
     ap.add_argument("--workload", choices=["headline", "batch", "large", "water", "wetsize",
-                                           "tracers", "setsox"],
+                                           "tracers", "setsox", "gaschem"],
                     default="headline",
                     help="headline: the metric's 21,600 x 72 step (default); batch, large, "
                          "water (aerosol water uptake), wetsize (calcsize + water uptake), "
-                         "tracers (state_q/qqcw <-> state planes) and setsox (aqueous sulfur "
-                         "chemistry) print their own line")
+                         "tracers (state_q/qqcw <-> state planes), setsox (aqueous sulfur "
+                         "chemistry) and gaschem (gas-phase chemistry) print their own line")
     args = ap.parse_args()
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) > 1:
         print("bench.py measures one GPU: the chain has no cross-GPU step and this build "
@@ -534,6 +622,8 @@ def main():
         return bench_tracers(args)
     if args.workload == "setsox":
         return bench_setsox(args)
+    if args.workload == "gaschem":
+        return bench_gas_chem(args)
     scene = scenes.generate(NCOL, NLEV, seed=0)
     d = scene_to_device(scene)
     # chained steps: calcsize diameters feed the next step's dry and wet

@@ -375,6 +375,40 @@ typedef struct mam4x_setsox_args {
  * stream-ordered; no workspace. */
 int mam4x_compute_setsox(mam4x_handle h, double dt, const mam4x_setsox_args *args, void *stream);
 
+/* ---------------------------------------------------------------------- */
+/* Gas-phase chemistry: mam4::microphysics::gas_phase_chemistry            */
+/* (gas_phase_chemistry.hpp:37-116) with the pp_linoz_mam4_resus_mom_soag  */
+/* mechanism (gas_chem_mechanism.hpp) and its implicit Euler solver        */
+/* (gas_chem.hpp:73-425), for every cell of a domain: rate constants, the  */
+/* Newton-Raphson solve of the 30 implicit species with the reference's   */
+/* step-cutting.  Its H2SO4 change is the gas production the microphysics */
+/* reads as pregaschem.  Planes: extents and layout as in mam4x_inputs.   */
+#define MAM4X_EXTCNT 9 /* species with external forcing                 */
+#define MAM4X_NFS 8    /* fixed species (invariants)                     */
+typedef struct mam4x_gas_chem_args {
+  int ncol, nlev;
+  int ldc;
+  int layout;
+  const double *temperature;     /* [K]                                    */
+  const double *photolysis_rate; /* H2O2 photolysis, photo_rates[0] [1/s] */
+  /* external in-situ forcing [#/cm3/s], the reference's extfrc order
+   * (NULL = 0); divided by the air density as the reference does */
+  const double *extfrc[MAM4X_EXTCNT];
+  /* fixed species [#/cm3]: 0 total density M, 3 H2O, 4 OH, 5 NO3, 6 HO2
+   * are read (others unused) */
+  const double *invariants[MAM4X_NFS];
+  /* washout rates [1/s] by gas_pcnst; entries 1-30 read, NULL = 0 */
+  const double *het_rates[MAM4X_GAS_PCNST];
+  /* mixing ratios [mol/mol] by gas_pcnst: 1-30 advanced in place (O3, 0,
+   * is not in the implicit class and is not used) */
+  double *qq[MAM4X_GAS_PCNST];
+} mam4x_gas_chem_args;
+
+/* gas_phase_chemistry over a domain with time step dt [s].  FP64;
+ * stream-ordered; no workspace. */
+int mam4x_compute_gas_chemistry(mam4x_handle h, double dt, const mam4x_gas_chem_args *args,
+                                void *stream);
+
 /* Batches: many scenes (domains of any column count, each with its own
  * inputs, outputs and layout) stepped by one set of launches.  The scenes'
  * cells form one work queue, so uneven scenes and uneven per-cell cost share

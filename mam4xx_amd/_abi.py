@@ -174,6 +174,25 @@ class SetsoxArgs(ctypes.Structure):
     ]
 
 
+EXTCNT = 9
+NFS = 8
+
+
+class GasChemArgs(ctypes.Structure):
+    _fields_ = [
+        ("ncol", ctypes.c_int),
+        ("nlev", ctypes.c_int),
+        ("ldc", ctypes.c_int),
+        ("layout", ctypes.c_int),
+        ("temperature", _vp),
+        ("photolysis_rate", _vp),
+        ("extfrc", _vp * EXTCNT),
+        ("invariants", _vp * NFS),
+        ("het_rates", _vp * GAS_PCNST),
+        ("qq", _vp * GAS_PCNST),
+    ]
+
+
 # entry points declared in include/mam4x.h, with ctypes signatures
 SIGNATURES = {
     "mam4x_abi_version": (ctypes.c_int, []),
@@ -202,6 +221,8 @@ SIGNATURES = {
                                                ctypes.POINTER(Fields), _vp]),
     "mam4x_fields_to_tracers": (ctypes.c_int, [_vp, ctypes.POINTER(Fields),
                                                ctypes.POINTER(Tracers), ctypes.c_int, _vp]),
+    "mam4x_compute_gas_chemistry": (ctypes.c_int, [_vp, ctypes.c_double,
+                                                   ctypes.POINTER(GasChemArgs), _vp]),
     "mam4x_compute_setsox": (ctypes.c_int, [_vp, ctypes.c_double, ctypes.POINTER(SetsoxArgs),
                                             _vp]),
     "mam4x_batch_create": (ctypes.c_int, [_vp, ctypes.c_int, ctypes.POINTER(Inputs),

@@ -215,6 +215,34 @@ class Chain:
                                               ctypes.c_void_p(s.cuda_stream))
         self._check(rc, "mam4x_fields_to_tracers")
 
+    def compute_gas_chemistry(self, dt, temperature, photolysis_rate, extfrc, invariants,
+                              het_rates, qq, layout=_abi.LAYOUT_LEV_COL, stream=None):
+        """Gas-phase chemistry (mam4x_compute_gas_chemistry).  temperature /
+        photolysis_rate: planes; extfrc [9][...], invariants [8][...],
+        het_rates / qq [31][...] tensors (qq advanced in place); None
+        extfrc / het_rates = 0."""
+        a = _abi.GasChemArgs()
+        t = temperature
+        assert t.dim() == 2 and t.dtype == torch.float64 and t.is_cuda and t.is_contiguous()
+        if layout == _abi.LAYOUT_LEV_COL:
+            a.nlev, a.ncol = t.shape
+        else:
+            a.ncol, a.nlev = t.shape
+        a.ldc, a.layout = t.shape[1], layout
+        a.temperature = t.data_ptr()
+        assert photolysis_rate.shape == t.shape and photolysis_rate.is_contiguous()
+        a.photolysis_rate = photolysis_rate.data_ptr()
+        for name, arr, n in (("extfrc", extfrc, _abi.EXTCNT), ("invariants", invariants, _abi.NFS),
+                             ("het_rates", het_rates, _abi.GAS_PCNST), ("qq", qq, _abi.GAS_PCNST)):
+            if arr is None:
+                continue
+            for i, p in enumerate(_plane_ptrs(arr, n)):
+                getattr(a, name)[i] = p
+        s = stream if stream is not None else torch.cuda.current_stream()
+        rc = self.lib.mam4x_compute_gas_chemistry(self.h, dt, ctypes.byref(a),
+                                                  ctypes.c_void_p(s.cuda_stream))
+        self._check(rc, "mam4x_compute_gas_chemistry")
+
     def compute_setsox(self, dt, met, qcw, qin, dqdt_aqso4=None, dqdt_aqh2so4=None,
                        layout=_abi.LAYOUT_LEV_COL, stream=None):
         """Aqueous sulfur chemistry (mam4x_compute_setsox).  met: dict of

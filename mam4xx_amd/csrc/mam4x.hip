@@ -32,6 +32,7 @@
 
 #include "../../include/mam4x.h"
 #include "chain_cell.hpp"
+#include "gas_chem.hpp"
 #include "setsox.hpp"
 #include "water_uptake.hpp"
 
@@ -709,6 +710,41 @@ __global__ void __launch_bounds__(kBlock) setsox_kernel(SoxArgs a) {
     if (a.aq[i]) a.aq[i][off] = k >= 0 ? r.aqso4[k] : 0.0;
     if (a.h2[i]) a.h2[i][off] = k >= 0 ? r.aqh2so4[k] : 0.0;
   }
+}
+
+// ------------------------------------------------------------------ gas chem
+// Gas-phase chemistry, one lane per cell (gas_chem.hpp).
+struct GasChemArgs {
+  const double *T, *photo;
+  const double *ext[gc::kExt];
+  const double *inv[5];  // M, H2O, OH, NO3, HO2
+  const double *het[MAM4X_GAS_PCNST];
+  double *qq[MAM4X_GAS_PCNST];
+  int ncol, nlev, ldc, layout;
+  double dt;
+};
+
+__global__ void __launch_bounds__(kBlock) gas_chem_kernel(GasChemArgs a) {
+  const long c = (long)blockIdx.x * kBlock + threadIdx.x;
+  if (c >= (long)a.ncol * a.nlev) return;
+  const int fast = a.layout == MAM4X_LAYOUT_COL_LEV ? a.nlev : a.ncol;
+  const long slow = c / fast;
+  const long off = slow * a.ldc + (c - slow * fast);
+  double inv[gc::kInv] = {a.inv[0][off], 0.0, 0.0, a.inv[1][off],
+                          a.inv[2][off], a.inv[3][off], a.inv[4][off], 0.0};
+  double ext[gc::kExt], het[31], y[31], rxt[gc::kRxn];
+#pragma unroll
+  for (int i = 0; i < gc::kExt; ++i) ext[i] = (a.ext[i] ? a.ext[i][off] : 0.0) / inv[0];
+  het[0] = y[0] = 0.0;
+#pragma unroll
+  for (int i = 1; i < 31; ++i) {
+    het[i] = a.het[i] ? a.het[i][off] : 0.0;
+    y[i] = a.qq[i][off];
+  }
+  gc::rates(a.T[off], a.photo[off], inv, rxt);
+  (void)gc::imp_sol(y, rxt, het, ext, a.dt);
+#pragma unroll
+  for (int i = 1; i < 31; ++i) a.qq[i][off] = y[i];
 }
 
 }  // namespace
@@ -1417,6 +1453,50 @@ int mam4x_fields_to_tracers(mam4x_handle h, const mam4x_fields *src, const mam4x
   const long ncell = (long)a.ncol * a.nlev;
   if (ncell == 0) return MAM4X_OK;
   hipLaunchKernelGGL(fields_to_tracers_kernel, dim3((unsigned)((ncell + kTile - 1) / kTile)),
+                     dim3(kBlock), 0, (hipStream_t)stream, a);
+  HIP_TRY(h, hipGetLastError());
+  return MAM4X_OK;
+}
+
+int mam4x_compute_gas_chemistry(mam4x_handle h, double dt, const mam4x_gas_chem_args *in,
+                                void *stream) {
+  if (!h) return MAM4X_ERR_ARG;
+  if (!in) return fail(h, MAM4X_ERR_ARG, "args is NULL");
+  if (!(dt > 0.0)) return fail(h, MAM4X_ERR_ARG, "dt must be > 0");
+  if (in->ncol < 0 || in->nlev < 0) return fail(h, MAM4X_ERR_ARG, "negative ncol or nlev");
+  if (in->layout != MAM4X_LAYOUT_LEV_COL && in->layout != MAM4X_LAYOUT_COL_LEV)
+    return fail(h, MAM4X_ERR_ARG, "unknown layout");
+  const bool lc = in->layout == MAM4X_LAYOUT_LEV_COL;
+  const int fast = lc ? in->ncol : in->nlev, slow = lc ? in->nlev : in->ncol;
+  const int ld = in->ldc ? in->ldc : fast;
+  if (ld < fast) return fail(h, MAM4X_ERR_ARG, "ldc smaller than the fastest extent");
+  if ((long long)ld * slow > 0x7fffffffLL)
+    return fail(h, MAM4X_ERR_ARG, "domain too large (ldc x slow extent must fit in int32)");
+  GasChemArgs a;
+  std::memset(&a, 0, sizeof(a));
+  a.T = in->temperature;
+  a.photo = in->photolysis_rate;
+  if (!a.T || !a.photo) return fail(h, MAM4X_ERR_ARG, "temperature or photolysis plane is NULL");
+  for (int i = 0; i < gc::kExt; ++i) a.ext[i] = in->extfrc[i];
+  const int used[5] = {0, 3, 4, 5, 6};
+  for (int k = 0; k < 5; ++k) {
+    a.inv[k] = in->invariants[used[k]];
+    if (!a.inv[k])
+      return fail(h, MAM4X_ERR_ARG, "invariant plane " + std::to_string(used[k]) + " is NULL");
+  }
+  for (int i = 1; i < MAM4X_GAS_PCNST; ++i) {
+    a.het[i] = in->het_rates[i];
+    a.qq[i] = in->qq[i];
+    if (!a.qq[i]) return fail(h, MAM4X_ERR_ARG, "qq plane " + std::to_string(i) + " is NULL");
+  }
+  a.ncol = in->ncol;
+  a.nlev = in->nlev;
+  a.ldc = ld;
+  a.layout = in->layout;
+  a.dt = dt;
+  const long ncell = (long)in->ncol * in->nlev;
+  if (ncell == 0) return MAM4X_OK;
+  hipLaunchKernelGGL(gas_chem_kernel, dim3((unsigned)((ncell + kBlock - 1) / kBlock)),
                      dim3(kBlock), 0, (hipStream_t)stream, a);
   HIP_TRY(h, hipGetLastError());
   return MAM4X_OK;

@@ -3095,3 +3095,207 @@ void orc_setsox(long ncell, double dt, const double *press, const double *tfld, 
     }
   }
 }
+
+/* ======================= gas-phase chemistry ============================= */
+/* gas_phase_chemistry (gas_phase_chemistry.hpp:37-116) with the
+ * pp_linoz_mam4_resus_mom_soag mechanism (gas_chem_mechanism.hpp) and the
+ * implicit solver imp_sol / newton_raphson_iter (gas_chem.hpp:14-425), per
+ * cell.  qq[31] is the gas_pcnst array (mol/mol); the implicit class is
+ * species 1..30 (clsmap_4), permute_4 the identity.  The mechanism's sparse
+ * LU has 32 entries: a diagonal plus the SO2 -> H2SO4 and DMS -> SO2
+ * couplings. */
+#define GC_RXNTOT 7
+#define GC_CLS 30
+#define GC_NZ 32
+#define GC_EXTCNT 9
+#define GC_NFS 8
+
+/* setrxt (gas_chem_mechanism.hpp:40-45), usrrxt (gas_chem.hpp:20-63) with
+ * the indices of gas_phase_chemistry.hpp:28-29, adjrxt (:63-71) */
+void orc_gc_rates(double temp, const double inv[GC_NFS], double rxt[GC_RXNTOT]) {
+  for (int i = 0; i < GC_RXNTOT; ++i) rxt[i] = 0.0;
+  rxt[2] = 2.9000000000e-12 * M_exp(-160.000000 / temp);
+  rxt[4] = 9.6000000000e-12 * M_exp(-234.000000 / temp);
+  rxt[6] = 1.9000000000e-13 * M_exp(520.000000 / temp);
+  const double mtot = inv[0];
+  { /* HO2 + HO2 -> H2O2 (index 1; water vapour invariant 3) */
+    const double ko = 3.5e-13 * M_exp(430.0 / temp);
+    const double kinf = 1.7e-33 * mtot * M_exp(1000. / temp);
+    const double fc = 1.0 + 1.4e-21 * inv[3] * M_exp(2200. / temp);
+    rxt[1] = (ko + kinf) * fc;
+  }
+  { /* DMS + OH -> 0.5 SO2 (index 5) */
+    const double ko = 1.0 + 5.5e-31 * M_exp(7460. / temp) * mtot * 0.21;
+    rxt[5] = 1.7e-42 * M_exp(7810. / temp) * mtot * 0.21 / ko;
+  }
+  { /* SO2 + OH -> SO4 (index 3) */
+    const double fc = 3.0e-31 * M_pow(300. / temp, 3.3);
+    const double ko = fc * mtot / (1.0 + fc * mtot / 1.5e-12);
+    const double lg = log10(fc * mtot / 1.5e-12);
+    rxt[3] = ko * M_pow(0.6, 1.0 / (1.0 + lg * lg));
+  }
+  rxt[2] *= inv[4];
+  rxt[3] *= inv[4];
+  rxt[4] *= inv[4];
+  rxt[5] *= inv[4];
+  rxt[6] *= inv[5];
+  rxt[1] *= inv[6] * inv[6] / mtot;
+}
+
+/* imp_prod_loss (gas_chem_mechanism.hpp:82-99) */
+static void gc_prod_loss(double prod[GC_CLS], double loss[GC_CLS], const double y[31],
+                         const double rxt[GC_RXNTOT], const double het[31]) {
+  loss[0] = (het[1] + rxt[0] + rxt[2]) * y[1];
+  prod[0] = 0.0;
+  loss[1] = het[2] * y[2];
+  prod[1] = rxt[3] * y[3];
+  loss[2] = (het[3] + rxt[3]) * y[3];
+  prod[2] = (rxt[4] + 0.500000 * rxt[5] + rxt[6]) * y[4];
+  loss[3] = (het[4] + rxt[4] + rxt[5] + rxt[6]) * y[4];
+  prod[3] = 0.0;
+  for (int i = 4; i < GC_CLS; ++i) {
+    loss[i] = het[i + 1] * y[i + 1];
+    prod[i] = 0.0;
+  }
+}
+
+/* indprd, class 4 (gas_chem_mechanism.hpp:100-143): the external forcing
+ * enters species H2O2 (via rxt[1]), SO2, DMS, SOAG and six aerosol tracers */
+static void gc_indprd(double prod[GC_CLS], const double rxt[GC_RXNTOT],
+                      const double extfrc[GC_EXTCNT]) {
+  for (int i = 0; i < GC_CLS; ++i) prod[i] = 0.0;
+  prod[0] = +rxt[1];
+  prod[2] = +extfrc[0];
+  prod[4] = +extfrc[8];
+  prod[5] = +extfrc[1];
+  prod[12] = +extfrc[5];
+  prod[13] = +extfrc[2];
+  prod[17] = +extfrc[6];
+  prod[26] = +extfrc[3];
+  prod[27] = +extfrc[4];
+  prod[29] = +extfrc[7];
+}
+
+/* linmat + nlnmat + lu_fac (gas_chem_mechanism.hpp:145-300): the factored
+ * system matrix; entries 2 and 4 are the couplings */
+static void gc_sys(double lu[GC_NZ], const double rxt[GC_RXNTOT], const double het[31],
+                   double dti) {
+  double m[GC_NZ];
+  m[0] = -(+rxt[0] + rxt[2] + het[1]);
+  m[1] = -(+het[2]);
+  m[2] = +rxt[3];
+  m[3] = -(+rxt[3] + het[3]);
+  m[4] = +rxt[4] + 0.500000 * rxt[5] + rxt[6];
+  m[5] = -(+rxt[4] + rxt[5] + rxt[6] + het[4]);
+  for (int i = 6; i < GC_NZ; ++i) m[i] = -(+het[i - 1]);
+  for (int i = 0; i < GC_NZ; ++i) lu[i] = (i == 2 || i == 4) ? m[i] : m[i] - dti;
+  for (int i = 0; i < GC_NZ; ++i)
+    if (i != 2 && i != 4) lu[i] = 1.0 / lu[i];
+}
+
+/* lu_slv (gas_chem_mechanism.hpp:183-228) */
+static void gc_lu_slv(const double lu[GC_NZ], double b[GC_CLS]) {
+  for (int k = 29; k >= 3; --k) b[k] *= lu[k + 2];
+  b[2] -= lu[4] * b[3];
+  b[2] *= lu[3];
+  b[1] -= lu[2] * b[2];
+  b[1] *= lu[1];
+  b[0] *= lu[0];
+}
+
+/* imp_sol (gas_chem.hpp:230-425) with newton_raphson_iter (:73-228).
+ * base_sol [31] advanced over delt in place; returns the number of time
+ * steps taken. */
+int orc_gc_imp_sol(double base_sol[31], const double rxt[GC_RXNTOT], const double het[31],
+                   const double extfrc[GC_EXTCNT], double delt) {
+  double ind_prd[GC_CLS], solution[GC_CLS], iter_inv[GC_CLS], prod[GC_CLS], loss[GC_CLS];
+  double lu[GC_NZ], forcing[GC_CLS];
+  gc_indprd(ind_prd, rxt, extfrc);
+  double dt = delt, interval_done = 0.0;
+  int cut_cnt = 0, fail_cnt = 0, stp_con_cnt = 0, nstep = 0;
+  for (int i = 0; i < 1000; ++i) {
+    ++nstep;
+    const double dti = 1.0 / dt;
+    for (int k = 0; k < GC_CLS; ++k) solution[k] = base_sol[k + 1];
+    for (int k = 0; k < GC_CLS; ++k) iter_inv[k] = dti * solution[k] + ind_prd[k];
+    int convergence = 0;
+    for (int it = 0; it < 11; ++it) {
+      gc_sys(lu, rxt, het, dti); /* factor[] is all true */
+      gc_prod_loss(prod, loss, base_sol, rxt, het);
+      for (int k = 0; k < GC_CLS; ++k)
+        forcing[k] = solution[k] * dti - (iter_inv[k] + prod[k] - loss[k]);
+      gc_lu_slv(lu, forcing);
+      for (int k = 0; k < GC_CLS; ++k) solution[k] += forcing[k];
+      for (int k = 0; k < GC_CLS; ++k)
+        if (solution[k] < 0.0) solution[k] = 0.0;
+      for (int k = 0; k < GC_CLS; ++k) base_sol[k + 1] = solution[k];
+      if (it > 0) {
+        convergence = 1;
+        for (int k = 0; k < GC_CLS; ++k) {
+          int conv = 1;
+          if (fabs(forcing[k]) > 1.0e-40) conv = fabs(forcing[k]) <= 1.0e-3 * fabs(solution[k]);
+          if (!conv) convergence = 0;
+        }
+        if (convergence) break;
+      }
+    }
+    if (!convergence) {
+      fail_cnt += fail_cnt; /* as the reference (stays 0) */
+      stp_con_cnt = 0;
+      if (cut_cnt < 5) {
+        cut_cnt += 1;
+        if (cut_cnt < 5)
+          dt *= 0.5;
+        else
+          dt *= 0.1;
+      }
+    }
+    interval_done += dt;
+    if (fabs(delt - interval_done) <= 0.0001) break;
+    if (convergence) stp_con_cnt += 1;
+    if (stp_con_cnt >= 2) {
+      dt *= 2.0;
+      stp_con_cnt = 0;
+    }
+    dt = kmin(dt, delt - interval_done);
+  }
+  for (int k = 0; k < GC_CLS; ++k) base_sol[k + 1] = solution[k];
+  return nstep;
+}
+
+/* gas_phase_chemistry for one cell: photo_rate = rxt[0] (phtcnt = 1);
+ * extfrc [1/cm3/s] scaled by the total density; qq[31] updated in place.
+ * Returns the time steps taken. */
+int orc_gas_phase_chemistry(double temp, double dt, double photo_rate,
+                            const double extfrc[GC_EXTCNT], const double inv[GC_NFS],
+                            const double het[31], double qq[31]) {
+  double rxt[GC_RXNTOT], ext[GC_EXTCNT];
+  orc_gc_rates(temp, inv, rxt);
+  for (int m = 0; m < GC_EXTCNT; ++m) ext[m] = extfrc[m] / inv[0];
+  rxt[0] = photo_rate;
+  return orc_gc_imp_sol(qq, rxt, het, ext, dt);
+}
+
+/* over a domain: temp, photo [ncell]; extfrc [9][ncell], inv [8][ncell],
+ * het [31][ncell], qq [31][ncell] in place; steps (optional) [ncell] */
+void orc_gas_phase_chemistry_domain(long ncell, double dt, const double *temp,
+                                    const double *photo, const double *extfrc,
+                                    const double *inv, const double *het, double *qq,
+                                    int *steps) {
+#ifdef _OPENMP
+#pragma omp parallel for schedule(static)
+#endif
+  for (long c = 0; c < ncell; ++c) {
+    if (g_noise_seed) noise_cell(c);
+    double e[GC_EXTCNT], v[GC_NFS], h[31], q[31];
+    for (int i = 0; i < GC_EXTCNT; ++i) e[i] = extfrc[(long)i * ncell + c];
+    for (int i = 0; i < GC_NFS; ++i) v[i] = inv[(long)i * ncell + c];
+    for (int i = 0; i < 31; ++i) {
+      h[i] = het[(long)i * ncell + c];
+      q[i] = qq[(long)i * ncell + c];
+    }
+    const int n = orc_gas_phase_chemistry(temp[c], dt, photo[c], e, v, h, q);
+    for (int i = 0; i < 31; ++i) qq[(long)i * ncell + c] = q[i];
+    if (steps) steps[c] = n;
+  }
+}

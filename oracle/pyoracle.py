@@ -202,6 +202,56 @@ class Oracle:
         sh = (31,) + tuple(shape)
         return qc.reshape(sh), qi.reshape(sh), d1.reshape(sh), d2.reshape(sh)
 
+    # ---- gas-phase chemistry (gas_chem.hpp, gas_phase_chemistry.hpp) ----
+    def _gc_sigs(self):
+        L = self.lib
+        if getattr(self, "_gc_ready", False):
+            return
+        L.orc_gc_rates.argtypes = [ctypes.c_double, _dp, _dp]
+        L.orc_gc_rates.restype = None
+        L.orc_gc_imp_sol.argtypes = [_dp, _dp, _dp, _dp, ctypes.c_double]
+        L.orc_gc_imp_sol.restype = ctypes.c_int
+        L.orc_gas_phase_chemistry.argtypes = [ctypes.c_double] * 3 + [_dp] * 4
+        L.orc_gas_phase_chemistry.restype = ctypes.c_int
+        L.orc_gas_phase_chemistry_domain.argtypes = [ctypes.c_long, ctypes.c_double] + [_dp] * 6 + \
+            [ctypes.POINTER(ctypes.c_int)]
+        L.orc_gas_phase_chemistry_domain.restype = None
+        self._gc_ready = True
+
+    def gc_rates(self, temp, inv):
+        """reaction rates rxt[7] after setrxt/usrrxt/adjrxt (rxt[0], the
+        photolysis rate, left 0)"""
+        self._gc_sigs()
+        out = np.zeros(7)
+        v = np.ascontiguousarray(inv, dtype=np.float64)
+        self.lib.orc_gc_rates(temp, _p(v), _p(out))
+        return out
+
+    def gc_imp_sol(self, base_sol, rxt, het, extfrc, delt):
+        """imp_sol on one cell: (new base_sol[31], time steps)"""
+        self._gc_sigs()
+        y = np.array(base_sol, dtype=np.float64)
+        r, h, e = (np.ascontiguousarray(a, dtype=np.float64) for a in (rxt, het, extfrc))
+        n = self.lib.orc_gc_imp_sol(_p(y), _p(r), _p(h), _p(e), delt)
+        return y, n
+
+    def gas_phase_chemistry(self, dt, temp, photo, extfrc, inv, het, qq):
+        """a domain: temp/photo [...], extfrc [9, ...], inv [8, ...], het /
+        qq [31, ...]; returns (qq, steps)"""
+        self._gc_sigs()
+        shape = np.shape(temp)
+        n = int(np.prod(shape))
+        t = np.ascontiguousarray(temp, dtype=np.float64).reshape(n)
+        ph = np.ascontiguousarray(photo, dtype=np.float64).reshape(n)
+        e = np.ascontiguousarray(extfrc, dtype=np.float64).reshape(9, n)
+        v = np.ascontiguousarray(inv, dtype=np.float64).reshape(8, n)
+        h = np.ascontiguousarray(het, dtype=np.float64).reshape(31, n)
+        q = np.array(qq, dtype=np.float64).reshape(31, n)
+        st = np.zeros(n, dtype=np.int32)
+        self.lib.orc_gas_phase_chemistry_domain(n, dt, _p(t), _p(ph), _p(e), _p(v), _p(h), _p(q),
+                                                st.ctypes.data_as(ctypes.POINTER(ctypes.c_int)))
+        return q.reshape((31,) + tuple(shape)), st.reshape(shape)
+
     def set_libm_noise(self, seed):
         """seed != 0: randomly move each exp/log/pow/cbrt/erfc result by one
         ulp (conditioning probe); 0: exact libm."""

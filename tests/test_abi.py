@@ -47,6 +47,8 @@ def test_struct_layouts_match_c(tmp_path):
                     ' offsetof(mam4x_fields, field));'
                     'printf("%zu %zu\\n", sizeof(mam4x_setsox_args),'
                     ' offsetof(mam4x_setsox_args, dqdt_aqh2so4));'
+                    'printf("%zu %zu\\n", sizeof(mam4x_gas_chem_args),'
+                    ' offsetof(mam4x_gas_chem_args, qq));'
                     'return 0;}\n')
     exe = tmp_path / "sz"
     subprocess.run(["gcc", "-I", os.path.join(ROOT, "include"), "-o", str(exe), str(prog)],
@@ -57,7 +59,8 @@ def test_struct_layouts_match_c(tmp_path):
           _abi.Inputs.precldchem.offset, _abi.Outputs.state_out.offset,
           _abi.Outputs.process_tendencies.offset, _abi.Outputs.process_stride.offset,
           ctypes.sizeof(_abi.Tracers), ctypes.sizeof(_abi.Fields), _abi.Fields.field.offset,
-          ctypes.sizeof(_abi.SetsoxArgs), _abi.SetsoxArgs.dqdt_aqh2so4.offset]
+          ctypes.sizeof(_abi.SetsoxArgs), _abi.SetsoxArgs.dqdt_aqh2so4.offset,
+          ctypes.sizeof(_abi.GasChemArgs), _abi.GasChemArgs.qq.offset]
     assert c == py
 
 
