@@ -584,6 +584,134 @@ def bench_gas_chem(args):
     print(json.dumps(res), flush=True)
 
 
+def bench_linoz(args):
+    """Stratospheric ozone (mam4x_compute_linoz; --workload linoz) over
+    21,600 synthetic columns x 72 levels, with the invariant densities
+    (mam4x_compute_invariants) of the same cells timed beside it: kernel
+    times from HIP events on the launch stream, HBM roofline of the values
+    moved, the oracle on the host cores."""
+    import ctypes
+    import numpy as np
+    import torch
+    from mam4xx_amd import _abi
+    from mam4xx_amd.chain import Chain
+    import oracle
+    rng = np.random.default_rng(0)
+    shp = (NLEV, NCOL)
+    ps = rng.uniform(95000, 103000, NCOL)
+    edges = 10.0 * (ps / 10.0) ** (np.arange(NLEV + 1)[:, None] / NLEV)
+    pmid, pdel = np.sqrt(edges[1:] * edges[:-1]), edges[1:] - edges[:-1]
+    temp = np.where(pmid < 2e4, rng.uniform(183, 215, shp), rng.uniform(215, 300, shp))
+    o3 = 10 ** rng.uniform(-8, -5, shp)
+    col_du = np.cumsum(2.12e20 * pdel * o3, axis=0) / 2.687e16
+    clim = np.stack([o3 * rng.uniform(0.7, 1.3, shp), temp + rng.normal(0, 5, shp),
+                     col_du * rng.uniform(0.8, 1.2, shp), rng.normal(0, 1e-12, shp),
+                     -10 ** rng.uniform(-8, -4.5, shp), rng.normal(0, 1e-13, shp),
+                     rng.normal(0, 1e-14, shp), 10 ** rng.uniform(-7, -5, shp)])
+    cols = dict(temp=temp, pmid=pmid, pdel=pdel, o3=o3, clim=clim,
+                sza=rng.uniform(0, np.pi * 100 / 180, NCOL), rlats=rng.uniform(-1.5, 1.5, NCOL),
+                col0=rng.uniform(0, 1e17, NCOL), ltrop=rng.integers(20, 50, NCOL))
+    conf = dict(chlorine_loading=3.0, psc_T=193.0, o3_sfc=3e-8, o3_tau=172800.0, o3_lbl=4)
+    qv = rng.uniform(0, 0.02, shp)
+    offline = 10 ** rng.uniform(-12, -6, (4,) + shp)
+    dev = lambda a, dt=torch.float64: torch.as_tensor(np.ascontiguousarray(a), dtype=dt).cuda()
+    d = {k: dev(v) for k, v in (("temp", temp), ("pmid", pmid), ("pdel", pdel), ("clim", clim),
+                                ("sza", cols["sza"]), ("lat", cols["rlats"]),
+                                ("col0", cols["col0"]), ("qv", qv), ("off", offline))}
+    ltrop = dev(cols["ltrop"], torch.int32)
+    o3_0 = dev(o3)
+    o3_d = o3_0.clone()
+    inv = torch.empty((8,) + shp, dtype=torch.float64, device="cuda")
+    ch = Chain()
+    s = torch.cuda.current_stream()
+    a = _abi.LinozArgs()
+    a.nlev, a.ncol, a.ldc, a.layout = NLEV, NCOL, NCOL, _abi.LAYOUT_LEV_COL
+    a.temperature, a.pressure, a.pdel = (d[k].data_ptr() for k in ("temp", "pmid", "pdel"))
+    for i in range(8):
+        a.climatology[i] = d["clim"][i].data_ptr()
+    a.zenith_angle, a.latitude, a.column_above_top = (d[k].data_ptr()
+                                                      for k in ("sza", "lat", "col0"))
+    a.ltrop = ltrop.data_ptr()
+    for k in ("chlorine_loading", "psc_T", "o3_sfc", "o3_tau"):
+        setattr(a, k, conf[k])
+    a.o3_lbl = conf["o3_lbl"]
+    a.o3 = o3_d.data_ptr()
+    b = _abi.SetinvArgs()
+    b.nlev, b.ncol, b.ldc, b.layout = NLEV, NCOL, NCOL, _abi.LAYOUT_LEV_COL
+    b.temperature, b.pressure, b.vapor_mixing_ratio = (d[k].data_ptr()
+                                                       for k in ("temp", "pmid", "qv"))
+    for i in range(4):
+        b.cnst_offline[i] = d["off"][i].data_ptr()
+    for i in range(8):
+        b.invariants[i] = inv[i].data_ptr()
+    sp = ctypes.c_void_p(s.cuda_stream)
+    launches = {"linoz": lambda: ch.lib.mam4x_compute_linoz(ch.h, DT, ctypes.byref(a), sp),
+                "setinv": lambda: ch.lib.mam4x_compute_invariants(ch.h, ctypes.byref(b), sp)}
+    ev = [torch.cuda.Event(enable_timing=True) for _ in range(2)]
+    ms = {}
+    for name, fn in launches.items():
+        for _ in range(args.warmup + 1):
+            o3_d.copy_(o3_0)
+            assert fn() == 0
+        torch.cuda.synchronize()
+        t = 0.0
+        for _ in range(args.steps):
+            o3_d.copy_(o3_0)  # every launch advances the same ozone
+            ev[0].record(s)
+            fn()
+            ev[1].record(s)
+            ev[1].synchronize()
+            t += ev[0].elapsed_time(ev[1])
+        ms[name] = t / args.steps
+    n = NCOL * NLEV
+    # linoz, compulsory values: every level reads pdel and the ozone (the
+    # overhead column); the levels above the tropopause also read T, p and 8
+    # climatology values and write the ozone; the surface-sink levels below
+    # the tropopause write the ozone.  The optional diagnostics are not
+    # requested.
+    lt = np.minimum(cols["ltrop"], NLEV).astype(np.int64)
+    lbl = conf["o3_lbl"]
+    sink_only = np.maximum(0, lbl - np.maximum(0, lt - (NLEV - lbl)))
+    bytes_launch = 8 * (2 * n + 11 * int(lt.sum()) + int(sink_only.sum()))
+    bytes_cell = bytes_launch / n
+    achieved = bytes_launch / (ms["linoz"] * 1e-3) / 1e9
+    # setinv: T, p, qv, 4 offline species read, 8 invariants written
+    inv_bytes = (3 + 4 + 8) * 8
+    res = {"metric": "stratospheric ozone (linoz) + overhead column: column-steps/sec at 72 lev",
+           "unit": "column-steps/s", "value": NCOL / (ms["linoz"] * 1e-3), "n_gpus": 1,
+           "higher_is_better": True, "dtype": "f64", "data": "synthetic", "steps": args.steps,
+           "warmup": args.warmup,
+           "config": {"workload": "21600 columns x 72 levels of synthetic columns, "
+                                  "lin_strat_chem_solve above the tropopause (levels 20-49) "
+                                  "and the 4-level surface sink, dt 1800 s",
+                      "columns": NCOL, "levels": NLEV},
+           "kernel_ms": ms["linoz"],
+           "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS,
+                        "unit": "GB/s", "frac": achieved / HBM_PEAK_GBS,
+                        "traffic": _traffic("linoz_kernel_%dx%d.txt" % (NCOL, NLEV)),
+                        "kernel": "linoz_block_kernel",
+                        "algorithmic_bytes_per_cell": bytes_cell},
+           "setinv": {"kernel_ms": ms["setinv"],
+                      "achieved_GBs": n * inv_bytes / (ms["setinv"] * 1e-3) / 1e9,
+                      "traffic": _traffic("setinv_kernel_%dx%d.txt" % (NCOL, NLEV)),
+                      "algorithmic_bytes_per_cell": inv_bytes}}
+    if not args.no_cpu:
+        orc = oracle.Oracle()
+        nth = min(16, os.cpu_count() or 1)
+        run = lambda: orc.linoz(DT, cols, conf)
+        run()
+        best = 1e30
+        for _ in range(5):
+            t0 = time.perf_counter()
+            run()
+            best = min(best, time.perf_counter() - t0)
+        res["cpu_baseline"] = {"value": NCOL / best, "unit": "column-steps/s", "cores": nth,
+                               "kind": "port", "sample": "the full 21600 x 72 domain, best of 5 "
+                               "after a warm-up (OpenMP over columns)", "cpu": _cpu_model()}
+    ch.close()
+    print(json.dumps(res), flush=True)
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -593,12 +721,13 @@ def main():
     ap.add_argument("--schedule", choices=["auto", "fused", "queue"], default="auto")
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline")
     ap.add_argument("--workload", choices=["headline", "batch", "large", "water", "wetsize",
-                                           "tracers", "setsox", "gaschem"],
+                                           "tracers", "setsox", "gaschem", "linoz"],
                     default="headline",
                     help="headline: the metric's 21,600 x 72 step (default); batch, large, "
                          "water (aerosol water uptake), wetsize (calcsize + water uptake), "
                          "tracers (state_q/qqcw <-> state planes), setsox (aqueous sulfur "
-                         "chemistry) and gaschem (gas-phase chemistry) print their own line")
+                         "chemistry), gaschem (gas-phase chemistry) and linoz (stratospheric "
+                         "ozone, with the invariant densities) print their own line")
     args = ap.parse_args()
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) > 1:
         print("bench.py measures one GPU: the chain has no cross-GPU step and this build "
@@ -622,6 +751,8 @@ def main():
         return bench_tracers(args)
     if args.workload == "setsox":
         return bench_setsox(args)
+    if args.workload == "linoz":
+        return bench_linoz(args)
     if args.workload == "gaschem":
         return bench_gas_chem(args)
     scene = scenes.generate(NCOL, NLEV, seed=0)

@@ -40,7 +40,7 @@
 extern "C" {
 #endif
 
-#define MAM4X_ABI_VERSION 6
+#define MAM4X_ABI_VERSION 7
 
 /* per-process tendency diagnostics (mam4x_outputs.process_tendencies) */
 #define MAM4X_GAS_PCNST 31
@@ -83,7 +83,7 @@ extern "C" {
 
 /* precision of a handle */
 #define MAM4X_PREC_FP64 0  /* everything in FP64 (the drop-in)             */
-#define MAM4X_PREC_MIXED 1 /* FP32 bulk, FP64 kept where stated in DESIGN  */
+#define MAM4X_PREC_MIXED 1 /* FP32 bulk; FP64 for the ill-conditioned parts */
 
 /* plane layouts (see Conventions) */
 #define MAM4X_LAYOUT_LEV_COL 0
@@ -408,6 +408,78 @@ typedef struct mam4x_gas_chem_args {
  * stream-ordered; no workspace. */
 int mam4x_compute_gas_chemistry(mam4x_handle h, double dt, const mam4x_gas_chem_args *args,
                                 void *stream);
+
+/* ---------------------------------------------------------------------- */
+/* Invariant densities: mam4::mo_setinv::setinv (mo_setinv.hpp:45-122) for */
+/* every cell -- the fixed species the gas-phase chemistry reads, from T,  */
+/* p, water vapour and four offline species.  Planes: extents and layout   */
+/* as in mam4x_inputs.                                                     */
+typedef struct mam4x_setinv_args {
+  int ncol, nlev;
+  int ldc;
+  int layout;
+  const double *temperature;        /* [K]                                */
+  const double *pressure;           /* mid-level [Pa]                     */
+  const double *vapor_mixing_ratio; /* qv [kg/kg]                         */
+  /* offline species [mol/mol], the reference's cnst_offline order: they
+   * fill invariants 7, 4 (OH), 5 (NO3), 6 (HO2).  NULL = 0. */
+  const double *cnst_offline[4];
+  /* [#/cm3]: 0 M, 1 N2, 2 O2, 3 H2O, 4-7 the offline species.  NULL planes
+   * are not written. */
+  double *invariants[MAM4X_NFS];
+} mam4x_setinv_args;
+
+/* setinv over a domain.  FP64; stream-ordered; no workspace. */
+int mam4x_compute_invariants(mam4x_handle h, const mam4x_setinv_args *args, void *stream);
+
+/* ---------------------------------------------------------------------- */
+/* Linearized stratospheric ozone (LINOZ): per column, the overhead ozone  */
+/* column (compute_o3_column_density, compute_o3_column_density.hpp:59-83) */
+/* from the ozone before the step, lin_strat_chem_solve at the levels above */
+/* the tropopause (lin_strat_chem.hpp:225-261: relaxation to the           */
+/* climatological steady state, polar-stratospheric-cloud loss), then      */
+/* lin_strat_sfcsink in the lowest o3_lbl levels (:289-316).  Level 0 is   */
+/* the model top.  Planes: extents and layout as in mam4x_inputs.          */
+#define MAM4X_NLINOZ 8 /* climatology fields, LinozData order              */
+typedef struct mam4x_linoz_args {
+  int ncol, nlev;
+  int ldc;
+  int layout;
+  const double *temperature; /* [K]                                       */
+  const double *pressure;    /* mid-level [Pa]                            */
+  const double *pdel;        /* layer thickness [Pa]                      */
+  /* o3_clim [mol/mol], t_clim [K], o3col_clim [DU], PmL_clim [mol/mol/s],
+   * dPmL_dO3 [1/s], dPmL_dT [mol/mol/s/K], dPmL_dO3col [mol/mol/s/DU],
+   * cariolle_pscs [1/s] (mo_gas_phase_chemdr.hpp:35-52) */
+  const double *climatology[MAM4X_NLINOZ];
+  /* per column [ncol]: solar zenith angle [rad], latitude [rad], number of
+   * levels above the tropopause (levels 0 .. ltrop-1 are solved), ozone
+   * column above the model top [molecules/cm2] (NULL = 0) */
+  const double *zenith_angle;
+  const double *latitude;
+  const int *ltrop;
+  const double *column_above_top;
+  /* LinozConf (mo_gas_phase_chemdr.hpp:55-70) */
+  double chlorine_loading; /* EESC [ppbv]; PSC loss only when > 0         */
+  double psc_T;            /* PSC temperature threshold [K]              */
+  double o3_sfc;           /* surface-sink target [mol/mol]              */
+  double o3_tau;           /* surface-sink time scale [s]                */
+  int o3_lbl;              /* surface-sink levels (0: no sink)           */
+  double *o3;              /* ozone [mol/mol], advanced in place          */
+  /* optional outputs (NULL: not written): planes of the overhead column
+   * [molecules/cm2] (every level), the linoz and PSC tendencies
+   * [mol/mol/s] and the steady-state ozone (0 below the tropopause); per
+   * column the surface sink's ozone loss [Tg/yr/m2] */
+  double *o3_column;
+  double *do3_linoz;
+  double *do3_linoz_psc;
+  double *ss_o3;
+  double *o3_sfcsink;
+} mam4x_linoz_args;
+
+/* linoz over a domain with time step dt [s].  One lane per column walks
+ * its levels.  FP64; stream-ordered; no workspace. */
+int mam4x_compute_linoz(mam4x_handle h, double dt, const mam4x_linoz_args *args, void *stream);
 
 /* Batches: many scenes (domains of any column count, each with its own
  * inputs, outputs and layout) stepped by one set of launches.  The scenes'

@@ -6,7 +6,7 @@ The structures below must match the C declarations field for field; the
 import ctypes
 import os
 
-ABI_VERSION = 6
+ABI_VERSION = 7
 
 NSTATE = 52
 NMODE = 4
@@ -193,6 +193,51 @@ class GasChemArgs(ctypes.Structure):
     ]
 
 
+NLINOZ = 8
+
+
+class SetinvArgs(ctypes.Structure):
+    _fields_ = [
+        ("ncol", ctypes.c_int),
+        ("nlev", ctypes.c_int),
+        ("ldc", ctypes.c_int),
+        ("layout", ctypes.c_int),
+        ("temperature", _vp),
+        ("pressure", _vp),
+        ("vapor_mixing_ratio", _vp),
+        ("cnst_offline", _vp * 4),
+        ("invariants", _vp * NFS),
+    ]
+
+
+class LinozArgs(ctypes.Structure):
+    _fields_ = [
+        ("ncol", ctypes.c_int),
+        ("nlev", ctypes.c_int),
+        ("ldc", ctypes.c_int),
+        ("layout", ctypes.c_int),
+        ("temperature", _vp),
+        ("pressure", _vp),
+        ("pdel", _vp),
+        ("climatology", _vp * NLINOZ),
+        ("zenith_angle", _vp),
+        ("latitude", _vp),
+        ("ltrop", _vp),
+        ("column_above_top", _vp),
+        ("chlorine_loading", ctypes.c_double),
+        ("psc_T", ctypes.c_double),
+        ("o3_sfc", ctypes.c_double),
+        ("o3_tau", ctypes.c_double),
+        ("o3_lbl", ctypes.c_int),
+        ("o3", _vp),
+        ("o3_column", _vp),
+        ("do3_linoz", _vp),
+        ("do3_linoz_psc", _vp),
+        ("ss_o3", _vp),
+        ("o3_sfcsink", _vp),
+    ]
+
+
 # entry points declared in include/mam4x.h, with ctypes signatures
 SIGNATURES = {
     "mam4x_abi_version": (ctypes.c_int, []),
@@ -223,6 +268,8 @@ SIGNATURES = {
                                                ctypes.POINTER(Tracers), ctypes.c_int, _vp]),
     "mam4x_compute_gas_chemistry": (ctypes.c_int, [_vp, ctypes.c_double,
                                                    ctypes.POINTER(GasChemArgs), _vp]),
+    "mam4x_compute_invariants": (ctypes.c_int, [_vp, ctypes.POINTER(SetinvArgs), _vp]),
+    "mam4x_compute_linoz": (ctypes.c_int, [_vp, ctypes.c_double, ctypes.POINTER(LinozArgs), _vp]),
     "mam4x_compute_setsox": (ctypes.c_int, [_vp, ctypes.c_double, ctypes.POINTER(SetsoxArgs),
                                             _vp]),
     "mam4x_batch_create": (ctypes.c_int, [_vp, ctypes.c_int, ctypes.POINTER(Inputs),

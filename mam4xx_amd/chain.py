@@ -243,6 +243,74 @@ class Chain:
                                                   ctypes.c_void_p(s.cuda_stream))
         self._check(rc, "mam4x_compute_gas_chemistry")
 
+    @staticmethod
+    def _extents(t, layout):
+        """(ncol, nlev, ldc) of a contiguous float64 CUDA plane"""
+        assert t.dim() == 2 and t.dtype == torch.float64 and t.is_cuda and t.is_contiguous()
+        if layout == _abi.LAYOUT_LEV_COL:
+            nlev, ncol = t.shape
+        else:
+            ncol, nlev = t.shape
+        return ncol, nlev, t.shape[1]
+
+    def compute_invariants(self, temperature, pressure, qv, cnst_offline, invariants,
+                           layout=_abi.LAYOUT_LEV_COL, stream=None):
+        """Invariant densities (mam4x_compute_invariants, setinv): planes T,
+        p, qv; cnst_offline [4][...] or None; invariants [8][...] output."""
+        a = _abi.SetinvArgs()
+        a.ncol, a.nlev, a.ldc = self._extents(temperature, layout)
+        a.layout = layout
+        for name, t in (("temperature", temperature), ("pressure", pressure),
+                        ("vapor_mixing_ratio", qv)):
+            assert t.shape == temperature.shape and t.is_contiguous()
+            setattr(a, name, t.data_ptr())
+        if cnst_offline is not None:
+            for i, p in enumerate(_plane_ptrs(cnst_offline, 4)):
+                a.cnst_offline[i] = p
+        for i, p in enumerate(_plane_ptrs(invariants, _abi.NFS)):
+            a.invariants[i] = p
+        s = stream if stream is not None else torch.cuda.current_stream()
+        rc = self.lib.mam4x_compute_invariants(self.h, ctypes.byref(a),
+                                               ctypes.c_void_p(s.cuda_stream))
+        self._check(rc, "mam4x_compute_invariants")
+
+    def compute_linoz(self, dt, temperature, pressure, pdel, climatology, zenith_angle,
+                      latitude, ltrop, o3, conf, column_above_top=None, o3_column=None,
+                      do3_linoz=None, do3_linoz_psc=None, ss_o3=None, o3_sfcsink=None,
+                      layout=_abi.LAYOUT_LEV_COL, stream=None):
+        """Stratospheric ozone (mam4x_compute_linoz): planes T, p, pdel, o3
+        (advanced in place) and the optional output planes; climatology
+        [8][...]; per column zenith_angle, latitude, ltrop (int32),
+        column_above_top and o3_sfcsink; conf: dict of chlorine_loading,
+        psc_T, o3_sfc, o3_tau, o3_lbl."""
+        a = _abi.LinozArgs()
+        a.ncol, a.nlev, a.ldc = self._extents(temperature, layout)
+        a.layout = layout
+        for name, t in (("temperature", temperature), ("pressure", pressure), ("pdel", pdel),
+                        ("o3", o3), ("o3_column", o3_column), ("do3_linoz", do3_linoz),
+                        ("do3_linoz_psc", do3_linoz_psc), ("ss_o3", ss_o3)):
+            if t is None:
+                continue
+            assert t.shape == temperature.shape and t.dtype == torch.float64 and t.is_contiguous()
+            setattr(a, name, t.data_ptr())
+        for i, p in enumerate(_plane_ptrs(climatology, _abi.NLINOZ)):
+            a.climatology[i] = p
+        for name, t, dt_ in (("zenith_angle", zenith_angle, torch.float64),
+                             ("latitude", latitude, torch.float64), ("ltrop", ltrop, torch.int32),
+                             ("column_above_top", column_above_top, torch.float64),
+                             ("o3_sfcsink", o3_sfcsink, torch.float64)):
+            if t is None:
+                continue
+            assert t.shape == (a.ncol,) and t.dtype == dt_ and t.is_cuda and t.is_contiguous()
+            setattr(a, name, t.data_ptr())
+        for k in ("chlorine_loading", "psc_T", "o3_sfc", "o3_tau"):
+            setattr(a, k, float(conf[k]))
+        a.o3_lbl = int(conf["o3_lbl"])
+        s = stream if stream is not None else torch.cuda.current_stream()
+        rc = self.lib.mam4x_compute_linoz(self.h, dt, ctypes.byref(a),
+                                          ctypes.c_void_p(s.cuda_stream))
+        self._check(rc, "mam4x_compute_linoz")
+
     def compute_setsox(self, dt, met, qcw, qin, dqdt_aqso4=None, dqdt_aqh2so4=None,
                        layout=_abi.LAYOUT_LEV_COL, stream=None):
         """Aqueous sulfur chemistry (mam4x_compute_setsox).  met: dict of

@@ -33,6 +33,7 @@
 #include "../../include/mam4x.h"
 #include "chain_cell.hpp"
 #include "gas_chem.hpp"
+#include "linoz.hpp"
 #include "setsox.hpp"
 #include "water_uptake.hpp"
 
@@ -745,6 +746,128 @@ __global__ void __launch_bounds__(kBlock) gas_chem_kernel(GasChemArgs a) {
   (void)gc::imp_sol(y, rxt, het, ext, a.dt);
 #pragma unroll
   for (int i = 1; i < 31; ++i) a.qq[i][off] = y[i];
+}
+
+// ------------------------------------------------------------------ setinv
+// Invariant densities, one lane per cell (linoz.hpp).
+struct SetinvArgs {
+  const double *T, *p, *qv;
+  const double *offline[4];
+  double *inv[MAM4X_NFS];
+  int ncol, nlev, ldc, layout;
+};
+
+__global__ void __launch_bounds__(kBlock) setinv_kernel(SetinvArgs a) {
+  const long c = (long)blockIdx.x * kBlock + threadIdx.x;
+  if (c >= (long)a.ncol * a.nlev) return;
+  const int fast = a.layout == MAM4X_LAYOUT_COL_LEV ? a.nlev : a.ncol;
+  const long slow = c / fast;
+  const long off = slow * a.ldc + (c - slow * fast);
+  double co[4], v[MAM4X_NFS];
+#pragma unroll
+  for (int i = 0; i < 4; ++i) co[i] = a.offline[i] ? a.offline[i][off] : 0.0;
+  lnz::invariants(a.T[off], a.p[off], a.qv[off], co, v);
+#pragma unroll
+  for (int i = 0; i < MAM4X_NFS; ++i)
+    if (a.inv[i]) a.inv[i][off] = v[i];
+}
+
+// ------------------------------------------------------------------ linoz
+// Stratospheric ozone (linoz.hpp).  A lane per column walking all its levels
+// leaves a 21,600-column domain with 340 waves whose every level waits on
+// memory.  The block kernel instead gives each 64-column block four waves,
+// each over a quarter of the levels: the waves form the levels' column
+// slabs into LDS, one wave runs the columns' running sums there in level
+// order (the reference's summation order, so the overhead column is
+// unchanged to the bit), then all four solve their levels, and one wave
+// applies the surface sink.  Columns taller than kLinozLdsLevels use the
+// lane-per-column kernel.
+constexpr int kLinozBlock = 64;
+constexpr int kLinozWaves = 4;
+constexpr int kLinozLdsLevels = 128;  // 64 KiB of slabs per block
+struct LinozArgs {
+  const double *T, *p, *pdel;
+  const double *clim[lnz::kNClim];
+  const double *sza, *lat, *col0;
+  const int *ltrop;
+  double *o3, *o3col, *do3, *do3_psc, *ss, *sink;
+  lnz::Conf cf;
+  int ncol, nlev, ldc, layout;
+  double dt;
+};
+
+__device__ __forceinline__ int clamp_ltrop(const LinozArgs &a, int j) {
+  const int lt = a.ltrop[j];
+  return lt < 0 ? 0 : lt > a.nlev ? a.nlev : lt;
+}
+
+__global__ void __launch_bounds__(kLinozBlock) linoz_kernel(LinozArgs a) {
+  const int j = blockIdx.x * kLinozBlock + threadIdx.x;
+  if (j >= a.ncol) return;
+  const bool lc = a.layout == MAM4X_LAYOUT_LEV_COL;
+  const long base = lc ? (long)j : (long)j * a.ldc;
+  const long ks = lc ? (long)a.ldc : 1L;
+  const double *clim[lnz::kNClim];
+#pragma unroll
+  for (int i = 0; i < lnz::kNClim; ++i) clim[i] = a.clim[i] + base;
+  double sink = 0.0;
+  lnz::column(a.nlev, ks, a.dt, a.sza[j], a.lat[j], clamp_ltrop(a, j), a.col0 ? a.col0[j] : 0.0,
+              a.cf, a.T + base, a.p + base, a.pdel + base, clim, a.o3 + base,
+              a.o3col ? a.o3col + base : nullptr, a.do3 ? a.do3 + base : nullptr,
+              a.do3_psc ? a.do3_psc + base : nullptr, a.ss ? a.ss + base : nullptr, &sink);
+  if (a.sink) a.sink[j] = sink;
+}
+
+__global__ void __launch_bounds__(kLinozBlock *kLinozWaves) linoz_block_kernel(LinozArgs a) {
+  extern __shared__ double slab[];  // [nlev][64]: level slabs, then overhead columns
+  const int lane = threadIdx.x % kLinozBlock, w = threadIdx.x / kLinozBlock;
+  const int j = blockIdx.x * kLinozBlock + lane;
+  const bool live = j < a.ncol;
+  const bool lc = a.layout == MAM4X_LAYOUT_LEV_COL;
+  const long base = live ? (lc ? (long)j : (long)j * a.ldc) : 0L;
+  const long ks = lc ? (long)a.ldc : 1L;
+  const int per = (a.nlev + kLinozWaves - 1) / kLinozWaves;
+  const int k0 = w * per, k1 = k0 + per < a.nlev ? k0 + per : a.nlev;
+  if (live)
+    for (int k = k0; k < k1; ++k) {
+      const long i = base + k * ks;
+      slab[k * kLinozBlock + lane] = lnz::kColumnFactor * a.pdel[i] * a.o3[i];
+    }
+  __syncthreads();
+  if (w == 0 && live) {  // the running sums, top down
+    const double col0 = a.col0 ? a.col0[j] : 0.0;
+    double above = 0.0;
+    for (int k = 0; k < a.nlev; ++k) {
+      const double delta = slab[k * kLinozBlock + lane];
+      slab[k * kLinozBlock + lane] = col0 + above + 0.5 * delta;
+      above += delta;
+    }
+  }
+  __syncthreads();
+  if (live) {
+    const int lt = clamp_ltrop(a, j);
+    const double sza = a.sza[j], lats = a.lat[j] * lnz::kRadToDeg;
+    for (int k = k0; k < k1; ++k) {
+      const long i = base + k * ks;
+      const double o3col = slab[k * kLinozBlock + lane];
+      if (a.o3col) a.o3col[i] = o3col;
+      lnz::LevelOut lo{0.0, 0.0, 0.0};
+      if (k < lt) {
+        double c[lnz::kNClim];
+#pragma unroll
+        for (int m = 0; m < lnz::kNClim; ++m) c[m] = a.clim[m][i];
+        a.o3[i] = lnz::solve_level(o3col, a.T[i], sza, a.p[i], a.dt, lats, c, a.cf, a.o3[i], lo);
+      }
+      if (a.do3) a.do3[i] = lo.do3;
+      if (a.do3_psc) a.do3_psc[i] = lo.do3_psc;
+      if (a.ss) a.ss[i] = lo.ss_o3;
+    }
+  }
+  __syncthreads();  // the sink reads ozone other waves wrote
+  if (w == 0 && live) {
+    const double sink = lnz::surface_sink(a.nlev, ks, a.dt, a.cf, a.pdel + base, a.o3 + base);
+    if (a.sink) a.sink[j] = sink;
+  }
 }
 
 }  // namespace
@@ -1498,6 +1621,97 @@ int mam4x_compute_gas_chemistry(mam4x_handle h, double dt, const mam4x_gas_chem_
   if (ncell == 0) return MAM4X_OK;
   hipLaunchKernelGGL(gas_chem_kernel, dim3((unsigned)((ncell + kBlock - 1) / kBlock)),
                      dim3(kBlock), 0, (hipStream_t)stream, a);
+  HIP_TRY(h, hipGetLastError());
+  return MAM4X_OK;
+}
+
+// extents, layout and leading dimension of a companion call's planes
+static int plane_extents(mam4x_handle h, int ncol, int nlev, int layout, int ldc, int &ld) {
+  if (ncol < 0 || nlev < 0) return fail(h, MAM4X_ERR_ARG, "negative ncol or nlev");
+  if (layout != MAM4X_LAYOUT_LEV_COL && layout != MAM4X_LAYOUT_COL_LEV)
+    return fail(h, MAM4X_ERR_ARG, "unknown layout");
+  const bool lc = layout == MAM4X_LAYOUT_LEV_COL;
+  const int fast = lc ? ncol : nlev, slow = lc ? nlev : ncol;
+  ld = ldc ? ldc : fast;
+  if (ld < fast) return fail(h, MAM4X_ERR_ARG, "ldc smaller than the fastest extent");
+  if ((long long)ld * slow > 0x7fffffffLL)
+    return fail(h, MAM4X_ERR_ARG, "domain too large (ldc x slow extent must fit in int32)");
+  return MAM4X_OK;
+}
+
+int mam4x_compute_invariants(mam4x_handle h, const mam4x_setinv_args *in, void *stream) {
+  if (!h) return MAM4X_ERR_ARG;
+  if (!in) return fail(h, MAM4X_ERR_ARG, "args is NULL");
+  int ld = 0;
+  if (int rc = plane_extents(h, in->ncol, in->nlev, in->layout, in->ldc, ld)) return rc;
+  SetinvArgs a;
+  std::memset(&a, 0, sizeof(a));
+  a.T = in->temperature;
+  a.p = in->pressure;
+  a.qv = in->vapor_mixing_ratio;
+  if (!a.T || !a.p || !a.qv)
+    return fail(h, MAM4X_ERR_ARG, "temperature, pressure or vapor plane is NULL");
+  for (int i = 0; i < 4; ++i) a.offline[i] = in->cnst_offline[i];
+  for (int i = 0; i < MAM4X_NFS; ++i) a.inv[i] = in->invariants[i];
+  a.ncol = in->ncol;
+  a.nlev = in->nlev;
+  a.ldc = ld;
+  a.layout = in->layout;
+  const long ncell = (long)in->ncol * in->nlev;
+  if (ncell == 0) return MAM4X_OK;
+  hipLaunchKernelGGL(setinv_kernel, dim3((unsigned)((ncell + kBlock - 1) / kBlock)),
+                     dim3(kBlock), 0, (hipStream_t)stream, a);
+  HIP_TRY(h, hipGetLastError());
+  return MAM4X_OK;
+}
+
+int mam4x_compute_linoz(mam4x_handle h, double dt, const mam4x_linoz_args *in, void *stream) {
+  if (!h) return MAM4X_ERR_ARG;
+  if (!in) return fail(h, MAM4X_ERR_ARG, "args is NULL");
+  if (!(dt > 0.0)) return fail(h, MAM4X_ERR_ARG, "dt must be > 0");
+  int ld = 0;
+  if (int rc = plane_extents(h, in->ncol, in->nlev, in->layout, in->ldc, ld)) return rc;
+  if (in->o3_lbl < 0 || in->o3_lbl > in->nlev)
+    return fail(h, MAM4X_ERR_ARG, "o3_lbl outside [0, nlev]");
+  if (in->o3_lbl > 0 && !(in->o3_tau > 0.0))
+    return fail(h, MAM4X_ERR_ARG, "o3_tau must be > 0 with a surface sink");
+  LinozArgs a;
+  std::memset(&a, 0, sizeof(a));
+  a.T = in->temperature;
+  a.p = in->pressure;
+  a.pdel = in->pdel;
+  a.o3 = in->o3;
+  if (!a.T || !a.p || !a.pdel || !a.o3)
+    return fail(h, MAM4X_ERR_ARG, "temperature, pressure, pdel or o3 plane is NULL");
+  for (int i = 0; i < lnz::kNClim; ++i) {
+    a.clim[i] = in->climatology[i];
+    if (!a.clim[i])
+      return fail(h, MAM4X_ERR_ARG, "climatology plane " + std::to_string(i) + " is NULL");
+  }
+  a.sza = in->zenith_angle;
+  a.lat = in->latitude;
+  a.ltrop = in->ltrop;
+  if (!a.sza || !a.lat || !a.ltrop)
+    return fail(h, MAM4X_ERR_ARG, "zenith_angle, latitude or ltrop is NULL");
+  a.col0 = in->column_above_top;
+  a.o3col = in->o3_column;
+  a.do3 = in->do3_linoz;
+  a.do3_psc = in->do3_linoz_psc;
+  a.ss = in->ss_o3;
+  a.sink = in->o3_sfcsink;
+  a.cf = lnz::Conf{in->chlorine_loading, in->psc_T, in->o3_sfc, in->o3_tau, in->o3_lbl};
+  a.ncol = in->ncol;
+  a.nlev = in->nlev;
+  a.ldc = ld;
+  a.layout = in->layout;
+  a.dt = dt;
+  if (in->ncol == 0) return MAM4X_OK;
+  const unsigned grid = (unsigned)((in->ncol + kLinozBlock - 1) / kLinozBlock);
+  if (in->nlev <= kLinozLdsLevels)
+    hipLaunchKernelGGL(linoz_block_kernel, dim3(grid), dim3(kLinozBlock * kLinozWaves),
+                       (size_t)in->nlev * kLinozBlock * sizeof(double), (hipStream_t)stream, a);
+  else
+    hipLaunchKernelGGL(linoz_kernel, dim3(grid), dim3(kLinozBlock), 0, (hipStream_t)stream, a);
   HIP_TRY(h, hipGetLastError());
   return MAM4X_OK;
 }

@@ -63,6 +63,7 @@ static inline double nz(double v) {
 #define M_cbrt(x) nz(cbrt(x))
 #define M_erfc(x) nz(erfc(x))
 #define M_sqrt(x) nz(sqrt(x))
+#define M_log10(x) nz(log10(x))
 
 /* ---------------- constants (mam4_constants.hpp, aero_modes.hpp) -------- */
 static const double PI = 3.14159265358979323846264;
@@ -3297,5 +3298,195 @@ void orc_gas_phase_chemistry_domain(long ncell, double dt, const double *temp,
     const int n = orc_gas_phase_chemistry(temp[c], dt, photo[c], e, v, h, q);
     for (int i = 0; i < 31; ++i) qq[(long)i * ncell + c] = q[i];
     if (steps) steps[c] = n;
+  }
+}
+
+/* ================ stratospheric ozone (linoz) and invariants =============
+ * compute_o3_column_density (compute_o3_column_density.hpp:59-83),
+ * lin_strat_chem_solve_kk / psc_activation / compute_steady_state_ozone
+ * (lin_strat_chem.hpp:31-222), lin_strat_sfcsink(_kk) (:263-316) and
+ * setinv_single_level (mo_setinv.hpp:45-86). */
+#define LNZ_PI 3.14159265358979323846264
+static const double lnz_rad2deg = 180. / LNZ_PI;
+
+/* compute_o3_column_density.hpp:59-83: o3_col_dens(k) = col0 + sum of the
+ * levels above + half of level k's own slab [mol/cm2] */
+void orc_o3_column_density(int nlev, const double *pdel, const double *vmr_o3, double col0,
+                           double *o3_col_dens) {
+  const double xfactor = 2.8704e21 / (9.80616 * 1.38044);
+  double partial_sum = 0.0;
+  for (int kk = 0; kk < nlev; ++kk) {
+    const double delta_kk = xfactor * pdel[kk] * vmr_o3[kk];
+    o3_col_dens[kk] = col0 + partial_sum + 0.5 * delta_kk;
+    partial_sum += delta_kk;
+  }
+}
+
+/* lin_strat_chem.hpp:31-95 */
+static void psc_activation(double lats, double temp, double pmid, double sza,
+                           double linoz_cariolle_psc, double delta_t, int excess_chlorine,
+                           double o3_old, double psc_T, double chlorine_loading,
+                           double *o3_new, double *do3_linoz_psc) {
+  const double lats_threshold = 40.0, chlorine_loading_1987 = 2.5977;
+  if (fabs(lats) > lats_threshold) {
+    if (excess_chlorine) {
+      if (temp <= psc_T) {
+        const double max_sza = 90.0 + M_sqrt(kmax(16.0 * M_log10(1e5 / pmid), 0.0));
+        if ((sza * lnz_rad2deg) <= max_sza) {
+          const double r = chlorine_loading / chlorine_loading_1987;
+          const double psc_loss = M_exp(-linoz_cariolle_psc * (r * r) * delta_t);
+          *o3_new = o3_old * psc_loss;
+          *do3_linoz_psc = (*o3_new - o3_old) / delta_t;
+        }
+      }
+    }
+  }
+}
+
+/* lin_strat_chem.hpp:97-222.  clim: o3_clim, t_clim, o3col_clim, PmL_clim,
+ * dPmL_dO3, dPmL_dT, dPmL_dO3col, cariolle_psc.  diag[3] = do3_linoz,
+ * do3_linoz_psc (0 unless the PSC loss applies), ss_o3. */
+void orc_linoz_solve_kk(double o3col, double temperature, double sza, double pmid,
+                        double delta_t, double rlats, const double clim[8],
+                        double chlorine_loading, double psc_T, double *o3_vmr, double diag[3]) {
+  const double convert_to_du = 1.0 / 2.687e16;
+  const double lats = rlats * lnz_rad2deg;
+  const int excess_chlorine = (chlorine_loading - 0.0000) > 0;
+  const double o3_old = *o3_vmr;
+  const double o3col_du = o3col * convert_to_du;
+  /* compute_steady_state_ozone */
+  const double delta_temperature = temperature - clim[1];
+  const double delta_o3col = o3col_du - clim[2];
+  const double ss_o3 =
+      clim[0] - (clim[3] + delta_o3col * clim[6] + delta_temperature * clim[5]) / clim[4];
+  const double delta_o3 = (ss_o3 - o3_old) * (1.0 - M_exp(clim[4] * delta_t));
+  double o3_new = o3_old + delta_o3;
+  double do3_psc = 0.0;
+  psc_activation(lats, temperature, pmid, sza, clim[7], delta_t, excess_chlorine, o3_old, psc_T,
+                 chlorine_loading, &o3_new, &do3_psc);
+  *o3_vmr = o3_new;
+  diag[0] = delta_o3 / delta_t;
+  diag[1] = do3_psc;
+  diag[2] = ss_o3;
+}
+
+/* lin_strat_chem.hpp:263-316: returns o3l_sfcsink [Tg/yr/m2] */
+double orc_linoz_sfcsink(int nlev, double delta_t, const double *pdel, double *o3l_vmr,
+                         double o3_sfc, int o3_lbl, double o3_tau) {
+  const double mwo3 = 48., mwdry = 0.028966 * 1e3, rgrav = 1.0 / 9.80616;
+  double do3mass = 0.0;
+  for (int kk = nlev - 1; kk > nlev - o3_lbl - 1; --kk) {
+    const double efactor = 1.0 - M_exp(-delta_t / o3_tau);
+    const double mass = pdel[kk] * rgrav;
+    const double do3 = (o3_sfc - o3l_vmr[kk]) * efactor;
+    o3l_vmr[kk] = o3l_vmr[kk] + do3;
+    do3mass += do3 * mass * mwo3 / mwdry;
+  }
+  return do3mass / delta_t * 1.0e-9 * (86400. * 365.0);
+}
+
+/* one column, levels contiguous: the column density from the ozone before
+ * the step, the solve above the tropopause (k < ltrop), then the surface
+ * sink (o3_lbl > 0).  Optional outputs: o3col, do3, do3_psc, ss [nlev]
+ * (0 at k >= ltrop), sfcsink. */
+void orc_linoz_column(int nlev, double dt, double sza, double rlats, int ltrop, double col0,
+                      double chlorine_loading, double psc_T, double o3_sfc, double o3_tau,
+                      int o3_lbl, const double *temp, const double *pmid, const double *pdel,
+                      const double *clim /* [8][nlev] */, double *o3, double *o3col,
+                      double *do3, double *do3_psc, double *ss, double *sfcsink) {
+  double *col = (double *)malloc(sizeof(double) * (nlev > 0 ? nlev : 1));
+  orc_o3_column_density(nlev, pdel, o3, col0, col);
+  for (int k = 0; k < nlev; ++k) {
+    double d[3] = {0.0, 0.0, 0.0};
+    if (k < ltrop) {
+      double c[8];
+      for (int j = 0; j < 8; ++j) c[j] = clim[(long)j * nlev + k];
+      orc_linoz_solve_kk(col[k], temp[k], sza, pmid[k], dt, rlats, c, chlorine_loading, psc_T,
+                         &o3[k], d);
+    }
+    if (o3col) o3col[k] = col[k];
+    if (do3) do3[k] = d[0];
+    if (do3_psc) do3_psc[k] = d[1];
+    if (ss) ss[k] = d[2];
+  }
+  const double s = o3_lbl > 0 ? orc_linoz_sfcsink(nlev, dt, pdel, o3, o3_sfc, o3_lbl, o3_tau)
+                              : 0.0;
+  if (sfcsink) *sfcsink = s;
+  free(col);
+}
+
+/* mo_setinv.hpp:45-86 with the default Config: inv[8] [#/cm3] */
+void orc_setinv_single_level(double tfld, double h2ovmr, double pmid,
+                             const double cnst_offline[4], double inv[8]) {
+  const double Pa_xfac = 10.0, boltz_cgs = 0.13806500000000001E-015;
+  const int coff_idx_list[4] = {7, 4, 5, 6};
+  for (int i = 0; i < 8; ++i) inv[i] = 0.0;
+  inv[0] = Pa_xfac * pmid / (boltz_cgs * tfld);
+  const double inv_m = inv[0];
+  inv[1] = 0.79 * inv_m;
+  inv[2] = 0.21 * inv_m;
+  inv[3] = h2ovmr * inv_m;
+  for (int i = 0; i < 4; ++i) inv[coff_idx_list[i]] = cnst_offline[i] * inv_m;
+}
+
+/* setinv over cells: T, p, qv (mass mixing ratio; vmr_from_mmr with the
+ * water molecular weight, mo_setinv.hpp:103) [ncell], cnst_offline
+ * [4][ncell], inv [8][ncell] */
+void orc_setinv(long ncell, const double *tfld, const double *pmid, const double *qv,
+                const double *cnst_offline, double *inv) {
+  for (long c = 0; c < ncell; ++c) {
+    double co[4], v[8];
+    for (int i = 0; i < 4; ++i) co[i] = cnst_offline[(long)i * ncell + c];
+    const double h2ovmr = qv[c] * 0.028966 / 0.018016;
+    orc_setinv_single_level(tfld[c], h2ovmr, pmid[c], co, v);
+    for (int i = 0; i < 8; ++i) inv[(long)i * ncell + c] = v[i];
+  }
+}
+
+/* orc_linoz_column over a domain of [nlev][ncol] planes (column fastest);
+ * clim [8][nlev][ncol]; per column sza, rlats, col0, ltrop; optional
+ * outputs as orc_linoz_column (planes) and sfcsink [ncol].  OpenMP over
+ * columns. */
+void orc_linoz_domain(int ncol, int nlev, double dt, double chlorine_loading, double psc_T,
+                      double o3_sfc, double o3_tau, int o3_lbl, const double *temp,
+                      const double *pmid, const double *pdel, const double *clim,
+                      const double *sza, const double *rlats, const double *col0,
+                      const int *ltrop, double *o3, double *o3col, double *do3,
+                      double *do3_psc, double *ss, double *sfcsink) {
+  const long plane = (long)nlev * ncol;
+#ifdef _OPENMP
+#pragma omp parallel
+#endif
+  {
+    double *buf = (double *)malloc(sizeof(double) * (size_t)nlev * 16 + 8);
+    double *t = buf, *pm = t + nlev, *pd = pm + nlev, *o = pd + nlev, *oc = o + nlev,
+           *d1 = oc + nlev, *d2 = d1 + nlev, *sv = d2 + nlev, *cl = sv + nlev;
+#ifdef _OPENMP
+#pragma omp for schedule(static)
+#endif
+    for (int j = 0; j < ncol; ++j) {
+      for (int k = 0; k < nlev; ++k) {
+        const long i = (long)k * ncol + j;
+        t[k] = temp[i];
+        pm[k] = pmid[i];
+        pd[k] = pdel[i];
+        o[k] = o3[i];
+        for (int c = 0; c < 8; ++c) cl[(long)c * nlev + k] = clim[c * plane + i];
+      }
+      double s = 0.0;
+      orc_linoz_column(nlev, dt, sza[j], rlats[j], ltrop[j], col0 ? col0[j] : 0.0,
+                       chlorine_loading, psc_T, o3_sfc, o3_tau, o3_lbl, t, pm, pd, cl, o, oc, d1,
+                       d2, sv, &s);
+      for (int k = 0; k < nlev; ++k) {
+        const long i = (long)k * ncol + j;
+        o3[i] = o[k];
+        if (o3col) o3col[i] = oc[k];
+        if (do3) do3[i] = d1[k];
+        if (do3_psc) do3_psc[i] = d2[k];
+        if (ss) ss[i] = sv[k];
+      }
+      if (sfcsink) sfcsink[j] = s;
+    }
+    free(buf);
   }
 }

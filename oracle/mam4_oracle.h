@@ -208,6 +208,31 @@ void orc_wet_size(int ncol, int nlev, double dt, const double *state,
                   const double *cloud_fraction, double *dgn_i, double *dgn_c,
                   double *dgncur_awet, double *qaerwat, double *wetdens, double *tend);
 
+/* ---- stratospheric ozone (linoz) and invariants ---- */
+
+/* compute_o3_column_density (compute_o3_column_density.hpp:59-83) */
+void orc_o3_column_density(int nlev, const double *pdel, const double *vmr_o3, double col0,
+                           double *o3_col_dens);
+/* lin_strat_chem_solve_kk with psc_activation (lin_strat_chem.hpp:31-222);
+ * clim[8] in LinozData order; diag = do3_linoz, do3_linoz_psc, ss_o3 */
+void orc_linoz_solve_kk(double o3col, double temperature, double sza, double pmid,
+                        double delta_t, double rlats, const double clim[8],
+                        double chlorine_loading, double psc_T, double *o3_vmr, double diag[3]);
+/* lin_strat_sfcsink (lin_strat_chem.hpp:263-316); returns Tg/yr/m2 */
+double orc_linoz_sfcsink(int nlev, double delta_t, const double *pdel, double *o3l_vmr,
+                         double o3_sfc, int o3_lbl, double o3_tau);
+/* one column: column density, solve above the tropopause, surface sink */
+void orc_linoz_column(int nlev, double dt, double sza, double rlats, int ltrop, double col0,
+                      double chlorine_loading, double psc_T, double o3_sfc, double o3_tau,
+                      int o3_lbl, const double *temp, const double *pmid, const double *pdel,
+                      const double *clim, double *o3, double *o3col, double *do3,
+                      double *do3_psc, double *ss, double *sfcsink);
+/* setinv_single_level (mo_setinv.hpp:45-86) and its loop over cells */
+void orc_setinv_single_level(double tfld, double h2ovmr, double pmid,
+                             const double cnst_offline[4], double inv[8]);
+void orc_setinv(long ncell, const double *tfld, const double *pmid, const double *qv,
+                const double *cnst_offline, double *inv);
+
 #ifdef __cplusplus
 }
 #endif

@@ -252,6 +252,87 @@ class Oracle:
                                                 st.ctypes.data_as(ctypes.POINTER(ctypes.c_int)))
         return q.reshape((31,) + tuple(shape)), st.reshape(shape)
 
+    # ---- stratospheric ozone and invariants (lin_strat_chem.hpp, mo_setinv.hpp) ----
+    def _lnz_sigs(self):
+        L = self.lib
+        if getattr(self, "_lnz_ready", False):
+            return
+        D, I = ctypes.c_double, ctypes.c_int
+        L.orc_o3_column_density.argtypes = [I, _dp, _dp, D, _dp]
+        L.orc_o3_column_density.restype = None
+        L.orc_linoz_solve_kk.argtypes = [D] * 6 + [_dp, D, D, _dp, _dp]
+        L.orc_linoz_solve_kk.restype = None
+        L.orc_linoz_sfcsink.argtypes = [I, D, _dp, _dp, D, I, D]
+        L.orc_linoz_sfcsink.restype = D
+        L.orc_linoz_column.argtypes = [I, D, D, D, I, D, D, D, D, D, I] + [_dp] * 10
+        L.orc_linoz_column.restype = None
+        L.orc_linoz_domain.argtypes = [I, I, D, D, D, D, D, I] + [_dp] * 7 + \
+            [ctypes.POINTER(ctypes.c_int)] + [_dp] * 6
+        L.orc_linoz_domain.restype = None
+        L.orc_setinv_single_level.argtypes = [D, D, D, _dp, _dp]
+        L.orc_setinv_single_level.restype = None
+        L.orc_setinv.argtypes = [ctypes.c_long] + [_dp] * 5
+        L.orc_setinv.restype = None
+        self._lnz_ready = True
+
+    def o3_column_density(self, pdel, vmr_o3, col0=0.0):
+        self._lnz_sigs()
+        p, v = (np.ascontiguousarray(a, dtype=np.float64) for a in (pdel, vmr_o3))
+        out = np.empty_like(p)
+        self.lib.orc_o3_column_density(len(p), _p(p), _p(v), col0, _p(out))
+        return out
+
+    def linoz_solve_kk(self, o3col, temp, sza, pmid, dt, rlats, clim, chlorine_loading, psc_T,
+                       o3):
+        """one level: (new o3, [do3_linoz, do3_linoz_psc, ss_o3])"""
+        self._lnz_sigs()
+        c = np.ascontiguousarray(clim, dtype=np.float64)
+        o = np.array([o3], dtype=np.float64)
+        d = np.zeros(3)
+        self.lib.orc_linoz_solve_kk(o3col, temp, sza, pmid, dt, rlats, _p(c), chlorine_loading,
+                                    psc_T, _p(o), _p(d))
+        return o[0], d
+
+    def linoz_sfcsink(self, dt, pdel, o3, o3_sfc, o3_lbl, o3_tau):
+        """(new o3 column, sink [Tg/yr/m2])"""
+        self._lnz_sigs()
+        p = np.ascontiguousarray(pdel, dtype=np.float64)
+        o = np.array(o3, dtype=np.float64)
+        s = self.lib.orc_linoz_sfcsink(len(p), dt, _p(p), _p(o), o3_sfc, o3_lbl, o3_tau)
+        return o, s
+
+    def linoz(self, dt, cols, conf):
+        """columns: dict of temp, pmid, pdel, o3 [nlev, ncol], clim [8, nlev,
+        ncol], sza, rlats, col0 [ncol], ltrop (int) [ncol]; conf: dict of
+        chlorine_loading, psc_T, o3_sfc, o3_tau, o3_lbl.  Returns (o3, o3col,
+        do3, do3_psc, ss_o3 [nlev, ncol], sfcsink [ncol])."""
+        self._lnz_sigs()
+        nlev, ncol = np.shape(cols["o3"])
+        f = lambda k: np.ascontiguousarray(cols[k], dtype=np.float64)
+        o3 = np.array(cols["o3"], dtype=np.float64)
+        outs = [np.empty((nlev, ncol)) for _ in range(4)]
+        sink = np.empty(ncol)
+        lt = np.ascontiguousarray(cols["ltrop"], dtype=np.int32)
+        self.lib.orc_linoz_domain(ncol, nlev, dt, conf["chlorine_loading"], conf["psc_T"],
+                                  conf["o3_sfc"], conf["o3_tau"], int(conf["o3_lbl"]),
+                                  *[_p(f(k)) for k in ("temp", "pmid", "pdel", "clim", "sza",
+                                                       "rlats", "col0")],
+                                  lt.ctypes.data_as(ctypes.POINTER(ctypes.c_int)), _p(o3),
+                                  *[_p(b) for b in outs], _p(sink))
+        return (o3, *outs, sink)
+
+    def setinv(self, tfld, pmid, qv, cnst_offline):
+        """invariants [8, ...] [#/cm3] from T, p, qv (kg/kg) and the four
+        offline species [4, ...] (vmr)"""
+        self._lnz_sigs()
+        shape = np.shape(tfld)
+        n = int(np.prod(shape))
+        t, p, q = (np.ascontiguousarray(a, dtype=np.float64).reshape(n) for a in (tfld, pmid, qv))
+        co = np.ascontiguousarray(cnst_offline, dtype=np.float64).reshape(4, n)
+        inv = np.empty((8, n))
+        self.lib.orc_setinv(n, _p(t), _p(p), _p(q), _p(co), _p(inv))
+        return inv.reshape((8,) + tuple(shape))
+
     def set_libm_noise(self, seed):
         """seed != 0: randomly move each exp/log/pow/cbrt/erfc result by one
         ulp (conditioning probe); 0: exact libm."""

@@ -49,14 +49,18 @@ extern "C" long long div_check(long n, unsigned long long seed, double *first) {
   double *d_first = nullptr;
   if (hipMalloc(&d_bad, sizeof(unsigned long long)) != hipSuccess) return -1;
   if (hipMalloc(&d_first, 2 * sizeof(double)) != hipSuccess) return -1;
-  hipMemset(d_bad, 0, sizeof(unsigned long long));
-  hipMemset(d_first, 0, 2 * sizeof(double));
-  const unsigned grid = (unsigned)((n + 255) / 256);
-  hipLaunchKernelGGL(div_check_kernel, dim3(grid), dim3(256), 0, 0, n, seed, d_bad, d_first);
+  long long rc = 0;
+  if (hipMemset(d_bad, 0, sizeof(unsigned long long)) != hipSuccess ||
+      hipMemset(d_first, 0, 2 * sizeof(double)) != hipSuccess)
+    rc = -1;
   unsigned long long bad = 0;
-  if (hipMemcpy(&bad, d_bad, sizeof(bad), hipMemcpyDeviceToHost) != hipSuccess) return -1;
-  hipMemcpy(first, d_first, 2 * sizeof(double), hipMemcpyDeviceToHost);
-  hipFree(d_bad);
-  hipFree(d_first);
-  return (long long)bad;
+  if (rc == 0) {
+    const unsigned grid = (unsigned)((n + 255) / 256);
+    hipLaunchKernelGGL(div_check_kernel, dim3(grid), dim3(256), 0, 0, n, seed, d_bad, d_first);
+    if (hipMemcpy(&bad, d_bad, sizeof(bad), hipMemcpyDeviceToHost) != hipSuccess ||
+        hipMemcpy(first, d_first, 2 * sizeof(double), hipMemcpyDeviceToHost) != hipSuccess)
+      rc = -1;
+  }
+  if (hipFree(d_bad) != hipSuccess || hipFree(d_first) != hipSuccess) rc = -1;
+  return rc ? rc : (long long)bad;
 }

@@ -1,6 +1,6 @@
 """Comparison helpers for the HIP chain against the CPU oracle.
 
-Error measure (documented in DESIGN.md): for every output plane the error of a
+Error measure: for every output plane the error of a
 tendency is measured on the state it produces,
 
     err = |tend_gpu - tend_ref| * dt / max(|q_in|, |q_ref_post|, floor)

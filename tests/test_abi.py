@@ -49,6 +49,11 @@ def test_struct_layouts_match_c(tmp_path):
                     ' offsetof(mam4x_setsox_args, dqdt_aqh2so4));'
                     'printf("%zu %zu\\n", sizeof(mam4x_gas_chem_args),'
                     ' offsetof(mam4x_gas_chem_args, qq));'
+                    'printf("%zu %zu\\n", sizeof(mam4x_setinv_args),'
+                    ' offsetof(mam4x_setinv_args, invariants));'
+                    'printf("%zu %zu %zu %zu\\n", sizeof(mam4x_linoz_args),'
+                    ' offsetof(mam4x_linoz_args, chlorine_loading),'
+                    ' offsetof(mam4x_linoz_args, o3_lbl), offsetof(mam4x_linoz_args, o3_sfcsink));'
                     'return 0;}\n')
     exe = tmp_path / "sz"
     subprocess.run(["gcc", "-I", os.path.join(ROOT, "include"), "-o", str(exe), str(prog)],
@@ -60,13 +65,16 @@ def test_struct_layouts_match_c(tmp_path):
           _abi.Outputs.process_tendencies.offset, _abi.Outputs.process_stride.offset,
           ctypes.sizeof(_abi.Tracers), ctypes.sizeof(_abi.Fields), _abi.Fields.field.offset,
           ctypes.sizeof(_abi.SetsoxArgs), _abi.SetsoxArgs.dqdt_aqh2so4.offset,
-          ctypes.sizeof(_abi.GasChemArgs), _abi.GasChemArgs.qq.offset]
+          ctypes.sizeof(_abi.GasChemArgs), _abi.GasChemArgs.qq.offset,
+          ctypes.sizeof(_abi.SetinvArgs), _abi.SetinvArgs.invariants.offset,
+          ctypes.sizeof(_abi.LinozArgs), _abi.LinozArgs.chlorine_loading.offset,
+          _abi.LinozArgs.o3_lbl.offset, _abi.LinozArgs.o3_sfcsink.offset]
     assert c == py
 
 
 def test_abi_version_and_default_config():
     lib = _abi.load()
-    assert lib.mam4x_abi_version() == _abi.ABI_VERSION == 6
+    assert lib.mam4x_abi_version() == _abi.ABI_VERSION == 7
     cfg = _abi.Config()
     assert lib.mam4x_default_config(ctypes.byref(cfg)) == _abi.OK
     assert (cfg.do_cond, cfg.do_rename, cfg.do_newnuc, cfg.do_coag) == (1, 1, 1, 1)
