@@ -712,6 +712,94 @@ def bench_linoz(args):
     print(json.dumps(res), flush=True)
 
 
+def bench_washout(args):
+    """Washout rates (mam4x_compute_washout, sethet; --workload washout) over
+    21,600 synthetic columns x 72 levels: kernel time from HIP events on the
+    launch stream, HBM roofline of the values moved, the oracle on the host
+    cores."""
+    import ctypes
+    import numpy as np
+    import torch
+    from mam4xx_amd import _abi
+    from mam4xx_amd.chain import Chain
+    import oracle
+    rng = np.random.default_rng(0)
+    shp = (NLEV, NCOL)
+    ps = rng.uniform(95000, 103000, NCOL)
+    edges = 10.0 * (ps / 10.0) ** (np.arange(NLEV + 1)[:, None] / NLEV)
+    press = np.sqrt(edges[1:] * edges[:-1])
+    T = np.where(press < 2e4, rng.uniform(200, 220, shp), rng.uniform(220, 300, shp))
+    cloudy = rng.uniform(size=shp) < 0.35
+    cols = dict(press=press, zmid=7000.0 * np.log(ps / press), tfld=T,
+                cmfdqr=np.where(cloudy, 10 ** rng.uniform(-10, -7, shp), 0.0),
+                nrain=np.where(cloudy, 10 ** rng.uniform(-10, -7, shp), 0.0),
+                nevapr=np.where(rng.uniform(size=shp) < 0.3, 10 ** rng.uniform(-10, -7.5, shp),
+                                0.0),
+                inv_m=press / (1.380649e-23 * T) * 1e-6, h2o2=10 ** rng.uniform(-10, -9, shp),
+                so2=np.zeros(shp), rlat=rng.uniform(-1.5, 1.5, NCOL),
+                phis=rng.uniform(0, 3e4, NCOL))
+    dev = lambda a: torch.as_tensor(np.ascontiguousarray(a), dtype=torch.float64).cuda()
+    d = {k: dev(v) for k, v in cols.items()}
+    het = torch.empty((31,) + shp, dtype=torch.float64, device="cuda")
+    ch = Chain()
+    s = torch.cuda.current_stream()
+    a = _abi.SethetArgs()
+    a.nlev, a.ncol, a.ldc, a.layout = NLEV, NCOL, NCOL, _abi.LAYOUT_LEV_COL
+    for f, k in (("pressure", "press"), ("zmid", "zmid"), ("temperature", "tfld"),
+                 ("cmfdqr", "cmfdqr"), ("nrain", "nrain"), ("nevapr", "nevapr"),
+                 ("air_density", "inv_m"), ("h2o2", "h2o2"), ("latitude", "rlat"),
+                 ("phis", "phis")):
+        setattr(a, f, d[k].data_ptr())
+    for i in range(31):
+        a.het_rates[i] = het[i].data_ptr()
+    fn = lambda: ch.lib.mam4x_compute_washout(ch.h, DT, ctypes.byref(a),
+                                              ctypes.c_void_p(s.cuda_stream))
+    ev = [torch.cuda.Event(enable_timing=True) for _ in range(2)]
+    for _ in range(args.warmup + 1):
+        assert fn() == 0
+    torch.cuda.synchronize()
+    ms = 0.0
+    for _ in range(args.steps):
+        ev[0].record(s)
+        fn()
+        ev[1].record(s)
+        ev[1].synchronize()
+        ms += ev[0].elapsed_time(ev[1])
+    ms /= args.steps
+    n = NCOL * NLEV
+    # per cell: pressure, zmid, T, three precipitation rates, M and H2O2 read;
+    # 31 rates written
+    bytes_cell = (8 + 31) * 8
+    achieved = n * bytes_cell / (ms * 1e-3) / 1e9
+    res = {"metric": "washout rates (sethet): column-steps/sec at 72 lev",
+           "unit": "column-steps/s", "value": NCOL / (ms * 1e-3), "n_gpus": 1,
+           "higher_is_better": True, "dtype": "f64", "data": "synthetic", "steps": args.steps,
+           "warmup": args.warmup,
+           "config": {"workload": "21600 columns x 72 levels of synthetic columns, 35 % of "
+                                  "levels precipitating, dt 1800 s",
+                      "columns": NCOL, "levels": NLEV},
+           "kernel_ms": ms,
+           "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS,
+                        "unit": "GB/s", "frac": achieved / HBM_PEAK_GBS,
+                        "traffic": _traffic("sethet_kernel_%dx%d.txt" % (NCOL, NLEV)),
+                        "algorithmic_bytes_per_cell": bytes_cell}}
+    if not args.no_cpu:
+        orc = oracle.Oracle()
+        nth = min(16, os.cpu_count() or 1)
+        run = lambda: orc.sethet(DT, cols)
+        run()
+        best = 1e30
+        for _ in range(5):
+            t0 = time.perf_counter()
+            run()
+            best = min(best, time.perf_counter() - t0)
+        res["cpu_baseline"] = {"value": NCOL / best, "unit": "column-steps/s", "cores": nth,
+                               "kind": "port", "sample": "the full 21600 x 72 domain, best of 5 "
+                               "after a warm-up (OpenMP over columns)", "cpu": _cpu_model()}
+    ch.close()
+    print(json.dumps(res), flush=True)
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -721,13 +809,14 @@ def main():
     ap.add_argument("--schedule", choices=["auto", "fused", "queue"], default="auto")
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline")
     ap.add_argument("--workload", choices=["headline", "batch", "large", "water", "wetsize",
-                                           "tracers", "setsox", "gaschem", "linoz"],
+                                           "tracers", "setsox", "gaschem", "linoz", "washout"],
                     default="headline",
                     help="headline: the metric's 21,600 x 72 step (default); batch, large, "
                          "water (aerosol water uptake), wetsize (calcsize + water uptake), "
                          "tracers (state_q/qqcw <-> state planes), setsox (aqueous sulfur "
                          "chemistry), gaschem (gas-phase chemistry) and linoz (stratospheric "
-                         "ozone, with the invariant densities) print their own line")
+                         "ozone, with the invariant densities) and washout (sethet) print "
+                         "their own line")
     args = ap.parse_args()
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) > 1:
         print("bench.py measures one GPU: the chain has no cross-GPU step and this build "
@@ -753,6 +842,8 @@ def main():
         return bench_setsox(args)
     if args.workload == "linoz":
         return bench_linoz(args)
+    if args.workload == "washout":
+        return bench_washout(args)
     if args.workload == "gaschem":
         return bench_gas_chem(args)
     scene = scenes.generate(NCOL, NLEV, seed=0)

@@ -481,6 +481,38 @@ typedef struct mam4x_linoz_args {
  * its levels.  FP64; stream-ordered; no workspace. */
 int mam4x_compute_linoz(mam4x_handle h, double dt, const mam4x_linoz_args *args, void *stream);
 
+/* ---------------------------------------------------------------------- */
+/* Washout rates: mam4::mo_sethet::sethet (mo_sethet.hpp:217-585) for every */
+/* column -- the het_rates the gas-phase chemistry reads.  H2O2 (1), H2SO4  */
+/* (2) and SO2 (3) get rates; every other species 0.  Level 0 is the model  */
+/* top.  Planes: extents and layout as in mam4x_inputs.                    */
+typedef struct mam4x_sethet_args {
+  int ncol, nlev;
+  int ldc;
+  int layout;
+  const double *pressure;    /* mid-level [Pa]                              */
+  /* the reference's zmid: sethet passes Atmosphere::height, and
+   * sethet_detail's layer depths and fall distances treat it as km */
+  const double *zmid;
+  const double *temperature; /* [K]                                         */
+  const double *cmfdqr;      /* convective precipitation production [kg/kg/s] */
+  const double *nrain;       /* stratiform precipitation production [kg/kg/s] */
+  const double *nevapr;      /* precipitation evaporation [kg/kg/s]          */
+  const double *air_density; /* invariant M [#/cm3]                          */
+  const double *h2o2;        /* H2O2 [mol/mol] (the reference also reads SO2
+                                only for a washout whose result it drops)  */
+  const double *latitude;    /* per column [ncol] [rad]                      */
+  const double *phis;        /* per column [ncol] surface geopotential [m2/s2] */
+  /* washout rates [1/s] by gas_pcnst: planes 1, 2, 3 required (they hold
+   * the column's scratch during the call); other non-NULL planes are
+   * zeroed */
+  double *het_rates[MAM4X_GAS_PCNST];
+} mam4x_sethet_args;
+
+/* sethet over a domain with time step dt [s].  One lane per column.  FP64;
+ * stream-ordered; no workspace. */
+int mam4x_compute_washout(mam4x_handle h, double dt, const mam4x_sethet_args *args, void *stream);
+
 /* Batches: many scenes (domains of any column count, each with its own
  * inputs, outputs and layout) stepped by one set of launches.  The scenes'
  * cells form one work queue, so uneven scenes and uneven per-cell cost share

@@ -238,6 +238,26 @@ class LinozArgs(ctypes.Structure):
     ]
 
 
+class SethetArgs(ctypes.Structure):
+    _fields_ = [
+        ("ncol", ctypes.c_int),
+        ("nlev", ctypes.c_int),
+        ("ldc", ctypes.c_int),
+        ("layout", ctypes.c_int),
+        ("pressure", _vp),
+        ("zmid", _vp),
+        ("temperature", _vp),
+        ("cmfdqr", _vp),
+        ("nrain", _vp),
+        ("nevapr", _vp),
+        ("air_density", _vp),
+        ("h2o2", _vp),
+        ("latitude", _vp),
+        ("phis", _vp),
+        ("het_rates", _vp * GAS_PCNST),
+    ]
+
+
 # entry points declared in include/mam4x.h, with ctypes signatures
 SIGNATURES = {
     "mam4x_abi_version": (ctypes.c_int, []),
@@ -270,6 +290,8 @@ SIGNATURES = {
                                                    ctypes.POINTER(GasChemArgs), _vp]),
     "mam4x_compute_invariants": (ctypes.c_int, [_vp, ctypes.POINTER(SetinvArgs), _vp]),
     "mam4x_compute_linoz": (ctypes.c_int, [_vp, ctypes.c_double, ctypes.POINTER(LinozArgs), _vp]),
+    "mam4x_compute_washout": (ctypes.c_int, [_vp, ctypes.c_double, ctypes.POINTER(SethetArgs),
+                                             _vp]),
     "mam4x_compute_setsox": (ctypes.c_int, [_vp, ctypes.c_double, ctypes.POINTER(SetsoxArgs),
                                             _vp]),
     "mam4x_batch_create": (ctypes.c_int, [_vp, ctypes.c_int, ctypes.POINTER(Inputs),

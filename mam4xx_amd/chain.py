@@ -311,6 +311,30 @@ class Chain:
                                           ctypes.c_void_p(s.cuda_stream))
         self._check(rc, "mam4x_compute_linoz")
 
+    def compute_washout(self, dt, planes, latitude, phis, het_rates,
+                        layout=_abi.LAYOUT_LEV_COL, stream=None):
+        """Washout rates (mam4x_compute_washout, sethet): planes: dict of
+        pressure, zmid, temperature, cmfdqr, nrain, nevapr, air_density,
+        h2o2 tensors; latitude, phis [ncol]; het_rates [31][...] output."""
+        a = _abi.SethetArgs()
+        t = planes["temperature"]
+        a.ncol, a.nlev, a.ldc = self._extents(t, layout)
+        a.layout = layout
+        for k in ("pressure", "zmid", "temperature", "cmfdqr", "nrain", "nevapr",
+                  "air_density", "h2o2"):
+            v = planes[k]
+            assert v.shape == t.shape and v.dtype == torch.float64 and v.is_contiguous(), k
+            setattr(a, k, v.data_ptr())
+        for name, v in (("latitude", latitude), ("phis", phis)):
+            assert v.shape == (a.ncol,) and v.dtype == torch.float64 and v.is_contiguous()
+            setattr(a, name, v.data_ptr())
+        for i, p in enumerate(_plane_ptrs(het_rates, _abi.GAS_PCNST)):
+            a.het_rates[i] = p
+        s = stream if stream is not None else torch.cuda.current_stream()
+        rc = self.lib.mam4x_compute_washout(self.h, dt, ctypes.byref(a),
+                                            ctypes.c_void_p(s.cuda_stream))
+        self._check(rc, "mam4x_compute_washout")
+
     def compute_setsox(self, dt, met, qcw, qin, dqdt_aqso4=None, dqdt_aqh2so4=None,
                        layout=_abi.LAYOUT_LEV_COL, stream=None):
         """Aqueous sulfur chemistry (mam4x_compute_setsox).  met: dict of

@@ -23,6 +23,7 @@
 // chain_fused_kernel runs everything in one lane per cell (small domains).
 #include <hip/hip_runtime.h>
 
+#include <cmath>
 #include <cstdio>
 #include <cstring>
 #include <array>
@@ -34,6 +35,7 @@
 #include "chain_cell.hpp"
 #include "gas_chem.hpp"
 #include "linoz.hpp"
+#include "sethet.hpp"
 #include "setsox.hpp"
 #include "water_uptake.hpp"
 
@@ -867,6 +869,33 @@ __global__ void __launch_bounds__(kLinozBlock *kLinozWaves) linoz_block_kernel(L
   if (w == 0 && live) {
     const double sink = lnz::surface_sink(a.nlev, ks, a.dt, a.cf, a.pdel + base, a.o3 + base);
     if (a.sink) a.sink[j] = sink;
+  }
+}
+
+// ------------------------------------------------------------------ sethet
+// Washout rates, one lane per column (sethet.hpp); blocks of one wave
+// spread the columns over the CUs.
+struct SethetArgs {
+  const double *press, *zmid, *T, *cmfdqr, *nrain, *nevapr, *inv, *h2o2, *lat, *phis;
+  double *het[MAM4X_GAS_PCNST];
+  double xkgm, dt;
+  int ncol, nlev, ldc, layout;
+};
+
+__global__ void __launch_bounds__(kLinozBlock) sethet_kernel(SethetArgs a) {
+  const int j = blockIdx.x * kLinozBlock + threadIdx.x;
+  if (j >= a.ncol) return;
+  const bool lc = a.layout == MAM4X_LAYOUT_LEV_COL;
+  const long base = lc ? (long)j : (long)j * a.ldc;
+  const long ks = lc ? (long)a.ldc : 1L;
+  sh::column(a.nlev, ks, a.lat[j], a.phis[j], a.dt, a.xkgm, a.press + base, a.zmid + base,
+             a.T + base, a.cmfdqr + base, a.nrain + base, a.nevapr + base, a.inv + base,
+             a.h2o2 + base, a.het[1] + base, a.het[2] + base, a.het[3] + base);
+  for (int m = 0; m < MAM4X_GAS_PCNST; ++m) {
+    if (m >= 1 && m <= 3) continue;
+    double *p = a.het[m];
+    if (p)
+      for (int k = 0; k < a.nlev; ++k) p[base + k * ks] = 0.0;
   }
 }
 
@@ -1712,6 +1741,49 @@ int mam4x_compute_linoz(mam4x_handle h, double dt, const mam4x_linoz_args *in, v
                        (size_t)in->nlev * kLinozBlock * sizeof(double), (hipStream_t)stream, a);
   else
     hipLaunchKernelGGL(linoz_kernel, dim3(grid), dim3(kLinozBlock), 0, (hipStream_t)stream, a);
+  HIP_TRY(h, hipGetLastError());
+  return MAM4X_OK;
+}
+
+int mam4x_compute_washout(mam4x_handle h, double dt, const mam4x_sethet_args *in, void *stream) {
+  if (!h) return MAM4X_ERR_ARG;
+  if (!in) return fail(h, MAM4X_ERR_ARG, "args is NULL");
+  if (!(dt > 0.0)) return fail(h, MAM4X_ERR_ARG, "dt must be > 0");
+  int ld = 0;
+  if (int rc = plane_extents(h, in->ncol, in->nlev, in->layout, in->ldc, ld)) return rc;
+  SethetArgs a;
+  std::memset(&a, 0, sizeof(a));
+  const double *need[10] = {in->pressure, in->zmid,        in->temperature, in->cmfdqr,
+                            in->nrain,    in->nevapr,      in->air_density, in->h2o2,
+                            in->latitude, in->phis};
+  for (int i = 0; i < 10; ++i)
+    if (!need[i]) return fail(h, MAM4X_ERR_ARG, "an input plane or column array is NULL");
+  a.press = in->pressure;
+  a.zmid = in->zmid;
+  a.T = in->temperature;
+  a.cmfdqr = in->cmfdqr;
+  a.nrain = in->nrain;
+  a.nevapr = in->nevapr;
+  a.inv = in->air_density;
+  a.h2o2 = in->h2o2;
+  a.lat = in->latitude;
+  a.phis = in->phis;
+  for (int m = 0; m < MAM4X_GAS_PCNST; ++m) a.het[m] = in->het_rates[m];
+  if (!a.het[1] || !a.het[2] || !a.het[3])
+    return fail(h, MAM4X_ERR_ARG, "het_rates planes 1, 2 and 3 are required");
+  // the drops' mass-transfer coefficient (mo_sethet.hpp:330-331), once on
+  // the host
+  const double xrm = .189, xum = 748.0, xvv = 6.18e-2, xdg = .112;
+  a.xkgm = xdg / xrm * 2.0 +
+           xdg / xrm * .6 * std::sqrt(xrm * xum / xvv) * std::pow((xvv / xdg), (1.0 / 3.0));
+  a.dt = dt;
+  a.ncol = in->ncol;
+  a.nlev = in->nlev;
+  a.ldc = ld;
+  a.layout = in->layout;
+  if (in->ncol == 0) return MAM4X_OK;
+  hipLaunchKernelGGL(sethet_kernel, dim3((unsigned)((in->ncol + kLinozBlock - 1) / kLinozBlock)),
+                     dim3(kLinozBlock), 0, (hipStream_t)stream, a);
   HIP_TRY(h, hipGetLastError());
   return MAM4X_OK;
 }

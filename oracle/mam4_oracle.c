@@ -3490,3 +3490,199 @@ void orc_linoz_domain(int ncol, int nlev, double dt, double chlorine_loading, do
     free(buf);
   }
 }
+
+/* ============================ washout (sethet) ============================
+ * mo_sethet::sethet_detail (mo_sethet.hpp:217-500) for one column with the
+ * wrapper's indices (:549-556): H2O2 (1), H2SO4 (2), SO2 (3) are the wet-
+ * deposited species.  Levels contiguous; level 0 is the model top. */
+#define SH_MASS_H2O (0.018016 * 1000)
+static const double sh_avo = 6.022214076e23;
+static const double sh_boltz_cgs = 1.380649e-23 * 1.e7;
+
+/* mo_sethet.hpp:36-52 */
+static double sh_calc_het_rates(double satf, double rain, double xhen, double tmp_hetrates,
+                                double work1, double work2) {
+  const double h2o_mol = 1.0e3 / SH_MASS_H2O;
+  const double work3 = satf * kmax(rain / (h2o_mol * (work1 + 1.0 / (xhen * work2))), 0.0);
+  return work3 + tmp_hetrates;
+}
+
+/* mo_sethet.hpp:59-103 */
+void orc_precip_rescale(int nlev, const double *cmfdqr, const double *nrain,
+                        const double *nevapr, double *precip) {
+  double total_rain = 0, total_pos = 0;
+  for (int kk = 0; kk < nlev; ++kk) {
+    precip[kk] = cmfdqr[kk] + nrain[kk] - nevapr[kk];
+    total_rain += precip[kk];
+  }
+  for (int kk = 0; kk < nlev; ++kk) {
+    if (precip[kk] < 0.0) precip[kk] = 0.0;
+    total_pos += precip[kk];
+  }
+  for (int kk = 0; kk < nlev; ++kk)
+    precip[kk] = total_rain <= 0.0 ? 0.0 : precip[kk] * total_rain / total_pos;
+}
+
+/* mo_sethet.hpp:107-170: in-place washout of xgas by the rain of every
+ * raining level kk >= ktop onto the levels below it */
+void orc_gas_washout(int nlev, int ktop, double xkgm, const double *xliq, const double *rain,
+                     const double *xhen_i, const double *tfld_i, const double *delz_i,
+                     double *xgas) {
+  const double const0 = sh_boltz_cgs * 1.0e-6, geo_fac = 6.0, xrm = .189, xum = 748.0;
+  const double cm3_2_m3 = 1.0e-6, avo2 = sh_avo * 1.0e-3 * cm3_2_m3;
+  double *scratch = (double *)malloc(sizeof(double) * (nlev > 0 ? nlev : 1));
+  for (int kk = ktop; kk < nlev; ++kk) {
+    if (rain[kk] != 0.0) {
+      double accumulator = 0.0;
+      for (int k = kk; k < nlev; ++k) {
+        const double xca = geo_fac * xkgm * xgas[k] / (xrm * xum) * delz_i[k] * xliq[kk] *
+                           cm3_2_m3;
+        accumulator += xca;
+        scratch[k] = accumulator;
+      }
+      for (int k = kk; k < nlev; ++k) {
+        const double xeqca = xgas[k] /
+                             (xliq[kk] * avo2 + 1.0 / (xhen_i[k] * const0 * tfld_i[k])) *
+                             xliq[kk] * avo2;
+        if (scratch[k] < xeqca) {
+          const double xca = geo_fac * xkgm * xgas[k] / (xrm * xum) * delz_i[k] * xliq[kk] *
+                             cm3_2_m3;
+          xgas[k] = kmax(xgas[k] - xca, 0.0);
+        }
+      }
+    }
+  }
+  free(scratch);
+}
+
+/* mo_sethet.hpp:173-197: the lowest level above p_limit (0 if none) */
+int orc_find_ktop(int nlev, double rlat, const double *press) {
+  const double d2r = 3.14159265358979323846264 / 180.0;
+  const double p_limit = fabs(rlat) > 60.0 * d2r ? 300.0e2 : 100.0e2;
+  for (int kk = nlev - 1; kk >= 0; kk--)
+    if (press[kk] < p_limit) return kk;
+  return 0;
+}
+
+/* the rain drops' mass-transfer coefficient (mo_sethet.hpp:330-331) */
+double orc_sethet_xkgm(void) {
+  const double xrm = .189, xum = 748.0, xvv = 6.18e-2, xdg = .112;
+  return xdg / xrm * 2.0 + xdg / xrm * .6 * M_sqrt(xrm * xum / xvv) * M_pow((xvv / xdg), (1.0 / 3.0));
+}
+
+/* one column: press [Pa], zmid (the reference's zmid argument, see
+ * sethet_detail), tfld, cmfdqr, nrain, nevapr [kg/kg/s], inv_m [#/cm3],
+ * qin_h2o2 / qin_so2 [vmr] (levels contiguous); het [31][nlev] out */
+void orc_sethet_column(int nlev, double rlat, double phis, double delt, const double *press,
+                       const double *zmid, const double *tfld, const double *cmfdqr,
+                       const double *nrain, const double *nevapr, const double *inv_m,
+                       const double *qin_h2o2, const double *qin_so2, double *het) {
+  const double xrm = .189, xum = 748.0, t0 = 298.0, xph0 = 1.0e-5, satf_hno3 = .016,
+               satf_h2o2 = .016, const0 = sh_boltz_cgs * 1.0e-6, hno3_diss = 15.4,
+               mass_air = 29.0, km2cm = 1.0e5, m2km = 1.0e-3, m3_2_cm3 = 1.0e6,
+               MISSING = -999999.0, large_value_lifetime = 1.0e29;
+  const double avo2 = sh_avo * 1.0e-3 * 1.0e-6, rga = 1.0 / 9.80616;
+  (void)xrm;
+  const size_t n = (size_t)(nlev > 0 ? nlev : 1);
+  double *w = (double *)calloc(n * 15, sizeof(double));
+  double *t_factor = w, *xk0_hno3 = w + n, *xk0_so2 = w + 2 * n, *so2_diss = w + 3 * n,
+         *xgas2 = w + 4 * n, *xgas3 = w + 5 * n, *delz = w + 6 * n, *xh2o2 = w + 7 * n,
+         *xso2 = w + 8 * n, *xliq = w + 9 * n, *rain = w + 10 * n, *precip = w + 11 * n,
+         *xhen_h2o2 = w + 12 * n, *xhen_hno3 = w + 13 * n, *xhen_so2 = w + 14 * n;
+  double *tmp1 = (double *)calloc(n, sizeof(double)), *tmp2 = (double *)calloc(n, sizeof(double));
+  const int wetdep_map[3] = {1, 2, 3};
+  for (int kk = 0; kk < nlev; ++kk)
+    for (int mm = 0; mm < 31; ++mm) het[(long)mm * nlev + kk] = 0.0;
+  for (int kk = 0; kk < nlev; ++kk)
+    for (int mm = 0; mm < 3; ++mm) het[(long)wetdep_map[mm] * nlev + kk] = MISSING;
+  const double xkgm = orc_sethet_xkgm();
+  const int ktop = orc_find_ktop(nlev, rlat, press);
+  orc_precip_rescale(nlev, cmfdqr, nrain, nevapr, precip);
+  for (int kk = 0; kk < nlev; ++kk) {
+    rain[kk] = mass_air * precip[kk] * inv_m[kk] / SH_MASS_H2O;
+    xliq[kk] = precip[kk] * delt * inv_m[kk] / sh_avo * mass_air * m3_2_cm3;
+    xh2o2[kk] = qin_h2o2[kk] * inv_m[kk];
+    xso2[kk] = qin_so2[kk] * inv_m[kk];
+  }
+  const double zsurf = m2km * phis * rga;
+  for (int kk = ktop; kk < nlev - 1; ++kk) delz[kk] = fabs((zmid[kk] - zmid[kk + 1]) * km2cm);
+  delz[nlev - 1] = fabs((zmid[nlev - 1] - zsurf) * km2cm);
+  for (int kk = ktop; kk < nlev; ++kk) {
+    t_factor[kk] = (t0 - tfld[kk]) / (t0 * tfld[kk]);
+    xhen_h2o2[kk] = 7.45e4 * M_exp(6620.0 * t_factor[kk]);
+    xk0_hno3[kk] = 2.1e5 * M_exp(8700.0 * t_factor[kk]);
+    xhen_hno3[kk] = xk0_hno3[kk] * (1.0 + hno3_diss / xph0);
+    xk0_so2[kk] = 1.23 * M_exp(3120.0 * t_factor[kk]);
+    so2_diss[kk] = 1.23e-2 * M_exp(1960.0 * t_factor[kk]);
+    xhen_so2[kk] = xk0_so2[kk] * (1.0 + so2_diss[kk] / xph0);
+  }
+  for (int kk = 0; kk < nlev; ++kk) {
+    xgas2[kk] = xh2o2[kk];
+    xgas3[kk] = xso2[kk];
+  }
+  orc_gas_washout(nlev, ktop, xkgm, xliq, rain, xhen_h2o2, tfld, delz, xgas2);
+  orc_gas_washout(nlev, ktop, xkgm, xliq, rain, xhen_so2, tfld, delz, xgas3);
+  for (int kk = ktop; kk < nlev; ++kk) {
+    double stay = 1.0;
+    if (rain[kk] != 0.0) {
+      stay = ((zmid[kk] - zsurf) * km2cm) / (xum * delt);
+      stay = kmin(stay, 1.0);
+    }
+    const double xdtm = delz[kk] / xum;
+    const double xxx2 = (xh2o2[kk] - xgas2[kk]);
+    const double yh2o2 = xxx2 != 0.0 ? xh2o2[kk] / xxx2 * xdtm : large_value_lifetime;
+    tmp1[kk] = kmax(1.0 / yh2o2, 0.0) * stay;
+    const double xxx3 = (xso2[kk] - xgas3[kk]);
+    const double yso2 = xxx3 != 0.0 ? xso2[kk] / xxx3 * xdtm : large_value_lifetime;
+    tmp2[kk] = kmax(1.0 / yso2, 0.0) * stay;
+  }
+  for (int kk = ktop; kk < nlev; ++kk) {
+    if (rain[kk] <= 0.0) {
+      for (int mm = 0; mm < 31; ++mm) het[(long)mm * nlev + kk] = 0.0;
+      continue;
+    }
+    const double work1 = avo2 * xliq[kk], work2 = const0 * tfld[kk];
+    het[1L * nlev + kk] = sh_calc_het_rates(satf_h2o2, rain[kk], xhen_h2o2[kk], tmp1[kk], work1,
+                                            work2);
+    het[3L * nlev + kk] = het[1L * nlev + kk]; /* SO2 takes the H2O2 rate */
+    het[2L * nlev + kk] = sh_calc_het_rates(satf_hno3, rain[kk], xhen_hno3[kk], 0.0, work1,
+                                            work2);
+  }
+  for (int kk = 0; kk < ktop + 1 && kk < nlev; ++kk)
+    for (int mm = 0; mm < 3; ++mm) het[(long)wetdep_map[mm] * nlev + kk] = 0.0;
+  free(w);
+  free(tmp1);
+  free(tmp2);
+}
+
+/* orc_sethet_column over [nlev][ncol] planes (column fastest); rlat, phis
+ * [ncol]; het [31][nlev][ncol] */
+void orc_sethet_domain(int ncol, int nlev, double delt, const double *rlat, const double *phis,
+                       const double *press, const double *zmid, const double *tfld,
+                       const double *cmfdqr, const double *nrain, const double *nevapr,
+                       const double *inv_m, const double *qin_h2o2, const double *qin_so2,
+                       double *het) {
+  const long plane = (long)nlev * ncol;
+#ifdef _OPENMP
+#pragma omp parallel
+#endif
+  {
+    double *b = (double *)malloc(sizeof(double) * (size_t)(nlev > 0 ? nlev : 1) * (9 + 31));
+    double *in[9];
+    for (int i = 0; i < 9; ++i) in[i] = b + (long)i * nlev;
+    double *h = b + 9L * nlev;
+    const double *src[9] = {press, zmid, tfld, cmfdqr, nrain, nevapr, inv_m, qin_h2o2, qin_so2};
+#ifdef _OPENMP
+#pragma omp for schedule(static)
+#endif
+    for (int j = 0; j < ncol; ++j) {
+      for (int i = 0; i < 9; ++i)
+        for (int k = 0; k < nlev; ++k) in[i][k] = src[i][(long)k * ncol + j];
+      orc_sethet_column(nlev, rlat[j], phis[j], delt, in[0], in[1], in[2], in[3], in[4], in[5],
+                        in[6], in[7], in[8], h);
+      for (int m = 0; m < 31; ++m)
+        for (int k = 0; k < nlev; ++k) het[m * plane + (long)k * ncol + j] = h[(long)m * nlev + k];
+    }
+    free(b);
+  }
+}

@@ -333,6 +333,63 @@ class Oracle:
         self.lib.orc_setinv(n, _p(t), _p(p), _p(q), _p(co), _p(inv))
         return inv.reshape((8,) + tuple(shape))
 
+    # ---- washout rates (mo_sethet.hpp) ----
+    def _sh_sigs(self):
+        L = self.lib
+        if getattr(self, "_sh_ready", False):
+            return
+        D, I = ctypes.c_double, ctypes.c_int
+        L.orc_precip_rescale.argtypes = [I, _dp, _dp, _dp, _dp]
+        L.orc_precip_rescale.restype = None
+        L.orc_gas_washout.argtypes = [I, I, D] + [_dp] * 6
+        L.orc_gas_washout.restype = None
+        L.orc_find_ktop.argtypes = [I, D, _dp]
+        L.orc_find_ktop.restype = I
+        L.orc_sethet_xkgm.argtypes = []
+        L.orc_sethet_xkgm.restype = D
+        L.orc_sethet_column.argtypes = [I, D, D, D] + [_dp] * 10
+        L.orc_sethet_column.restype = None
+        L.orc_sethet_domain.argtypes = [I, I, D] + [_dp] * 12
+        L.orc_sethet_domain.restype = None
+        self._sh_ready = True
+
+    def precip_rescale(self, cmfdqr, nrain, nevapr):
+        self._sh_sigs()
+        a = [np.ascontiguousarray(x, dtype=np.float64) for x in (cmfdqr, nrain, nevapr)]
+        out = np.empty_like(a[0])
+        self.lib.orc_precip_rescale(len(out), *[_p(x) for x in a], _p(out))
+        return out
+
+    def gas_washout(self, ktop, xkgm, xliq, rain, xhen, tfld, delz, xgas):
+        self._sh_sigs()
+        a = [np.ascontiguousarray(x, dtype=np.float64) for x in (xliq, rain, xhen, tfld, delz)]
+        g = np.array(xgas, dtype=np.float64)
+        self.lib.orc_gas_washout(len(g), ktop, xkgm, *[_p(x) for x in a], _p(g))
+        return g
+
+    def find_ktop(self, rlat, press):
+        self._sh_sigs()
+        p = np.ascontiguousarray(press, dtype=np.float64)
+        return self.lib.orc_find_ktop(len(p), rlat, _p(p))
+
+    def sethet_xkgm(self):
+        self._sh_sigs()
+        return self.lib.orc_sethet_xkgm()
+
+    def sethet(self, delt, cols):
+        """cols: dict of press, zmid, tfld, cmfdqr, nrain, nevapr, inv_m,
+        h2o2, so2 [nlev, ncol] and rlat, phis [ncol]; returns het_rates
+        [31, nlev, ncol] [1/s]"""
+        self._sh_sigs()
+        nlev, ncol = np.shape(cols["press"])
+        f = lambda k: np.ascontiguousarray(cols[k], dtype=np.float64)
+        het = np.empty((31, nlev, ncol))
+        self.lib.orc_sethet_domain(ncol, nlev, delt, _p(f("rlat")), _p(f("phis")),
+                                   *[_p(f(k)) for k in ("press", "zmid", "tfld", "cmfdqr",
+                                                        "nrain", "nevapr", "inv_m", "h2o2",
+                                                        "so2")], _p(het))
+        return het
+
     def set_libm_noise(self, seed):
         """seed != 0: randomly move each exp/log/pow/cbrt/erfc result by one
         ulp (conditioning probe); 0: exact libm."""

@@ -54,6 +54,8 @@ def test_struct_layouts_match_c(tmp_path):
                     'printf("%zu %zu %zu %zu\\n", sizeof(mam4x_linoz_args),'
                     ' offsetof(mam4x_linoz_args, chlorine_loading),'
                     ' offsetof(mam4x_linoz_args, o3_lbl), offsetof(mam4x_linoz_args, o3_sfcsink));'
+                    'printf("%zu %zu\\n", sizeof(mam4x_sethet_args),'
+                    ' offsetof(mam4x_sethet_args, het_rates));'
                     'return 0;}\n')
     exe = tmp_path / "sz"
     subprocess.run(["gcc", "-I", os.path.join(ROOT, "include"), "-o", str(exe), str(prog)],
@@ -68,7 +70,8 @@ def test_struct_layouts_match_c(tmp_path):
           ctypes.sizeof(_abi.GasChemArgs), _abi.GasChemArgs.qq.offset,
           ctypes.sizeof(_abi.SetinvArgs), _abi.SetinvArgs.invariants.offset,
           ctypes.sizeof(_abi.LinozArgs), _abi.LinozArgs.chlorine_loading.offset,
-          _abi.LinozArgs.o3_lbl.offset, _abi.LinozArgs.o3_sfcsink.offset]
+          _abi.LinozArgs.o3_lbl.offset, _abi.LinozArgs.o3_sfcsink.offset,
+          ctypes.sizeof(_abi.SethetArgs), _abi.SethetArgs.het_rates.offset]
     assert c == py
 
 
