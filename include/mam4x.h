@@ -513,6 +513,30 @@ typedef struct mam4x_sethet_args {
  * stream-ordered; no workspace. */
 int mam4x_compute_washout(mam4x_handle h, double dt, const mam4x_sethet_args *args, void *stream);
 
+/* ---------------------------------------------------------------------- */
+/* External forcing: mam4::mo_setext::setext (mo_setext.hpp:22-117) for    */
+/* every cell -- the extfrc the gas-phase chemistry reads.  Each forced     */
+/* species' rate is the sum of its sectors' data, in sector order; data     */
+/* stored bottom-up (the reference's file_alt_data) are read flipped.       */
+/* Planes: extents and layout as in mam4x_inputs.                          */
+#define MAM4X_MAX_SECTORS 4
+typedef struct mam4x_setext_args {
+  int ncol, nlev;
+  int ldc;
+  int layout;
+  /* per forced species mm (the reference's Forcing): destination entry
+   * frc_ndx[mm] (1-based, 1..MAM4X_EXTCNT), sector count (0..4), and
+   * whether its data run bottom-up */
+  int frc_ndx[MAM4X_EXTCNT];
+  int nsectors[MAM4X_EXTCNT];
+  int file_alt_data[MAM4X_EXTCNT];
+  const double *sectors[MAM4X_EXTCNT][MAM4X_MAX_SECTORS]; /* [#/cm3/s] planes */
+  double *extfrc[MAM4X_EXTCNT];                            /* out, [#/cm3/s] */
+} mam4x_setext_args;
+
+/* setext over a domain.  FP64; stream-ordered; no workspace. */
+int mam4x_compute_forcing(mam4x_handle h, const mam4x_setext_args *args, void *stream);
+
 /* Batches: many scenes (domains of any column count, each with its own
  * inputs, outputs and layout) stepped by one set of launches.  The scenes'
  * cells form one work queue, so uneven scenes and uneven per-cell cost share

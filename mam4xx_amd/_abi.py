@@ -258,6 +258,23 @@ class SethetArgs(ctypes.Structure):
     ]
 
 
+MAX_SECTORS = 4
+
+
+class SetextArgs(ctypes.Structure):
+    _fields_ = [
+        ("ncol", ctypes.c_int),
+        ("nlev", ctypes.c_int),
+        ("ldc", ctypes.c_int),
+        ("layout", ctypes.c_int),
+        ("frc_ndx", ctypes.c_int * EXTCNT),
+        ("nsectors", ctypes.c_int * EXTCNT),
+        ("file_alt_data", ctypes.c_int * EXTCNT),
+        ("sectors", (_vp * MAX_SECTORS) * EXTCNT),
+        ("extfrc", _vp * EXTCNT),
+    ]
+
+
 # entry points declared in include/mam4x.h, with ctypes signatures
 SIGNATURES = {
     "mam4x_abi_version": (ctypes.c_int, []),
@@ -292,6 +309,7 @@ SIGNATURES = {
     "mam4x_compute_linoz": (ctypes.c_int, [_vp, ctypes.c_double, ctypes.POINTER(LinozArgs), _vp]),
     "mam4x_compute_washout": (ctypes.c_int, [_vp, ctypes.c_double, ctypes.POINTER(SethetArgs),
                                              _vp]),
+    "mam4x_compute_forcing": (ctypes.c_int, [_vp, ctypes.POINTER(SetextArgs), _vp]),
     "mam4x_compute_setsox": (ctypes.c_int, [_vp, ctypes.c_double, ctypes.POINTER(SetsoxArgs),
                                             _vp]),
     "mam4x_batch_create": (ctypes.c_int, [_vp, ctypes.c_int, ctypes.POINTER(Inputs),

@@ -335,6 +335,31 @@ class Chain:
                                             ctypes.c_void_p(s.cuda_stream))
         self._check(rc, "mam4x_compute_washout")
 
+    def compute_forcing(self, sectors, extfrc, frc_ndx=None, file_alt_data=None,
+                        layout=_abi.LAYOUT_LEV_COL, stream=None):
+        """External forcing (mam4x_compute_forcing, setext): sectors: 9 lists
+        (one per forced species) of up to 4 planes; extfrc [9][...] output;
+        frc_ndx: 1-based destination entries (default 1..9); file_alt_data:
+        per species, data stored bottom-up."""
+        a = _abi.SetextArgs()
+        t = extfrc[0]
+        a.ncol, a.nlev, a.ldc = self._extents(t, layout)
+        a.layout = layout
+        assert len(sectors) == _abi.EXTCNT
+        for m in range(_abi.EXTCNT):
+            a.frc_ndx[m] = frc_ndx[m] if frc_ndx is not None else m + 1
+            a.file_alt_data[m] = int(bool(file_alt_data[m])) if file_alt_data is not None else 0
+            a.nsectors[m] = len(sectors[m])
+            for i, v in enumerate(sectors[m]):
+                assert v.shape == t.shape and v.dtype == torch.float64 and v.is_contiguous()
+                a.sectors[m][i] = v.data_ptr()
+        for i, p in enumerate(_plane_ptrs(extfrc, _abi.EXTCNT)):
+            a.extfrc[i] = p
+        s = stream if stream is not None else torch.cuda.current_stream()
+        rc = self.lib.mam4x_compute_forcing(self.h, ctypes.byref(a),
+                                            ctypes.c_void_p(s.cuda_stream))
+        self._check(rc, "mam4x_compute_forcing")
+
     def compute_setsox(self, dt, met, qcw, qin, dqdt_aqso4=None, dqdt_aqh2so4=None,
                        layout=_abi.LAYOUT_LEV_COL, stream=None):
         """Aqueous sulfur chemistry (mam4x_compute_setsox).  met: dict of

@@ -899,6 +899,36 @@ __global__ void __launch_bounds__(kLinozBlock) sethet_kernel(SethetArgs a) {
   }
 }
 
+// ------------------------------------------------------------------ setext
+// External forcing, one lane per cell: each forced species' sectors summed
+// in order (extfrc_set, mo_setext.hpp:22-54).
+struct SetextArgs {
+  const double *sec[MAM4X_EXTCNT][MAM4X_MAX_SECTORS];
+  double *out[MAM4X_EXTCNT];  // by species mm: its destination plane
+  int nsec[MAM4X_EXTCNT], alt[MAM4X_EXTCNT];
+  int ncol, nlev, ldc, layout;
+};
+
+__global__ void __launch_bounds__(kBlock) setext_kernel(SetextArgs a) {
+  const long c = (long)blockIdx.x * kBlock + threadIdx.x;
+  if (c >= (long)a.ncol * a.nlev) return;
+  const bool cl = a.layout == MAM4X_LAYOUT_COL_LEV;
+  const int fast = cl ? a.nlev : a.ncol;
+  const long slow = c / fast;
+  const int f = (int)(c - slow * fast);
+  const int k = cl ? f : (int)slow, j = cl ? (int)slow : f;
+  const long off = cl ? (long)j * a.ldc + k : (long)k * a.ldc + j;
+  const int kf = a.nlev - 1 - k;  // the level read from bottom-up data
+  const long off_alt = cl ? (long)j * a.ldc + kf : (long)kf * a.ldc + j;
+  for (int m = 0; m < MAM4X_EXTCNT; ++m) {
+    if (!a.out[m]) continue;
+    const long o = a.alt[m] ? off_alt : off;
+    double v = 0.0;
+    for (int s = 0; s < a.nsec[m]; ++s) v += a.sec[m][s][o];
+    a.out[m][off] = v;
+  }
+}
+
 }  // namespace
 
 enum { kPre = 0, kPost = 1, kFused = 2 };
@@ -1784,6 +1814,42 @@ int mam4x_compute_washout(mam4x_handle h, double dt, const mam4x_sethet_args *in
   if (in->ncol == 0) return MAM4X_OK;
   hipLaunchKernelGGL(sethet_kernel, dim3((unsigned)((in->ncol + kLinozBlock - 1) / kLinozBlock)),
                      dim3(kLinozBlock), 0, (hipStream_t)stream, a);
+  HIP_TRY(h, hipGetLastError());
+  return MAM4X_OK;
+}
+
+int mam4x_compute_forcing(mam4x_handle h, const mam4x_setext_args *in, void *stream) {
+  if (!h) return MAM4X_ERR_ARG;
+  if (!in) return fail(h, MAM4X_ERR_ARG, "args is NULL");
+  int ld = 0;
+  if (int rc = plane_extents(h, in->ncol, in->nlev, in->layout, in->ldc, ld)) return rc;
+  SetextArgs a;
+  std::memset(&a, 0, sizeof(a));
+  bool used[MAM4X_EXTCNT] = {};
+  for (int m = 0; m < MAM4X_EXTCNT; ++m) {
+    const int nn = in->frc_ndx[m] - 1;
+    if (nn < 0 || nn >= MAM4X_EXTCNT)
+      return fail(h, MAM4X_ERR_ARG, "frc_ndx outside 1.." + std::to_string(MAM4X_EXTCNT));
+    if (used[nn]) return fail(h, MAM4X_ERR_ARG, "two forcings write one extfrc entry");
+    used[nn] = true;
+    if (in->nsectors[m] < 0 || in->nsectors[m] > MAM4X_MAX_SECTORS)
+      return fail(h, MAM4X_ERR_ARG, "nsectors outside 0..4");
+    a.nsec[m] = in->nsectors[m];
+    a.alt[m] = in->file_alt_data[m] != 0;
+    for (int s = 0; s < a.nsec[m]; ++s) {
+      a.sec[m][s] = in->sectors[m][s];
+      if (!a.sec[m][s]) return fail(h, MAM4X_ERR_ARG, "a sector plane is NULL");
+    }
+    a.out[m] = in->extfrc[nn];
+  }
+  a.ncol = in->ncol;
+  a.nlev = in->nlev;
+  a.ldc = ld;
+  a.layout = in->layout;
+  const long ncell = (long)in->ncol * in->nlev;
+  if (ncell == 0) return MAM4X_OK;
+  hipLaunchKernelGGL(setext_kernel, dim3((unsigned)((ncell + kBlock - 1) / kBlock)),
+                     dim3(kBlock), 0, (hipStream_t)stream, a);
   HIP_TRY(h, hipGetLastError());
   return MAM4X_OK;
 }

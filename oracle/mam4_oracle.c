@@ -3686,3 +3686,24 @@ void orc_sethet_domain(int ncol, int nlev, double delt, const double *rlat, cons
     free(b);
   }
 }
+
+/* ====================== external forcing (setext) =========================
+ * mo_setext::extfrc_set (mo_setext.hpp:22-54) for one column: each forced
+ * species' rate is the sum of its sectors, read bottom-up where the data
+ * are stored that way (file_alt_data).  sectors[mm][isec] -> [nlev];
+ * frcing [9][nlev]. */
+void orc_setext_column(int nlev, const int frc_ndx[9], const int file_alt_data[9],
+                       const int nsectors[9], const double *const sectors[9][4],
+                       double *frcing) {
+  for (int kk = 0; kk < nlev; ++kk)
+    for (int mm = 0; mm < 9; ++mm) {
+      const int nn = frc_ndx[mm] - 1;
+      frcing[(long)nn * nlev + kk] = 0.0;
+      for (int isec = 0; isec < nsectors[mm]; ++isec) {
+        if (file_alt_data[mm])
+          frcing[(long)nn * nlev + kk] += sectors[mm][isec][nlev - 1 - kk];
+        else
+          frcing[(long)nn * nlev + kk] += sectors[mm][isec][kk];
+      }
+    }
+}

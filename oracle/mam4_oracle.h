@@ -232,6 +232,33 @@ void orc_setinv_single_level(double tfld, double h2ovmr, double pmid,
                              const double cnst_offline[4], double inv[8]);
 void orc_setinv(long ncell, const double *tfld, const double *pmid, const double *qv,
                 const double *cnst_offline, double *inv);
+void orc_linoz_domain(int ncol, int nlev, double dt, double chlorine_loading, double psc_T,
+                      double o3_sfc, double o3_tau, int o3_lbl, const double *temp,
+                      const double *pmid, const double *pdel, const double *clim,
+                      const double *sza, const double *rlats, const double *col0,
+                      const int *ltrop, double *o3, double *o3col, double *do3,
+                      double *do3_psc, double *ss, double *sfcsink);
+
+/* ---- washout (mo_sethet.hpp) and external forcing (mo_setext.hpp) ---- */
+void orc_precip_rescale(int nlev, const double *cmfdqr, const double *nrain,
+                        const double *nevapr, double *precip);
+void orc_gas_washout(int nlev, int ktop, double xkgm, const double *xliq, const double *rain,
+                     const double *xhen_i, const double *tfld_i, const double *delz_i,
+                     double *xgas);
+int orc_find_ktop(int nlev, double rlat, const double *press);
+double orc_sethet_xkgm(void);
+void orc_sethet_column(int nlev, double rlat, double phis, double delt, const double *press,
+                       const double *zmid, const double *tfld, const double *cmfdqr,
+                       const double *nrain, const double *nevapr, const double *inv_m,
+                       const double *qin_h2o2, const double *qin_so2, double *het);
+void orc_sethet_domain(int ncol, int nlev, double delt, const double *rlat, const double *phis,
+                       const double *press, const double *zmid, const double *tfld,
+                       const double *cmfdqr, const double *nrain, const double *nevapr,
+                       const double *inv_m, const double *qin_h2o2, const double *qin_so2,
+                       double *het);
+void orc_setext_column(int nlev, const int frc_ndx[9], const int file_alt_data[9],
+                       const int nsectors[9], const double *const sectors[9][4],
+                       double *frcing);
 
 #ifdef __cplusplus
 }

@@ -390,6 +390,36 @@ class Oracle:
                                                         "so2")], _p(het))
         return het
 
+    # ---- external forcing (mo_setext.hpp) ----
+    def setext(self, sectors, frc_ndx=None, file_alt_data=None):
+        """sectors: 9 lists (one per forced species) of up to 4 [nlev, ncol]
+        arrays; returns extfrc [9, nlev, ncol] (entries no forcing targets
+        are 0)"""
+        L = self.lib
+        L.orc_setext_column.argtypes = [ctypes.c_int, ctypes.POINTER(ctypes.c_int),
+                                        ctypes.POINTER(ctypes.c_int),
+                                        ctypes.POINTER(ctypes.c_int),
+                                        ctypes.POINTER((ctypes.c_void_p * 4) * 9), _dp]
+        L.orc_setext_column.restype = None
+        shape = next(np.shape(s[0]) for s in sectors if len(s))
+        nlev, ncol = shape
+        ndx = (ctypes.c_int * 9)(*(frc_ndx if frc_ndx is not None else range(1, 10)))
+        alt = (ctypes.c_int * 9)(*([int(bool(x)) for x in file_alt_data]
+                                   if file_alt_data is not None else [0] * 9))
+        nsec = (ctypes.c_int * 9)(*[len(s) for s in sectors])
+        out = np.zeros((9, nlev, ncol))
+        for j in range(ncol):
+            cols = [[np.ascontiguousarray(np.asarray(x, dtype=np.float64)[:, j]) for x in s]
+                    for s in sectors]
+            tab = ((ctypes.c_void_p * 4) * 9)()
+            for m in range(9):
+                for i, c in enumerate(cols[m]):
+                    tab[m][i] = c.ctypes.data
+            fr = np.zeros((9, nlev))
+            L.orc_setext_column(nlev, ndx, alt, nsec, ctypes.byref(tab), _p(fr))
+            out[:, :, j] = fr
+        return out
+
     def set_libm_noise(self, seed):
         """seed != 0: randomly move each exp/log/pow/cbrt/erfc result by one
         ulp (conditioning probe); 0: exact libm."""

@@ -56,6 +56,8 @@ def test_struct_layouts_match_c(tmp_path):
                     ' offsetof(mam4x_linoz_args, o3_lbl), offsetof(mam4x_linoz_args, o3_sfcsink));'
                     'printf("%zu %zu\\n", sizeof(mam4x_sethet_args),'
                     ' offsetof(mam4x_sethet_args, het_rates));'
+                    'printf("%zu %zu %zu\\n", sizeof(mam4x_setext_args),'
+                    ' offsetof(mam4x_setext_args, sectors), offsetof(mam4x_setext_args, extfrc));'
                     'return 0;}\n')
     exe = tmp_path / "sz"
     subprocess.run(["gcc", "-I", os.path.join(ROOT, "include"), "-o", str(exe), str(prog)],
@@ -71,7 +73,9 @@ def test_struct_layouts_match_c(tmp_path):
           ctypes.sizeof(_abi.SetinvArgs), _abi.SetinvArgs.invariants.offset,
           ctypes.sizeof(_abi.LinozArgs), _abi.LinozArgs.chlorine_loading.offset,
           _abi.LinozArgs.o3_lbl.offset, _abi.LinozArgs.o3_sfcsink.offset,
-          ctypes.sizeof(_abi.SethetArgs), _abi.SethetArgs.het_rates.offset]
+          ctypes.sizeof(_abi.SethetArgs), _abi.SethetArgs.het_rates.offset,
+          ctypes.sizeof(_abi.SetextArgs), _abi.SetextArgs.sectors.offset,
+          _abi.SetextArgs.extfrc.offset]
     assert c == py
 
 
