@@ -49,7 +49,7 @@ def _gpu_linoz(torch, cols, col_lev=False, conf=CONF, outputs=True):
         (sink.cpu().numpy() if sink is not None else None)
 
 
-@pytest.mark.parametrize("nlev", [72, 150])  # LDS block kernel; lane-per-column above 128
+@pytest.mark.parametrize("nlev", [72, 128, 150])  # LDS block kernel up to 128 levels (64 KiB)
 def test_linoz_matches_oracle(orc, torch_cuda, nlev):
     rng = np.random.default_rng(21 + nlev)
     cols = columns(rng, nlev, 500)
