@@ -777,15 +777,15 @@ __global__ void __launch_bounds__(kBlock) setinv_kernel(SetinvArgs a) {
 // ------------------------------------------------------------------ linoz
 // Stratospheric ozone (linoz.hpp).  A lane per column walking all its levels
 // leaves a 21,600-column domain with 340 waves whose every level waits on
-// memory.  The block kernel instead gives each 64-column block four waves,
-// each over a quarter of the levels: the waves form the levels' column
+// memory.  The block kernel instead gives each 64-column block eight waves,
+// each over an eighth of the levels: the waves form the levels' column
 // slabs into LDS, one wave runs the columns' running sums there in level
 // order (the reference's summation order, so the overhead column is
-// unchanged to the bit), then all four solve their levels, and one wave
+// unchanged to the bit), then all eight solve their levels, and one wave
 // applies the surface sink.  Columns taller than kLinozLdsLevels use the
 // lane-per-column kernel.
 constexpr int kLinozBlock = 64;
-constexpr int kLinozWaves = 4;
+constexpr int kLinozWaves = 8;  // 2 and 4 waves measured slower
 constexpr int kLinozLdsLevels = 128;  // 64 KiB of slabs per block
 struct LinozArgs {
   const double *T, *p, *pdel;
