@@ -9,7 +9,9 @@
   H2SO4, SOA with SOAG (in the chain's carbon units), POM, BC, dust, NaCl,
   MOM, over all modes, interstitial and cloud-borne -- in every cell;
 * the work-queue schedule's results do not depend on the order in which the
-  queue hands out cells: two runs agree bit for bit.
+  queue hands out cells: two runs agree bit for bit;
+* the column kernels of the chemistry feeders (linoz, washout) at the same
+  size agree with the oracle on a sample of columns.
 """
 import numpy as np
 import pytest
@@ -117,3 +119,47 @@ def test_full_size_wet_size_sample_matches_oracle(orc, torch_cuda):
         assert np.median(err) < 1e-15, name
         assert (err > 1e-12).mean() < 1e-3, (name, np.sort(err.ravel())[-3:])
         assert err.max() < 1e-6, (name, err.max())
+
+
+def test_full_size_chemistry_feeders_sample_matches_oracle(orc, torch_cuda):
+    """linoz (LDS block kernel) and washout over 21,600 columns; columns are
+    independent, so the sampled columns must match the oracle run on them
+    alone (bars of test_gpu_linoz.py and test_gpu_sethet.py)"""
+    from mam4xx_amd.chain import Chain
+    from .test_linoz import CONF, columns as linoz_columns
+    from .test_sethet import columns as sethet_columns
+    from .test_gpu_sethet import PLANES
+    torch = torch_cuda
+    dev = lambda a, dt=torch.float64: torch.as_tensor(np.ascontiguousarray(a), dtype=dt).cuda()
+    sub = lambda cols: {k: np.ascontiguousarray(v[..., SAMPLE]) for k, v in cols.items()}
+    ch = Chain()
+    # linoz
+    cols = linoz_columns(np.random.default_rng(61), NLEV, NCOL)
+    o3 = dev(cols["o3"])
+    ch.compute_linoz(DT, dev(cols["temp"]), dev(cols["pmid"]), dev(cols["pdel"]),
+                     dev(cols["clim"]), dev(cols["sza"]), dev(cols["rlats"]),
+                     dev(cols["ltrop"], torch.int32), o3, CONF,
+                     column_above_top=dev(cols["col0"]))
+    # washout
+    wcols = sethet_columns(np.random.default_rng(62), NLEV, NCOL)
+    het = torch.empty((31, NLEV, NCOL), dtype=torch.float64, device="cuda")
+    ch.compute_washout(DT, {k: dev(wcols[v]) for k, v in PLANES.items()}, dev(wcols["rlat"]),
+                       dev(wcols["phis"]), het)
+    torch.cuda.synchronize()
+    ch.close()
+    got = o3.cpu().numpy()[:, SAMPLE]
+    ref = orc.linoz(DT, sub(cols), CONF)[0]
+    o3_0 = cols["o3"][:, SAMPLE]
+    assert (np.abs(got - ref) <= 1e-12 * np.maximum(np.abs(o3_0), np.abs(ref))).all()
+    assert np.isfinite(o3.cpu().numpy()).all()
+    hg = het.cpu().numpy()[..., SAMPLE]
+    hr = orc.sethet(DT, sub(wcols))
+    env = np.zeros_like(hr)
+    try:
+        for s in range(1, 7):
+            orc.set_libm_noise(s)
+            env = np.maximum(env, np.abs(orc.sethet(DT, sub(wcols)) - hr))
+    finally:
+        orc.set_libm_noise(0)
+    assert (np.abs(hg - hr) <= np.maximum(1e-12 * np.abs(hr), 10 * env)).all()
+    assert np.isfinite(het.cpu().numpy()).all() and (hr[1] > 0).sum() > 100
