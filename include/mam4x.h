@@ -543,8 +543,11 @@ int mam4x_compute_forcing(mam4x_handle h, const mam4x_setext_args *args, void *s
  * the GPU.  A batch is planned once for fixed pointers and dt; runs only
  * launch kernels -- no allocation, copy, memset or synchronisation -- so a
  * run can be captured in a hipGraph and replayed (with stats disabled).
- *   mam4x_batch_create: validates and uploads the argument blocks and sizes
- *     the handle's workspace (allocates and synchronizes).
+ *   mam4x_batch_create: validates and uploads the argument blocks and
+ *     allocates the batch's own work-queue workspace (allocates and
+ *     synchronizes).  A batch never uses the handle's workspace, so a graph
+ *     that captured its run stays valid whatever the handle does later
+ *     (larger steps, other batches); it is valid until mam4x_batch_destroy.
  *   mam4x_batch_run: one step of every scene; dt must equal the planned dt.
  * No counterpart in the reference, which steps one column per Kokkos team. */
 typedef struct mam4x_batch_s *mam4x_batch;

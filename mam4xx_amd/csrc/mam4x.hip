@@ -1905,21 +1905,34 @@ int mam4x_compute_setsox(mam4x_handle h, double dt, const mam4x_setsox_args *in,
   return MAM4X_OK;
 }
 
+// A batch owns its argument blocks and its work-queue workspace (SOA loop
+// planes, cost buckets, queue, queue counters), allocated once at creation.
+// A run captured in a hipGraph keeps these pointers, so nothing a later call
+// on the handle does (a larger step growing the handle's own workspace,
+// another batch) can move them.
 struct mam4x_batch_s {
   mam4x_handle h = nullptr;
   int nscene = 0;
   double dt = 0.0;
-  bool pre = false, do_cond = false;
+  bool pre = false, fused = false, queue = false;
   ChainArgs *d_args = nullptr;
   unsigned *d_blk0 = nullptr;
   unsigned nblock = 0;
   long ncell = 0;  // padded to whole blocks per scene
+  double *ws = nullptr;
+  unsigned char *bucket = nullptr;
+  int *wqueue = nullptr;
+  QueueCtl *ctl = nullptr;
 };
 
 int mam4x_batch_destroy(mam4x_batch b) {
   if (!b) return MAM4X_ERR_ARG;
   if (b->d_args) (void)hipFree(b->d_args);
   if (b->d_blk0) (void)hipFree(b->d_blk0);
+  if (b->ws) (void)hipFree(b->ws);
+  if (b->bucket) (void)hipFree(b->bucket);
+  if (b->wqueue) (void)hipFree(b->wqueue);
+  if (b->ctl) (void)hipFree(b->ctl);
   delete b;
   return MAM4X_OK;
 }
@@ -1955,14 +1968,20 @@ int mam4x_batch_create(mam4x_handle h, int nscene, const mam4x_inputs *in,
   b->nscene = nscene;
   b->dt = dt;
   b->pre = pre;
-  b->do_cond = args[0].cfg.do_cond != 0;
   b->nblock = blk0[nscene];
   b->ncell = (long)b->nblock * kBlock;
+  b->fused = h->cfg.schedule == MAM4X_SCHED_FUSED ||
+             (h->cfg.schedule == MAM4X_SCHED_AUTO && b->ncell < kFusedBelow);
+  b->queue = !b->fused && args[0].cfg.do_cond != 0 && b->ncell > 0;
   if (hipMalloc(&b->d_args, sizeof(ChainArgs) * nscene) != hipSuccess ||
-      hipMalloc(&b->d_blk0, sizeof(unsigned) * (nscene + 1)) != hipSuccess) {
+      hipMalloc(&b->d_blk0, sizeof(unsigned) * (nscene + 1)) != hipSuccess ||
+      (b->queue && (hipMalloc(&b->ws, sizeof(double) * W_NPLANE * b->ncell) != hipSuccess ||
+                    hipMalloc(&b->bucket, (size_t)b->ncell) != hipSuccess ||
+                    hipMalloc(&b->wqueue, sizeof(int) * b->ncell) != hipSuccess ||
+                    hipMalloc(&b->ctl, sizeof(QueueCtl)) != hipSuccess))) {
     (void)hipGetLastError();
     mam4x_batch_destroy(b);
-    return fail(h, MAM4X_ERR_NOMEM, "batch argument blocks");
+    return fail(h, MAM4X_ERR_NOMEM, "batch argument blocks or workspace");
   }
   if (hipMemcpy(b->d_args, args.data(), sizeof(ChainArgs) * nscene, hipMemcpyHostToDevice) !=
           hipSuccess ||
@@ -1970,17 +1989,6 @@ int mam4x_batch_create(mam4x_handle h, int nscene, const mam4x_inputs *in,
                 hipMemcpyHostToDevice) != hipSuccess) {
     mam4x_batch_destroy(b);
     return fail(h, MAM4X_ERR_HIP, "batch upload failed");
-  }
-  if (b->do_cond && b->ncell > 0) {
-    int rc = reserve_work(h, b->ncell, nullptr);
-    if (rc) {
-      mam4x_batch_destroy(b);
-      return rc;
-    }
-    if (hipDeviceSynchronize() != hipSuccess) {
-      mam4x_batch_destroy(b);
-      return fail(h, MAM4X_ERR_HIP, "batch set-up");
-    }
   }
   *batch = b;
   return MAM4X_OK;
@@ -1992,25 +2000,22 @@ int mam4x_batch_run(mam4x_batch b, double t, double dt, void *stream) {
   mam4x_handle h = b->h;
   if (dt != b->dt) return fail(h, MAM4X_ERR_ARG, "batch planned for another dt");
   if (b->ncell == 0) return MAM4X_OK;
-  if (b->do_cond && h->cap < b->ncell)
-    return fail(h, MAM4X_ERR_ARG, "workspace shrank below the batch (handle reused?)");
   hipStream_t s = (hipStream_t)stream;
   const bool mixed = h->cfg.precision == MAM4X_PREC_MIXED;
-  const bool fused = h->cfg.schedule == MAM4X_SCHED_FUSED ||
-                     (h->cfg.schedule == MAM4X_SCHED_AUTO && b->ncell < kFusedBelow);
   const SceneBatch src{b->d_args, b->d_blk0, b->nscene};
   Work w;
   std::memset(&w, 0, sizeof(w));
   w.ncell = b->ncell;
-  w.ctl = h->ctl;
+  w.ctl = b->ctl;
   w.dtfull = (double)(int)(dt / 1);  // QUIRK (mam4_amicphys.hpp:1319): int sub-step
   if (h->stats) w.stats = h->d_stats;
-  const bool queue = !fused && b->do_cond;
+  const bool queue = b->queue;
+  const bool fused = b->fused;
   if (queue) {
-    w.ws = h->ws;
-    w.bucket = h->bucket;
-    w.queue = h->queue;
-    hipLaunchKernelGGL(queue_reset_kernel, dim3(1), dim3(64), 0, s, h->ctl);
+    w.ws = b->ws;
+    w.bucket = b->bucket;
+    w.queue = b->wqueue;
+    hipLaunchKernelGGL(queue_reset_kernel, dim3(1), dim3(64), 0, s, b->ctl);
     if (mixed)
       launch_cells<Mixed>(kPre, b->pre, b->nblock, s, src, w);
     else

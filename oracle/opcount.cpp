@@ -65,6 +65,17 @@ static inline CD fabs(CD a) { return CD::mk(std::fabs(a.v), a.dyn); }
 static inline CD round(CD a) { return CD::mk(std::round(a.v), a.dyn); }
 static inline bool isfinite(CD a) { return std::isfinite(a.v); }
 static inline CD nextafter(CD a, double b) { return CD::mk(std::nextafter(a.v, b), a.dyn); }
+// functions of the oracle's other entry points (water uptake, chemistry),
+// which the chain's step never calls; counted at weight 1
+static inline CD operator+(CD a) { return a; }
+static inline CD hypot(CD a, CD b) { return op(std::hypot(a.v, b.v), a.dyn || b.dyn); }
+static inline CD atan2(CD a, CD b) { return op(std::atan2(a.v, b.v), a.dyn || b.dyn); }
+static inline CD cos(CD a) { return op(std::cos(a.v), a.dyn); }
+static inline CD sin(CD a) { return op(std::sin(a.v), a.dyn); }
+static inline CD log10(CD a) { return op(std::log10(a.v), a.dyn); }
+static inline CD acos(CD a) { return op(std::acos(a.v), a.dyn); }
+static inline CD tanh(CD a) { return op(std::tanh(a.v), a.dyn); }
+static inline bool isnan(CD a) { return std::isnan(a.v); }
 
 #define double CD
 #define _Thread_local thread_local

@@ -5,7 +5,9 @@
   queue);
 * a batch run captured in a CUDA graph and replayed over chained steps
   matches an eager batch stepping an identical copy, replay by replay, with
-  eager work and synchronisation between the replays.
+  eager work and synchronisation between the replays;
+* a batch owns its workspace: a captured run stays valid after the same
+  handle grows its own workspace with a larger single-domain step.
 """
 import numpy as np
 import pytest
@@ -89,6 +91,53 @@ def test_batch_graph_replays_match_eager(torch_cuda):
         for da, db in zip(copies[0], copies[1]):
             assert torch.equal(da["state"], db["state"]), step
             assert torch.equal(da["dgncur_a"], db["dgncur_a"]), step
+    for b in batches:
+        b.close()
+
+
+def test_batch_graph_survives_handle_workspace_growth(torch_cuda):
+    from mam4xx_amd import _abi
+    from mam4xx_amd.chain import Chain, scene_to_device
+    torch = torch_cuda
+    scs = scenes.uneven_batch(nscenes=4, seed=6, total_cols=400)
+    copies = []
+    for _ in range(2):
+        ds = [scene_to_device(s) for s in scs]
+        for d in ds:
+            d["dgncur_awet"] = d["dgncur_a"]
+        copies.append(ds)
+    chans = [Chain(schedule=_abi.SCHED_QUEUE), Chain(schedule=_abi.SCHED_QUEUE)]
+    batches = []
+    for ch, ds in zip(chans, copies):
+        outs = [_outputs(torch, s) for s in scs]
+        batches.append(ch.batch(ds, DT, [o[0] for o in outs], dgn_i=[d["dgncur_a"] for d in ds],
+                                state_out=[d["state"] for d in ds]))
+    graph = torch.cuda.CUDAGraph()
+    side = torch.cuda.Stream()
+    side.wait_stream(torch.cuda.current_stream())
+    with torch.cuda.stream(side):
+        batches[0].run()
+    torch.cuda.current_stream().wait_stream(side)
+    batches[1].run()
+    torch.cuda.synchronize()
+    with torch.cuda.graph(graph):
+        batches[0].run()
+    # a single-domain step larger than the batch on the capturing handle:
+    # the handle's own workspace grows (its old buffers are freed)
+    big = scene_to_device(scenes.generate(1200, 72, seed=21))
+    tb, gb = _outputs(torch, scenes.generate(1200, 72, seed=21))
+    for _ in range(2):
+        chans[0].compute_tendencies(big, DT, tb, dgn_i=gb)
+        torch.cuda.synchronize()
+        for step in range(2):
+            graph.replay()
+            batches[1].run()
+            torch.cuda.synchronize()
+            for da, db in zip(copies[0], copies[1]):
+                assert torch.equal(da["state"], db["state"]), step
+                assert torch.equal(da["dgncur_a"], db["dgncur_a"]), step
+        big = scene_to_device(scenes.generate(2400, 72, seed=22))
+        tb, gb = _outputs(torch, scenes.generate(2400, 72, seed=22))
     for b in batches:
         b.close()
 
