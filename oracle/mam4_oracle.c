@@ -163,6 +163,8 @@ static double vapor_saturation_pressure_magnus(double T, double P) {
   return 1.00071 * M_exp(4.5e-8 * P) * ew;
 }
 
+double orc_magnus_es(double T, double P) { return vapor_saturation_pressure_magnus(T, P); }
+
 /* ======================= subarea set-up (mam4_amicphys.hpp) ============== */
 
 /* mam4_amicphys.hpp:241-317 */
@@ -369,10 +371,20 @@ static void set_subarea_gases_and_aerosols(
 /* ======================= gas-aerosol exchange ============================ */
 
 /* gasaerexch.hpp:222-236 */
+/* Test switch: 0 (default) keeps the reference's constants; 1 replaces the
+ * air terms of Fuller's diffusivity -- 0.035 for 1/M_air, 2.7189 for
+ * V_air^(1/3), and 1.013e5 Pa per atmosphere -- by 1/28.966, 20.1^(1/3) and
+ * 1.01325e5, the values under which the Fortran MAM4 known answers of
+ * src/tests/mam4_amicphys_1gridcell.cpp:1788-2386 were produced. */
+static int g_fortran_air = 0;
+void orc_set_fortran_air_constants(int on) { g_fortran_air = on; }
+
 static double gas_diffusivity(double T, double p_atm, double mw, double vd) {
   const double onethird = 1.0 / 3.0;
-  const double dgas = (1.0e-3 * M_pow(T, 1.75) * M_sqrt(1. / mw + 0.035)) /
-                      (p_atm * M_pow((M_pow(vd, onethird) + 2.7189), 2.0));
+  const double inv_mw_air = g_fortran_air ? 1.0 / 28.966 : 0.035;
+  const double vd_air_cbrt = g_fortran_air ? pow(20.1, onethird) : 2.7189;
+  const double dgas = (1.0e-3 * M_pow(T, 1.75) * M_sqrt(1. / mw + inv_mw_air)) /
+                      (p_atm * M_pow((M_pow(vd, onethird) + vd_air_cbrt), 2.0));
   return dgas * 1.0e-4;
 }
 /* gasaerexch.hpp:240-250 */
@@ -632,7 +644,7 @@ static int mam_gasaerexch_1subarea(int jtsubstep, double dtsubstep, double temp,
   qgas_avg[0] = 0.0;
   qgas_avg[1] = 0.0;
   if (jtsubstep == 0) {
-    const double p_in_atm = pmid / 1.013e5;
+    const double p_in_atm = pmid / (g_fortran_air ? 1.01325e5 : 1.013e5);
     for (int igas = 0; igas < 2; ++igas) {
       const double dif = gas_diffusivity(temp, p_in_atm, mw_gas[igas], vol_molar_gas[igas]);
       const double spd = mean_molecular_speed(temp, mw_gas[igas]);
@@ -1395,6 +1407,18 @@ static double tail_fraction(double diameter, double log_dia_cutoff,
   return 0.5 * M_erfc(tail);
 }
 
+/* Renaming's tail fraction of the Aitken -> accumulation pair for a mode of
+ * geometric-mean diameter `diameter` (number, or volume with volume != 0),
+ * and the pair's cutoff diameter (rename.hpp:142-150, 392-474). */
+double orc_rename_tail_fraction(double diameter, int volume, double *cutoff) {
+  static const int dest_mode_of_mode[4] = {-1, 0, -1, -1};
+  rename_consts c;
+  find_renaming_pairs(dest_mode_of_mode, &c);
+  if (cutoff) *cutoff = c.diameter_cutoff[1];
+  return tail_fraction(diameter, c.ln_dia_cutoff[1], c.fmode_dist_tail_fac[1],
+                       c.ln_diameter_tail_fac[1], volume);
+}
+
 /* rename.hpp:225-389 */
 static void do_inter_mode_transfer(const int dest_mode_of_mode[4], int is_cloudy,
                                    double smallest_dryvol, const rename_consts *c,
@@ -1505,7 +1529,8 @@ static int mam_amicphys_1subarea(
     double qaer_delaa[7][4][4], const double qnumcw3[4], double qnumcw_cur[4],
     double qnumcw_delaa[4][1], const double qaercw2[7][4],
     const double qaercw3[7][4], double qaercw_cur[7][4],
-    double qaercw_delaa[7][4][1]) {
+    double qaercw_delaa[7][4][1], double *qnum_cond_only,
+    double *qaer_cond_only, double *qnum_coag_aged, double *qaer_coag_aged) {
   int niter = 0;
   const double r_universal = 8314.46759100000;
   const double aircon = pmid / (r_universal * temp);
@@ -1565,6 +1590,12 @@ static int mam_amicphys_1subarea(
           qaer_delsub_cond[s][m] = qaer_cur[s][m] - qaer_sv1[s][m];
       del_h2so4_aeruptk = qgas_cur[IGAS_H2SO4] -
                           (qgas_sv1[IGAS_H2SO4] + netprod[IGAS_H2SO4] * dtsubstep);
+      /* condensation's own increments, before aging re-attributes the
+       * primary-carbon share (the Fortran's iqtend_cond_only column) */
+      if (qnum_cond_only)
+        for (int m = 0; m < 4; ++m) qnum_cond_only[m] = qnum_delsub_cond[m];
+      if (qaer_cond_only)
+        memcpy(qaer_cond_only, qaer_delsub_cond, sizeof(qaer_delsub_cond));
     } else {
       for (int g = 0; g < 2; ++g) qgas_avg[g] = qgas_cur[g];
       memset(qaer_delsub_cond, 0, sizeof(qaer_delsub_cond));
@@ -1658,6 +1689,12 @@ static int mam_amicphys_1subarea(
       orc_pcarbon_aging_1subarea(nmono, dgn_a, qnum_cur, qnum_delsub_cond,
                                  qnum_delsub_coag, qaer_cur, qaer_delsub_cond,
                                  qaer_delsub_coag, qaer_delsub_coag_in);
+    /* coagulation's increments after aging re-attributed their primary-carbon
+     * share: the Fortran's iqtend_coag column (the reference accumulates its
+     * column before aging, :1631-1640) */
+    if (qnum_coag_aged)
+      for (int m = 0; m < 4; ++m) qnum_coag_aged[m] = qnum_delsub_coag[m];
+    if (qaer_coag_aged) memcpy(qaer_coag_aged, qaer_delsub_coag, sizeof(qaer_delsub_coag));
     if (do_cond) {
       for (int m = 0; m < 4; ++m) qnum_delaa[m][0] += qnum_delsub_cond[m];
       for (int s = 0; s < 7; ++s)
@@ -1665,6 +1702,41 @@ static int mam_amicphys_1subarea(
     }
   }
   return niter;
+}
+
+/* One call of mam_amicphys_1subarea (mam4_amicphys.hpp:1126-1694) with flat
+ * arrays, for the known-answer vectors of one clear and one cloudy subarea.
+ * Gases qgas[2] = {SOAG, H2SO4}; masses qaer[7][4] in the amicphys species
+ * order (SOA, SO4, POM, BC, NaCl, dust, MOM) x mode (acc, Aitken, coarse,
+ * primary carbon); increments qgas_delaa[2][4], qnum_delaa[4][4],
+ * qaer_delaa[7][4][4] over (cond, rename, newnuc, coag); cloud-borne
+ * increments [4][1] and [7][4][1].  qnum_cond_only[4] / qaer_cond_only[7][4]
+ * (may be NULL) receive condensation's increments before aging, and
+ * qnum_coag_aged[4] / qaer_coag_aged[7][4] (may be NULL) coagulation's after
+ * it.  Returns the SOA sub-step count. */
+int orc_amicphys_1subarea(int gaexch_h2so4_uptake_optaa, int do_cond, int do_rename,
+                          int do_newnuc, int do_coag, double deltat, int iscldy,
+                          unsigned nmono, double temp, double pmid, double zmid,
+                          double pblh, double relhum, const double *dgn_a,
+                          const double *dgn_awet, const double *wetdens,
+                          const double *qgas1, const double *qgas3, double *qgas_cur,
+                          double *qgas_delaa, const double *qnum3, double *qnum_cur,
+                          double *qnum_delaa, const double *qaer2, const double *qaer3,
+                          double *qaer_cur, double *qaer_delaa, const double *qnumcw3,
+                          double *qnumcw_cur, double *qnumcw_delaa,
+                          const double *qaercw2, const double *qaercw3,
+                          double *qaercw_cur, double *qaercw_delaa,
+                          double *qnum_cond_only, double *qaer_cond_only,
+                          double *qnum_coag_aged, double *qaer_coag_aged) {
+  return mam_amicphys_1subarea(
+      gaexch_h2so4_uptake_optaa, do_cond, do_rename, do_newnuc, do_coag, deltat, iscldy,
+      nmono, temp, pmid, zmid, pblh, relhum, dgn_a, dgn_awet, wetdens, qgas1, qgas3,
+      qgas_cur, (double(*)[4])qgas_delaa, qnum3, qnum_cur, (double(*)[4])qnum_delaa,
+      (const double(*)[4])qaer2, (const double(*)[4])qaer3, (double(*)[4])qaer_cur,
+      (double(*)[4][4])qaer_delaa, qnumcw3, qnumcw_cur, (double(*)[1])qnumcw_delaa,
+      (const double(*)[4])qaercw2, (const double(*)[4])qaercw3, (double(*)[4])qaercw_cur,
+      (double(*)[4][1])qaercw_delaa, qnum_cond_only, qaer_cond_only, qnum_coag_aged,
+      qaer_coag_aged);
 }
 
 /* mam4_amicphys.hpp:1699-1991 */
@@ -1764,7 +1836,7 @@ static int mam_amicphys_1gridcell(
         dgn_a, dgn_awet, wetdens, qgas1, qgas3, qgas4, qgas_delaa, qnum3, qnum4,
         qnum_delaa, (const double(*)[4])qaer2, (const double(*)[4])qaer3, qaer4,
         qaer_delaa, qnumcw3, qnumcw4, qnumcw_delaa, (const double(*)[4])qaercw2,
-        (const double(*)[4])qaercw3, qaercw4, qaercw_delaa);
+        (const double(*)[4])qaercw3, qaercw4, qaercw_delaa, NULL, NULL, NULL, NULL);
     if (do_map_gas)
       for (int g = 0; g < 2; ++g) {
         const int ll = LMAP_GAS[g];

@@ -54,6 +54,38 @@ void orc_amicphys_intr(const orc_config *cfg, double deltat, double temp,
                        const double wetdens_host[4], double *qgcm_tendaa,
                        double *qqcwgcm_tendaa, int stats[2]);
 
+/* One call of mam_amicphys_1subarea (mam4_amicphys.hpp:1126-1694): gases
+ * {SOAG, H2SO4}, masses [7][4] in the amicphys species order x mode,
+ * increments over (cond, rename, newnuc, coag).  Returns SOA sub-steps. */
+int orc_amicphys_1subarea(int gaexch_h2so4_uptake_optaa, int do_cond, int do_rename,
+                          int do_newnuc, int do_coag, double deltat, int iscldy,
+                          unsigned nmono, double temp, double pmid, double zmid,
+                          double pblh, double relhum, const double *dgn_a,
+                          const double *dgn_awet, const double *wetdens,
+                          const double *qgas1, const double *qgas3, double *qgas_cur,
+                          double *qgas_delaa, const double *qnum3, double *qnum_cur,
+                          double *qnum_delaa, const double *qaer2, const double *qaer3,
+                          double *qaer_cur, double *qaer_delaa, const double *qnumcw3,
+                          double *qnumcw_cur, double *qnumcw_delaa,
+                          const double *qaercw2, const double *qaercw3,
+                          double *qaercw_cur, double *qaercw_delaa,
+                          double *qnum_cond_only, double *qaer_cond_only,
+                          double *qnum_coag_aged, double *qaer_coag_aged);
+
+/* Test switch for the air terms of the gas diffusivity: 0 = the reference's
+ * constants (default), 1 = the Fortran MAM4 values (1/28.966, 20.1^(1/3),
+ * 1.01325e5 Pa/atm). */
+void orc_set_fortran_air_constants(int on);
+
+/* Improved Magnus saturation vapour pressure over water with the moist-air
+ * enhancement factor [Pa] (conversions.hpp:148-180). */
+double orc_magnus_es(double T, double P);
+
+/* Renaming's erfc tail fraction of the Aitken -> accumulation pair at
+ * geometric-mean diameter `diameter` (number, or volume with volume != 0);
+ * *cutoff (may be NULL) receives the pair's cutoff diameter. */
+double orc_rename_tail_fraction(double diameter, int volume, double *cutoff);
+
 /* Per-level calcsize: the body of CalcSize::compute_tendencies
  * (calcsize.hpp:962-1218) for one level k.  Inputs are prognostics in the
  * AeroProcess layout (n[mode], q[mode][species-in-mode]); tendencies must be

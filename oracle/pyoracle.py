@@ -110,6 +110,57 @@ class Oracle:
         L.orc_calcsize_sub_level.restype = None
         L.orc_wet_size.argtypes = [ctypes.c_int, ctypes.c_int, ctypes.c_double] + [_dp] * 11
         L.orc_wet_size.restype = None
+        L.orc_amicphys_1subarea.argtypes = ([ctypes.c_int] * 5 + [ctypes.c_double, ctypes.c_int,
+                                                                  ctypes.c_uint]
+                                            + [ctypes.c_double] * 5 + [_dp] * 25)
+        L.orc_amicphys_1subarea.restype = ctypes.c_int
+        L.orc_magnus_es.argtypes = [ctypes.c_double, ctypes.c_double]
+        L.orc_magnus_es.restype = ctypes.c_double
+        L.orc_rename_tail_fraction.argtypes = [ctypes.c_double, ctypes.c_int, _dp]
+        L.orc_rename_tail_fraction.restype = ctypes.c_double
+        L.orc_set_fortran_air_constants.argtypes = [ctypes.c_int]
+        L.orc_set_fortran_air_constants.restype = None
+
+    def amicphys_1subarea(self, c, do_cond=1, do_rename=1, do_newnuc=1, do_coag=1, optaa=2,
+                          nmono=8, fortran_air=False):
+        """One mam_amicphys_1subarea call (mam4_amicphys.hpp:1126-1694) on the
+        inputs of a dict such as the known-answer cases: gases {SOAG, H2SO4},
+        masses [7][4] (amicphys species order x mode).  Returns the updated
+        gases, numbers and masses, the increments with a fifth column holding
+        condensation's own increments before aging, coagulation's increments
+        after aging ("coag_aged"), and the SOA sub-steps.  fortran_air selects
+        the Fortran MAM4 air terms of the gas diffusivity for this call."""
+        f = lambda k, shape: np.ascontiguousarray(np.asarray(c[k], dtype=np.float64)  # noqa
+                                                  .reshape(shape))
+        z = lambda *shape: np.zeros(shape)  # noqa: E731
+        dgn_a, dgn_awet, wetdens = f("dgn_a", 4), f("dgn_awet", 4), f("wetdens", 4)
+        qgas1, qgas3, qnum3 = f("qgas1", 2), f("qgas3", 2), f("qnum3", 4)
+        qaer2, qaer3 = f("qaer2", (7, 4)), f("qaer3", (7, 4))
+        qgas, qgas_d, qnum, qnum_d = z(2), z(2, 4), z(4), z(4, 4)
+        qaer, qaer_d = z(7, 4), z(7, 4, 4)
+        qnumcw3, qnumcw, qnumcw_d = z(4), z(4), z(4, 1)
+        qaercw2, qaercw3, qaercw, qaercw_d = z(7, 4), z(7, 4), z(7, 4), z(7, 4, 1)
+        qnum_co, qaer_co, qnum_ca, qaer_ca = z(4), z(7, 4), z(4), z(7, 4)
+        self.lib.orc_set_fortran_air_constants(1 if fortran_air else 0)
+        try:
+            n = self._subarea_call(c, optaa, do_cond, do_rename, do_newnuc, do_coag, nmono, (
+                dgn_a, dgn_awet, wetdens, qgas1, qgas3, qgas, qgas_d, qnum3, qnum, qnum_d,
+                qaer2, qaer3, qaer, qaer_d, qnumcw3, qnumcw, qnumcw_d, qaercw2, qaercw3,
+                qaercw, qaercw_d, qnum_co, qaer_co, qnum_ca, qaer_ca))
+        finally:
+            self.lib.orc_set_fortran_air_constants(0)
+        # gases: condensation has no aging share, so cond_only = cond
+        qgas_d5 = np.concatenate([qgas_d, qgas_d[:, :1] * (do_cond != 0)], axis=1)
+        qnum_d5 = np.concatenate([qnum_d, qnum_co[:, None]], axis=1)
+        qaer_d5 = np.concatenate([qaer_d, qaer_co[:, :, None]], axis=2)
+        return {"qgas4": qgas, "qgas_delaa": qgas_d5, "qnum4": qnum, "qnum_delaa": qnum_d5,
+                "qaer4": qaer, "qaer_delaa": qaer_d5, "qnumcw4": qnumcw, "qaercw4": qaercw,
+                "qnum_coag_aged": qnum_ca, "qaer_coag_aged": qaer_ca, "substeps": n}
+
+    def _subarea_call(self, c, optaa, do_cond, do_rename, do_newnuc, do_coag, nmono, arrays):
+        return self.lib.orc_amicphys_1subarea(
+            optaa, do_cond, do_rename, do_newnuc, do_coag, c["deltat"], int(c["iscldy"]), nmono,
+            c["temp"], c["pmid"], c["zmid"], c["pblh"], c["relhum"], *[_p(a) for a in arrays])
 
     # ---- aqueous sulfur chemistry (mo_setsox.hpp) ----
     def _sox_sigs(self):
