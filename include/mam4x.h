@@ -237,6 +237,22 @@ int mam4x_compute_tendencies(mam4x_handle h, double t, double dt,
                              const mam4x_inputs *in, const mam4x_outputs *out,
                              void *stream);
 
+/* mam4x_compute_tendencies without any allocation.  No counterpart in the
+ * reference: it is for callers that cannot allocate device memory (code under
+ * HIP graph capture, tight serving loops, harnesses that forbid hipMalloc).
+ * Same step and results as mam4x_compute_tendencies on the same handle; the
+ * work-queue workspace lives in a fixed device array of the library
+ * (MAM4X_NOALLOC_MAX_CELLS cells, 61.3 MB per GPU, held until the process
+ * ends) instead of the handle's growing one, and no phase-timing events are
+ * recorded.  Domains above MAM4X_NOALLOC_MAX_CELLS cells (ncol x nlev) are
+ * rejected with MAM4X_ERR_ARG.  Every call on one GPU shares that array, whose
+ * contents persist between calls: calls that overlap on one GPU (other
+ * streams, other handles, other threads) race and must be ordered. */
+#define MAM4X_NOALLOC_MAX_CELLS 524288
+int mam4x_compute_tendencies_noalloc(mam4x_handle h, double t, double dt,
+                                     const mam4x_inputs *in, const mam4x_outputs *out,
+                                     void *stream);
+
 /* Directional derivative of mam4x_compute_tendencies (no counterpart in the
  * reference, which has no derivative code): the same step, evaluated in
  * forward-mode dual numbers, writes the tendencies and diameters to *out

@@ -36,6 +36,7 @@ PACK_TENDENCY = 1
 
 GAS_PCNST = 31
 NPROCESS = 5
+NOALLOC_MAX_CELLS = 524288  # MAM4X_NOALLOC_MAX_CELLS
 PROC_CONDENSATION = 0
 PROC_RENAMING = 1
 PROC_NUCLEATION = 2
@@ -289,6 +290,9 @@ SIGNATURES = {
     "mam4x_compute_tendencies": (ctypes.c_int, [_vp, ctypes.c_double, ctypes.c_double,
                                                 ctypes.POINTER(Inputs), ctypes.POINTER(Outputs),
                                                 _vp]),
+    "mam4x_compute_tendencies_noalloc": (ctypes.c_int, [_vp, ctypes.c_double, ctypes.c_double,
+                                                        ctypes.POINTER(Inputs),
+                                                        ctypes.POINTER(Outputs), _vp]),
     "mam4x_compute_tangent": (ctypes.c_int, [_vp, ctypes.c_double, ctypes.c_double,
                                              ctypes.POINTER(Inputs), ctypes.POINTER(Tangent),
                                              ctypes.POINTER(Outputs), _vp]),

@@ -126,9 +126,11 @@ class Chain:
 
     def compute_tendencies(self, scene, dt, tend, dgn_i=None, dgn_c=None, state_out=None,
                            precldchem=None, pregaschem=None, t=0.0, stream=None,
-                           layout=_abi.LAYOUT_LEV_COL, process=None):
-        """One step (mam4x_compute_tendencies).  process: optional per-process
-        tendency diagnostics, [5][31][nlev][ncol] (see _process_ptrs)."""
+                           layout=_abi.LAYOUT_LEV_COL, process=None, noalloc=False):
+        """One step (mam4x_compute_tendencies, or with noalloc the
+        allocation-free mam4x_compute_tendencies_noalloc).  process: optional
+        per-process tendency diagnostics, [5][31][nlev][ncol] (see
+        _process_ptrs)."""
         inp = self.make_inputs(scene, precldchem, pregaschem, layout)
         out = _abi.Outputs()
         if process is not None:
@@ -147,9 +149,10 @@ class Chain:
             for i, p in enumerate(_plane_ptrs(state_out, L.NSTATE)):
                 out.state_out[i] = p
         s = stream if stream is not None else torch.cuda.current_stream()
-        rc = self.lib.mam4x_compute_tendencies(self.h, t, dt, ctypes.byref(inp),
-                                               ctypes.byref(out), ctypes.c_void_p(s.cuda_stream))
-        self._check(rc, "mam4x_compute_tendencies")
+        name = "mam4x_compute_tendencies_noalloc" if noalloc else "mam4x_compute_tendencies"
+        rc = getattr(self.lib, name)(self.h, t, dt, ctypes.byref(inp), ctypes.byref(out),
+                                     ctypes.c_void_p(s.cuda_stream))
+        self._check(rc, name)
 
     @staticmethod
     def _tracers(state_q, qqcw):

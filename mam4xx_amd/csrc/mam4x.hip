@@ -28,6 +28,7 @@
 #include <cstring>
 #include <array>
 #include <new>
+#include <mutex>
 #include <string>
 #include <vector>
 
@@ -960,6 +961,43 @@ static void launch_cells(int kind, bool pre, unsigned grid, hipStream_t s, const
   }
 }
 
+// Device memory of the allocation-free step (mam4x_compute_tendencies_noalloc):
+// the work-queue workspace for up to kScratchCells cells -- W_NPLANE planes of
+// doubles, the queue, the cost buckets -- and the queue counters.  Every GPU the
+// library runs on holds one copy (117 B per cell: 61.3 MB) until the process
+// ends; every call on a GPU shares it, so calls that overlap on one GPU race.
+static constexpr long kScratchCells = MAM4X_NOALLOC_MAX_CELLS;
+static constexpr size_t kScratchWsBytes = sizeof(double) * W_NPLANE * kScratchCells;
+static constexpr size_t kScratchQueueOff = kScratchWsBytes;
+static constexpr size_t kScratchBucketOff = kScratchQueueOff + sizeof(int) * kScratchCells;
+static constexpr size_t kScratchCtlOff =
+    (kScratchBucketOff + kScratchCells + 255) / 256 * 256;
+static constexpr size_t kScratchBytes = kScratchCtlOff + sizeof(QueueCtl);
+static __device__ __align__(256) unsigned char scratch[kScratchBytes];
+static_assert(kScratchBytes < 62 * 1000 * 1000, "scratch stays at 61.3 MB");
+
+// The address of `scratch` on the current device.  In host code the array's
+// name is only a stand-in; each device holds its own copy at its own address,
+// resolved once per device and kept.
+static int scratch_address(int device, unsigned char **out) {
+  constexpr int kMaxDevices = 64;
+  static std::mutex mu;
+  static unsigned char *addr[kMaxDevices] = {};
+  *out = nullptr;
+  if (device < 0 || device >= kMaxDevices) return MAM4X_ERR_ARG;
+  std::lock_guard<std::mutex> lock(mu);
+  if (!addr[device]) {
+    void *p = nullptr;
+    if (hipGetSymbolAddress(&p, HIP_SYMBOL(scratch)) != hipSuccess || !p) {
+      (void)hipGetLastError();
+      return MAM4X_ERR_HIP;
+    }
+    addr[device] = static_cast<unsigned char *>(p);
+  }
+  *out = addr[device];
+  return MAM4X_OK;
+}
+
 struct mam4x_handle_s {
   mam4x_config cfg;
   int device = 0;
@@ -974,6 +1012,8 @@ struct mam4x_handle_s {
   int *queue = nullptr;
   QueueCtl *ctl = nullptr;
   unsigned soa_grid = 0;  // resident blocks of soa_queue_kernel on the device
+  // the device's allocation-free workspace (`scratch`), NULL if unresolved
+  unsigned char *scratch = nullptr;
   // adjoint: a plane of ones (the unit seed direction), grown on demand
   double *ones = nullptr;
   long ones_cap = 0;
@@ -1146,6 +1186,8 @@ int mam4x_create(const mam4x_config *config, mam4x_handle *out) {
   if (h->cfg.soa_blocks_per_cu > 0 && h->cfg.soa_blocks_per_cu < per_cu)
     per_cu = h->cfg.soa_blocks_per_cu;
   h->soa_grid = (unsigned)(ncu * per_cu);
+  // only the allocation-free step uses it; that entry point reports a failure
+  if (scratch_address(h->device, &h->scratch) != MAM4X_OK) h->scratch = nullptr;
   ChainConsts k;
   chain_consts_init(&k);
   if (hipMalloc(&h->d_consts, sizeof(ChainConsts)) != hipSuccess ||
@@ -1503,12 +1545,21 @@ int mam4x_compute_wet_size(mam4x_handle h, double dt, const mam4x_inputs *in,
   return MAM4X_OK;
 }
 
-int mam4x_compute_tendencies(mam4x_handle h, double t, double dt, const mam4x_inputs *in,
-                             const mam4x_outputs *out, void *stream) {
-  (void)t;
-  if (!h) return MAM4X_ERR_ARG;
+// One step on one domain.  With noalloc the work-queue workspace and its
+// counters live in the device's `scratch` array and no phase events are
+// recorded: the call allocates nothing and only launches kernels.
+static int compute_step(mam4x_handle h, double dt, const mam4x_inputs *in,
+                        const mam4x_outputs *out, void *stream, bool noalloc) {
   int rc = check_inputs(h, in);
   if (rc) return rc;
+  if (noalloc) {
+    if (!h->scratch)
+      return fail(h, MAM4X_ERR_HIP, "the device's scratch workspace could not be resolved");
+    if ((long)in->ncol * in->nlev > kScratchCells)
+      return fail(h, MAM4X_ERR_ARG, "domain of " + std::to_string((long)in->ncol * in->nlev) +
+                  " cells exceeds the allocation-free limit of " +
+                  std::to_string(kScratchCells) + " cells");
+  }
   if (!(dt > 0.0)) return fail(h, MAM4X_ERR_ARG, "dt must be > 0");
   const bool mixed = h->cfg.precision == MAM4X_PREC_MIXED;
   ChainArgs a;
@@ -1529,12 +1580,17 @@ int mam4x_compute_tendencies(mam4x_handle h, double t, double dt, const mam4x_in
   Work w;
   std::memset(&w, 0, sizeof(w));
   w.ncell = ncell;
-  w.ctl = h->ctl;
+  w.ctl = noalloc ? reinterpret_cast<QueueCtl *>(h->scratch + kScratchCtlOff) : h->ctl;
   w.dtfull = (double)(int)(dt / 1);  // QUIRK (mam4_amicphys.hpp:1319): int sub-step
   if (h->stats) w.stats = h->d_stats;
   const OneDomain src{a};
   const bool queue = !fused && a.cfg.do_cond;
-  if (queue) {
+  if (queue && noalloc) {
+    // W_NPLANE planes of ncell doubles fit the kScratchCells-sized region
+    w.ws = reinterpret_cast<double *>(h->scratch);
+    w.queue = reinterpret_cast<int *>(h->scratch + kScratchQueueOff);
+    w.bucket = h->scratch + kScratchBucketOff;
+  } else if (queue) {
     rc = reserve_work(h, ncell, s);
     if (rc) return rc;
     w.ws = h->ws;
@@ -1542,7 +1598,7 @@ int mam4x_compute_tendencies(mam4x_handle h, double t, double dt, const mam4x_in
     w.queue = h->queue;
   }
   hipEvent_t *ev = nullptr;
-  if (h->stats) {
+  if (h->stats && !noalloc) {
     if (h->pool.empty()) {
       std::array<hipEvent_t, 4> q{};
       for (auto &e : q) HIP_TRY(h, hipEventCreate(&e));
@@ -1554,7 +1610,8 @@ int mam4x_compute_tendencies(mam4x_handle h, double t, double dt, const mam4x_in
     HIP_TRY(h, hipEventRecord(ev[0], s));
   }
   if (queue) {
-    hipLaunchKernelGGL(queue_reset_kernel, dim3(1), dim3(64), 0, s, h->ctl);
+    // the counters start at zero: cleared by a kernel (no memset node when captured)
+    hipLaunchKernelGGL(queue_reset_kernel, dim3(1), dim3(64), 0, s, w.ctl);
     if (mixed)
       launch_cells<Mixed>(kPre, pre, grid, s, src, w);
     else
@@ -1575,6 +1632,21 @@ int mam4x_compute_tendencies(mam4x_handle h, double t, double dt, const mam4x_in
   HIP_TRY(h, hipGetLastError());
   if (ev) HIP_TRY(h, hipEventRecord(ev[3], s));
   return MAM4X_OK;
+}
+
+int mam4x_compute_tendencies(mam4x_handle h, double t, double dt, const mam4x_inputs *in,
+                             const mam4x_outputs *out, void *stream) {
+  (void)t;
+  if (!h) return MAM4X_ERR_ARG;
+  return compute_step(h, dt, in, out, stream, false);
+}
+
+int mam4x_compute_tendencies_noalloc(mam4x_handle h, double t, double dt,
+                                     const mam4x_inputs *in, const mam4x_outputs *out,
+                                     void *stream) {
+  (void)t;
+  if (!h) return MAM4X_ERR_ARG;
+  return compute_step(h, dt, in, out, stream, true);
 }
 
 static int tracer_args(mam4x_handle h, const mam4x_tracers *tr, const mam4x_fields *fl,
