@@ -839,6 +839,118 @@ def bench_washout(args):
     print(json.dumps(res), flush=True)
 
 
+def bench_valu(args):
+    """Scalar VALU rates of the forms the precision policies compile to
+    (tests/cpp/valu_rates.hip, built with the library's flags: packed FP32
+    disabled): FP64 / FP32 FMA and the device-libm exp and log."""
+    import ctypes
+    import numpy as np
+    import torch
+    lib_path = os.path.join(ROOT, "tests", "_build", "libvalurates.so")
+    if not os.path.exists(lib_path):
+        raise SystemExit("%s not built: run __graft_entry__.build()" % lib_path)
+    torch.cuda.init()
+    lib = ctypes.CDLL(lib_path)
+    lib.valu_rates.restype = ctypes.c_int
+    ops, chk = np.zeros(6), np.zeros(6)
+    dp = ctypes.POINTER(ctypes.c_double)
+    rc = lib.valu_rates(ops.ctypes.data_as(dp), chk.ctypes.data_as(dp))
+    assert rc == 0 and np.all(np.isfinite(chk)), (rc, chk)
+    names = ("fma_f64", "fma_f32", "exp_f64", "exp_f32", "log_f64", "log_f32")
+    rates = {n: float(v) for n, v in zip(names, ops)}
+    out = {
+        "metric": "scalar VALU operations/s (FP64 vs FP32 forms of the chain's precision policies)",
+        "value": rates["fma_f64"] * 2 / 1e12,
+        "unit": "TFLOP/s (FP64 FMA)",
+        "n_gpus": 1,
+        "higher_is_better": True,
+        "vs_baseline": None,
+        "dtype": "f64+f32",
+        "data": "synthetic",
+        "config": {"workload": "8192 blocks x 256 lanes x 8 independent chains; packed FP32 "
+                               "forms disabled (the library's flags)"},
+        "rates_gops": {n: v / 1e9 for n, v in rates.items()},
+        "fma_tflops": {"f64": rates["fma_f64"] * 2 / 1e12, "f32": rates["fma_f32"] * 2 / 1e12},
+        "peak_tflops": {"f64_vector": FP64_PEAK_TFLOPS, "f32_vector_packed": 157.3},
+        "f32_over_f64": {"fma": rates["fma_f32"] / rates["fma_f64"],
+                         "exp": rates["exp_f32"] / rates["exp_f64"],
+                         "log": rates["log_f32"] / rates["log_f64"]},
+    }
+    print(json.dumps(out), flush=True)
+
+
+def bench_adjoint(args):
+    """Derivatives of the step (--workload adjoint) over the 21,600 x 72
+    scene: the FP64 step (mam4x_compute_tendencies), the tangent along one
+    direction (mam4x_compute_tangent) and the reverse sweep with every
+    gradient plane (mam4x_compute_adjoint), each timed over K calls from the
+    same seeded state.  The reverse sweep synchronises its stream once per
+    call (to size its sub-step records), so the host clock is the figure."""
+    import torch
+    from mam4xx_amd import layout as L
+    from mam4xx_amd import scenes
+    from mam4xx_amd.chain import Chain, scene_to_device
+    scene = scenes.generate(NCOL, NLEV, seed=0)
+    d = scene_to_device(scene)
+    ch = Chain()
+    shape = (L.NSTATE, NLEV, NCOL)
+    g = torch.Generator(device="cuda").manual_seed(1)
+    tend = torch.empty(shape, dtype=torch.float64, device="cuda")
+    dtend = torch.empty_like(tend)
+    v = d["state"] * torch.randn(shape, dtype=torch.float64, device="cuda", generator=g)
+    vT = torch.randn((NLEV, NCOL), dtype=torch.float64, device="cuda", generator=g)
+    w = torch.randn(shape, dtype=torch.float64, device="cuda", generator=g)
+    w = w / d["state"].abs().clamp_min(1e-30)
+    gs = torch.empty_like(tend)
+    gT, gp, gq = (torch.empty((NLEV, NCOL), dtype=torch.float64, device="cuda")
+                  for _ in range(3))
+    runs = {
+        "forward": lambda: ch.compute_tendencies(d, DT, tend),
+        "tangent": lambda: ch.compute_tangent(d, DT, tend, dtend, dstate=v, dtemperature=vT),
+        "adjoint": lambda: ch.compute_adjoint(d, DT, w, state_bar=gs, temperature_bar=gT,
+                                              pressure_bar=gp, qv_bar=gq),
+    }
+    stream = torch.cuda.current_stream()
+    ms = {}
+    for name, run in runs.items():
+        for _ in range(args.warmup):
+            run()
+        torch.cuda.synchronize()
+        e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+        t0 = time.perf_counter()
+        e0.record(stream)
+        for _ in range(args.steps):
+            run()
+        e1.record(stream)
+        torch.cuda.synchronize()
+        ms[name] = {"host_ms": (time.perf_counter() - t0) * 1e3 / args.steps,
+                    "event_ms": e0.elapsed_time(e1) / args.steps}
+    finite = all(bool(torch.isfinite(x).all().item()) for x in (gs, gT, gp, gq))
+    a = ms["adjoint"]["host_ms"]
+    out = {
+        "metric": "reverse-mode (vector-Jacobian product) column-steps/sec at 72 lev, FP64",
+        "value": NCOL / (a * 1e-3),
+        "unit": "column-steps/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": a,
+        "higher_is_better": True,
+        "vs_baseline": None,
+        "dtype": "f64",
+        "data": "synthetic (seeded generator of the reference study's value distributions)",
+        "config": {"workload": "21600 columns x 72 levels, one 1800 s step from the seeded "
+                               "state; cotangent of all 52 tendencies, gradients w.r.t. the "
+                               "state, T, p and qv", "columns": NCOL, "levels": NLEV},
+        "times_ms": ms,
+        "cost_vs_forward": {k: ms[k]["host_ms"] / ms["forward"]["host_ms"]
+                            for k in ("tangent", "adjoint")},
+        "gradients_finite": finite,
+    }
+    ch.close()
+    print(json.dumps(out), flush=True)
+
+
 def _leg(args, name, prec, sched, d, state0, dgn0):
     """One precision's timed run: W warm-up steps from the seeded state, the
     state restored, an untimed step 1 (its tendencies kept), the state
@@ -1038,14 +1150,16 @@ def main():
     ap.add_argument("--no-mixed", action="store_true",
                     help="headline: time the FP64 build only")
     ap.add_argument("--workload", choices=["headline", "batch", "large", "water", "wetsize",
-                                           "tracers", "setsox", "gaschem", "linoz", "washout"],
+                                           "tracers", "setsox", "gaschem", "linoz", "washout",
+                                           "valu", "adjoint"],
                     default="headline",
                     help="headline: the metric's 21,600 x 72 step (default); batch, large, "
                          "water (aerosol water uptake), wetsize (calcsize + water uptake), "
                          "tracers (state_q/qqcw <-> state planes), setsox (aqueous sulfur "
                          "chemistry), gaschem (gas-phase chemistry) and linoz (stratospheric "
-                         "ozone, with the invariant densities) and washout (sethet) print "
-                         "their own line")
+                         "ozone, with the invariant densities), washout (sethet), valu "
+                         "(scalar FP64 / FP32 VALU rates) and adjoint (tangent and reverse "
+                         "sweep against the step) print their own line")
     args = ap.parse_args()
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) > 1:
         print("bench.py measures one GPU: the chain has no cross-GPU step and this build "
@@ -1075,6 +1189,10 @@ def main():
         return bench_washout(args)
     if args.workload == "gaschem":
         return bench_gas_chem(args)
+    if args.workload == "valu":
+        return bench_valu(args)
+    if args.workload == "adjoint":
+        return bench_adjoint(args)
     return bench_headline(args)
 
 if __name__ == "__main__":

@@ -264,14 +264,17 @@ int mam4x_compute_tangent(mam4x_handle h, double t, double dt, const mam4x_input
                           const mam4x_tangent *dir, const mam4x_outputs *out, void *stream);
 
 /* Vector-Jacobian product of mam4x_compute_tendencies (no counterpart in the
- * reference): for every gradient plane requested in *adj, one pass of the
- * tangent step seeded with the unit direction of that input, whose
- * derivative is contracted with adj->tendencies in the same lane (so no
- * Jacobian is stored).  The result is exact to round-off for the branch each
- * cell takes, like the tangent; its cost is one tangent step per requested
- * plane (up to 55), not a reverse sweep.  Uses a workspace plane of the
- * input extent held by the handle (allocated on first use); calls on one
- * handle must be stream-ordered. */
+ * reference): a reverse sweep of the FP64 step, one lane per cell.  The lane
+ * recomputes its cell's step, keeping the state each SOA sub-step starts
+ * from (6 doubles per sub-step), then back-propagates adj->tendencies
+ * through every process in reverse order; all requested gradient planes
+ * come out of that one sweep.  Derivatives are those of the branch each cell
+ * takes, as for the tangent.  A first forward work-queue pass counts every
+ * cell's SOA sub-steps; the stream is synchronised once to size the
+ * sub-step records, so the call cannot be captured in a graph.  The records
+ * and counters are held by the handle (grown on demand); calls on one handle
+ * must be stream-ordered.  A cell whose recomputed loop does not retrace the
+ * counted sub-steps gets NaN gradients rather than wrong ones. */
 int mam4x_compute_adjoint(mam4x_handle h, double t, double dt, const mam4x_inputs *in,
                           const mam4x_adjoint *adj, void *stream);
 

@@ -8,14 +8,11 @@ vapour.
 * Forward mode (``torch.autograd.forward_ad``): one call of the tangent
   kernel (mam4x_compute_tangent) per direction.  The functorch transforms
   (``torch.func``) are not supported: the kernels need plain device tensors.
-* Reverse mode (``backward``): cells are independent, so each cell's Jacobian
-  is a 52 x 55 block.  ``backward`` is one call of mam4x_compute_adjoint,
-  which forms grad_x_j(cell) = sum_f g_f(cell) * d tend_f(cell) / d x_j(cell)
-  by one tangent pass per input field that requires a gradient (up to 55),
-  each seeded with the unit direction of that field in every cell and
-  contracted with g in the same lane.  This is exact (the same derivative as
-  forward mode); its cost is up to 55 tangent passes, not the few forward
-  passes a hand-written reverse sweep would take.
+* Reverse mode (``backward``): one call of mam4x_compute_adjoint, a
+  hand-written reverse sweep of the step (one lane per cell, the SOA
+  sub-step states kept as checkpoints), which returns every requested
+  gradient plane at once.  Cells are independent, so each cell's Jacobian is
+  a 52 x 55 block and the product needs no communication between lanes.
 
 Derivatives are those of the branch each cell takes (include/mam4x.h,
 mam4x_compute_tangent).

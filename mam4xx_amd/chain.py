@@ -417,10 +417,11 @@ class Chain:
 
     def compute_tangent(self, scene, dt, tend, dtend, dstate=None, dtemperature=None,
                         dpressure=None, dqv=None, dgn_i=None, dgn_c=None, t=0.0, stream=None,
-                        layout=_abi.LAYOUT_LEV_COL):
+                        layout=_abi.LAYOUT_LEV_COL, precldchem=None):
         """Tendencies and their derivative along (dstate, dT, dp, dqv) (None =
-        zero direction), through mam4x_compute_tangent."""
-        inp = self.make_inputs(scene, None, None, layout)
+        zero direction), through mam4x_compute_tangent.  precldchem: the
+        optional pre-cloud-chemistry state (held constant)."""
+        inp = self.make_inputs(scene, precldchem, None, layout)
         out = _abi.Outputs()
         for i, p in enumerate(_plane_ptrs(tend, L.NSTATE)):
             out.tendencies[i] = p
@@ -448,13 +449,14 @@ class Chain:
 
     def compute_adjoint(self, scene, dt, tbar, state_bar=None, temperature_bar=None,
                         pressure_bar=None, qv_bar=None, t=0.0, stream=None,
-                        layout=_abi.LAYOUT_LEV_COL):
+                        layout=_abi.LAYOUT_LEV_COL, precldchem=None):
         """Per-cell vector-Jacobian product through mam4x_compute_adjoint:
         tbar ([52, ...] cotangent of the tendencies, or a list of 52 planes
         with None for zero planes) in; the gradients of
         sum(tbar * tendencies) with respect to the state, T, p and qv written
-        to the given tensors (None = not formed)."""
-        inp = self.make_inputs(scene, None, None, layout)
+        to the given tensors (None = not formed).  precldchem: the optional
+        pre-cloud-chemistry state (held constant)."""
+        inp = self.make_inputs(scene, precldchem, None, layout)
         adj = _abi.Adjoint()
         if isinstance(tbar, (list, tuple)):  # one plane or None per field
             assert len(tbar) == L.NSTATE

@@ -32,7 +32,9 @@
 #include <string>
 #include <vector>
 
+
 #include "../../include/mam4x.h"
+#include "chain_adjoint.hpp"
 #include "chain_cell.hpp"
 #include "gas_chem.hpp"
 #include "linoz.hpp"
@@ -385,6 +387,402 @@ __global__ void __launch_bounds__(kBlock) chain_post_kernel(Src src, Work w) {
 
 // AeroProcess::validate checks (gasaerexch.hpp:96-120): negative T, p, qv or
 // any negative prognostic.  One count per offending cell.
+// Reverse sweep (mam4x_compute_adjoint).
+//
+// A cell without a SOA loop (overcast, or condensation off) runs the whole
+// sweep in one lane: chain_adjoint_kernel.  A cell with one runs it in the
+// step's own phases, so that the SOA loops, whose length varies from one
+// sub-step to 1000 between cells, keep the work queue's lane refill:
+//   1. the step's set-up kernel and queue (chain_pre_kernel,
+//      queue_scatter_kernel);
+//   2. soa_log_kernel: the loop on the queue, each wave logging one row per
+//      iteration of its refill loop -- per lane the state a sub-step starts
+//      from (6 doubles) and a tag (sub-step of cell c / cell c ended /
+//      idle) -- into chunks of kLogRows rows taken from a pool, chained
+//      backwards; the loops' final states go to the R_* planes;
+//   3. adj_post_kernel: the reverse of cell_finish and clear_post, one lane
+//      per cell; its adjoints go to the PostBar planes;
+//   4. soa_replay_kernel: wave w replays wave w's log backwards, row by row,
+//      so every lane meets its cells' sub-steps in reverse order and the
+//      rows are read as they were written, whole;  the adjoints of each
+//      loop's inputs go to the I_* planes;
+//   5. adj_pre_kernel: the reverse of clear_pre, clear_build and cell_setup,
+//      writing the gradients.
+struct AdjArgs {
+  double *sbar[NSTATE];
+  double *Tbar, *pbar, *qvbar;
+  const unsigned char *bucket;  // chain_adjoint_kernel: kNoSoa cells only; NULL: all
+};
+
+// the SOA loop's final state
+enum { R_G = 0, R_A = 1, R_GAVG = 5, R_AVG = 6, R_DTSUM = 7, R_NPLANE = 8 };
+// the adjoints of the SOA loop's inputs, from its reverse
+enum { I_G = 0, I_G0 = 1, I_A = 2, I_UPTK = 6, I_AOPOA = 10, I_NPLANE = 14 };
+// PostBar: S_b (4 numbers, 7 x 4 masses), G_b (8), Xb (25), st_b, uptk_b (4),
+// soa_g_b, soa_gavg_b, soa_a_b (4)
+constexpr int kPbPlanes = 32 + 8 + 25 + NSTATE + 4 + 6;
+constexpr int kPbSoa = kPbPlanes - 6;  // plane of soa_g_b; soa_gavg_b, soa_a_b follow
+constexpr int kAdjPlanes = R_NPLANE + kPbPlanes + I_NPLANE;
+
+// The log: chunks of kLogRows rows; a row holds, for the 64 lanes of a wave,
+// kLogVals planes of doubles (lane-contiguous) and a tag each.
+constexpr int kLogRows = 32, kLogVals = 6;
+constexpr int kTagIdle = (int)0x80000000;  // tag: lane idle; c >= 0: a sub-step
+                                           // of cell c; ~c: cell c ended
+struct AdjWork {
+  const double *ws;             // the SOA loop's inputs (ws_load_in)
+  double *res;                  // R_NPLANE planes
+  double *pb;                   // kPbPlanes planes
+  double *inb;                  // I_NPLANE planes
+  double *rec;                  // log values [cap][kLogRows][kLogVals][64]
+  int *tag;                     // log tags [cap][kLogRows][64]
+  int *prev;                    // [cap]: the wave's chunk before this one (-1: none)
+  int2 *wend;                   // [waves]: (last chunk or -1, rows used in it)
+  unsigned *next;               // chunks taken from the pool
+  unsigned cap;                 // chunks in the pool
+  unsigned nwave;               // waves of the log and replay kernels
+  const unsigned char *bucket;  // kNoSoa: no SOA loop
+  const int *queue;             // cells with a SOA loop, longest first
+  QueueCtl *ctl;
+  long ncell;
+  double dtfull;
+};
+static_assert(sizeof(ChainArgs) + sizeof(AdjWork) + sizeof(AdjArgs) <= 4000,
+              "the adjoint kernels' arguments fit the kernel-argument segment");
+
+// PostBar <-> its planes (STORE: to them)
+template <bool STORE>
+__device__ __forceinline__ void pb_io(double *pl, long n, long c, dev::rev::PostBar &pb) {
+  int k = 0;
+  auto io = [&](double &v) {
+    double *q = pl + (long)k * n + c;
+    if (STORE)
+      *q = v;
+    else
+      v = *q;
+    ++k;
+  };
+#pragma unroll
+  for (int m = 0; m < 4; ++m) io(pb.S_b.n[m]);
+#pragma unroll
+  for (int sp = 0; sp < 7; ++sp)
+#pragma unroll
+    for (int m = 0; m < 4; ++m) io(pb.S_b.q[sp][m]);
+  io(pb.G_b.g1[0]);
+  io(pb.G_b.g1[1]);
+  io(pb.G_b.g[0]);
+  io(pb.G_b.g[1]);
+  io(pb.G_b.hcl);
+  io(pb.G_b.scl);
+  io(pb.G_b.hcd);
+  io(pb.G_b.scd);
+  io(pb.Xb.T);
+  io(pb.Xb.p);
+  io(pb.Xb.rh_clear);
+  io(pb.Xb.h1);
+  io(pb.Xb.s1);
+  io(pb.Xb.h2);
+  io(pb.Xb.s2);
+  io(pb.Xb.h3);
+  io(pb.Xb.s3);
+#pragma unroll
+  for (int m = 0; m < 4; ++m) {
+    io(pb.Xb.fa_n[m]);
+    io(pb.Xb.fc_n[m]);
+    io(pb.Xb.fa_q[m]);
+    io(pb.Xb.fc_q[m]);
+  }
+#pragma unroll
+  for (int f = 0; f < NSTATE; ++f) io(pb.st_b[f]);
+#pragma unroll
+  for (int m = 0; m < 4; ++m) io(pb.uptk_b[m]);
+  io(pb.soa_g_b);  // plane kPbSoa
+  io(pb.soa_gavg_b);
+#pragma unroll
+  for (int m = 0; m < 4; ++m) io(pb.soa_a_b[m]);
+}
+
+// Phase 2: soa_queue_kernel's loop (same refill, same sub-steps) with the
+// log written at the end of every iteration.  A pool that runs out stops the
+// wave's logging, but the loop still drains the queue and still counts the
+// chunks it would have taken: the caller sees *next > cap, the pool this
+// run needed, and runs the phase again with a pool that large.
+__global__ void __launch_bounds__(kBlock) soa_log_kernel(AdjWork g) {
+  unsigned nq = 0, nlong = 0, nsplit = 0;
+  for (int j = 0; j < kNBucket; ++j) {
+    nq += g.ctl->hist[j];
+    if (j >= kLongBucket) nlong += g.ctl->hist[j];
+    if (j >= kShortBucket) nsplit += g.ctl->hist[j];
+  }
+  const unsigned nwave = gridDim.x * (kBlock / 64);
+  const unsigned wid = blockIdx.x * (kBlock / 64) + (threadIdx.x >> 6);
+  bool first = MAM4X_STATIC_FIRST != 0;
+  const int lane = threadIdx.x & 63;
+  const long n = g.ncell;
+  unsigned cbeg = 0, cend = 0;  // the wave's current chunk of the queue (wave-uniform)
+  bool lexhausted = false, exhausted = false;
+  bool active = false;
+  int cell = 0;
+  dev::SoaState<double> s;
+  int chunk = -1, row = kLogRows;  // the wave's log position (wave-uniform)
+  bool logging = true;
+  while (true) {
+    const unsigned long long need = __ballot(!active);
+    if (need && cbeg == cend && !exhausted) {
+      unsigned b0 = nsplit, e0 = nsplit;
+      if (first) {
+        first = false;
+        b0 = wid * (unsigned)kChunk;
+      } else if (!lexhausted) {
+        if (lane == 0) b0 = atomicAdd(&g.ctl->head, (unsigned)kChunk);
+        b0 = (MAM4X_STATIC_FIRST ? nwave * (unsigned)kChunk : 0u) +
+             __builtin_amdgcn_readfirstlane(b0);
+      }
+      if (b0 < nsplit) {
+        e0 = min(b0 + (unsigned)kChunk, nsplit);
+      } else {
+        lexhausted = true;
+        if (lane == 0) b0 = atomicAdd(&g.ctl->shead, (unsigned)kShortChunk);
+        b0 = nsplit + __builtin_amdgcn_readfirstlane(b0);
+        e0 = min(b0 + (unsigned)kShortChunk, nq);
+      }
+      if (MAM4X_SOA_PRIO && b0 < nlong) __builtin_amdgcn_s_setprio(2);
+      if (b0 >= nq) {
+        exhausted = true;
+      } else {
+        cbeg = b0;
+        cend = e0;
+      }
+    }
+    if (need && cbeg < cend) {
+      const unsigned rank = (unsigned)__popcll(need & ((1ull << lane) - 1ull));
+      const unsigned avail = cend - cbeg;
+      if (!active && rank < avail) {
+        cell = g.queue[cbeg + rank];
+        dev::SoaIn<double> in;
+        ws_load_in(g.ws, n, cell, in);
+        dev::soa_init(s, in);
+        active = true;
+      }
+      cbeg += min((unsigned)__popcll(need), avail);
+    }
+    if (!__any(active)) break;
+    double r[kLogVals];
+    r[0] = s.g;
+#pragma unroll
+    for (int m = 0; m < 4; ++m) r[1 + m] = s.a[m];
+    r[5] = s.tcur;
+    int tag = kTagIdle;
+    if (active) {
+      if (dev::soa_tick(s, g.dtfull)) {
+        tag = cell;
+      } else {
+        tag = ~cell;
+        g.res[R_G * n + cell] = s.g;
+#pragma unroll
+        for (int m = 0; m < 4; ++m) g.res[(R_A + m) * n + cell] = s.a[m];
+        g.res[R_GAVG * n + cell] = dev::soa_gavg(s);
+        g.res[R_AVG * n + cell] = s.avg;
+        g.res[R_DTSUM * n + cell] = s.dtsum;
+        active = false;
+      }
+    }
+    if (row == kLogRows) {  // a new chunk, linked back to the last one
+      unsigned id = 0;
+      if (lane == 0) id = atomicAdd(g.next, 1u);
+      id = __builtin_amdgcn_readfirstlane(id);
+      row = 0;
+      if (id < g.cap && logging) {
+        if (lane == 0) g.prev[id] = chunk;
+        chunk = (int)id;
+      } else {
+        logging = false;  // the pool ran out: from here on only count chunks
+      }
+    }
+    if (logging) {
+      const long base = (long)chunk * kLogRows + row;
+      g.tag[base * 64 + lane] = tag;
+      if (tag >= 0) {
+#pragma unroll
+        for (int j = 0; j < kLogVals; ++j) g.rec[(base * kLogVals + j) * 64 + lane] = r[j];
+      }
+    }
+    ++row;
+  }
+  if (lane == 0 && wid < g.nwave) g.wend[wid] = make_int2(chunk, chunk >= 0 ? row : 0);
+}
+
+// Phase 4.  One lane's reverse of the loops it ran in phase 2: seeds from the
+// PostBar planes and the final state, then sub-steps in reverse, then the
+// adjoints of the loop's inputs stored to the I_* planes.
+struct SoaRev {
+  const AdjWork &g;
+  int cell;
+  dev::SoaState<double> k;
+  dev::rev::SoaBar b;
+  __device__ explicit SoaRev(const AdjWork &g_) : g(g_), cell(-1) {}
+  __device__ __forceinline__ void begin(int c) {
+    cell = c;
+    const long n = g.ncell;
+    dev::SoaIn<double> in;
+    ws_load_in(g.ws, n, c, in);
+    dev::soa_init(k, in);
+    const double *pl = g.pb + (long)kPbSoa * n + c;
+    double a_b[4];
+#pragma unroll
+    for (int m = 0; m < 4; ++m) a_b[m] = pl[(2 + m) * n];
+    dev::rev::soa_rev_seed(g.res[R_AVG * n + c], g.res[R_DTSUM * n + c], pl[0], a_b, pl[n], b);
+  }
+  __device__ __forceinline__ void finish() {
+    const long n = g.ncell;
+    dev::SoaIn<double> in_b = {};
+    dev::rev::soa_rev_init(k, b, in_b);
+    double *pl = g.inb + cell;
+    pl[I_G * n] = in_b.g;
+    pl[I_G0 * n] = in_b.g0;
+#pragma unroll
+    for (int m = 0; m < 4; ++m) {
+      pl[(I_A + m) * n] = in_b.a[m];
+      pl[(I_UPTK + m) * n] = in_b.uptk[m];
+      pl[(I_AOPOA + m) * n] = in_b.aopoa[m];
+    }
+    cell = -1;
+  }
+};
+
+__global__ void __launch_bounds__(kBlock) soa_replay_kernel(AdjWork g) {
+  const unsigned wid = blockIdx.x * (kBlock / 64) + (threadIdx.x >> 6);
+  const int lane = threadIdx.x & 63;
+  if (wid >= g.nwave) return;
+  const int2 e = g.wend[wid];
+  int chunk = e.x, nrow = e.y;
+  SoaRev t(g);
+  while (chunk >= 0) {
+    for (int r = nrow - 1; r >= 0; --r) {
+      const long base = (long)chunk * kLogRows + r;
+      const int tag = g.tag[base * 64 + lane];
+      if (tag >= 0) {
+        dev::rev::SoaCkpt c;
+        c.g = g.rec[(base * kLogVals + 0) * 64 + lane];
+#pragma unroll
+        for (int m = 0; m < 4; ++m) c.a[m] = g.rec[(base * kLogVals + 1 + m) * 64 + lane];
+        c.tcur = g.rec[(base * kLogVals + 5) * 64 + lane];
+        dev::rev::soa_tick_rev(t.k, c, g.dtfull, t.b);
+      } else {
+        if (t.cell >= 0) t.finish();
+        if (tag != kTagIdle) t.begin(~tag);
+      }
+    }
+    chunk = g.prev[chunk];
+    nrow = kLogRows;
+  }
+  if (t.cell >= 0) t.finish();
+}
+
+// the queue's refill counters back to the start, for another pass over it
+__global__ void queue_rewind_kernel(QueueCtl *q, unsigned *next) {
+  if (threadIdx.x == 0) {
+    q->head = 0;
+    q->shead = 0;
+    *next = 0;
+  }
+}
+
+__device__ __forceinline__ void load_tbar(const ChainArgs &a, long off, double tb[NSTATE]) {
+#pragma unroll
+  for (int f = 0; f < NSTATE; ++f) tb[f] = a.tbar[f] ? a.tbar[f][off] : 0.0;
+}
+
+__device__ __forceinline__ void store_grad(const AdjArgs &o, long off, const double sb[NSTATE],
+                                           double Tb, double pb, double qb) {
+#pragma unroll
+  for (int f = 0; f < NSTATE; ++f)
+    if (o.sbar[f]) o.sbar[f][off] = sb[f];
+  if (o.Tbar) o.Tbar[off] = Tb;
+  if (o.pbar) o.pbar[off] = pb;
+  if (o.qvbar) o.qvbar[off] = qb;
+}
+
+// Phase 3 (cells with a SOA loop).
+template <bool PRE>
+__global__ void __launch_bounds__(kBlock) adj_post_kernel(ChainArgs a, AdjWork g) {
+  const long c = (long)blockIdx.x * kBlock + threadIdx.x;
+  const long n = g.ncell;
+  if (c >= n || g.bucket[c] == kNoSoa) return;
+  int icol;
+  long off;
+  cell_index(a, c, icol, off);
+  double tb[NSTATE];
+  load_tbar(a, off, tb);
+  dev::CellCtx<Fp64> X;
+  dev::cell_setup<PRE, Fp64>(a, off, icol, X);
+  dev::Aer<double> S0;
+  dev::ClearGas<double> G0;
+  dev::clear_build<Fp64>(a, off, X, S0, G0);
+  dev::SoaIn<double> in;
+  ws_load_in(g.ws, n, c, in);
+  double sa[4];
+#pragma unroll
+  for (int m = 0; m < 4; ++m) sa[m] = g.res[(R_A + m) * n + c];
+  dev::rev::PostBar pb;
+  dev::rev::adjoint_post_phase<PRE>(a, off, X, S0, G0, in, g.res[R_G * n + c],
+                                    g.res[R_GAVG * n + c], sa, tb, pb);
+  pb_io<true>(g.pb, n, c, pb);
+}
+
+// Phase 5 (cells with a SOA loop).
+template <bool PRE>
+__global__ void __launch_bounds__(kBlock) adj_pre_kernel(ChainArgs a, AdjWork g, AdjArgs o) {
+  const long c = (long)blockIdx.x * kBlock + threadIdx.x;
+  const long n = g.ncell;
+  if (c >= n || g.bucket[c] == kNoSoa) return;
+  int icol;
+  long off;
+  cell_index(a, c, icol, off);
+  dev::CellCtx<Fp64> X;
+  dev::cell_setup<PRE, Fp64>(a, off, icol, X);
+  dev::Aer<double> S0;
+  dev::ClearGas<double> G0;
+  dev::clear_build<Fp64>(a, off, X, S0, G0);
+  dev::rev::PostBar pb;
+  pb_io<false>(g.pb, n, c, pb);
+  const double *pl = g.inb + c;
+  dev::SoaIn<double> in_b;
+  in_b.g = pl[I_G * n];
+  in_b.g0 = pl[I_G0 * n];
+#pragma unroll
+  for (int m = 0; m < 4; ++m) {
+    in_b.a[m] = pl[(I_A + m) * n];
+    in_b.uptk[m] = pb.uptk_b[m] + pl[(I_UPTK + m) * n];
+    in_b.aopoa[m] = pl[(I_AOPOA + m) * n];
+  }
+  double sb[NSTATE], Tb = 0.0, pB = 0.0, qb = 0.0;
+  dev::rev::adjoint_pre_phase<PRE>(a, off, X, S0, in_b, pb, sb, Tb, pB, qb);
+  store_grad(o, off, sb, Tb, pB, qb);
+}
+
+// Cells without a SOA loop: the whole sweep in one lane.
+template <bool PRE>
+__global__ void __launch_bounds__(kBlock) chain_adjoint_kernel(ChainArgs a, AdjArgs o) {
+  const long c = (long)blockIdx.x * kBlock + threadIdx.x;
+  const long ncell = (long)a.ncol * a.nlev;
+  if (c >= ncell) return;
+  if (o.bucket && o.bucket[c] != kNoSoa) return;
+  int icol;
+  long off;
+  cell_index(a, c, icol, off);
+  double tb[NSTATE], sb[NSTATE];
+  load_tbar(a, off, tb);
+#pragma unroll
+  for (int f = 0; f < NSTATE; ++f) sb[f] = 0.0;
+  double Tb = 0.0, pb = 0.0, qb = 0.0;
+  // no SOA loop runs here, so no record is saved or read
+  auto save = [](int, const dev::rev::SoaCkpt &) {};
+  auto load = [](int) { return dev::rev::SoaCkpt{}; };
+  dev::rev::adjoint_cell<PRE>(a, off, icol, tb, save, load, sb, Tb, pb, qb);
+  store_grad(o, off, sb, Tb, pb, qb);
+}
+
 __global__ void __launch_bounds__(kBlock) validate_kernel(ChainArgs a,
                                                           unsigned long long *count) {
   const long ncell = (long)a.ncol * a.nlev;
@@ -400,11 +798,6 @@ __global__ void __launch_bounds__(kBlock) validate_kernel(ChainArgs a,
   }
   for (int d = 32; d > 0; d >>= 1) bad += __shfl_xor(bad, d, 64);
   if ((threadIdx.x & 63) == 0 && bad) atomicAdd(count, bad);
-}
-
-__global__ void __launch_bounds__(kBlock) fill_kernel(double *p, long n, double v) {
-  const long i = (long)blockIdx.x * kBlock + threadIdx.x;
-  if (i < n) p[i] = v;
 }
 
 // Aerosol water uptake (water_uptake.hpp): one lane per cell.
@@ -1012,11 +1405,19 @@ struct mam4x_handle_s {
   int *queue = nullptr;
   QueueCtl *ctl = nullptr;
   unsigned soa_grid = 0;  // resident blocks of soa_queue_kernel on the device
+  unsigned log_grid = 0;  // resident blocks of soa_log_kernel on the device
   // the device's allocation-free workspace (`scratch`), NULL if unresolved
   unsigned char *scratch = nullptr;
-  // adjoint: a plane of ones (the unit seed direction), grown on demand
-  double *ones = nullptr;
-  long ones_cap = 0;
+  // adjoint: the phases' planes (kAdjPlanes x adj_cap doubles), and the SOA
+  // log: log_cap chunks of values and tags, their back links, each wave's
+  // end and the pool counter (grown on demand)
+  double *adjws = nullptr;
+  long adj_cap = 0;
+  double *log_rec = nullptr;
+  int *log_tag = nullptr, *log_prev = nullptr;
+  int2 *log_wend = nullptr;
+  unsigned *log_next = nullptr;
+  unsigned log_cap = 0;
   // phase timing (stats enabled): 4 events per call, recycled through a pool
   std::vector<std::array<hipEvent_t, 4>> pending, pool;
   std::string err;
@@ -1186,6 +1587,13 @@ int mam4x_create(const mam4x_config *config, mam4x_handle *out) {
   if (h->cfg.soa_blocks_per_cu > 0 && h->cfg.soa_blocks_per_cu < per_cu)
     per_cu = h->cfg.soa_blocks_per_cu;
   h->soa_grid = (unsigned)(ncu * per_cu);
+  int log_cu = 0;
+  if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&log_cu, soa_log_kernel, kBlock, 0) !=
+      hipSuccess) {
+    delete h;
+    return MAM4X_ERR_HIP;
+  }
+  h->log_grid = (unsigned)(ncu * (log_cu > 0 ? log_cu : 1));
   // only the allocation-free step uses it; that entry point reports a failure
   if (scratch_address(h->device, &h->scratch) != MAM4X_OK) h->scratch = nullptr;
   ChainConsts k;
@@ -1213,7 +1621,9 @@ int mam4x_destroy(mam4x_handle h) {
   if (h->d_count) (void)hipFree(h->d_count);
   if (h->d_stats) (void)hipFree(h->d_stats);
   if (h->ctl) (void)hipFree(h->ctl);
-  if (h->ones) (void)hipFree(h->ones);
+  for (void *q : {(void *)h->adjws, (void *)h->log_rec, (void *)h->log_tag,
+                  (void *)h->log_prev, (void *)h->log_wend, (void *)h->log_next})
+    if (q) (void)hipFree(q);
   free_work(h);
   destroy_events(h);
   delete h;
@@ -1385,6 +1795,53 @@ int mam4x_compute_tangent(mam4x_handle h, double t, double dt, const mam4x_input
   return MAM4X_OK;
 }
 
+// Planes of the reverse sweep for ncell cells, and its log's per-wave ends
+// and pool counter.  Growing frees the old planes only after the stream has
+// drained.
+static int reserve_adjoint(mam4x_handle h, long ncell, hipStream_t s) {
+  if (!h->log_wend) {
+    if (hipMalloc(&h->log_wend, sizeof(int2) * h->log_grid * (kBlock / 64)) != hipSuccess ||
+        hipMalloc(&h->log_next, sizeof(unsigned)) != hipSuccess) {
+      (void)hipGetLastError();
+      return fail(h, MAM4X_ERR_NOMEM, "adjoint log allocation failed");
+    }
+  }
+  if (ncell <= h->adj_cap) return MAM4X_OK;
+  if (h->adj_cap) HIP_TRY(h, hipStreamSynchronize(s));
+  if (h->adjws) (void)hipFree(h->adjws);
+  h->adjws = nullptr;
+  h->adj_cap = 0;
+  if (hipMalloc(&h->adjws, sizeof(double) * kAdjPlanes * (size_t)ncell) != hipSuccess) {
+    (void)hipGetLastError();
+    return fail(h, MAM4X_ERR_NOMEM, "adjoint workspace allocation failed");
+  }
+  h->adj_cap = ncell;
+  return MAM4X_OK;
+}
+
+// The SOA log's pool: at least nchunk chunks (the stream is idle).
+static int reserve_log(mam4x_handle h, unsigned long long nchunk) {
+  if (nchunk <= h->log_cap) return MAM4X_OK;
+  if (nchunk > 0x7fffffffULL)
+    return fail(h, MAM4X_ERR_NOMEM, "adjoint log too large (" + std::to_string(nchunk) +
+                " chunks)");
+  for (void *q : {(void *)h->log_rec, (void *)h->log_tag, (void *)h->log_prev})
+    if (q) (void)hipFree(q);
+  h->log_rec = nullptr;
+  h->log_tag = h->log_prev = nullptr;
+  h->log_cap = 0;
+  const size_t rows = (size_t)nchunk * kLogRows;
+  if (hipMalloc(&h->log_rec, sizeof(double) * kLogVals * 64 * rows) != hipSuccess ||
+      hipMalloc(&h->log_tag, sizeof(int) * 64 * rows) != hipSuccess ||
+      hipMalloc(&h->log_prev, sizeof(int) * (size_t)nchunk) != hipSuccess) {
+    (void)hipGetLastError();
+    return fail(h, MAM4X_ERR_NOMEM, "adjoint log allocation failed (" +
+                std::to_string(nchunk) + " chunks)");
+  }
+  h->log_cap = (unsigned)nchunk;
+  return MAM4X_OK;
+}
+
 int mam4x_compute_adjoint(mam4x_handle h, double t, double dt, const mam4x_inputs *in,
                           const mam4x_adjoint *adj, void *stream) {
   (void)t;
@@ -1400,46 +1857,88 @@ int mam4x_compute_adjoint(mam4x_handle h, double t, double dt, const mam4x_input
   hipStream_t s = (hipStream_t)stream;
   const long ncell = (long)in->ncol * in->nlev;
   if (ncell == 0) return MAM4X_OK;
-  // extent of the plane offsets: slow extent x ldc
-  const long extent = (long)(in->layout == MAM4X_LAYOUT_COL_LEV ? in->ncol : in->nlev) * a.ldc;
-  if (extent > h->ones_cap) {
-    if (h->ones) {
-      HIP_TRY(h, hipStreamSynchronize(s));
-      (void)hipFree(h->ones);
-      h->ones = nullptr;
-      h->ones_cap = 0;
-    }
-    if (hipMalloc(&h->ones, sizeof(double) * extent) != hipSuccess) {
-      (void)hipGetLastError();
-      h->ones = nullptr;
-      return fail(h, MAM4X_ERR_NOMEM, "adjoint seed plane allocation failed");
-    }
-    h->ones_cap = extent;
-    hipLaunchKernelGGL(fill_kernel, dim3((unsigned)((extent + kBlock - 1) / kBlock)),
-                       dim3(kBlock), 0, s, h->ones, extent, 1.0);
-    HIP_TRY(h, hipGetLastError());
-  }
+  if (ncell > 0x7fffffffL) return fail(h, MAM4X_ERR_ARG, "domain too large for the adjoint");
   const unsigned grid = (unsigned)((ncell + kBlock - 1) / kBlock);
-  Work w;
-  std::memset(&w, 0, sizeof(w));
-  w.ncell = ncell;
-  if (h->stats) w.stats = h->d_stats;
   const bool pre = has_pre(in);
-  // one seeded tangent pass per requested gradient plane: state fields, T, p, qv
-  for (int i = 0; i < NSTATE + 3; ++i) {
-    double *xbar = i < NSTATE        ? adj->state[i]
-                   : i == NSTATE     ? adj->temperature
-                   : i == NSTATE + 1 ? adj->pressure
-                                     : adj->vapor_mixing_ratio;
-    if (!xbar) continue;
-    for (int f = 0; f < NSTATE; ++f) a.dstate[f] = f == i ? h->ones : nullptr;
-    a.dT = i == NSTATE ? h->ones : nullptr;
-    a.dp = i == NSTATE + 1 ? h->ones : nullptr;
-    a.dqv = i == NSTATE + 2 ? h->ones : nullptr;
-    a.xbar = xbar;
-    launch_fused<Tangent>(pre, grid, s, OneDomain{a}, w);
+  AdjArgs o;
+  std::memset(&o, 0, sizeof(o));
+  for (int f = 0; f < NSTATE; ++f) o.sbar[f] = adj->state[f];
+  o.Tbar = adj->temperature;
+  o.pbar = adj->pressure;
+  o.qvbar = adj->vapor_mixing_ratio;
+  if (a.cfg.do_cond) {
+    rc = reserve_work(h, ncell, s);
+    if (rc) return rc;
+    rc = reserve_adjoint(h, ncell, s);
+    if (rc) return rc;
+    if (!h->log_cap) {
+      // a first pool for ~40 sub-steps per cell; grown below when short
+      rc = reserve_log(h, (unsigned long long)ncell * 40 / (64 * kLogRows) + 64);
+      if (rc) return rc;
+    }
+    // 1. set-up and queue
+    Work w;
+    std::memset(&w, 0, sizeof(w));
+    w.ncell = ncell;
+    w.ctl = h->ctl;
+    w.dtfull = (double)(int)(dt / 1);  // QUIRK (mam4_amicphys.hpp:1319): int sub-step
+    w.ws = h->ws;
+    w.bucket = h->bucket;
+    w.queue = h->queue;
+    const OneDomain src{a};
+    hipLaunchKernelGGL(queue_reset_kernel, dim3(1), dim3(64), 0, s, h->ctl);
+    launch_cells<Fp64>(kPre, pre, grid, s, src, w);
+    hipLaunchKernelGGL(queue_scatter_kernel, dim3(grid), dim3(kBlock), 0, s, w);
     HIP_TRY(h, hipGetLastError());
+    AdjWork g;
+    g.ws = h->ws;
+    g.res = h->adjws;
+    g.pb = h->adjws + (size_t)R_NPLANE * ncell;
+    g.inb = h->adjws + (size_t)(R_NPLANE + kPbPlanes) * ncell;
+    g.wend = h->log_wend;
+    g.next = h->log_next;
+    g.nwave = h->log_grid * (kBlock / 64);
+    g.bucket = h->bucket;
+    g.queue = h->queue;
+    g.ctl = h->ctl;
+    g.ncell = ncell;
+    g.dtfull = w.dtfull;
+    // 2. the loop, logged; once more with a larger pool if it ran out
+    for (int attempt = 0;; ++attempt) {
+      g.rec = h->log_rec;
+      g.tag = h->log_tag;
+      g.prev = h->log_prev;
+      g.cap = h->log_cap;
+      hipLaunchKernelGGL(queue_rewind_kernel, dim3(1), dim3(64), 0, s, h->ctl, h->log_next);
+      hipLaunchKernelGGL(soa_log_kernel, dim3(h->log_grid), dim3(kBlock), 0, s, g);
+      HIP_TRY(h, hipGetLastError());
+      unsigned used = 0;
+      HIP_TRY(h, hipMemcpyAsync(&used, h->log_next, sizeof(used), hipMemcpyDeviceToHost, s));
+      HIP_TRY(h, hipStreamSynchronize(s));
+      if (used <= h->log_cap) break;
+      if (attempt > 0) return fail(h, MAM4X_ERR_HIP, "adjoint log: pool sizing failed");
+      rc = reserve_log(h, (unsigned long long)used + used / 4 + 64);
+      if (rc) return rc;
+    }
+    // 3. reverse of cell_finish and clear_post
+    if (pre)
+      hipLaunchKernelGGL((adj_post_kernel<true>), dim3(grid), dim3(kBlock), 0, s, a, g);
+    else
+      hipLaunchKernelGGL((adj_post_kernel<false>), dim3(grid), dim3(kBlock), 0, s, a, g);
+    // 4. reverse of the loops, wave by wave
+    hipLaunchKernelGGL(soa_replay_kernel, dim3(h->log_grid), dim3(kBlock), 0, s, g);
+    // 5. reverse of clear_pre, clear_build and cell_setup
+    if (pre)
+      hipLaunchKernelGGL((adj_pre_kernel<true>), dim3(grid), dim3(kBlock), 0, s, a, g, o);
+    else
+      hipLaunchKernelGGL((adj_pre_kernel<false>), dim3(grid), dim3(kBlock), 0, s, a, g, o);
+    o.bucket = h->bucket;  // the rest: overcast cells
   }
+  if (pre)
+    hipLaunchKernelGGL((chain_adjoint_kernel<true>), dim3(grid), dim3(kBlock), 0, s, a, o);
+  else
+    hipLaunchKernelGGL((chain_adjoint_kernel<false>), dim3(grid), dim3(kBlock), 0, s, a, o);
+  HIP_TRY(h, hipGetLastError());
   return MAM4X_OK;
 }
 

@@ -1,13 +1,14 @@
 // CPU build of the HIP kernel's per-cell code (mam4xx_amd/csrc/chain_cell.hpp)
 // for the test suite, in the FP64 (prec 0), mixed (prec 1) or tangent
-// (prec 2) policy.  It separates logic errors from libm differences: run on
+// (prec 2) policy, and of its reverse sweep (chain_adjoint.hpp).  It separates logic errors from libm differences: run on
 // the CPU with glibc, the kernel's restructured FP64 chain must agree with
 // the oracle to round-off.  Test infrastructure only; the library never uses
 // it.
 #include <cmath>
 #include <cstring>
+#include <vector>
 
-#include "../../mam4xx_amd/csrc/chain_cell.hpp"
+#include "../../mam4xx_amd/csrc/chain_adjoint.hpp"
 
 using namespace mam4x;
 
@@ -125,4 +126,105 @@ extern "C" long cell_host_run(const int *cfgv, int prec, int ncol, int nlev, dou
                               const double *wd, double *tend, double *dgi, double *dgc) {
   return cell_host_run2(cfgv, prec, ncol, nlev, dt, state, T, p, z, qv, cld, pblh, dga, dgw, wd,
                         tend, dgi, dgc, nullptr, nullptr, nullptr, nullptr, nullptr);
+}
+
+// Tangent with an optional pre-cloud-chemistry state (precld, NULL = none).
+extern "C" long cell_host_tangent(const int *cfgv, int ncol, int nlev, double dt,
+                                  const double *state, const double *T, const double *p,
+                                  const double *z, const double *qv, const double *cld,
+                                  const double *pblh, const double *dga, const double *dgw,
+                                  const double *wd, const double *precld, const double *dstate,
+                                  const double *dT, const double *dp, const double *dqv,
+                                  double *tend, double *dtend) {
+  static ChainConsts K;
+  chain_consts_init(&K);
+  const long n = (long)ncol * nlev;
+  ChainArgs a;
+  std::memset(&a, 0, sizeof(a));
+  for (int f = 0; f < NSTATE; ++f) {
+    a.state[f] = state + f * n;
+    a.tend[f] = tend + f * n;
+    if (dstate) a.dstate[f] = dstate + f * n;
+    a.dtend[f] = dtend + f * n;
+    if (precld) a.precld[f] = precld + f * n;
+  }
+  a.T = T; a.p = p; a.z = z; a.qv = qv; a.cld = cld; a.pblh = pblh;
+  a.dT = dT; a.dp = dp; a.dqv = dqv;
+  for (int m = 0; m < 4; ++m) {
+    a.dga[m] = dga + m * n;
+    a.dgw[m] = dgw + m * n;
+    a.wd[m] = wd + m * n;
+  }
+  a.K = &K;
+  a.ncol = ncol; a.nlev = nlev; a.ldc = ncol; a.layout = 0;
+  a.cfg.do_cond = cfgv[0]; a.cfg.do_rename = cfgv[1]; a.cfg.do_newnuc = cfgv[2];
+  a.cfg.do_coag = cfgv[3]; a.cfg.gaexch_h2so4_uptake_optaa = cfgv[4];
+  a.cfg.nmono = (unsigned)cfgv[5];
+  a.do_calcsize = cfgv[6]; a.do_xfer = cfgv[7]; a.do_adjust = cfgv[8];
+  a.dt = dt;
+  long iters = 0;
+  for (int k = 0; k < nlev; ++k)
+    for (int i = 0; i < ncol; ++i) {
+      const long off = (long)k * ncol + i;
+      iters += precld ? dev::chain_cell_fused<true, Tangent>(a, off, i)
+                      : dev::chain_cell_fused<false, Tangent>(a, off, i);
+    }
+  return iters;
+}
+
+// Reverse sweep: tbar ([52][nlev][ncol]) is the cotangent of the tendencies;
+// sbar ([52][nlev][ncol]), Tbar, pbar, qvbar receive the gradient of
+// <tbar, tendencies> per cell.  The SOA loop's states are kept on a vector.
+extern "C" long cell_host_adjoint(const int *cfgv, int ncol, int nlev, double dt,
+                                  const double *state, const double *T, const double *p,
+                                  const double *z, const double *qv, const double *cld,
+                                  const double *pblh, const double *dga, const double *dgw,
+                                  const double *wd, const double *precld, const double *tbar,
+                                  double *sbar, double *Tbar, double *pbar, double *qvbar) {
+  static ChainConsts K;
+  chain_consts_init(&K);
+  const long n = (long)ncol * nlev;
+  ChainArgs a;
+  std::memset(&a, 0, sizeof(a));
+  for (int f = 0; f < NSTATE; ++f) {
+    a.state[f] = state + f * n;
+    if (precld) a.precld[f] = precld + f * n;
+  }
+  a.T = T; a.p = p; a.z = z; a.qv = qv; a.cld = cld; a.pblh = pblh;
+  for (int m = 0; m < 4; ++m) {
+    a.dga[m] = dga + m * n;
+    a.dgw[m] = dgw + m * n;
+    a.wd[m] = wd + m * n;
+  }
+  a.K = &K;
+  a.ncol = ncol; a.nlev = nlev; a.ldc = ncol; a.layout = 0;
+  a.cfg.do_cond = cfgv[0]; a.cfg.do_rename = cfgv[1]; a.cfg.do_newnuc = cfgv[2];
+  a.cfg.do_coag = cfgv[3]; a.cfg.gaexch_h2so4_uptake_optaa = cfgv[4];
+  a.cfg.nmono = (unsigned)cfgv[5];
+  a.do_calcsize = cfgv[6]; a.do_xfer = cfgv[7]; a.do_adjust = cfgv[8];
+  a.dt = dt;
+  std::vector<dev::rev::SoaCkpt> ck;
+  auto save = [&](int i, const dev::rev::SoaCkpt &c) {
+    if ((int)ck.size() <= i) ck.resize(i + 1);
+    ck[i] = c;
+  };
+  auto load = [&](int i) { return ck[i]; };
+  long iters = 0;
+  for (int k = 0; k < nlev; ++k)
+    for (int i = 0; i < ncol; ++i) {
+      const long off = (long)k * ncol + i;
+      double tb[NSTATE], sb[NSTATE];
+      for (int f = 0; f < NSTATE; ++f) {
+        tb[f] = tbar[f * n + off];
+        sb[f] = 0.0;
+      }
+      double tB = 0.0, pB = 0.0, qB = 0.0;
+      iters += precld ? dev::rev::adjoint_cell<true>(a, off, i, tb, save, load, sb, tB, pB, qB)
+                      : dev::rev::adjoint_cell<false>(a, off, i, tb, save, load, sb, tB, pB, qB);
+      for (int f = 0; f < NSTATE; ++f) sbar[f * n + off] = sb[f];
+      Tbar[off] = tB;
+      pbar[off] = pB;
+      qvbar[off] = qB;
+    }
+  return iters;
 }
