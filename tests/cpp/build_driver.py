@@ -1,7 +1,8 @@
 """Build recipe of the GPU test drivers: the C++ host-interface driver
 (tests/cpp/adapter_driver.cpp -> tests/_build/libadapter_driver.so, linked
-against mam4xx_amd/lib/libmam4x.so, build that first) and the division
-check kernel (tests/cpp/div_check.hip -> tests/_build/libdivcheck.so).
+against mam4xx_amd/lib/libmam4x.so, build that first), the division
+check kernel (tests/cpp/div_check.hip -> tests/_build/libdivcheck.so) and the
+VALU-rate probe (tests/cpp/valu_rates.hip -> tests/_build/libvalurates.so).
 Called by __graft_entry__.build(); the GPU tests load the results and fail
 when they are missing.
 """
@@ -33,6 +34,14 @@ def build(verbose=False):
         print(" ".join(cmd), flush=True)
     subprocess.run(cmd, check=True)
     os.replace(div + ".tmp", div)
+    rates = os.path.join(os.path.dirname(OUT), "libvalurates.so")
+    cmd = [HIPCC, "-O3", "-std=c++17", "-fPIC", "-shared", "--offload-arch=gfx950",
+           "-Xclang", "-target-feature", "-Xclang", "-packed-fp32-ops", "-ffp-contract=off",
+           "-o", rates + ".tmp", os.path.join(HERE, "valu_rates.hip")]
+    if verbose:
+        print(" ".join(cmd), flush=True)
+    subprocess.run(cmd, check=True)
+    os.replace(rates + ".tmp", rates)
     return OUT
 
 
