@@ -174,7 +174,10 @@ struct SoaBar {
 
 // Reverse of one soa_tick from the state c it started at, given the
 // adjoints b of the state after it; on return b holds the adjoints of the
-// state before it (constants' adjoints accumulated).
+// state before it (constants' adjoints accumulated).  The forward part
+// reproduces soa_tick's values bit for bit (num::div_r with the shared
+// reciprocal is num::div); every divisor is inverted once and its
+// reciprocal serves the forward quotient and all reverse uses.
 __host__ __device__ __forceinline__ void soa_tick_rev(const SoaState<double> &k,
                                                       const SoaCkpt &c, double dtfull,
                                                       SoaBar &b) {
@@ -182,64 +185,65 @@ __host__ __device__ __forceinline__ void soa_tick_rev(const SoaState<double> &k,
   // ---- forward, as soa_tick ----
   const double gprv = c.g;
   const double g_soa0 = kmax(gprv, 0.0);
-  double a_soa[4], sat0[4], phi[4], gst[4], mden[4], term[4];
-#pragma unroll
-  for (int n = 0; n < 4; ++n) {
-    a_soa[n] = k.skip[n] ? 0.0 : kmax(c.a[n], 0.0);
-    sat0[n] = num::div(k.g0, kmax(k.aopoa[n] + a_soa[n], a_min1));
-    gst[n] = sat0[n] * a_soa[n];
-    mden[n] = kmax(g_soa0, kmax(gst[n], g_min1));
-    phi[n] = num::div(g_soa0 - gst[n], mden[n]);
-    term[n] = soa_term(k.skip[n], k.uk[n] * fabs(phi[n]));
-  }
+  double a_soa[4], sat0[4], ryy[4], phi[4], gst[4], mden[4], rm[4];
   double tot = g_soa0, tmpb = 0.0;
 #pragma unroll
   for (int n = 0; n < 4; ++n) {
+    a_soa[n] = k.skip[n] ? 0.0 : kmax(c.a[n], 0.0);
+    const double yy = kmax(k.aopoa[n] + a_soa[n], a_min1);
+    ryy[n] = num::rcp(yy);
+    sat0[n] = num::div_r(k.g0, yy, ryy[n]);
+    gst[n] = sat0[n] * a_soa[n];
+    mden[n] = kmax(g_soa0, kmax(gst[n], g_min1));
+    rm[n] = num::rcp(mden[n]);
+    phi[n] = num::div_r(g_soa0 - gst[n], mden[n], rm[n]);
+  }
+#pragma unroll
+  for (int n = 0; n < 4; ++n) {
     tot = tot + soa_term(k.skip[n], a_soa[n]);
-    tmpb = tmpb + term[n];
+    tmpb = tmpb + soa_term(k.skip[n], k.uk[n] * fabs(phi[n]));
   }
   const double tmpa = kmax(0.0, tmpb);
   const double dtmax = dtfull - c.tcur;
   const bool last = dtmax * tmpa <= alpha_astem;
-  const double dtcur = last ? dtmax : num::div(alpha_astem, tmpa);
-  double beta[4], sat[4], del[4], at[4];
+  double rtmpa = 0.0, dtcur = dtmax;
+  if (!last) {  // then tmpa > alpha / dtmax > 0
+    rtmpa = num::rcp(tmpa);
+    dtcur = num::div_r(alpha_astem, tmpa, rtmpa);
+  }
+  double beta[4], sat[4], del[4], at[4], rxx[4], rden[4], qa[4];
+  double sa = 0.0, sb = 0.0;
 #pragma unroll
   for (int n = 0; n < 4; ++n) {
     beta[n] = dtcur * k.uk[n];
-    const double gstar = sat0[n] * a_soa[n];
-    del[n] = g_soa0 - gstar;
+    del[n] = g_soa0 - gst[n];
     at[n] = a_soa[n] + beta[n] * del[n];
-    const double sat1 = num::div(k.g0, kmax(k.aopoa[n] + at[n], a_min1));
+    const double xx = kmax(k.aopoa[n] + at[n], a_min1);
+    rxx[n] = num::rcp(xx);
+    const double sat1 = num::div_r(k.g0, xx, rxx[n]);
     sat[n] = del[n] > 0.0 ? sat1 : sat0[n];
-  }
-  double sa = 0.0, sb = 0.0, den[4], qa[4], qb[4];
-#pragma unroll
-  for (int n = 0; n < 4; ++n) {
-    den[n] = 1.0 + beta[n] * sat[n];
-    qa[n] = num::div(a_soa[n], den[n]);
-    qb[n] = num::div(beta[n], den[n]);
+    const double den = 1.0 + beta[n] * sat[n];
+    rden[n] = num::rcp(den);
+    qa[n] = num::div_r(a_soa[n], den, rden[n]);
     sa = sa + soa_term(k.skip[n], qa[n]);
-    sb = sb + soa_term(k.skip[n], qb[n]);
+    sb = sb + soa_term(k.skip[n], num::div_r(beta[n], den, rden[n]));
   }
-  const double gq = num::div(tot - sa, 1.0 + sb);
+  const double r1sb = num::rcp(1.0 + sb);
+  const double gq = num::div_r(tot - sa, 1.0 + sb, r1sb);
   const double g_new = kmax(0.0, gq);
 
   // ---- reverse ----
-  // (the forward above reproduces soa_tick's values bit for bit; below, one
-  // reciprocal per divisor serves all of its uses)
   double dtcur_b = 0.0, gprv_b = 0.0, gnew_b = b.g;
   // avg += dtcur (gprv + 0.5 (g_new - gprv)); dtsum += dtcur
   gprv_b += b.avg * 0.5 * dtcur;
   gnew_b += b.avg * 0.5 * dtcur;
   dtcur_b += b.avg * (gprv + 0.5 * (g_new - gprv)) + b.dtsum;
-  double a_soa_b[4] = {0, 0, 0, 0}, beta_b[4] = {0, 0, 0, 0}, den_b[4] = {0, 0, 0, 0};
-  double rden[4];
+  double a_soa_b[4], beta_b[4], den_b[4];
   // a = (a_soa + beta g_new) / den
 #pragma unroll
   for (int n = 0; n < 4; ++n) {
-    rden[n] = 1.0;
+    a_soa_b[n] = beta_b[n] = den_b[n] = 0.0;
     if (k.skip[n]) continue;
-    rden[n] = 1.0 / den[n];
     const double anew = (a_soa[n] + beta[n] * g_new) * rden[n];
     const double ab = b.a[n] * rden[n];
     a_soa_b[n] += ab;
@@ -249,10 +253,10 @@ __host__ __device__ __forceinline__ void soa_tick_rev(const SoaState<double> &k,
   }
   // g_new = kmax(0, (tot - sa) / (1 + sb))
   const double gq_b = cmax_pass(0.0, gq, gnew_b);
-  const double tot_b = gq_b / (1.0 + sb);
+  const double tot_b = gq_b * r1sb;
   const double sa_b = -tot_b;
   const double sb_b = -tot_b * gq;
-  double sat_b[4] = {0, 0, 0, 0}, sat0_b[4] = {0, 0, 0, 0};
+  double g_soa_b = 0.0, sat0_b[4] = {0, 0, 0, 0};
 #pragma unroll
   for (int n = 0; n < 4; ++n) {
     if (k.skip[n]) continue;
@@ -260,30 +264,21 @@ __host__ __device__ __forceinline__ void soa_tick_rev(const SoaState<double> &k,
     a_soa_b[n] += sa_b * rden[n];
     den_b[n] -= sa_b * qa[n] * rden[n];
     beta_b[n] += sb_b * rden[n];
-    den_b[n] -= sb_b * qb[n] * rden[n];
+    den_b[n] -= sb_b * (beta[n] * rden[n]) * rden[n];
     // den = 1 + beta sat
     beta_b[n] += den_b[n] * sat[n];
-    sat_b[n] += den_b[n] * beta[n];
-  }
-  double g_soa_b = 0.0;
-#pragma unroll
-  for (int n = 0; n < 4; ++n) {
-    if (k.skip[n]) continue;
+    const double sat_b = den_b[n] * beta[n];
     // sat = del > 0 ? g0 / kmax(aopoa + at, a_min1) : sat0
     double at_b = 0.0, del_b = 0.0;
     if (del[n] > 0.0) {
-      const double xx = k.aopoa[n] + at[n];
-      if (xx < a_min1) {
-        b.g0 += sat_b[n] / a_min1;
-      } else {
-        const double rxx = 1.0 / xx;  // sat = g0 / xx here
-        b.g0 += sat_b[n] * rxx;
-        const double xx_b = -sat_b[n] * sat[n] * rxx;
+      b.g0 += sat_b * rxx[n];
+      if (!(k.aopoa[n] + at[n] < a_min1)) {
+        const double xx_b = -sat_b * sat[n] * rxx[n];
         b.aopoa[n] += xx_b;
         at_b = xx_b;
       }
     } else {
-      sat0_b[n] += sat_b[n];
+      sat0_b[n] += sat_b;
     }
     // at = a_soa + beta del; del = g_soa - sat0 a_soa
     a_soa_b[n] += at_b;
@@ -303,7 +298,7 @@ __host__ __device__ __forceinline__ void soa_tick_rev(const SoaState<double> &k,
   } else {
     tcur_b += b.tcur;
     dtcur_b += b.tcur;
-    tmpa_b -= dtcur_b * dtcur / tmpa;
+    tmpa_b -= dtcur_b * dtcur * rtmpa;
   }
   const double tmpb_b = cmax_pass(0.0, tmpb, tmpa_b);
   // tot = g_soa + sum a_soa
@@ -316,7 +311,7 @@ __host__ __device__ __forceinline__ void soa_tick_rev(const SoaState<double> &k,
     b.uk[n] += tmpb_b * fabs(phi[n]);
     const double phi_b = tmpb_b * k.uk[n] * (phi[n] < 0 ? -1.0 : 1.0);
     // phi = (g_soa - gstar) / kmax(g_soa, kmax(gstar, g_min1))
-    const double nb = phi_b / mden[n];
+    const double nb = phi_b * rm[n];
     const double md_b = -nb * phi[n];
     g_soa_b += nb;
     double gst_b = -nb;
@@ -330,13 +325,9 @@ __host__ __device__ __forceinline__ void soa_tick_rev(const SoaState<double> &k,
     sat0_b[n] += gst_b * a_soa[n];
     a_soa_b[n] += gst_b * sat0[n];
     // sat0 = g0 / kmax(aopoa + a_soa, a_min1)
-    const double yy = k.aopoa[n] + a_soa[n];
-    if (yy < a_min1) {
-      b.g0 += sat0_b[n] / a_min1;
-    } else {
-      const double ryy = 1.0 / yy;
-      b.g0 += sat0_b[n] * ryy;
-      const double yy_b = -sat0_b[n] * sat0[n] * ryy;
+    b.g0 += sat0_b[n] * ryy[n];
+    if (!(k.aopoa[n] + a_soa[n] < a_min1)) {
+      const double yy_b = -sat0_b[n] * sat0[n] * ryy[n];
       b.aopoa[n] += yy_b;
       a_soa_b[n] += yy_b;
     }
@@ -2270,23 +2261,24 @@ __host__ __device__ __forceinline__ void cell_finish_rev(
   }
 }
 
-// Adjoints that the reverse of clear_post leaves for the reverse of the SOA
-// loop (soa_g_b, soa_a_b, soa_gavg_b) and for the reverse of clear_pre,
-// clear_build and cell_setup (the rest).  Between the phases of the GPU's
-// split sweep they are held in HBM.
+// What the phase before the SOA loop's reverse hands on: the adjoints of the
+// loop's results (soa_g_b, soa_a_b, soa_gavg_b) and of its uptake-rate input
+// from the later processes (uptk_b), and the gradient contributed by every
+// path that avoids the loop (g, T_b, p_b, qv_b).  The reverses of clear_pre,
+// clear_build and cell_setup are linear in their adjoints, so the second
+// phase adds the loop's contribution to this partial gradient.  Between the
+// phases of the GPU's split sweep it is held in HBM.
 struct PostBar {
-  Aer<double> S_b;
-  ClearGas<double> G_b;
-  CtxBar Xb;
-  double st_b[NSTATE];
+  double g[NSTATE], T_b, p_b, qv_b;
   double uptk_b[4];
   double soa_g_b, soa_gavg_b, soa_a_b[4];
 };
 
 // Phase "post" of a cell with a clear subarea (X.cat != 1): clear_post run
 // forward from S0 / G0 (clear_build's outputs), in (clear_pre's; zero without
-// condensation) and the SOA loop's results, then the reverse of cell_finish
-// and clear_post from the cotangent tb.
+// condensation) and the SOA loop's results; the reverse of cell_finish and
+// clear_post from the cotangent tb; then the reverse of clear_build and
+// cell_setup for every adjoint that does not pass through the SOA loop.
 template <bool PRE>
 __host__ __device__ __forceinline__ void adjoint_post_phase(
     const ChainArgs &a, long off, const CellCtx<Fp64> &X, const Aer<double> &S0,
@@ -2297,38 +2289,50 @@ __host__ __device__ __forceinline__ void adjoint_post_phase(
   ClearGas<double> G = G0;
   clear_post<Fp64>(K, a.cfg, a.dt, X.T, X.p, X.z, X.pblh, X.rh_clear, X.dga, X.dgw, X.wd, G.g1,
                    G.g, S, in.uptk, sg, sgavg, sa);
-  zero(pb.S_b);
-  zero(pb.G_b);
-  zero(pb.Xb);
+  Aer<double> S_b;
+  ClearGas<double> G_b;
+  CtxBar Xb;
+  zero(S_b);
+  zero(G_b);
+  zero(Xb);
 #pragma unroll
-  for (int f = 0; f < NSTATE; ++f) pb.st_b[f] = 0.0;
+  for (int f = 0; f < NSTATE; ++f) pb.g[f] = 0.0;
 #pragma unroll
   for (int n = 0; n < 4; ++n) {
     pb.uptk_b[n] = 0.0;
     pb.soa_a_b[n] = 0.0;
   }
   pb.soa_g_b = pb.soa_gavg_b = 0.0;
-  cell_finish_rev<PRE>(a, off, X, S, G, tb, pb.S_b, pb.G_b, pb.Xb, pb.st_b);
+  pb.T_b = pb.p_b = pb.qv_b = 0.0;
+  cell_finish_rev<PRE>(a, off, X, S, G, tb, S_b, G_b, Xb, pb.g);
   clear_post_rev(K, a.cfg, a.dt, X.T, X.p, X.z, X.pblh, X.rh_clear, X.dga, X.dgw, X.wd, G0.g1,
-                 G0.g, S0, in.uptk, sg, sgavg, sa, pb.S_b, pb.G_b.g, pb.G_b.g1, pb.uptk_b,
-                 pb.soa_g_b, pb.soa_gavg_b, pb.soa_a_b, pb.Xb.T, pb.Xb.p, pb.Xb.rh_clear);
+                 G0.g, S0, in.uptk, sg, sgavg, sa, S_b, G_b.g, G_b.g1, pb.uptk_b, pb.soa_g_b,
+                 pb.soa_gavg_b, pb.soa_a_b, Xb.T, Xb.p, Xb.rh_clear);
+  clear_build_rev(a, off, X, S_b, G_b, Xb, pb.g);
+  cell_setup_rev<PRE>(a, off, X, Xb, pb.g, pb.T_b, pb.p_b, pb.qv_b);
 }
 
-// Phase "pre" of a cell with a clear subarea: from the post phase's adjoints
-// and those of the SOA loop's inputs (in_b, whose uptk entry must already
-// hold pb.uptk_b), the reverse of clear_pre (with condensation), clear_build
-// and cell_setup.  st_b, T_b, p_b, qv_b receive the gradient.
+// Phase "pre" of a cell with a clear subarea and condensation: the gradient
+// through the SOA loop's inputs, from their adjoints in_b (its uptk entry
+// holding the later processes' part too), by the reverse of clear_pre,
+// clear_build and cell_setup.  st_b, T_b, p_b, qv_b receive it (zeroed here);
+// the cell's gradient is this plus the post phase's partial one.
 template <bool PRE>
 __host__ __device__ __forceinline__ void adjoint_pre_phase(
     const ChainArgs &a, long off, const CellCtx<Fp64> &X, const Aer<double> &S0,
-    const SoaIn<double> &in_b, PostBar &pb, double st_b[NSTATE], double &T_b, double &p_b,
-    double &qv_b) {
-  if (a.cfg.do_cond)
-    clear_pre_rev(*a.K, X.T, X.p, X.dgw, S0, in_b, pb.S_b, pb.G_b.g[0], pb.Xb.T, pb.Xb.p);
-  clear_build_rev(a, off, X, pb.S_b, pb.G_b, pb.Xb, pb.st_b);
+    const SoaIn<double> &in_b, double st_b[NSTATE], double &T_b, double &p_b, double &qv_b) {
+  Aer<double> S_b;
+  ClearGas<double> G_b;
+  CtxBar Xb;
+  zero(S_b);
+  zero(G_b);
+  zero(Xb);
 #pragma unroll
-  for (int f = 0; f < NSTATE; ++f) st_b[f] = pb.st_b[f];
-  cell_setup_rev<PRE>(a, off, X, pb.Xb, st_b, T_b, p_b, qv_b);
+  for (int f = 0; f < NSTATE; ++f) st_b[f] = 0.0;
+  T_b = p_b = qv_b = 0.0;
+  clear_pre_rev(*a.K, X.T, X.p, X.dgw, S0, in_b, S_b, G_b.g[0], Xb.T, Xb.p);
+  clear_build_rev(a, off, X, S_b, G_b, Xb, st_b);
+  cell_setup_rev<PRE>(a, off, X, Xb, st_b, T_b, p_b, qv_b);
 }
 
 // The whole step's adjoint for one cell: the forward step recomputed (the
@@ -2375,12 +2379,24 @@ __host__ __device__ __forceinline__ int adjoint_cell(const ChainArgs &a, long of
     }
     PostBar pb;
     adjoint_post_phase<PRE>(a, off, X, S0, G0, in, sg, sgavg, sa, tb, pb);
-    SoaIn<double> in_b = {};
 #pragma unroll
-    for (int n = 0; n < 4; ++n) in_b.uptk[n] = pb.uptk_b[n];
-    if (a.cfg.do_cond)
+    for (int f = 0; f < NSTATE; ++f) st_b[f] = pb.g[f];
+    T_b = pb.T_b;
+    p_b = pb.p_b;
+    qv_b = pb.qv_b;
+    if (a.cfg.do_cond) {
+      SoaIn<double> in_b = {};
+#pragma unroll
+      for (int n = 0; n < 4; ++n) in_b.uptk[n] = pb.uptk_b[n];
       soa_loop_rev(in, dtfull, st, load, nt, pb.soa_g_b, pb.soa_a_b, pb.soa_gavg_b, in_b);
-    adjoint_pre_phase<PRE>(a, off, X, S0, in_b, pb, st_b, T_b, p_b, qv_b);
+      double sb2[NSTATE], Tb2, pb2, qb2;
+      adjoint_pre_phase<PRE>(a, off, X, S0, in_b, sb2, Tb2, pb2, qb2);
+#pragma unroll
+      for (int f = 0; f < NSTATE; ++f) st_b[f] += sb2[f];
+      T_b += Tb2;
+      p_b += pb2;
+      qv_b += qb2;
+    }
     return nt;
   }
   // overcast: no clear subarea, no SOA loop

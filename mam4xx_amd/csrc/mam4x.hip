@@ -31,7 +31,7 @@
 #include <mutex>
 #include <string>
 #include <vector>
-
+#include <algorithm>
 
 #include "../../include/mam4x.h"
 #include "chain_adjoint.hpp"
@@ -75,6 +75,14 @@ constexpr int kLongBucket = 15;
 #endif
 #ifndef MAM4X_STATIC_FIRST
 #define MAM4X_STATIC_FIRST 1
+#endif
+// Minimum waves per SIMD the reverse sweep's heaviest kernels are compiled
+// for (1: as many registers as the compiler wants).
+#ifndef MAM4X_REPLAY_WAVES
+#define MAM4X_REPLAY_WAVES 1
+#endif
+#ifndef MAM4X_ADJ_PRE_WAVES
+#define MAM4X_ADJ_PRE_WAVES 2
 #endif
 constexpr int kShortBucket = MAM4X_SHORT_BUCKET;
 constexpr int kShortChunk = MAM4X_SHORT_CHUNK;
@@ -418,9 +426,10 @@ struct AdjArgs {
 enum { R_G = 0, R_A = 1, R_GAVG = 5, R_AVG = 6, R_DTSUM = 7, R_NPLANE = 8 };
 // the adjoints of the SOA loop's inputs, from its reverse
 enum { I_G = 0, I_G0 = 1, I_A = 2, I_UPTK = 6, I_AOPOA = 10, I_NPLANE = 14 };
-// PostBar: S_b (4 numbers, 7 x 4 masses), G_b (8), Xb (25), st_b, uptk_b (4),
-// soa_g_b, soa_gavg_b, soa_a_b (4)
-constexpr int kPbPlanes = 32 + 8 + 25 + NSTATE + 4 + 6;
+// PostBar: the partial gradient (52 + 3), uptk_b (4), soa_g_b, soa_gavg_b,
+// soa_a_b (4)
+constexpr int kPbPlanes = NSTATE + 3 + 4 + 6;
+constexpr int kPbUptk = NSTATE + 3;    // plane of uptk_b[0]
 constexpr int kPbSoa = kPbPlanes - 6;  // plane of soa_g_b; soa_gavg_b, soa_a_b follow
 constexpr int kAdjPlanes = R_NPLANE + kPbPlanes + I_NPLANE;
 
@@ -450,56 +459,21 @@ struct AdjWork {
 static_assert(sizeof(ChainArgs) + sizeof(AdjWork) + sizeof(AdjArgs) <= 4000,
               "the adjoint kernels' arguments fit the kernel-argument segment");
 
-// PostBar <-> its planes (STORE: to them)
-template <bool STORE>
-__device__ __forceinline__ void pb_io(double *pl, long n, long c, dev::rev::PostBar &pb) {
-  int k = 0;
-  auto io = [&](double &v) {
-    double *q = pl + (long)k * n + c;
-    if (STORE)
-      *q = v;
-    else
-      v = *q;
-    ++k;
-  };
+// PostBar -> its planes
+__device__ __forceinline__ void pb_store(double *pl, long n, long c,
+                                         const dev::rev::PostBar &pb) {
+  double *q = pl + c;
 #pragma unroll
-  for (int m = 0; m < 4; ++m) io(pb.S_b.n[m]);
+  for (int f = 0; f < NSTATE; ++f) q[(long)f * n] = pb.g[f];
+  q[(long)NSTATE * n] = pb.T_b;
+  q[(long)(NSTATE + 1) * n] = pb.p_b;
+  q[(long)(NSTATE + 2) * n] = pb.qv_b;
 #pragma unroll
-  for (int sp = 0; sp < 7; ++sp)
+  for (int m = 0; m < 4; ++m) q[(long)(kPbUptk + m) * n] = pb.uptk_b[m];
+  q[(long)kPbSoa * n] = pb.soa_g_b;
+  q[(long)(kPbSoa + 1) * n] = pb.soa_gavg_b;
 #pragma unroll
-    for (int m = 0; m < 4; ++m) io(pb.S_b.q[sp][m]);
-  io(pb.G_b.g1[0]);
-  io(pb.G_b.g1[1]);
-  io(pb.G_b.g[0]);
-  io(pb.G_b.g[1]);
-  io(pb.G_b.hcl);
-  io(pb.G_b.scl);
-  io(pb.G_b.hcd);
-  io(pb.G_b.scd);
-  io(pb.Xb.T);
-  io(pb.Xb.p);
-  io(pb.Xb.rh_clear);
-  io(pb.Xb.h1);
-  io(pb.Xb.s1);
-  io(pb.Xb.h2);
-  io(pb.Xb.s2);
-  io(pb.Xb.h3);
-  io(pb.Xb.s3);
-#pragma unroll
-  for (int m = 0; m < 4; ++m) {
-    io(pb.Xb.fa_n[m]);
-    io(pb.Xb.fc_n[m]);
-    io(pb.Xb.fa_q[m]);
-    io(pb.Xb.fc_q[m]);
-  }
-#pragma unroll
-  for (int f = 0; f < NSTATE; ++f) io(pb.st_b[f]);
-#pragma unroll
-  for (int m = 0; m < 4; ++m) io(pb.uptk_b[m]);
-  io(pb.soa_g_b);  // plane kPbSoa
-  io(pb.soa_gavg_b);
-#pragma unroll
-  for (int m = 0; m < 4; ++m) io(pb.soa_a_b[m]);
+  for (int m = 0; m < 4; ++m) q[(long)(kPbSoa + 2 + m) * n] = pb.soa_a_b[m];
 }
 
 // Phase 2: soa_queue_kernel's loop (same refill, same sub-steps) with the
@@ -650,31 +624,54 @@ struct SoaRev {
   }
 };
 
-__global__ void __launch_bounds__(kBlock) soa_replay_kernel(AdjWork g) {
+// one lane's entry of a log row
+__device__ __forceinline__ void log_fetch(const AdjWork &g, int chunk, int r, int lane, int &tag,
+                                          double v[kLogVals]) {
+  const long base = (long)chunk * kLogRows + r;
+  tag = g.tag[base * 64 + lane];
+#pragma unroll
+  for (int j = 0; j < kLogVals; ++j) v[j] = g.rec[(base * kLogVals + j) * 64 + lane];
+}
+
+// The wave's rows newest first; the next row's loads are issued before the
+// current row's sub-step is reversed, so they are in flight meanwhile.
+__global__ void __launch_bounds__(kBlock, MAM4X_REPLAY_WAVES) soa_replay_kernel(AdjWork g) {
   const unsigned wid = blockIdx.x * (kBlock / 64) + (threadIdx.x >> 6);
   const int lane = threadIdx.x & 63;
   if (wid >= g.nwave) return;
   const int2 e = g.wend[wid];
-  int chunk = e.x, nrow = e.y;
+  int chunk = e.x, r = e.y - 1;
+  if (chunk < 0 || r < 0) return;
   SoaRev t(g);
-  while (chunk >= 0) {
-    for (int r = nrow - 1; r >= 0; --r) {
-      const long base = (long)chunk * kLogRows + r;
-      const int tag = g.tag[base * 64 + lane];
-      if (tag >= 0) {
-        dev::rev::SoaCkpt c;
-        c.g = g.rec[(base * kLogVals + 0) * 64 + lane];
-#pragma unroll
-        for (int m = 0; m < 4; ++m) c.a[m] = g.rec[(base * kLogVals + 1 + m) * 64 + lane];
-        c.tcur = g.rec[(base * kLogVals + 5) * 64 + lane];
-        dev::rev::soa_tick_rev(t.k, c, g.dtfull, t.b);
-      } else {
-        if (t.cell >= 0) t.finish();
-        if (tag != kTagIdle) t.begin(~tag);
-      }
+  int tag;
+  double v[kLogVals];
+  log_fetch(g, chunk, r, lane, tag, v);
+  while (true) {
+    int nchunk = chunk, nr = r - 1;
+    if (nr < 0) {
+      nchunk = g.prev[chunk];
+      nr = kLogRows - 1;
     }
-    chunk = g.prev[chunk];
-    nrow = kLogRows;
+    int ntag = kTagIdle;
+    double nv[kLogVals];
+    if (nchunk >= 0) log_fetch(g, nchunk, nr, lane, ntag, nv);
+    if (tag >= 0) {
+      dev::rev::SoaCkpt c;
+      c.g = v[0];
+#pragma unroll
+      for (int m = 0; m < 4; ++m) c.a[m] = v[1 + m];
+      c.tcur = v[5];
+      dev::rev::soa_tick_rev(t.k, c, g.dtfull, t.b);
+    } else {
+      if (t.cell >= 0) t.finish();
+      if (tag != kTagIdle) t.begin(~tag);
+    }
+    if (nchunk < 0) break;
+    chunk = nchunk;
+    r = nr;
+    tag = ntag;
+#pragma unroll
+    for (int j = 0; j < kLogVals; ++j) v[j] = nv[j];
   }
   if (t.cell >= 0) t.finish();
 }
@@ -727,12 +724,12 @@ __global__ void __launch_bounds__(kBlock) adj_post_kernel(ChainArgs a, AdjWork g
   dev::rev::PostBar pb;
   dev::rev::adjoint_post_phase<PRE>(a, off, X, S0, G0, in, g.res[R_G * n + c],
                                     g.res[R_GAVG * n + c], sa, tb, pb);
-  pb_io<true>(g.pb, n, c, pb);
+  pb_store(g.pb, n, c, pb);
 }
 
 // Phase 5 (cells with a SOA loop).
 template <bool PRE>
-__global__ void __launch_bounds__(kBlock) adj_pre_kernel(ChainArgs a, AdjWork g, AdjArgs o) {
+__global__ void __launch_bounds__(kBlock, MAM4X_ADJ_PRE_WAVES) adj_pre_kernel(ChainArgs a, AdjWork g, AdjArgs o) {
   const long c = (long)blockIdx.x * kBlock + threadIdx.x;
   const long n = g.ncell;
   if (c >= n || g.bucket[c] == kNoSoa) return;
@@ -744,20 +741,25 @@ __global__ void __launch_bounds__(kBlock) adj_pre_kernel(ChainArgs a, AdjWork g,
   dev::Aer<double> S0;
   dev::ClearGas<double> G0;
   dev::clear_build<Fp64>(a, off, X, S0, G0);
-  dev::rev::PostBar pb;
-  pb_io<false>(g.pb, n, c, pb);
   const double *pl = g.inb + c;
+  const double *pp = g.pb + c;
   dev::SoaIn<double> in_b;
   in_b.g = pl[I_G * n];
   in_b.g0 = pl[I_G0 * n];
 #pragma unroll
   for (int m = 0; m < 4; ++m) {
     in_b.a[m] = pl[(I_A + m) * n];
-    in_b.uptk[m] = pb.uptk_b[m] + pl[(I_UPTK + m) * n];
+    in_b.uptk[m] = pp[(long)(kPbUptk + m) * n] + pl[(I_UPTK + m) * n];
     in_b.aopoa[m] = pl[(I_AOPOA + m) * n];
   }
-  double sb[NSTATE], Tb = 0.0, pB = 0.0, qb = 0.0;
-  dev::rev::adjoint_pre_phase<PRE>(a, off, X, S0, in_b, pb, sb, Tb, pB, qb);
+  double sb[NSTATE], Tb, pB, qb;
+  dev::rev::adjoint_pre_phase<PRE>(a, off, X, S0, in_b, sb, Tb, pB, qb);
+  // plus the post phase's partial gradient
+#pragma unroll
+  for (int f = 0; f < NSTATE; ++f) sb[f] = pp[(long)f * n] + sb[f];
+  Tb = pp[(long)NSTATE * n] + Tb;
+  pB = pp[(long)(NSTATE + 1) * n] + pB;
+  qb = pp[(long)(NSTATE + 2) * n] + qb;
   store_grad(o, off, sb, Tb, pB, qb);
 }
 
@@ -1872,8 +1874,13 @@ int mam4x_compute_adjoint(mam4x_handle h, double t, double dt, const mam4x_input
     rc = reserve_adjoint(h, ncell, s);
     if (rc) return rc;
     if (!h->log_cap) {
-      // a first pool for ~40 sub-steps per cell; grown below when short
-      rc = reserve_log(h, (unsigned long long)ncell * 40 / (64 * kLogRows) + 64);
+      // a first pool for 64 sub-steps per cell (at most 6 GiB); grown below
+      // when short
+      const unsigned long long chunk_bytes = (unsigned long long)kLogRows * 64 *
+                                             (kLogVals * sizeof(double) + sizeof(int));
+      unsigned long long first = (unsigned long long)ncell * 64 / (64 * kLogRows) + 64;
+      first = std::min(first, (6ULL << 30) / chunk_bytes);
+      rc = reserve_log(h, first);
       if (rc) return rc;
     }
     // 1. set-up and queue

@@ -136,20 +136,31 @@ __host__ __device__ __forceinline__ Dual round(Dual a) { return Dual(::round(a.v
 // numerator +0 as well (IEEE: -0).  For other operands the result may differ
 // from a / b (b = 0 or inf gives NaN), so callers use it only where the
 // operands are bounded and numerators are never -0 (the SOA sub-step).
-__host__ __device__ __forceinline__ double div(double a, double b) {
+// div() in two steps: rcp(b), the reciprocal with its two Newton steps, and
+// div_r(a, b, r), the quotient from it; div_r(a, b, rcp(b)) is div(a, b),
+// so a divisor used more than once is inverted once.
+__host__ __device__ __forceinline__ double rcp(double b) {
 #if defined(__HIP_DEVICE_COMPILE__)
   double r = __builtin_amdgcn_rcp(b);
   double e = __builtin_fma(-b, r, 1.0);
   r = __builtin_fma(r, e, r);
   e = __builtin_fma(-b, r, 1.0);
-  r = __builtin_fma(r, e, r);
+  return __builtin_fma(r, e, r);
+#else
+  return 1.0 / b;
+#endif
+}
+__host__ __device__ __forceinline__ double div_r(double a, double b, double r) {
+#if defined(__HIP_DEVICE_COMPILE__)
   const double q = a * r;
   const double res = __builtin_fma(-b, q, a);
   return __builtin_fma(res, r, q);
 #else
+  (void)r;
   return a / b;
 #endif
 }
+__host__ __device__ __forceinline__ double div(double a, double b) { return div_r(a, b, rcp(b)); }
 __host__ __device__ __forceinline__ F32 div(F32 a, F32 b) { return a / b; }
 __host__ __device__ __forceinline__ Dual div(Dual a, Dual b) { return a / b; }
 
