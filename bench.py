@@ -892,7 +892,7 @@ def bench_adjoint(args):
     from mam4xx_amd.chain import Chain, scene_to_device
     scene = scenes.generate(NCOL, NLEV, seed=0)
     d = scene_to_device(scene)
-    ch = Chain()
+    ch = Chain(ntsubstep=args.ntsubstep)
     shape = (L.NSTATE, NLEV, NCOL)
     g = torch.Generator(device="cuda").manual_seed(1)
     tend = torch.empty(shape, dtype=torch.float64, device="cuda")
@@ -939,9 +939,11 @@ def bench_adjoint(args):
         "vs_baseline": None,
         "dtype": "f64",
         "data": "synthetic (seeded generator of the reference study's value distributions)",
-        "config": {"workload": "21600 columns x 72 levels, one 1800 s step from the seeded "
-                               "state; cotangent of all 52 tendencies, gradients w.r.t. the "
-                               "state, T, p and qv", "columns": NCOL, "levels": NLEV},
+        "config": {"workload": "21600 columns x 72 levels, one 1800 s step with %d condensation "
+                               "sub-step(s) from the seeded state; cotangent of all 52 "
+                               "tendencies, gradients w.r.t. the state, T, p and qv"
+                               % args.ntsubstep, "columns": NCOL, "levels": NLEV,
+                   "ntsubstep": args.ntsubstep},
         "times_ms": ms,
         "cost_vs_forward": {k: ms[k]["host_ms"] / ms["forward"]["host_ms"]
                             for k in ("tangent", "adjoint")},
@@ -1147,6 +1149,8 @@ def main():
     ap.add_argument("--warmup", type=int, default=3)
     ap.add_argument("--schedule", choices=["auto", "fused", "queue"], default="auto")
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline")
+    ap.add_argument("--ntsubstep", type=int, default=1,
+                    help="adjoint workload: condensation sub-steps per step")
     ap.add_argument("--no-mixed", action="store_true",
                     help="headline: time the FP64 build only")
     ap.add_argument("--workload", choices=["headline", "batch", "large", "water", "wetsize",

@@ -40,7 +40,8 @@
 extern "C" {
 #endif
 
-#define MAM4X_ABI_VERSION 7
+#define MAM4X_ABI_VERSION 8
+#define MAM4X_MAX_NTSUBSTEP 64
 
 /* per-process tendency diagnostics (mam4x_outputs.process_tendencies) */
 #define MAM4X_GAS_PCNST 31
@@ -113,6 +114,18 @@ typedef struct mam4x_config {
   int schedule;  /* MAM4X_SCHED_*                                         */
   int soa_blocks_per_cu; /* persistent SOA kernel: resident blocks per CU,
                             0 = as many as fit (performance only)        */
+  int ntsubstep; /* condensation sub-steps per step, 1..MAM4X_MAX_NTSUBSTEP
+                    (0 is taken as 1).  The reference fixes 1
+                    (mam4_amicphys.hpp:1288).  With n > 1 each subarea's
+                    chain (condensation, renaming, nucleation, coagulation,
+                    aging) runs n times over the int sub-step dt / n; the
+                    H2SO4 uptake rates and their sum are those of the first
+                    sub-step, held for the others (the Fortran original's
+                    scope of uptkaer / uptkrate_h2so4, which the C++ leaves
+                    undefined after the first: :1344, :1377,
+                    gasaerexch.hpp:428); the cloudy subarea's renaming takes
+                    1/n of the cloud-chemistry growth per sub-step
+                    (:1446-1455).  n > 1 runs the one-lane-per-cell schedule. */
 } mam4x_config;
 
 /* Atmosphere (atmosphere.hpp:16-137) and modal diagnostics the chain reads. */

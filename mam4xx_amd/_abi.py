@@ -6,7 +6,7 @@ The structures below must match the C declarations field for field; the
 import ctypes
 import os
 
-ABI_VERSION = 7
+ABI_VERSION = 8
 
 NSTATE = 52
 NMODE = 4
@@ -62,6 +62,7 @@ class Config(ctypes.Structure):
         ("precision", ctypes.c_int),
         ("schedule", ctypes.c_int),
         ("soa_blocks_per_cu", ctypes.c_int),
+        ("ntsubstep", ctypes.c_int),
     ]
 
 

@@ -1315,26 +1315,23 @@ __host__ __device__ __forceinline__ void age_pcarbon_rev(const ChainConsts &K, u
 }
 
 // ------------------------------------------------------------------ subarea
-// clear_post (no diagnostics): S_in / g_in are its inputs; S_b / g_b hold the
-// outputs' adjoints on entry and the inputs' on return.
-__host__ __device__ __forceinline__ void clear_post_rev(
-    const ChainConsts &K, const Cfg &cfg, double deltat, double temp, double pmid, double zmid,
-    double pblh, double relhum, const double dgn_a[4], const double dgn_awet[4],
-    const double wetdens[4], const double g1[2], const double g_in[2], const Aer<double> &S_in,
-    const double uptk[4], double soa_g, double soa_gavg, const double soa_a[4],
-    Aer<double> &S_b, double g_b[2], double g1_b[2], double uptk_b[4], double &soa_g_b,
-    double &soa_gavg_b, double soa_a_b[4], double &temp_b, double &pmid_b, double &relhum_b) {
+// One sub-step of clear_post (clear_post_sub, no diagnostics) over dtsub:
+// g_in / S_in are the gases and aerosol it starts from, netprod the H2SO4
+// net production from clear_post_begin.  S_b / g_b hold the outputs'
+// adjoints on entry and the inputs' on return; netprod_b, uptk_b, the SOA
+// loop results' adjoints and temp_b, pmid_b, relhum_b accumulate.
+__host__ __device__ __forceinline__ void clear_post_sub_rev(
+    const ChainConsts &K, const Cfg &cfg, double dtsub, double netprod, double temp,
+    double pmid, double zmid, double pblh, double relhum, const double dgn_a[4],
+    const double dgn_awet[4], const double wetdens[4], const double g_in[2],
+    const Aer<double> &S_in, const double uptk[4], double soa_g, double soa_gavg,
+    const double soa_a[4], Aer<double> &S_b, double g_b[2], double uptk_b[4],
+    double &netprod_b, double &soa_g_b, double &soa_gavg_b, double soa_a_b[4],
+    double &temp_b, double &pmid_b, double &relhum_b) {
   // ---- forward, keeping the state before each process ----
   const double aircon = pmid / (8314.46759100000 * temp);
   double g[2] = {g_in[0], g_in[1]};
-  double netprod = 0.0;
-  const bool np = cfg.do_cond && cfg.gaexch_h2so4_uptake_optaa == 2;
-  if (np) {
-    netprod = (g[1] - g1[1]) / deltat;
-    g[1] = (netprod >= 0) ? g1[1] : 0.0;
-  }
   const double g_np1 = g[1];
-  const double dtsub = (double)(int)(deltat / 1);
   double uptkrate = 0.0, gavg[2];
   Aer<double> S = S_in;
   double dc_so4[4] = {0, 0, 0, 0}, dc_soa[4] = {0, 0, 0, 0}, so4_0[4] = {0, 0, 0, 0};
@@ -1394,7 +1391,6 @@ __host__ __device__ __forceinline__ void clear_post_rev(
   double dc_so4_b[4] = {0, 0, 0, 0}, dc_soa_b[4] = {0, 0, 0, 0};
   dc_so4_b[NAIT] += gi_b[A_SO4];
   dc_soa_b[NAIT] += gi_b[A_SOA];
-  double netprod_b = 0.0;
   if (cfg.do_cond) {
     double so4_b[4];
 #pragma unroll
@@ -1418,13 +1414,38 @@ __host__ __device__ __forceinline__ void clear_post_rev(
     g_b[0] += gavg_b[0];
     g_b[1] += gavg_b[1];
   }
-  if (np) {
+  pmid_b += aircon_b / (8314.46759100000 * temp);
+  temp_b -= aircon_b * aircon / temp;
+}
+
+// clear_post_begin: g_b (H2SO4) and netprod_b -> the adjoints of the gases
+// before it (g_b) and before gas chemistry (g1_b).
+__host__ __device__ __forceinline__ void clear_post_begin_rev(const Cfg &cfg, double deltat,
+                                                              double netprod, double netprod_b,
+                                                              double g_b[2], double g1_b[2]) {
+  if (cfg.do_cond && cfg.gaexch_h2so4_uptake_optaa == 2) {
     if (netprod >= 0) g1_b[1] += g_b[1];
     g_b[1] = netprod_b / deltat;
     g1_b[1] -= netprod_b / deltat;
   }
-  pmid_b += aircon_b / (8314.46759100000 * temp);
-  temp_b -= aircon_b * aircon / temp;
+}
+
+// clear_post (no diagnostics, one sub-step): S_in / g_in are its inputs;
+// S_b / g_b hold the outputs' adjoints on entry and the inputs' on return.
+__host__ __device__ __forceinline__ void clear_post_rev(
+    const ChainConsts &K, const Cfg &cfg, double deltat, double temp, double pmid, double zmid,
+    double pblh, double relhum, const double dgn_a[4], const double dgn_awet[4],
+    const double wetdens[4], const double g1[2], const double g_in[2], const Aer<double> &S_in,
+    const double uptk[4], double soa_g, double soa_gavg, const double soa_a[4],
+    Aer<double> &S_b, double g_b[2], double g1_b[2], double uptk_b[4], double &soa_g_b,
+    double &soa_gavg_b, double soa_a_b[4], double &temp_b, double &pmid_b, double &relhum_b) {
+  double g[2] = {g_in[0], g_in[1]};
+  const double netprod = clear_post_begin(cfg, deltat, g1, g);
+  double netprod_b = 0.0;
+  clear_post_sub_rev(K, cfg, (double)(int)(deltat / 1), netprod, temp, pmid, zmid, pblh, relhum,
+                     dgn_a, dgn_awet, wetdens, g, S_in, uptk, soa_g, soa_gavg, soa_a, S_b, g_b,
+                     uptk_b, netprod_b, soa_g_b, soa_gavg_b, soa_a_b, temp_b, pmid_b, relhum_b);
+  clear_post_begin_rev(cfg, deltat, netprod, netprod_b, g_b, g1_b);
 }
 
 // ------------------------------------------------------------------ calcsize
@@ -1977,6 +1998,18 @@ __host__ __device__ __forceinline__ void clear_build_rev(const ChainArgs &a, lon
   }
 }
 
+// soa_in_state: from the adjoints of the loop inputs it sets (in_b.g, .a,
+// .aopoa), those of the subarea's state S and SOAG
+__host__ __device__ __forceinline__ void soa_in_state_rev(const Aer<double> &S,
+                                                          const SoaIn<double> &in_b,
+                                                          Aer<double> &S_b, double &g_soag_b) {
+  g_soag_b += in_b.g;
+#pragma unroll
+  for (int n = 0; n < 4; ++n) S_b.q[A_SOA][n] += in_b.a[n];
+  S_b.q[A_POM][NACC] += maxc_pass(S.q[A_POM][NACC], 0.0, 0.1 * in_b.aopoa[NACC]);
+  S_b.q[A_POM][NCOA] += maxc_pass(S.q[A_POM][NCOA], 0.0, 0.1 * in_b.aopoa[NCOA]);
+}
+
 // clear_pre: from the adjoints of the SOA loop's inputs (in_b; its uptk
 // entry also carries the H2SO4 uptake's other uses), those of S, the SOAG
 // it started from, temp and pmid
@@ -1990,11 +2023,7 @@ __host__ __device__ __forceinline__ void clear_pre_rev(const ChainConsts &K, dou
   double aircon_b = 0.0;
   h2so4_uptake_rev(K, temp, pmid, aircon, dgn_awet, S.n, in_b.uptk, temp_b, pmid_b, aircon_b,
                    S_b.n);
-  g_soag_b += in_b.g;
-#pragma unroll
-  for (int n = 0; n < 4; ++n) S_b.q[A_SOA][n] += in_b.a[n];
-  S_b.q[A_POM][NACC] += maxc_pass(S.q[A_POM][NACC], 0.0, 0.1 * in_b.aopoa[NACC]);
-  S_b.q[A_POM][NCOA] += maxc_pass(S.q[A_POM][NCOA], 0.0, 0.1 * in_b.aopoa[NCOA]);
+  soa_in_state_rev(S, in_b, S_b, g_soag_b);
   soa_g0_rev(K, temp, pmid, in_b.g0, temp_b, pmid_b);
   pmid_b += aircon_b / (8314.46759100000 * temp);
   temp_b -= aircon_b * aircon / temp;
@@ -2010,6 +2039,7 @@ __host__ __device__ __forceinline__ void cell_finish_rev(
     ClearGas<double> &G_b, CtxBar &Xb, double st_b[NSTATE]) {
   const ChainConsts &K = *a.K;
   const double dt = a.dt;
+  const int ns = nsub(a.cfg);
   // ---- forward ----
   double gi_out[NSTATE];
   if (X.cat != 1) {
@@ -2057,14 +2087,16 @@ __host__ __device__ __forceinline__ void cell_finish_rev(
           const double qc2 = ld2<PRE, Fp64>(a, off, f_qc(m, s), to_vmr_mass(m, s));
           const double a2 = (X.cat == 2 ? q2 * X.fc_q[m] : q2) * fcvt_aer(sp);
           const double c2 = (X.cat == 2 ? qc2 / X.fcldy : qc2) * fcvt_aer(sp);
-          gi[sp] = (C.q[sp][m] - a2) / 1;
-          gc[sp] = (Cc.q[sp][m] - c2) / 1;
+          gi[sp] = (C.q[sp][m] - a2) / ns;
+          gc[sp] = (Cc.q[sp][m] - c2) / ns;
         }
       }
     }
     C0 = C;
     Cc0 = Cc;
-    if (a.cfg.do_rename) rename_ait_to_acc<Fp64>(K, true, C, gi, Cc, gc);
+    // one renaming per sub-step, each with 1/ns of the cloud-chemistry growth
+    if (a.cfg.do_rename)
+      for (int j = 0; j < ns; ++j) rename_ait_to_acc<Fp64>(K, true, C, gi, Cc, gc);
     const double w = X.fcldy;
 #pragma unroll
     for (int m = 0; m < 4; ++m) {
@@ -2178,7 +2210,14 @@ __host__ __device__ __forceinline__ void cell_finish_rev(
       }
     }
     double gi_b[7] = {0, 0, 0, 0, 0, 0, 0}, gc_b[7] = {0, 0, 0, 0, 0, 0, 0};
-    if (a.cfg.do_rename) rename_rev(K, true, C0, gi, Cc0, gc, C_b, gi_b, Cc_b, gc_b);
+    if (a.cfg.do_rename) {
+      // sub-step j's renaming starts from C0 renamed j times (recomputed)
+      for (int j = ns - 1; j >= 0; --j) {
+        Aer<double> Cj = C0, Ccj = Cc0;
+        for (int i = 0; i < j; ++i) rename_ait_to_acc<Fp64>(K, true, Cj, gi, Ccj, gc);
+        rename_rev(K, true, Cj, gi, Ccj, gc, C_b, gi_b, Cc_b, gc_b);
+      }
+    }
 #pragma unroll
     for (int m = 0; m < 4; ++m) {
       const double n3 = ld3<Fp64>(a, off, f_ni(m), TO_VMR_NUM);
@@ -2197,10 +2236,11 @@ __host__ __device__ __forceinline__ void cell_finish_rev(
         double cq_b = C_b.q[sp][m], ccq_b = Cc_b.q[sp][m];
         if (m == NAIT) {
           // gi = C.q - a2, gc = Cc.q - c2
-          cq_b += gi_b[sp];
-          ccq_b += gc_b[sp];
+          const double gib = gi_b[sp] / ns, gcb = gc_b[sp] / ns;
+          cq_b += gib;
+          ccq_b += gcb;
           const double q2 = ld2<PRE, Fp64>(a, off, f_qi(m, s), to_vmr_mass(m, s));
-          const double a2_b = -gi_b[sp] * fcvt_aer(sp), c2_b = -gc_b[sp] * fcvt_aer(sp);
+          const double a2_b = -gib * fcvt_aer(sp), c2_b = -gcb * fcvt_aer(sp);
           if (X.cat == 2) {
             Xb.fc_q[m] += a2_b * q2;
             ld2_rev<PRE>(a, off, f_qi(m, s), to_vmr_mass(m, s), a2_b * X.fc_q[m], st_b);
@@ -2335,80 +2375,208 @@ __host__ __device__ __forceinline__ void adjoint_pre_phase(
   cell_setup_rev<PRE>(a, off, X, Xb, st_b, T_b, p_b, qv_b);
 }
 
-// The whole step's adjoint for one cell: the forward step recomputed (the
-// SOA loop saving the state each sub-step starts from through save(i, c)),
-// then the reverse sweep (the loop's states read back through load(i)).
-// tb: cotangent of the 52 tendencies; st_b, T_b, p_b, qv_b receive the
-// gradient of <tb, tendencies> w.r.t. the state, T, p and qv (st_b zeroed by
-// the caller).  Returns the SOA sub-step count.  The GPU runs the same
-// phases as separate kernels, the SOA loop's on a work queue.
-template <bool PRE, class Save, class Load>
+// The state a SOA sub-step starts from, as the reverse sweep keeps it.
+__host__ __device__ __forceinline__ SoaCkpt soa_ckpt(const SoaState<double> &st) {
+  SoaCkpt c;
+  c.g = st.g;
+#pragma unroll
+  for (int n = 0; n < 4; ++n) c.a[n] = st.a[n];
+  c.tcur = st.tcur;
+  return c;
+}
+
+// A SOA loop's sub-step states for the reverse sweep in fixed storage
+// (2 C records): record() runs the loop keeping up to C states; a longer
+// loop (nt > C, at most C^2 sub-steps: C = 32 covers the 1000-sub-step cap)
+// is run a second time keeping every s-th state, s = ceil(nt / C), and
+// get(i), called for i = nt - 1 down to 0, recomputes each s-long segment
+// from its first state once, when it first needs it.
+template <int C>
+struct SoaTape2 {
+  SoaState<double> init;
+  SoaCkpt coarse[C], fine[C];
+  double dtfull;
+  int nt, s, seg;
+  __host__ __device__ int record(SoaState<double> &st, double dt) {
+    init = st;
+    dtfull = dt;
+    nt = 0;
+    s = 1;
+    seg = -1;
+    while (true) {
+      const SoaCkpt c = soa_ckpt(st);
+      if (!soa_tick(st, dtfull)) break;
+      if (nt < C) coarse[nt] = c;
+      ++nt;
+    }
+    if (nt > C) {
+      s = (nt + C - 1) / C;
+      SoaState<double> t = init;
+      for (int i = 0; i < nt; ++i) {
+        if (i % s == 0) coarse[i / s] = soa_ckpt(t);
+        soa_tick(t, dtfull);
+      }
+    }
+    return nt;
+  }
+  __host__ __device__ SoaCkpt get(int i) {
+    if (s == 1) return coarse[i];
+    const int sg = i / s;
+    if (sg != seg) {
+      SoaState<double> t = init;
+      t.g = coarse[sg].g;
+#pragma unroll
+      for (int n = 0; n < 4; ++n) t.a[n] = coarse[sg].a[n];
+      t.tcur = coarse[sg].tcur;
+      t.niter = sg * s;
+      const int lo = sg * s, hi = lo + s < nt ? lo + s : nt;
+      for (int k = lo; k < hi; ++k) {
+        fine[k - lo] = soa_ckpt(t);
+        soa_tick(t, dtfull);
+      }
+      seg = sg;
+    }
+    return fine[i - sg * s];
+  }
+};
+
+// One sub-step's subarea state kept in registers (ntsubstep = 1 only).
+struct OneSubTape {
+  Aer<double> S;
+  double g[2];
+  __host__ __device__ void save(int, const Aer<double> &S_, const double g_[2]) {
+    S = S_;
+    g[0] = g_[0];
+    g[1] = g_[1];
+  }
+  __host__ __device__ void load(int, Aer<double> &S_, double g_[2]) const {
+    S_ = S;
+    g_[0] = g[0];
+    g_[1] = g[1];
+  }
+};
+
+// The whole step's adjoint for one cell, with ntsubstep sub-steps: the
+// forward step recomputed, keeping the subarea's aerosol and gases at the
+// start of every sub-step (sub.save / sub.load, 34 doubles each); then the
+// reverse sweep, sub-step by sub-step from the last, each sub-step's SOA
+// loop recomputed from its start (soa.record) and read back in reverse
+// (soa.get(i), i = nt - 1 .. 0).  tb: cotangent of the 52 tendencies;
+// st_b, T_b, p_b, qv_b receive the gradient of <tb, tendencies> w.r.t. the
+// state, T, p and qv (st_b zeroed by the caller).  Returns the SOA sub-step
+// count.  The GPU runs the one-sub-step case as separate kernels, the SOA
+// loop's on a work queue.
+template <bool PRE, class SoaTape, class SubTape>
 __host__ __device__ __forceinline__ int adjoint_cell(const ChainArgs &a, long off, int icol,
-                                                     const double tb[NSTATE], Save save,
-                                                     Load load, double st_b[NSTATE],
+                                                     const double tb[NSTATE], SoaTape &soa,
+                                                     SubTape &sub, double st_b[NSTATE],
                                                      double &T_b, double &p_b, double &qv_b) {
   const ChainConsts &K = *a.K;
+  const Cfg &cfg = a.cfg;
   CellCtx<Fp64> X;
   cell_setup<PRE, Fp64>(a, off, icol, X);
-  int nt = 0;
+  CtxBar Xb;
+  zero(Xb);
+  int niter = 0;
   if (X.cat != 1) {
     Aer<double> S0;
     ClearGas<double> G0;
     clear_build<Fp64>(a, off, X, S0, G0);
-    SoaIn<double> in = {};
-    SoaState<double> st;
-    double sg = 0.0, sgavg = 0.0, sa[4] = {0, 0, 0, 0};
-    const double dtfull = (double)(int)(a.dt / 1);
-    if (a.cfg.do_cond) {
-      clear_pre<Fp64>(K, X.T, X.p, X.dgw, S0, G0.g[0], in);
-      soa_init(st, in);
-      while (true) {
-        SoaCkpt c;
-        c.g = st.g;
+    SoaIn<double> in0 = {};
+    if (cfg.do_cond) clear_pre<Fp64>(K, X.T, X.p, X.dgw, S0, G0.g[0], in0);
+    const int ns = nsub(cfg);
+    const double dtsub = dt_sub(cfg, a.dt);
+    double g[2] = {G0.g[0], G0.g[1]};
+    const double netprod = clear_post_begin(cfg, a.dt, G0.g1, g);
+    // ---- forward over the sub-steps ----
+    Aer<double> S = S0;
+    for (int j = 0; j < ns; ++j) {
+      sub.save(j, S, g);
+      SoaIn<double> in = in0;
+      if (j > 0 && cfg.do_cond) soa_in_state<Fp64>(S, g[0], in);
+      double sg = 0.0, sgavg = 0.0, sa[4] = {0, 0, 0, 0};
+      if (cfg.do_cond) {
+        SoaState<double> st;
+        soa_init(st, in);
+        while (soa_tick(st, dtsub)) {
+        }
+        niter += st.niter;
+        sg = st.g;
+        sgavg = soa_gavg(st);
 #pragma unroll
-        for (int n = 0; n < 4; ++n) c.a[n] = st.a[n];
-        c.tcur = st.tcur;
-        if (!soa_tick(st, dtfull)) break;
-        save(nt, c);
-        ++nt;
+        for (int n = 0; n < 4; ++n) sa[n] = st.a[n];
       }
-      sg = st.g;
-      sgavg = soa_gavg(st);
-#pragma unroll
-      for (int n = 0; n < 4; ++n) sa[n] = st.a[n];
+      clear_post_sub<Fp64>(K, cfg, dtsub, netprod, X.T, X.p, X.z, X.pblh, X.rh_clear, X.dga,
+                           X.dgw, X.wd, g, S, in0.uptk, sg, sgavg, sa);
     }
-    PostBar pb;
-    adjoint_post_phase<PRE>(a, off, X, S0, G0, in, sg, sgavg, sa, tb, pb);
+    ClearGas<double> G = G0;
+    G.g[0] = g[0];
+    G.g[1] = g[1];
+    // ---- reverse ----
+    Aer<double> S_b;
+    ClearGas<double> G_b;
+    zero(S_b);
+    zero(G_b);
+    cell_finish_rev<PRE>(a, off, X, S, G, tb, S_b, G_b, Xb, st_b);
+    double netprod_b = 0.0;
+    SoaIn<double> in0_b = {};
+    for (int j = ns - 1; j >= 0; --j) {
+      Aer<double> Sj;
+      double gj[2];
+      sub.load(j, Sj, gj);
+      SoaIn<double> in = in0;
+      if (j > 0 && cfg.do_cond) soa_in_state<Fp64>(Sj, gj[0], in);
+      double sg = 0.0, sgavg = 0.0, sa[4] = {0, 0, 0, 0};
+      SoaState<double> st;
+      int nt = 0;
+      if (cfg.do_cond) {
+        soa_init(st, in);
+        nt = soa.record(st, dtsub);
+        sg = st.g;
+        sgavg = soa_gavg(st);
 #pragma unroll
-    for (int f = 0; f < NSTATE; ++f) st_b[f] = pb.g[f];
-    T_b = pb.T_b;
-    p_b = pb.p_b;
-    qv_b = pb.qv_b;
-    if (a.cfg.do_cond) {
-      SoaIn<double> in_b = {};
+        for (int n = 0; n < 4; ++n) sa[n] = st.a[n];
+      }
+      double sg_b = 0.0, sgavg_b = 0.0, sa_b[4] = {0, 0, 0, 0}, uptk_b[4] = {0, 0, 0, 0};
+      clear_post_sub_rev(K, cfg, dtsub, netprod, X.T, X.p, X.z, X.pblh, X.rh_clear, X.dga, X.dgw,
+                         X.wd, gj, Sj, in0.uptk, sg, sgavg, sa, S_b, G_b.g, uptk_b, netprod_b,
+                         sg_b, sgavg_b, sa_b, Xb.T, Xb.p, Xb.rh_clear);
 #pragma unroll
-      for (int n = 0; n < 4; ++n) in_b.uptk[n] = pb.uptk_b[n];
-      soa_loop_rev(in, dtfull, st, load, nt, pb.soa_g_b, pb.soa_a_b, pb.soa_gavg_b, in_b);
-      double sb2[NSTATE], Tb2, pb2, qb2;
-      adjoint_pre_phase<PRE>(a, off, X, S0, in_b, sb2, Tb2, pb2, qb2);
+      for (int n = 0; n < 4; ++n) in0_b.uptk[n] += uptk_b[n];
+      if (cfg.do_cond) {
+        SoaIn<double> in_b = {};
+        auto get = [&](int i) { return soa.get(i); };
+        soa_loop_rev(in, dtsub, st, get, nt, sg_b, sa_b, sgavg_b, in_b);
+        // the rates and g0 are the first sub-step's for all
 #pragma unroll
-      for (int f = 0; f < NSTATE; ++f) st_b[f] += sb2[f];
-      T_b += Tb2;
-      p_b += pb2;
-      qv_b += qb2;
+        for (int n = 0; n < 4; ++n) in0_b.uptk[n] += in_b.uptk[n];
+        in0_b.g0 += in_b.g0;
+        if (j > 0) {
+          soa_in_state_rev(Sj, in_b, S_b, G_b.g[0]);
+        } else {
+          in0_b.g += in_b.g;
+#pragma unroll
+          for (int n = 0; n < 4; ++n) {
+            in0_b.a[n] += in_b.a[n];
+            in0_b.aopoa[n] += in_b.aopoa[n];
+          }
+        }
+      }
     }
-    return nt;
+    clear_post_begin_rev(cfg, a.dt, netprod, netprod_b, G_b.g, G_b.g1);
+    if (cfg.do_cond)
+      clear_pre_rev(K, X.T, X.p, X.dgw, S0, in0_b, S_b, G_b.g[0], Xb.T, Xb.p);
+    clear_build_rev(a, off, X, S_b, G_b, Xb, st_b);
+  } else {
+    // overcast: no clear subarea, no SOA loop
+    Aer<double> S, S_b;
+    ClearGas<double> G, G_b;
+    zero(S_b);
+    zero(G_b);
+    cell_finish_rev<PRE>(a, off, X, S, G, tb, S_b, G_b, Xb, st_b);
   }
-  // overcast: no clear subarea, no SOA loop
-  Aer<double> S, S_b;
-  ClearGas<double> G, G_b;
-  zero(S_b);
-  zero(G_b);
-  CtxBar Xb;
-  zero(Xb);
-  cell_finish_rev<PRE>(a, off, X, S, G, tb, S_b, G_b, Xb, st_b);
   cell_setup_rev<PRE>(a, off, X, Xb, st_b, T_b, p_b, qv_b);
-  return 0;
+  return niter;
 }
 
 }  // namespace rev

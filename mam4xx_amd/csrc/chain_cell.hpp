@@ -245,7 +245,7 @@ __device__ __forceinline__ void clear_build(const ChainArgs &a, long off, const 
 // Cloudy subarea (renaming only), recombination, calcsize and the writes.
 // S/G: the clear subarea after clear_post (unused for overcast cells).  The
 // grid-cell sums, the calcsize step and the tendencies are in the wide type.
-template <bool PRE, class P>
+template <bool PRE, class P, bool MULTI = false>
 __device__ __forceinline__ void cell_finish(const ChainArgs &a, long off, const CellCtx<P> &X,
                                             const Aer<typename P::R> &S,
                                             const ClearGas<typename P::W> &G) {
@@ -302,20 +302,24 @@ __device__ __forceinline__ void cell_finish(const ChainArgs &a, long off, const 
           const R qc2 = ld2<PRE, P>(a, off, f_qc(m, s), to_vmr_mass(m, s));
           const R a2 = (X.cat == 2 ? q2 * X.fc_q[m] : q2) * fcvt_aer(sp);
           const R c2 = (X.cat == 2 ? qc2 / X.fcldy : qc2) * fcvt_aer(sp);
-          gi[sp] = (C.q[sp][m] - a2) / 1;
-          gc[sp] = (Cc.q[sp][m] - c2) / 1;
+          gi[sp] = (C.q[sp][m] - a2) / (MULTI ? nsub(a.cfg) : 1);
+          gc[sp] = (Cc.q[sp][m] - c2) / (MULTI ? nsub(a.cfg) : 1);
         }
       }
     }
+    // one renaming per sub-step, each with the sub-step's share of the
+    // cloud-chemistry growth (mam4_amicphys.hpp:1446-1455)
     if (a.cfg.do_rename) {
-      if constexpr (P::diag) {
-        const Aer<R> C0 = C, Cc0 = Cc;
-        rename_ait_to_acc<P>(K, true, C, gi, Cc, gc);
-        const ProcSink D = proc_sink(a, off, X.fcldy);
-        D.add_aer(1, C, C0);
-        D.add_aer(4, Cc, Cc0);
-      } else {
-        rename_ait_to_acc<P>(K, true, C, gi, Cc, gc);
+      for (int j = 0; j < (MULTI ? nsub(a.cfg) : 1); ++j) {
+        if constexpr (P::diag) {
+          const Aer<R> C0 = C, Cc0 = Cc;
+          rename_ait_to_acc<P>(K, true, C, gi, Cc, gc);
+          const ProcSink D = proc_sink(a, off, X.fcldy);
+          D.add_aer(1, C, C0);
+          D.add_aer(4, Cc, Cc0);
+        } else {
+          rename_ait_to_acc<P>(K, true, C, gi, Cc, gc);
+        }
       }
     }
     const double w = X.fcldy;
@@ -420,7 +424,10 @@ __device__ __forceinline__ void cell_finish(const ChainArgs &a, long off, const 
 }
 
 // The whole step for one cell in one lane.  Returns the SOA sub-step count.
-template <bool PRE, class P>
+// MULTI: ntsubstep > 1 (a separate instantiation: the sub-step loop keeps
+// the subarea's state live across sub-steps, which costs the one-sub-step
+// kernel registers it does not need).
+template <bool PRE, class P, bool MULTI = false>
 __device__ __forceinline__ int chain_cell_fused(const ChainArgs &a, long off, int icol) {
   using W = typename P::W;
   const ChainConsts &K = *a.K;
@@ -438,24 +445,55 @@ __device__ __forceinline__ int chain_cell_fused(const ChainArgs &a, long off, in
     clear_build<P>(a, off, X, S, G);
     SoaIn<W> in = {};
     W sg = 0.0, sgavg = 0.0, sa[4] = {0, 0, 0, 0};
-    if (a.cfg.do_cond) {
-      clear_pre<P>(K, X.T, X.p, X.dgw, S, G.g[0], in);
-      SoaState<W> st;
-      soa_init(st, in);
-      const double dtfull = (double)(int)(a.dt / 1);
-      while (soa_tick(st, dtfull)) {
-      }
-      niter = st.niter;
-      sg = st.g;
-      sgavg = soa_gavg(st);
+    if constexpr (!MULTI) {
+      if (a.cfg.do_cond) {
+        clear_pre<P>(K, X.T, X.p, X.dgw, S, G.g[0], in);
+        SoaState<W> st;
+        soa_init(st, in);
+        const double dtfull = (double)(int)(a.dt / 1);
+        while (soa_tick(st, dtfull)) {
+        }
+        niter = st.niter;
+        sg = st.g;
+        sgavg = soa_gavg(st);
 #pragma unroll
-      for (int n = 0; n < 4; ++n) sa[n] = st.a[n];
+        for (int n = 0; n < 4; ++n) sa[n] = st.a[n];
+      }
+      clear_post<P>(K, a.cfg, a.dt, X.T, X.p, X.z, X.pblh, X.rh_clear, X.dga, X.dgw, X.wd, G.g1,
+                    G.g, S, in.uptk, sg, sgavg, sa, &D);
+    } else {
+      if (a.cfg.do_cond) clear_pre<P>(K, X.T, X.p, X.dgw, S, G.g[0], in);
+      const W netprod = clear_post_begin(a.cfg, a.dt, G.g1, G.g);
+      const int ns = nsub(a.cfg);
+      const double dtsub = dt_sub(a.cfg, a.dt);
+      for (int j = 0; j < ns; ++j) {
+        if (a.cfg.do_cond) {
+          if (j > 0) soa_in_state<P>(S, G.g[0], in);
+          SoaState<W> st;
+          soa_init(st, in);
+          while (soa_tick(st, dtsub)) {
+          }
+          niter += st.niter;
+          sg = st.g;
+          sgavg = soa_gavg(st);
+#pragma unroll
+          for (int n = 0; n < 4; ++n) sa[n] = st.a[n];
+        }
+        clear_post_sub<P>(K, a.cfg, dtsub, netprod, X.T, X.p, X.z, X.pblh, X.rh_clear, X.dga,
+                          X.dgw, X.wd, G.g, S, in.uptk, sg, sgavg, sa, &D);
+      }
     }
-    clear_post<P>(K, a.cfg, a.dt, X.T, X.p, X.z, X.pblh, X.rh_clear, X.dga, X.dgw, X.wd, G.g1,
-                  G.g, S, in.uptk, sg, sgavg, sa, &D);
   }
-  cell_finish<PRE, P>(a, off, X, S, G);
+  cell_finish<PRE, P, MULTI>(a, off, X, S, G);
   return niter;
+}
+
+// chain_cell_fused with the instantiation the configuration needs (host
+// builds; the kernels choose it at launch).
+template <bool PRE, class P>
+__device__ __forceinline__ int chain_cell_step(const ChainArgs &a, long off, int icol) {
+  return nsub(a.cfg) > 1 ? chain_cell_fused<PRE, P, true>(a, off, icol)
+                         : chain_cell_fused<PRE, P, false>(a, off, icol);
 }
 
 }  // namespace dev

@@ -55,7 +55,17 @@ struct Cfg {
   int do_cond, do_rename, do_newnuc, do_coag;
   int gaexch_h2so4_uptake_optaa;
   unsigned nmono;
+  int ntsubstep;  // condensation sub-steps per step; 0 or 1: one (the reference's)
 };
+
+// Sub-steps of the subarea chain per step (mam4_amicphys.hpp:1288 fixes 1)
+// and their length, an int (:1319).
+__host__ __device__ __forceinline__ int nsub(const Cfg &c) {
+  return c.ntsubstep > 1 ? c.ntsubstep : 1;
+}
+__host__ __device__ __forceinline__ double dt_sub(const Cfg &c, double deltat) {
+  return (double)(int)(deltat / nsub(c));
+}
 
 // Per-process tendency diagnostics of one cell: the gas_aero_exchange_*
 // arrays of modal_aero_amicphys_intr (mam4_amicphys.hpp:1940-1986,
@@ -842,8 +852,28 @@ __device__ __forceinline__ void age_pcarbon(const ChainConsts &K, unsigned nmono
 // mam_amicphys_1subarea (mam4_amicphys.hpp:1126-1694) for a clear subarea,
 // split around the SOA loop.
 //
+// The SOA loop's inputs taken from the subarea's state: SOAG, SOA per mode
+// and the oxygenated POM (the rest of SoaIn stays).
+template <class P>
+__device__ __forceinline__ void soa_in_state(const Aer<typename P::R> &S, typename P::W g_soag,
+                                             SoaIn<typename P::W> &o) {
+  using W = typename P::W;
+  o.g = g_soag;
+#pragma unroll
+  for (int n = 0; n < 4; ++n) o.a[n] = num::to<W>(S.q[A_SOA][n]);
+  // opoa_frac = 0.1, 0 for the pcarbon mode (gasaerexch_soaexch.hpp:123-138)
+  o.aopoa[NACC] = num::to<W>(0.1 * kmax(S.q[A_POM][NACC], 0.0));
+  o.aopoa[NAIT] = 0.1 * kmax(0.0, 0.0);
+  o.aopoa[NCOA] = num::to<W>(0.1 * kmax(S.q[A_POM][NCOA], 0.0));
+  o.aopoa[NPCA] = num::to<W>(0.0 * kmax(S.q[A_POM][NPCA], 0.0));
+}
+
 // clear_pre: the part of mam_gasaerexch_1subarea before the SOA loop
-// (gasaerexch.hpp:387-477): H2SO4 uptake rates and the loop's inputs.
+// (gasaerexch.hpp:387-477): H2SO4 uptake rates and the loop's inputs.  With
+// more than one sub-step the rates are those of the first, held for the
+// others (the Fortran original's scope; the C++ reference leaves them
+// undefined after the first, mam4_amicphys.hpp:1377, gasaerexch.hpp:428),
+// and soa_in_state refreshes the rest of the inputs before each later one.
 template <class P>
 __device__ __forceinline__ void clear_pre(const ChainConsts &K, typename P::R temp,
                                           typename P::R pmid, const typename P::R dgn_awet[4],
@@ -854,43 +884,43 @@ __device__ __forceinline__ void clear_pre(const ChainConsts &K, typename P::R te
   const R aircon = pmid / (8314.46759100000 * temp);
   R uptk[4];
   h2so4_uptake<P>(K, temp, pmid, aircon, dgn_awet, S.n, uptk);
-  o.g = g_soag;
 #pragma unroll
-  for (int n = 0; n < 4; ++n) {
-    o.uptk[n] = num::to<W>(uptk[n]);
-    o.a[n] = num::to<W>(S.q[A_SOA][n]);
-  }
-  // opoa_frac = 0.1, 0 for the pcarbon mode (gasaerexch_soaexch.hpp:123-138)
-  o.aopoa[NACC] = num::to<W>(0.1 * kmax(S.q[A_POM][NACC], 0.0));
-  o.aopoa[NAIT] = 0.1 * kmax(0.0, 0.0);
-  o.aopoa[NCOA] = num::to<W>(0.1 * kmax(S.q[A_POM][NCOA], 0.0));
-  o.aopoa[NPCA] = num::to<W>(0.0 * kmax(S.q[A_POM][NPCA], 0.0));
+  for (int n = 0; n < 4; ++n) o.uptk[n] = num::to<W>(uptk[n]);
+  soa_in_state<P>(S, g_soag, o);
   o.g0 = num::to<W>(soa_g0(K, temp, pmid));
 }
 
-// clear_post: everything after the SOA loop -- the H2SO4 update, renaming,
-// nucleation, coagulation and aging.  uptk: H2SO4 uptake rates from
-// clear_pre; soa_g/soa_gavg/soa_a: SOA loop results (ignored without
-// condensation).  g1/g: gases {SOAG, H2SO4} before gas chemistry / current.
-// D: the diagnostics sink (P::diag only), with the clear area fraction set.
-template <class P>
-__device__ __forceinline__ void clear_post(
-    const ChainConsts &K, const Cfg &cfg, double deltat, typename P::R temp, typename P::R pmid,
-    typename P::R zmid, typename P::R pblh, typename P::R relhum,
-    const typename P::R dgn_a[4], const typename P::R dgn_awet[4],
-    const typename P::R wetdens[4], const typename P::W g1[2], typename P::W g[2],
-    Aer<typename P::R> &S, const typename P::W uptk[4], typename P::W soa_g,
-    typename P::W soa_gavg, const typename P::W soa_a[4], const ProcSink *D = nullptr) {
-  using R = typename P::R;
-  using W = typename P::W;
-  const R aircon = pmid / (8314.46759100000 * temp);
+// Before the sub-steps (mam4_amicphys.hpp:1279-1286): with simultaneous
+// H2SO4 uptake, the net production by other processes over the step, and
+// H2SO4 restarts from its value before gas chemistry.
+template <class W>
+__device__ __forceinline__ W clear_post_begin(const Cfg &cfg, double deltat, const W g1[2],
+                                              W g[2]) {
   W netprod = 0.0;
   if (cfg.do_cond && cfg.gaexch_h2so4_uptake_optaa == 2) {
     netprod = (g[1] - g1[1]) / deltat;
     g[1] = (netprod >= 0) ? g1[1] : W(0);  // QUIRK (:1283)
   }
+  return netprod;
+}
+
+// One sub-step of everything after the SOA loop -- the H2SO4 update,
+// renaming, nucleation, coagulation and aging -- over dtsub.  uptk: H2SO4
+// uptake rates from clear_pre; soa_g/soa_gavg/soa_a: the sub-step's SOA loop
+// results (ignored without condensation).  g: gases {SOAG, H2SO4}, current.
+// D: the diagnostics sink (P::diag only), with the clear area fraction set.
+template <class P>
+__device__ __forceinline__ void clear_post_sub(
+    const ChainConsts &K, const Cfg &cfg, double dtsub, typename P::W netprod,
+    typename P::R temp, typename P::R pmid, typename P::R zmid, typename P::R pblh,
+    typename P::R relhum, const typename P::R dgn_a[4], const typename P::R dgn_awet[4],
+    const typename P::R wetdens[4], typename P::W g[2], Aer<typename P::R> &S,
+    const typename P::W uptk[4], typename P::W soa_g, typename P::W soa_gavg,
+    const typename P::W soa_a[4], const ProcSink *D = nullptr) {
+  using R = typename P::R;
+  using W = typename P::W;
+  const R aircon = pmid / (8314.46759100000 * temp);
   [[maybe_unused]] const W gsv[2] = {g[0], g[1]};  // the gases condensation starts from
-  const double dtsub = (double)(int)(deltat / 1);  // QUIRK (:1319): int sub-step
   W uptkrate_h2so4 = 0;
   W gavg[2];
   R dc_so4[4] = {0, 0, 0, 0}, dc_soa[4] = {0, 0, 0, 0};
@@ -990,6 +1020,21 @@ __device__ __forceinline__ void clear_post(
   } else {
     age_pcarbon(K, cfg.nmono, dgn_a[NPCA], S, dc_so4[NPCA], dc_soa[NPCA], cin_so4, cin_soa);
   }
+}
+
+// clear_post: the whole of it for one sub-step over deltat (the reference's
+// ntsubstep = 1).  g1/g: gases {SOAG, H2SO4} before gas chemistry / current.
+template <class P>
+__device__ __forceinline__ void clear_post(
+    const ChainConsts &K, const Cfg &cfg, double deltat, typename P::R temp, typename P::R pmid,
+    typename P::R zmid, typename P::R pblh, typename P::R relhum,
+    const typename P::R dgn_a[4], const typename P::R dgn_awet[4],
+    const typename P::R wetdens[4], const typename P::W g1[2], typename P::W g[2],
+    Aer<typename P::R> &S, const typename P::W uptk[4], typename P::W soa_g,
+    typename P::W soa_gavg, const typename P::W soa_a[4], const ProcSink *D = nullptr) {
+  const typename P::W netprod = clear_post_begin(cfg, deltat, g1, g);
+  clear_post_sub<P>(K, cfg, (double)(int)(deltat / 1), netprod, temp, pmid, zmid, pblh, relhum,
+                    dgn_a, dgn_awet, wetdens, g, S, uptk, soa_g, soa_gavg, soa_a, D);
 }
 
 // ------------------------------------------------------------------ calcsize

@@ -187,7 +187,7 @@ __device__ __forceinline__ void add_stats(unsigned long long *stats, unsigned lo
   if ((threadIdx.x & 63) == 0 && v) atomicAdd(stats, v);
 }
 
-template <bool PRE, class P, class Src>
+template <bool PRE, class P, class Src, bool MULTI>
 __global__ void __launch_bounds__(kBlock) chain_fused_kernel(Src src, Work w) {
   long c = (long)blockIdx.x * kBlock + threadIdx.x;
   const ChainArgs &a = src.scene(c);
@@ -197,7 +197,7 @@ __global__ void __launch_bounds__(kBlock) chain_fused_kernel(Src src, Work w) {
     int icol;
     long off;
     cell_index(a, c, icol, off);
-    niter = (unsigned long long)dev::chain_cell_fused<PRE, P>(a, off, icol);
+    niter = (unsigned long long)dev::chain_cell_fused<PRE, P, MULTI>(a, off, icol);
   }
   add_stats(w.stats, niter);
 }
@@ -763,7 +763,13 @@ __global__ void __launch_bounds__(kBlock, MAM4X_ADJ_PRE_WAVES) adj_pre_kernel(Ch
   store_grad(o, off, sb, Tb, pB, qb);
 }
 
-// Cells without a SOA loop: the whole sweep in one lane.
+// A SOA loop tape for cells that run none (never called).
+struct NoSoaTape {
+  __device__ int record(dev::SoaState<double> &, double) { return 0; }
+  __device__ dev::rev::SoaCkpt get(int) { return dev::rev::SoaCkpt{}; }
+};
+
+// Cells without a SOA loop (one sub-step): the whole sweep in one lane.
 template <bool PRE>
 __global__ void __launch_bounds__(kBlock) chain_adjoint_kernel(ChainArgs a, AdjArgs o) {
   const long c = (long)blockIdx.x * kBlock + threadIdx.x;
@@ -778,10 +784,62 @@ __global__ void __launch_bounds__(kBlock) chain_adjoint_kernel(ChainArgs a, AdjA
 #pragma unroll
   for (int f = 0; f < NSTATE; ++f) sb[f] = 0.0;
   double Tb = 0.0, pb = 0.0, qb = 0.0;
-  // no SOA loop runs here, so no record is saved or read
-  auto save = [](int, const dev::rev::SoaCkpt &) {};
-  auto load = [](int) { return dev::rev::SoaCkpt{}; };
-  dev::rev::adjoint_cell<PRE>(a, off, icol, tb, save, load, sb, Tb, pb, qb);
+  NoSoaTape soa;
+  dev::rev::OneSubTape sub;
+  dev::rev::adjoint_cell<PRE>(a, off, icol, tb, soa, sub, sb, Tb, pb, qb);
+  store_grad(o, off, sb, Tb, pb, qb);
+}
+
+// The subarea's aerosol and gases at the start of each sub-step, in HBM:
+// kSubVals planes per sub-step, one double per cell.
+constexpr int kSubVals = 4 + 28 + 2;
+struct PlaneSubTape {
+  double *base;
+  long n, c;
+  __device__ void save(int j, const dev::Aer<double> &S, const double g[2]) {
+    double *q = base + (long)j * kSubVals * n + c;
+#pragma unroll
+    for (int m = 0; m < 4; ++m) q[m * n] = S.n[m];
+#pragma unroll
+    for (int sp = 0; sp < 7; ++sp)
+#pragma unroll
+      for (int m = 0; m < 4; ++m) q[(4 + 4 * sp + m) * n] = S.q[sp][m];
+    q[32 * n] = g[0];
+    q[33 * n] = g[1];
+  }
+  __device__ void load(int j, dev::Aer<double> &S, double g[2]) const {
+    const double *q = base + (long)j * kSubVals * n + c;
+#pragma unroll
+    for (int m = 0; m < 4; ++m) S.n[m] = q[m * n];
+#pragma unroll
+    for (int sp = 0; sp < 7; ++sp)
+#pragma unroll
+      for (int m = 0; m < 4; ++m) S.q[sp][m] = q[(4 + 4 * sp + m) * n];
+    g[0] = q[32 * n];
+    g[1] = q[33 * n];
+  }
+};
+
+// ntsubstep > 1: every cell's whole sweep in one lane, a checkpoint per
+// sub-step in HBM (subws: ntsubstep x kSubVals planes) and each sub-step's
+// SOA loop recomputed inside it from fixed per-lane storage (SoaTape2).
+template <bool PRE>
+__global__ void __launch_bounds__(kBlock) chain_adjoint_sub_kernel(ChainArgs a, AdjArgs o,
+                                                                   double *subws) {
+  const long c = (long)blockIdx.x * kBlock + threadIdx.x;
+  const long ncell = (long)a.ncol * a.nlev;
+  if (c >= ncell) return;
+  int icol;
+  long off;
+  cell_index(a, c, icol, off);
+  double tb[NSTATE], sb[NSTATE];
+  load_tbar(a, off, tb);
+#pragma unroll
+  for (int f = 0; f < NSTATE; ++f) sb[f] = 0.0;
+  double Tb = 0.0, pb = 0.0, qb = 0.0;
+  dev::rev::SoaTape2<32> soa;
+  PlaneSubTape sub{subws, ncell, c};
+  dev::rev::adjoint_cell<PRE>(a, off, icol, tb, soa, sub, sb, Tb, pb, qb);
   store_grad(o, off, sb, Tb, pb, qb);
 }
 
@@ -1330,17 +1388,28 @@ __global__ void __launch_bounds__(kBlock) setext_kernel(SetextArgs a) {
 enum { kPre = 0, kPost = 1, kFused = 2 };
 
 template <class P, class Src>
-static void launch_fused(bool pre, unsigned grid, hipStream_t s, const Src &src, const Work &w) {
-  if (pre)
-    hipLaunchKernelGGL((chain_fused_kernel<true, P, Src>), dim3(grid), dim3(kBlock), 0, s, src, w);
-  else
-    hipLaunchKernelGGL((chain_fused_kernel<false, P, Src>), dim3(grid), dim3(kBlock), 0, s, src, w);
+static void launch_fused(bool pre, bool multi, unsigned grid, hipStream_t s, const Src &src,
+                         const Work &w) {
+  if (multi) {  // ntsubstep > 1
+    if (pre)
+      hipLaunchKernelGGL((chain_fused_kernel<true, P, Src, true>), dim3(grid), dim3(kBlock), 0, s,
+                         src, w);
+    else
+      hipLaunchKernelGGL((chain_fused_kernel<false, P, Src, true>), dim3(grid), dim3(kBlock), 0,
+                         s, src, w);
+  } else if (pre) {
+    hipLaunchKernelGGL((chain_fused_kernel<true, P, Src, false>), dim3(grid), dim3(kBlock), 0, s,
+                       src, w);
+  } else {
+    hipLaunchKernelGGL((chain_fused_kernel<false, P, Src, false>), dim3(grid), dim3(kBlock), 0,
+                       s, src, w);
+  }
 }
 
 // launches one of the per-cell kernels for policy P (FP64 wide type)
 template <class P, class Src>
 static void launch_cells(int kind, bool pre, unsigned grid, hipStream_t s, const Src &src,
-                         const Work &w) {
+                         const Work &w, bool multi = false) {
   if (kind == kPre) {
     if (pre)
       hipLaunchKernelGGL((chain_pre_kernel<true, P, Src>), dim3(grid), dim3(kBlock), 0, s, src, w);
@@ -1352,7 +1421,7 @@ static void launch_cells(int kind, bool pre, unsigned grid, hipStream_t s, const
     else
       hipLaunchKernelGGL((chain_post_kernel<false, P, Src>), dim3(grid), dim3(kBlock), 0, s, src, w);
   } else {
-    launch_fused<P>(pre, grid, s, src, w);
+    launch_fused<P>(pre, multi, grid, s, src, w);
   }
 }
 
@@ -1415,6 +1484,8 @@ struct mam4x_handle_s {
   // end and the pool counter (grown on demand)
   double *adjws = nullptr;
   long adj_cap = 0;
+  double *subws = nullptr;  // ntsubstep > 1: the sub-step checkpoints
+  size_t sub_cap = 0;       // doubles
   double *log_rec = nullptr;
   int *log_tag = nullptr, *log_prev = nullptr;
   int2 *log_wend = nullptr;
@@ -1534,6 +1605,7 @@ static void fill_args(mam4x_handle h, const mam4x_inputs *in, ChainArgs &a) {
   // follows newnuc_h2so4_conc_optaa.
   a.cfg.gaexch_h2so4_uptake_optaa = c.newnuc_h2so4_conc_optaa;
   a.cfg.nmono = c.n_so4_monolayers_pcage;
+  a.cfg.ntsubstep = c.ntsubstep;
   a.do_calcsize = c.do_calcsize;
   a.do_xfer = c.calcsize_do_aitacc_transfer;
   a.do_adjust = c.calcsize_do_adjust;
@@ -1558,6 +1630,7 @@ int mam4x_default_config(mam4x_config *out) {
   out->precision = MAM4X_PREC_FP64;
   out->schedule = MAM4X_SCHED_AUTO;
   out->soa_blocks_per_cu = 0;
+  out->ntsubstep = 1;
   return MAM4X_OK;
 }
 
@@ -1572,7 +1645,8 @@ int mam4x_create(const mam4x_config *config, mam4x_handle *out) {
     mam4x_default_config(&h->cfg);
   if ((h->cfg.precision != MAM4X_PREC_FP64 && h->cfg.precision != MAM4X_PREC_MIXED) ||
       h->cfg.schedule < MAM4X_SCHED_AUTO || h->cfg.schedule > MAM4X_SCHED_QUEUE ||
-      h->cfg.soa_blocks_per_cu < 0) {
+      h->cfg.soa_blocks_per_cu < 0 || h->cfg.ntsubstep < 0 ||
+      h->cfg.ntsubstep > MAM4X_MAX_NTSUBSTEP) {
     delete h;
     return MAM4X_ERR_ARG;
   }
@@ -1623,7 +1697,7 @@ int mam4x_destroy(mam4x_handle h) {
   if (h->d_count) (void)hipFree(h->d_count);
   if (h->d_stats) (void)hipFree(h->d_stats);
   if (h->ctl) (void)hipFree(h->ctl);
-  for (void *q : {(void *)h->adjws, (void *)h->log_rec, (void *)h->log_tag,
+  for (void *q : {(void *)h->adjws, (void *)h->subws, (void *)h->log_rec, (void *)h->log_tag,
                   (void *)h->log_prev, (void *)h->log_wend, (void *)h->log_next})
     if (q) (void)hipFree(q);
   free_work(h);
@@ -1792,7 +1866,7 @@ int mam4x_compute_tangent(mam4x_handle h, double t, double dt, const mam4x_input
   std::memset(&w, 0, sizeof(w));
   w.ncell = ncell;
   if (h->stats) w.stats = h->d_stats;
-  launch_fused<Tangent>(has_pre(in), grid, s, OneDomain{a}, w);
+  launch_fused<Tangent>(has_pre(in), dev::nsub(a.cfg) > 1, grid, s, OneDomain{a}, w);
   HIP_TRY(h, hipGetLastError());
   return MAM4X_OK;
 }
@@ -1868,6 +1942,28 @@ int mam4x_compute_adjoint(mam4x_handle h, double t, double dt, const mam4x_input
   o.Tbar = adj->temperature;
   o.pbar = adj->pressure;
   o.qvbar = adj->vapor_mixing_ratio;
+  if (dev::nsub(a.cfg) > 1) {
+    const size_t need = (size_t)dev::nsub(a.cfg) * kSubVals * (size_t)ncell;
+    if (need > h->sub_cap) {
+      if (h->sub_cap) HIP_TRY(h, hipStreamSynchronize(s));
+      if (h->subws) (void)hipFree(h->subws);
+      h->subws = nullptr;
+      h->sub_cap = 0;
+      if (hipMalloc(&h->subws, sizeof(double) * need) != hipSuccess) {
+        (void)hipGetLastError();
+        return fail(h, MAM4X_ERR_NOMEM, "adjoint sub-step checkpoint allocation failed");
+      }
+      h->sub_cap = need;
+    }
+    if (pre)
+      hipLaunchKernelGGL((chain_adjoint_sub_kernel<true>), dim3(grid), dim3(kBlock), 0, s, a, o,
+                         h->subws);
+    else
+      hipLaunchKernelGGL((chain_adjoint_sub_kernel<false>), dim3(grid), dim3(kBlock), 0, s, a,
+                         o, h->subws);
+    HIP_TRY(h, hipGetLastError());
+    return MAM4X_OK;
+  }
   if (a.cfg.do_cond) {
     rc = reserve_work(h, ncell, s);
     if (rc) return rc;
@@ -2082,7 +2178,8 @@ static int compute_step(mam4x_handle h, double dt, const mam4x_inputs *in,
   if (ncell == 0) return MAM4X_OK;
   const unsigned grid = (unsigned)((ncell + kBlock - 1) / kBlock);
   bool fused = h->cfg.schedule == MAM4X_SCHED_FUSED ||
-               (h->cfg.schedule == MAM4X_SCHED_AUTO && ncell < kFusedBelow);
+               (h->cfg.schedule == MAM4X_SCHED_AUTO && ncell < kFusedBelow) ||
+               h->cfg.ntsubstep > 1;  // the split phases hold one sub-step
   Work w;
   std::memset(&w, 0, sizeof(w));
   w.ncell = ncell;
@@ -2129,12 +2226,13 @@ static int compute_step(mam4x_handle h, double dt, const mam4x_inputs *in,
     hipLaunchKernelGGL(soa_queue_kernel, dim3(h->soa_grid), dim3(kBlock), 0, s, w, w.stats);
   if (ev) HIP_TRY(h, hipEventRecord(ev[2], s));
   const int kind = fused ? kFused : kPost;
+  const bool multi = dev::nsub(a.cfg) > 1;
   if (mixed)
-    launch_cells<Mixed>(kind, pre, grid, s, src, w);
+    launch_cells<Mixed>(kind, pre, grid, s, src, w, multi);
   else if (proc)  // the same step, also forming the per-process diagnostics
-    launch_cells<Fp64Diag>(kind, pre, grid, s, src, w);
+    launch_cells<Fp64Diag>(kind, pre, grid, s, src, w, multi);
   else
-    launch_cells<Fp64>(kind, pre, grid, s, src, w);
+    launch_cells<Fp64>(kind, pre, grid, s, src, w, multi);
   HIP_TRY(h, hipGetLastError());
   if (ev) HIP_TRY(h, hipEventRecord(ev[3], s));
   return MAM4X_OK;
@@ -2549,7 +2647,8 @@ int mam4x_batch_create(mam4x_handle h, int nscene, const mam4x_inputs *in,
   b->nblock = blk0[nscene];
   b->ncell = (long)b->nblock * kBlock;
   b->fused = h->cfg.schedule == MAM4X_SCHED_FUSED ||
-             (h->cfg.schedule == MAM4X_SCHED_AUTO && b->ncell < kFusedBelow);
+             (h->cfg.schedule == MAM4X_SCHED_AUTO && b->ncell < kFusedBelow) ||
+             h->cfg.ntsubstep > 1;  // the split phases hold one sub-step
   b->queue = !b->fused && args[0].cfg.do_cond != 0 && b->ncell > 0;
   if (hipMalloc(&b->d_args, sizeof(ChainArgs) * nscene) != hipSuccess ||
       hipMalloc(&b->d_blk0, sizeof(unsigned) * (nscene + 1)) != hipSuccess ||
@@ -2602,10 +2701,11 @@ int mam4x_batch_run(mam4x_batch b, double t, double dt, void *stream) {
     hipLaunchKernelGGL(soa_queue_kernel, dim3(h->soa_grid), dim3(kBlock), 0, s, w, w.stats);
   }
   const int kind = fused ? kFused : kPost;
+  const bool multi = h->cfg.ntsubstep > 1;
   if (mixed)
-    launch_cells<Mixed>(kind, b->pre, b->nblock, s, src, w);
+    launch_cells<Mixed>(kind, b->pre, b->nblock, s, src, w, multi);
   else
-    launch_cells<Fp64>(kind, b->pre, b->nblock, s, src, w);
+    launch_cells<Fp64>(kind, b->pre, b->nblock, s, src, w, multi);
   HIP_TRY(h, hipGetLastError());
   return MAM4X_OK;
 }

@@ -1520,7 +1520,7 @@ static void mam_rename_1subarea(int is_cloudy, double smallest_dryvol,
 /* mam4_amicphys.hpp:1126-1694 (ntsubstep = 1).  delaa arrays [x][4]. */
 static int mam_amicphys_1subarea(
     int gaexch_h2so4_uptake_optaa, int do_cond, int do_rename, int do_newnuc,
-    int do_coag, double deltat, int iscldy, unsigned nmono, double temp,
+    int do_coag, double deltat, int ntsubstep, int iscldy, unsigned nmono, double temp,
     double pmid, double zmid, double pblh, double relhum, const double dgn_a[4],
     const double dgn_awet[4], const double wetdens[4], const double qgas1[2],
     const double qgas3[2], double qgas_cur[2], double qgas_delaa[2][4],
@@ -1557,8 +1557,13 @@ static int mam_amicphys_1subarea(
   memset(qaer_delaa, 0, sizeof(double) * 7 * 4 * 4);
   memset(qnumcw_delaa, 0, sizeof(double) * 4);
   memset(qaercw_delaa, 0, sizeof(double) * 7 * 4);
-  /* QUIRK (:1319): the sub-step length is an int. */
-  const int dtsubstep = (int)(deltat / 1);
+  /* QUIRK (:1319): the sub-step length is an int.  The reference fixes
+   * ntsubstep = 1 (:1288); for more, the loop below runs the whole subarea
+   * chain ntsubstep times, with the H2SO4 uptake rates and their sum from the
+   * first sub-step held for the later ones (the Fortran original's scope of
+   * uptkaer / uptkrate_h2so4; the C++ declares both inside the loop, :1344,
+   * :1377, and computes them only at jtsubstep == 0, gasaerexch.hpp:428). */
+  const int dtsubstep = (int)(deltat / ntsubstep);
   double qgas_sv1[2], qnum_sv1[4], qaer_sv1[7][4];
   double del_h2so4_aeruptk;
   double qgas_avg[2];
@@ -1571,16 +1576,16 @@ static int mam_amicphys_1subarea(
   memset(qaer_delsub_grow4rnam, 0, sizeof(qaer_delsub_grow4rnam));
   memset(qaercw_delsub_grow4rnam, 0, sizeof(qaercw_delsub_grow4rnam));
   memset(qaer_delsub_coag_in, 0, sizeof(qaer_delsub_coag_in));
-  {
-    double uptkrate_h2so4 = 0;
+  double uptkrate_h2so4 = 0;
+  double uptkaer[2][4];
+  for (int jtsubstep = 0; jtsubstep < ntsubstep; ++jtsubstep) {
     if (do_cond) {
       for (int g = 0; g < 2; ++g) qgas_sv1[g] = qgas_cur[g];
       for (int m = 0; m < 4; ++m) qnum_sv1[m] = qnum_cur[m];
       memcpy(qaer_sv1, qaer_cur, sizeof(qaer_sv1));
-      double uptkaer[2][4];
-      niter = mam_gasaerexch_1subarea(0, (double)dtsubstep, temp, pmid, aircon,
-                                      qgas_cur, qgas_avg, netprod, qaer_cur,
-                                      qnum_cur, dgn_awet, uptkaer, &uptkrate_h2so4);
+      niter += mam_gasaerexch_1subarea(jtsubstep, (double)dtsubstep, temp, pmid, aircon,
+                                       qgas_cur, qgas_avg, netprod, qaer_cur,
+                                       qnum_cur, dgn_awet, uptkaer, &uptkrate_h2so4);
       for (int g = 0; g < 2; ++g)
         qgas_delaa[g][0] = qgas_delaa[g][0] +
                            (qgas_cur[g] - (qgas_sv1[g] + netprod[g] * dtsubstep));
@@ -1611,8 +1616,8 @@ static int mam_amicphys_1subarea(
         for (int s = 0; s < 7; ++s)
           for (int m = 0; m < 4; ++m) {
             qaer_delsub_grow4rnam[s][m] =
-                (qaer3[s][m] - qaer2[s][m]) / 1 + qaer_delsub_grow4rnam[s][m];
-            qaercw_delsub_grow4rnam[s][m] = (qaercw3[s][m] - qaercw2[s][m]) / 1;
+                (qaer3[s][m] - qaer2[s][m]) / ntsubstep + qaer_delsub_grow4rnam[s][m];
+            qaercw_delsub_grow4rnam[s][m] = (qaercw3[s][m] - qaercw2[s][m]) / ntsubstep;
           }
       }
       for (int m = 0; m < 4; ++m) qnum_sv1[m] = qnum_cur[m];
@@ -1729,7 +1734,7 @@ int orc_amicphys_1subarea(int gaexch_h2so4_uptake_optaa, int do_cond, int do_ren
                           double *qnum_cond_only, double *qaer_cond_only,
                           double *qnum_coag_aged, double *qaer_coag_aged) {
   return mam_amicphys_1subarea(
-      gaexch_h2so4_uptake_optaa, do_cond, do_rename, do_newnuc, do_coag, deltat, iscldy,
+      gaexch_h2so4_uptake_optaa, do_cond, do_rename, do_newnuc, do_coag, deltat, 1, iscldy,
       nmono, temp, pmid, zmid, pblh, relhum, dgn_a, dgn_awet, wetdens, qgas1, qgas3,
       qgas_cur, (double(*)[4])qgas_delaa, qnum3, qnum_cur, (double(*)[4])qnum_delaa,
       (const double(*)[4])qaer2, (const double(*)[4])qaer3, (double(*)[4])qaer_cur,
@@ -1832,7 +1837,8 @@ static int mam_amicphys_1gridcell(
      * as gaexch_h2so4_uptake_optaa matters. */
     niter += mam_amicphys_1subarea(
         cfg->newnuc_h2so4_conc_optaa, do_cond, do_rename, do_newnuc, do_coag,
-        deltat, iscldy[jsub], nmono, temp, pmid, zmid, pblh, relhumsub[jsub],
+        deltat, cfg->ntsubstep > 1 ? cfg->ntsubstep : 1, iscldy[jsub], nmono, temp, pmid,
+        zmid, pblh, relhumsub[jsub],
         dgn_a, dgn_awet, wetdens, qgas1, qgas3, qgas4, qgas_delaa, qnum3, qnum4,
         qnum_delaa, (const double(*)[4])qaer2, (const double(*)[4])qaer3, qaer4,
         qaer_delaa, qnumcw3, qnumcw4, qnumcw_delaa, (const double(*)[4])qaercw2,

@@ -36,6 +36,10 @@ typedef struct {
   int do_calcsize;
   int calcsize_do_aitacc_transfer;
   int calcsize_do_adjust;
+  /* condensation sub-steps per step (mam4_amicphys.hpp:1288 fixes 1; 0 is
+   * taken as 1): the subarea chain runs ntsubstep times with an int sub-step
+   * of deltat / ntsubstep, the H2SO4 uptake rates of the first held */
+  int ntsubstep;
 } orc_config;
 
 /* Per-cell chain driver: modal_aero_amicphys_intr (mam4_amicphys.hpp:2119).

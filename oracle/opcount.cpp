@@ -106,7 +106,7 @@ int main(int argc, char **argv) {
   for (long i = 0; i < total; ++i) buf[i] = CD::mk(raw[i], true);
   CD *st = buf.data(), *T = st + 52 * n, *p = T + n, *z = p + n, *qv = z + n, *cld = qv + n,
      *pblh = cld + n, *dga = pblh + ncol, *dgw = dga + 4 * n, *wd = dgw + 4 * n;
-  orc_config cfg = {1, 1, 1, 1, 2, 2, 8, 1, 1, 1};
+  orc_config cfg = {1, 1, 1, 1, 2, 2, 8, 1, 1, 1, 1};
   std::vector<CD> one(52 + 5 + 1 + 12), tend(52), di(4), dc(4);
   for (int k = 0; k < nlev; ++k)
     for (int i = 0; i < ncol; ++i) {

@@ -39,13 +39,14 @@ class OrcConfig(ctypes.Structure):
         ("do_calcsize", ctypes.c_int),
         ("calcsize_do_aitacc_transfer", ctypes.c_int),
         ("calcsize_do_adjust", ctypes.c_int),
+        ("ntsubstep", ctypes.c_int),
     ]
 
 
 def default_config(**kw):
     c = dict(do_cond=1, do_rename=1, do_newnuc=1, do_coag=1, gaexch_h2so4_uptake_optaa=2,
              newnuc_h2so4_conc_optaa=2, n_so4_monolayers_pcage=8, do_calcsize=1,
-             calcsize_do_aitacc_transfer=1, calcsize_do_adjust=1)
+             calcsize_do_aitacc_transfer=1, calcsize_do_adjust=1, ntsubstep=1)
     c.update(kw)
     return OrcConfig(**c)
 

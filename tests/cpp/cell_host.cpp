@@ -1,8 +1,9 @@
 // CPU build of the HIP kernel's per-cell code (mam4xx_amd/csrc/chain_cell.hpp)
 // for the test suite, in the FP64 (prec 0), mixed (prec 1) or tangent
-// (prec 2) policy, and of its reverse sweep (chain_adjoint.hpp).  It separates logic errors from libm differences: run on
-// the CPU with glibc, the kernel's restructured FP64 chain must agree with
-// the oracle to round-off.  Test infrastructure only; the library never uses
+// (prec 2) policy, and of its reverse sweep (chain_adjoint.hpp).  It
+// separates logic errors from libm differences: run on the CPU with glibc,
+// the kernel's restructured FP64 chain must agree with the oracle to
+// round-off.  Test infrastructure only; the library never uses
 // it.
 #include <cmath>
 #include <cstring>
@@ -11,6 +12,48 @@
 #include "../../mam4xx_amd/csrc/chain_adjoint.hpp"
 
 using namespace mam4x;
+
+// condensation sub-steps of the next calls (the config vectors predate it)
+static int g_ntsubstep = 1;
+extern "C" void cell_host_set_ntsubstep(int n) { g_ntsubstep = n; }
+
+// The reverse sweep's SOA states: 0 every state in a vector, 1 the GPU's
+// fixed two-level storage (dev::rev::SoaTape2<32>).
+static int g_tape = 0;
+extern "C" void cell_host_set_tape(int k) { g_tape = k; }
+
+struct VecSoaTape {
+  std::vector<dev::rev::SoaCkpt> v;
+  int record(dev::SoaState<double> &st, double dtfull) {
+    v.clear();
+    while (true) {
+      const dev::rev::SoaCkpt c = dev::rev::soa_ckpt(st);
+      if (!dev::soa_tick(st, dtfull)) break;
+      v.push_back(c);
+    }
+    return (int)v.size();
+  }
+  dev::rev::SoaCkpt get(int i) const { return v[i]; }
+};
+
+struct VecSubTape {
+  std::vector<dev::Aer<double>> S;
+  std::vector<double> g;
+  void save(int j, const dev::Aer<double> &S_, const double g_[2]) {
+    if ((int)S.size() <= j) {
+      S.resize(j + 1);
+      g.resize(2 * (j + 1));
+    }
+    S[j] = S_;
+    g[2 * j] = g_[0];
+    g[2 * j + 1] = g_[1];
+  }
+  void load(int j, dev::Aer<double> &S_, double g_[2]) const {
+    S_ = S[j];
+    g_[0] = g[2 * j];
+    g_[1] = g[2 * j + 1];
+  }
+};
 
 // Per-process tendency diagnostics (the Fp64Diag policy): proc receives
 // [5][31][nlev][ncol]; precld (NULL = none) is the optional pre-cloud-
@@ -44,14 +87,15 @@ extern "C" long cell_host_proc(const int *cfgv, int ncol, int nlev, double dt,
   a.cfg.do_cond = cfgv[0]; a.cfg.do_rename = cfgv[1]; a.cfg.do_newnuc = cfgv[2];
   a.cfg.do_coag = cfgv[3]; a.cfg.gaexch_h2so4_uptake_optaa = cfgv[4];
   a.cfg.nmono = (unsigned)cfgv[5];
+  a.cfg.ntsubstep = g_ntsubstep;
   a.do_calcsize = cfgv[6]; a.do_xfer = cfgv[7]; a.do_adjust = cfgv[8];
   a.dt = dt;
   long iters = 0;
   for (int k = 0; k < nlev; ++k)
     for (int i = 0; i < ncol; ++i) {
       const long off = (long)k * ncol + i;
-      iters += precld ? dev::chain_cell_fused<true, Fp64Diag>(a, off, i)
-                      : dev::chain_cell_fused<false, Fp64Diag>(a, off, i);
+      iters += precld ? dev::chain_cell_step<true, Fp64Diag>(a, off, i)
+                      : dev::chain_cell_step<false, Fp64Diag>(a, off, i);
     }
   return iters;
 }
@@ -91,15 +135,16 @@ extern "C" long cell_host_run3(const int *cfgv, int prec, int ncol, int nlev, do
   a.cfg.do_cond = cfgv[0]; a.cfg.do_rename = cfgv[1]; a.cfg.do_newnuc = cfgv[2];
   a.cfg.do_coag = cfgv[3]; a.cfg.gaexch_h2so4_uptake_optaa = cfgv[4];
   a.cfg.nmono = (unsigned)cfgv[5];
+  a.cfg.ntsubstep = g_ntsubstep;
   a.do_calcsize = cfgv[6]; a.do_xfer = cfgv[7]; a.do_adjust = cfgv[8];
   a.has_dgn = dgi || dgc; a.has_sout = 0; a.dt = dt;
   long iters = 0;
   for (int k = 0; k < nlev; ++k)
     for (int i = 0; i < ncol; ++i) {
       const long off = (long)k * ncol + i;
-      iters += prec == 2   ? dev::chain_cell_fused<false, Tangent>(a, off, i)
-               : prec == 1 ? dev::chain_cell_fused<false, Mixed>(a, off, i)
-                           : dev::chain_cell_fused<false, Fp64>(a, off, i);
+      iters += prec == 2   ? dev::chain_cell_step<false, Tangent>(a, off, i)
+               : prec == 1 ? dev::chain_cell_step<false, Mixed>(a, off, i)
+                           : dev::chain_cell_step<false, Fp64>(a, off, i);
     }
   return iters;
 }
@@ -160,14 +205,15 @@ extern "C" long cell_host_tangent(const int *cfgv, int ncol, int nlev, double dt
   a.cfg.do_cond = cfgv[0]; a.cfg.do_rename = cfgv[1]; a.cfg.do_newnuc = cfgv[2];
   a.cfg.do_coag = cfgv[3]; a.cfg.gaexch_h2so4_uptake_optaa = cfgv[4];
   a.cfg.nmono = (unsigned)cfgv[5];
+  a.cfg.ntsubstep = g_ntsubstep;
   a.do_calcsize = cfgv[6]; a.do_xfer = cfgv[7]; a.do_adjust = cfgv[8];
   a.dt = dt;
   long iters = 0;
   for (int k = 0; k < nlev; ++k)
     for (int i = 0; i < ncol; ++i) {
       const long off = (long)k * ncol + i;
-      iters += precld ? dev::chain_cell_fused<true, Tangent>(a, off, i)
-                      : dev::chain_cell_fused<false, Tangent>(a, off, i);
+      iters += precld ? dev::chain_cell_step<true, Tangent>(a, off, i)
+                      : dev::chain_cell_step<false, Tangent>(a, off, i);
     }
   return iters;
 }
@@ -201,14 +247,12 @@ extern "C" long cell_host_adjoint(const int *cfgv, int ncol, int nlev, double dt
   a.cfg.do_cond = cfgv[0]; a.cfg.do_rename = cfgv[1]; a.cfg.do_newnuc = cfgv[2];
   a.cfg.do_coag = cfgv[3]; a.cfg.gaexch_h2so4_uptake_optaa = cfgv[4];
   a.cfg.nmono = (unsigned)cfgv[5];
+  a.cfg.ntsubstep = g_ntsubstep;
   a.do_calcsize = cfgv[6]; a.do_xfer = cfgv[7]; a.do_adjust = cfgv[8];
   a.dt = dt;
-  std::vector<dev::rev::SoaCkpt> ck;
-  auto save = [&](int i, const dev::rev::SoaCkpt &c) {
-    if ((int)ck.size() <= i) ck.resize(i + 1);
-    ck[i] = c;
-  };
-  auto load = [&](int i) { return ck[i]; };
+  VecSoaTape vt;
+  dev::rev::SoaTape2<32> t2;
+  VecSubTape sub;
   long iters = 0;
   for (int k = 0; k < nlev; ++k)
     for (int i = 0; i < ncol; ++i) {
@@ -219,8 +263,12 @@ extern "C" long cell_host_adjoint(const int *cfgv, int ncol, int nlev, double dt
         sb[f] = 0.0;
       }
       double tB = 0.0, pB = 0.0, qB = 0.0;
-      iters += precld ? dev::rev::adjoint_cell<true>(a, off, i, tb, save, load, sb, tB, pB, qB)
-                      : dev::rev::adjoint_cell<false>(a, off, i, tb, save, load, sb, tB, pB, qB);
+      if (g_tape == 1)
+        iters += precld ? dev::rev::adjoint_cell<true>(a, off, i, tb, t2, sub, sb, tB, pB, qB)
+                        : dev::rev::adjoint_cell<false>(a, off, i, tb, t2, sub, sb, tB, pB, qB);
+      else
+        iters += precld ? dev::rev::adjoint_cell<true>(a, off, i, tb, vt, sub, sb, tB, pB, qB)
+                        : dev::rev::adjoint_cell<false>(a, off, i, tb, vt, sub, sb, tB, pB, qB);
       for (int f = 0; f < NSTATE; ++f) sbar[f * n + off] = sb[f];
       Tbar[off] = tB;
       pbar[off] = pB;
