@@ -202,6 +202,34 @@ __global__ void __launch_bounds__(kBlock) chain_fused_kernel(Src src, Work w) {
   add_stats(w.stats, niter);
 }
 
+// ntsubstep > 1 on the work queue's order: pass 0 runs the queued cells
+// longest first (by the first sub-step's predicted SOA cost), so a wave's
+// lanes run sub-step loops of similar length; pass 1 the cells without a SOA
+// loop.  Every cell's result is independent of the order.
+template <bool PRE, class P>
+__global__ void __launch_bounds__(kBlock) chain_fused_order_kernel(OneDomain src, Work w,
+                                                                   int pass) {
+  const ChainArgs &a = src.a;
+  const long i = (long)blockIdx.x * kBlock + threadIdx.x;
+  const long ncell = (long)a.ncol * a.nlev;
+  long c = -1;
+  if (pass == 0) {
+    unsigned nq = 0;
+    for (int j = 0; j < kNBucket; ++j) nq += w.ctl->hist[j];
+    if (i < (long)nq) c = w.queue[i];
+  } else if (i < ncell && w.bucket[i] == kNoSoa) {
+    c = i;
+  }
+  unsigned long long niter = 0;
+  if (c >= 0) {
+    int icol;
+    long off;
+    cell_index(a, c, icol, off);
+    niter = (unsigned long long)dev::chain_cell_fused<PRE, P, true>(a, off, icol);
+  }
+  add_stats(w.stats, niter);
+}
+
 __global__ void queue_reset_kernel(QueueCtl *q) {
   const int t = threadIdx.x;
   if (t < kNBucket) {
@@ -823,12 +851,28 @@ struct PlaneSubTape {
 // ntsubstep > 1: every cell's whole sweep in one lane, a checkpoint per
 // sub-step in HBM (subws: ntsubstep x kSubVals planes) and each sub-step's
 // SOA loop recomputed inside it from fixed per-lane storage (SoaTape2).
+// With a queue (condensation on), pass 0 takes the queued cells longest
+// first, so a wave's lanes run SOA loops of similar length, and pass 1 the
+// rest; without one, pass 1 takes every cell.
 template <bool PRE>
 __global__ void __launch_bounds__(kBlock) chain_adjoint_sub_kernel(ChainArgs a, AdjArgs o,
-                                                                   double *subws) {
-  const long c = (long)blockIdx.x * kBlock + threadIdx.x;
+                                                                   double *subws,
+                                                                   const int *queue,
+                                                                   const QueueCtl *ctl,
+                                                                   int pass) {
+  const long i = (long)blockIdx.x * kBlock + threadIdx.x;
   const long ncell = (long)a.ncol * a.nlev;
-  if (c >= ncell) return;
+  long c;
+  if (pass == 0) {
+    unsigned nq = 0;
+    for (int j = 0; j < kNBucket; ++j) nq += ctl->hist[j];
+    if (i >= (long)nq) return;
+    c = queue[i];
+  } else {
+    if (i >= ncell) return;
+    if (o.bucket && o.bucket[i] != kNoSoa) return;
+    c = i;
+  }
   int icol;
   long off;
   cell_index(a, c, icol, off);
@@ -1425,6 +1469,17 @@ static void launch_cells(int kind, bool pre, unsigned grid, hipStream_t s, const
   }
 }
 
+template <class P>
+static void launch_order(bool pre, unsigned grid, hipStream_t s, const OneDomain &src,
+                         const Work &w, int pass) {
+  if (pre)
+    hipLaunchKernelGGL((chain_fused_order_kernel<true, P>), dim3(grid), dim3(kBlock), 0, s, src,
+                       w, pass);
+  else
+    hipLaunchKernelGGL((chain_fused_order_kernel<false, P>), dim3(grid), dim3(kBlock), 0, s, src,
+                       w, pass);
+}
+
 // Device memory of the allocation-free step (mam4x_compute_tendencies_noalloc):
 // the work-queue workspace for up to kScratchCells cells -- W_NPLANE planes of
 // doubles, the queue, the cost buckets -- and the queue counters.  Every GPU the
@@ -1955,12 +2010,39 @@ int mam4x_compute_adjoint(mam4x_handle h, double t, double dt, const mam4x_input
       }
       h->sub_cap = need;
     }
+    const int *queue = nullptr;
+    const QueueCtl *ctl = nullptr;
+    if (a.cfg.do_cond) {
+      // the first sub-step's predicted SOA cost orders the cells
+      rc = reserve_work(h, ncell, s);
+      if (rc) return rc;
+      Work w;
+      std::memset(&w, 0, sizeof(w));
+      w.ncell = ncell;
+      w.ctl = h->ctl;
+      w.dtfull = dev::dt_sub(a.cfg, dt);
+      w.ws = h->ws;
+      w.bucket = h->bucket;
+      w.queue = h->queue;
+      hipLaunchKernelGGL(queue_reset_kernel, dim3(1), dim3(64), 0, s, h->ctl);
+      launch_cells<Fp64>(kPre, pre, grid, s, OneDomain{a}, w);
+      hipLaunchKernelGGL(queue_scatter_kernel, dim3(grid), dim3(kBlock), 0, s, w);
+      queue = h->queue;
+      ctl = h->ctl;
+      o.bucket = h->bucket;
+      if (pre)
+        hipLaunchKernelGGL((chain_adjoint_sub_kernel<true>), dim3(grid), dim3(kBlock), 0, s, a,
+                           o, h->subws, queue, ctl, 0);
+      else
+        hipLaunchKernelGGL((chain_adjoint_sub_kernel<false>), dim3(grid), dim3(kBlock), 0, s, a,
+                           o, h->subws, queue, ctl, 0);
+    }
     if (pre)
       hipLaunchKernelGGL((chain_adjoint_sub_kernel<true>), dim3(grid), dim3(kBlock), 0, s, a, o,
-                         h->subws);
+                         h->subws, queue, ctl, 1);
     else
       hipLaunchKernelGGL((chain_adjoint_sub_kernel<false>), dim3(grid), dim3(kBlock), 0, s, a,
-                         o, h->subws);
+                         o, h->subws, queue, ctl, 1);
     HIP_TRY(h, hipGetLastError());
     return MAM4X_OK;
   }
@@ -2178,13 +2260,15 @@ static int compute_step(mam4x_handle h, double dt, const mam4x_inputs *in,
   if (ncell == 0) return MAM4X_OK;
   const unsigned grid = (unsigned)((ncell + kBlock - 1) / kBlock);
   bool fused = h->cfg.schedule == MAM4X_SCHED_FUSED ||
-               (h->cfg.schedule == MAM4X_SCHED_AUTO && ncell < kFusedBelow) ||
-               h->cfg.ntsubstep > 1;  // the split phases hold one sub-step
+               (h->cfg.schedule == MAM4X_SCHED_AUTO && ncell < kFusedBelow);
+  // ntsubstep > 1: one lane runs a cell's whole step (the split phases hold
+  // one sub-step), in the queue's order unless the fused schedule is chosen
+  const bool multi = dev::nsub(a.cfg) > 1;
   Work w;
   std::memset(&w, 0, sizeof(w));
   w.ncell = ncell;
   w.ctl = noalloc ? reinterpret_cast<QueueCtl *>(h->scratch + kScratchCtlOff) : h->ctl;
-  w.dtfull = (double)(int)(dt / 1);  // QUIRK (mam4_amicphys.hpp:1319): int sub-step
+  w.dtfull = dev::dt_sub(a.cfg, dt);  // QUIRK (mam4_amicphys.hpp:1319): int sub-step
   if (h->stats) w.stats = h->d_stats;
   const OneDomain src{a};
   const bool queue = !fused && a.cfg.do_cond;
@@ -2222,17 +2306,27 @@ static int compute_step(mam4x_handle h, double dt, const mam4x_inputs *in,
     hipLaunchKernelGGL(queue_scatter_kernel, dim3(grid), dim3(kBlock), 0, s, w);
   }
   if (ev) HIP_TRY(h, hipEventRecord(ev[1], s));
-  if (queue)
+  if (queue && !multi)
     hipLaunchKernelGGL(soa_queue_kernel, dim3(h->soa_grid), dim3(kBlock), 0, s, w, w.stats);
   if (ev) HIP_TRY(h, hipEventRecord(ev[2], s));
-  const int kind = fused ? kFused : kPost;
-  const bool multi = dev::nsub(a.cfg) > 1;
-  if (mixed)
-    launch_cells<Mixed>(kind, pre, grid, s, src, w, multi);
-  else if (proc)  // the same step, also forming the per-process diagnostics
-    launch_cells<Fp64Diag>(kind, pre, grid, s, src, w, multi);
-  else
-    launch_cells<Fp64>(kind, pre, grid, s, src, w, multi);
+  if (multi && queue) {
+    for (int pass = 0; pass < 2; ++pass) {
+      if (mixed)
+        launch_order<Mixed>(pre, grid, s, src, w, pass);
+      else if (proc)
+        launch_order<Fp64Diag>(pre, grid, s, src, w, pass);
+      else
+        launch_order<Fp64>(pre, grid, s, src, w, pass);
+    }
+  } else {
+    const int kind = fused || multi ? kFused : kPost;
+    if (mixed)
+      launch_cells<Mixed>(kind, pre, grid, s, src, w, multi);
+    else if (proc)  // the same step, also forming the per-process diagnostics
+      launch_cells<Fp64Diag>(kind, pre, grid, s, src, w, multi);
+    else
+      launch_cells<Fp64>(kind, pre, grid, s, src, w, multi);
+  }
   HIP_TRY(h, hipGetLastError());
   if (ev) HIP_TRY(h, hipEventRecord(ev[3], s));
   return MAM4X_OK;
