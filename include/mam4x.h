@@ -178,6 +178,12 @@ typedef struct mam4x_outputs {
    * aging.  FP64 precision, single-domain calls only. */
   double *process_tendencies[MAM4X_NPROCESS];
   long long process_stride;
+  /* Optional: per cell, 1 when a SOA sub-stepping loop of the step stopped
+   * at its 1000-sub-step cap before the end of the (sub-)step, else 0 (the
+   * reference warns and goes on: gasaerexch_soaexch.hpp:93, 170-173).  One
+   * int plane in the tendencies' layout and leading dimension; NULL = not
+   * formed. */
+  int *soa_capped;
 } mam4x_outputs;
 
 /* Forward-mode tangent: the direction of the inputs and the derivative of

@@ -135,6 +135,7 @@ class Outputs(ctypes.Structure):
         ("state_out", _vp * NSTATE),
         ("process_tendencies", _vp * NPROCESS),
         ("process_stride", ctypes.c_longlong),
+        ("soa_capped", _vp),
     ]
 
 

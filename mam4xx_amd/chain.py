@@ -126,13 +126,19 @@ class Chain:
 
     def compute_tendencies(self, scene, dt, tend, dgn_i=None, dgn_c=None, state_out=None,
                            precldchem=None, pregaschem=None, t=0.0, stream=None,
-                           layout=_abi.LAYOUT_LEV_COL, process=None, noalloc=False):
+                           layout=_abi.LAYOUT_LEV_COL, process=None, noalloc=False,
+                           soa_capped=None):
         """One step (mam4x_compute_tendencies, or with noalloc the
         allocation-free mam4x_compute_tendencies_noalloc).  process: optional
         per-process tendency diagnostics, [5][31][nlev][ncol] (see
-        _process_ptrs)."""
+        _process_ptrs).  soa_capped: optional int32 [nlev][ncol], 1 where a
+        SOA loop stopped at its 1000-sub-step cap."""
         inp = self.make_inputs(scene, precldchem, pregaschem, layout)
         out = _abi.Outputs()
+        if soa_capped is not None:
+            assert soa_capped.dtype == torch.int32 and soa_capped.is_cuda
+            assert soa_capped.is_contiguous()
+            out.soa_capped = soa_capped.data_ptr()
         if process is not None:
             ptrs, out.process_stride = _process_ptrs(process)
             for k, p in enumerate(ptrs):

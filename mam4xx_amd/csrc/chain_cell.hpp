@@ -41,6 +41,8 @@ struct ChainArgs {
   // of 31 gas_pcnst planes, tracer i at proc[k] + i * pstride + off
   double *proc[5];
   long pstride;
+  // per cell: 1 when a SOA loop stopped at its sub-step cap (NULL = not formed)
+  int *soacap;
   const ChainConsts *K;
   int ncol, nlev, ldc;
   int layout;  // 0: [nlev][ldc] column fastest; 1: [ncol][ldc] level fastest
@@ -436,6 +438,7 @@ __device__ __forceinline__ int chain_cell_fused(const ChainArgs &a, long off, in
   Aer<typename P::R> S;
   ClearGas<W> G;
   int niter = 0;
+  bool capped = false;
   [[maybe_unused]] ProcSink D;
   if constexpr (P::diag) {
     D = proc_sink(a, off, X.fclea);
@@ -454,6 +457,7 @@ __device__ __forceinline__ int chain_cell_fused(const ChainArgs &a, long off, in
         while (soa_tick(st, dtfull)) {
         }
         niter = st.niter;
+        capped = st.niter > 1000;
         sg = st.g;
         sgavg = soa_gavg(st);
 #pragma unroll
@@ -474,6 +478,7 @@ __device__ __forceinline__ int chain_cell_fused(const ChainArgs &a, long off, in
           while (soa_tick(st, dtsub)) {
           }
           niter += st.niter;
+          capped = capped || st.niter > 1000;
           sg = st.g;
           sgavg = soa_gavg(st);
 #pragma unroll
@@ -485,6 +490,7 @@ __device__ __forceinline__ int chain_cell_fused(const ChainArgs &a, long off, in
     }
   }
   cell_finish<PRE, P, MULTI>(a, off, X, S, G);
+  if (a.soacap) a.soacap[off] = capped ? 1 : 0;
   return niter;
 }
 

@@ -87,6 +87,8 @@ constexpr int kLongBucket = 15;
 constexpr int kShortBucket = MAM4X_SHORT_BUCKET;
 constexpr int kShortChunk = MAM4X_SHORT_CHUNK;
 constexpr unsigned char kNoSoa = 0xff;
+constexpr unsigned char kCapped = 0x80;  // bucket bit: the SOA loop hit its cap
+static_assert(kNBucket <= kCapped, "bucket indices stay below the cap bit");
 
 // Workspace: 14 planes of one double per cell.  The set-up kernel writes
 // them in cell order (coalesced); the queue kernel's chunks hold cells of a
@@ -377,6 +379,9 @@ __global__ void __launch_bounds__(kBlock) soa_queue_kernel(Work w, unsigned long
       for (int m = 0; m < 4; ++m) w.ws[(W_A + m) * n + cell] = s.a[m];
       w.ws[W_GAVG * n + cell] = dev::soa_gavg(s);
       niter += (unsigned long long)s.niter;
+      // the loop stopped at its cap: marked in the cell's bucket byte
+      // (buckets are < kCapped; the set-up kernel rewrites it every step)
+      if (s.niter > 1000) w.bucket[cell] = (unsigned char)(w.bucket[cell] | kCapped);
       active = false;
     }
   }
@@ -419,6 +424,8 @@ __global__ void __launch_bounds__(kBlock) chain_post_kernel(Src src, Work w) {
                     G.g1, G.g, S, uptk, sg, sgavg, sa, &D);
   }
   dev::cell_finish<PRE, P>(a, off, X, S, G);
+  if (a.soacap)
+    a.soacap[off] = (a.cfg.do_cond && X.cat != 1 && (w.bucket[g] & kCapped)) ? 1 : 0;
 }
 
 // AeroProcess::validate checks (gasaerexch.hpp:96-120): negative T, p, qv or
@@ -1855,6 +1862,7 @@ static int fill_outputs(mam4x_handle h, const mam4x_outputs *out, ChainArgs &a) 
   if (nsout != 0 && nsout != NSTATE)
     return fail(h, MAM4X_ERR_ARG, "state_out must be all NULL or all set");
   a.has_sout = nsout == NSTATE;
+  a.soacap = out->soa_capped;
   a.has_dgn = 0;
   for (int m = 0; m < 4; ++m) {
     a.dgi[m] = out->dgncur_i[m];

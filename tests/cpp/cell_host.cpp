@@ -17,6 +17,11 @@ using namespace mam4x;
 static int g_ntsubstep = 1;
 extern "C" void cell_host_set_ntsubstep(int n) { g_ntsubstep = n; }
 
+// The next cell_host_run* calls' SOA-cap flag plane ([nlev][ncol] ints; NULL
+// = not formed).
+static int *g_soacap = nullptr;
+extern "C" void cell_host_set_soacap(int *p) { g_soacap = p; }
+
 // The reverse sweep's SOA states: 0 every state in a vector, 1 the GPU's
 // fixed two-level storage (dev::rev::SoaTape2<32>).
 static int g_tape = 0;
@@ -138,6 +143,7 @@ extern "C" long cell_host_run3(const int *cfgv, int prec, int ncol, int nlev, do
   a.cfg.ntsubstep = g_ntsubstep;
   a.do_calcsize = cfgv[6]; a.do_xfer = cfgv[7]; a.do_adjust = cfgv[8];
   a.has_dgn = dgi || dgc; a.has_sout = 0; a.dt = dt;
+  a.soacap = g_soacap;
   long iters = 0;
   for (int k = 0; k < nlev; ++k)
     for (int i = 0; i < ncol; ++i) {

@@ -222,3 +222,4 @@ def test_dot_product_per_cell(host3):
     assert np.median(rel) < 1e-14
     assert (rel > 1e-8).mean() <= 2e-3
     assert rel.max() < 1e-5
+
