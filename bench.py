@@ -184,8 +184,8 @@ def bench_batch(args):
                      "graph_column_steps_per_s": ncols / graph}
         b.close()
         ch.close()
-    best, ms = tune.tune(ds, DT, steps=5)
-    out["tuned"] = {"best": best, "ms_per_step": {str(tune.CANDIDATES[i]): v for i, v in ms.items()}}
+    best, tried = tune.tune(ds, DT, steps=5)
+    out["tuned"] = {"best": best, "tried": [[c, ms] for c, ms in tried]}
     out["value"] = out["queue"]["graph_column_steps_per_s"]
     print(json.dumps(out), flush=True)
 

@@ -40,7 +40,7 @@
 extern "C" {
 #endif
 
-#define MAM4X_ABI_VERSION 8
+#define MAM4X_ABI_VERSION 9
 #define MAM4X_MAX_NTSUBSTEP 64
 
 /* per-process tendency diagnostics (mam4x_outputs.process_tendencies) */
@@ -126,6 +126,20 @@ typedef struct mam4x_config {
                     gasaerexch.hpp:428); the cloudy subarea's renaming takes
                     1/n of the cloud-chemistry growth per sub-step
                     (:1446-1455).  n > 1 runs the one-lane-per-cell schedule. */
+  /* Performance only (results are identical; 0 = default):
+   *  soa_chunk        queue entries a wave takes per refill from the long
+   *                   part of the SOA work queue (default 64);
+   *  soa_short_chunk  the same for the short part (default 256);
+   *  soa_short_bucket first cost bucket of the long part, 1..24 (default 10:
+   *                   predicted <= ~30 sub-steps go to the short part);
+   *  levels_per_wave  1, 2, 4 or 8: the per-cell kernels give each wave a
+   *                   tile of that many levels x 64 / n columns (layout
+   *                   MAM4X_LAYOUT_LEV_COL, where the tiles cover the domain
+   *                   exactly; otherwise 1 = 64 columns of one level). */
+  int soa_chunk;
+  int soa_short_chunk;
+  int soa_short_bucket;
+  int levels_per_wave;
 } mam4x_config;
 
 /* Atmosphere (atmosphere.hpp:16-137) and modal diagnostics the chain reads. */

@@ -6,7 +6,7 @@ The structures below must match the C declarations field for field; the
 import ctypes
 import os
 
-ABI_VERSION = 8
+ABI_VERSION = 9
 
 NSTATE = 52
 NMODE = 4
@@ -63,6 +63,10 @@ class Config(ctypes.Structure):
         ("schedule", ctypes.c_int),
         ("soa_blocks_per_cu", ctypes.c_int),
         ("ntsubstep", ctypes.c_int),
+        ("soa_chunk", ctypes.c_int),
+        ("soa_short_chunk", ctypes.c_int),
+        ("soa_short_bucket", ctypes.c_int),
+        ("levels_per_wave", ctypes.c_int),
     ]
 
 

@@ -46,6 +46,7 @@ struct ChainArgs {
   const ChainConsts *K;
   int ncol, nlev, ldc;
   int layout;  // 0: [nlev][ldc] column fastest; 1: [ncol][ldc] level fastest
+  int lpw;     // layout 0: levels per run of 64 cells (1: 64 columns of one level)
   int do_calcsize, do_xfer, do_adjust, has_dgn, has_sout;
   double dt;
   dev::Cfg cfg;

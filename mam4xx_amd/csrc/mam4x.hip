@@ -51,7 +51,7 @@ constexpr int kNBucket = 24;  // half-octave cost buckets: 2 log2(1 + predicted 
 #ifndef MAM4X_QUEUE_CHUNK
 #define MAM4X_QUEUE_CHUNK 64
 #endif
-constexpr int kChunk = MAM4X_QUEUE_CHUNK;  // queue entries a wave takes per refill
+constexpr int kChunk = MAM4X_QUEUE_CHUNK;  // default queue entries a wave takes per refill
 // Waves whose first chunk comes from buckets >= kLongBucket (predicted >= ~180
 // sub-steps) raise their issue priority: a cell's sub-steps are a serial
 // chain, so the longest cells set the kernel's critical path and should not
@@ -132,7 +132,25 @@ struct QueueCtl {
   unsigned shead;           // next free slot of the short part
 };
 
+// The queue's granularity (mam4x_config.soa_chunk, soa_short_chunk,
+// soa_short_bucket; performance only): queue entries a wave takes per refill
+// from the long part and from the short part, the first bucket of the long
+// part, and the first bucket whose waves raise their issue priority.
+struct QueueShape {
+  int chunk, short_chunk, short_bucket, long_bucket;
+};
+
+static QueueShape queue_shape(const mam4x_config &c) {
+  QueueShape q;
+  q.chunk = c.soa_chunk > 0 ? c.soa_chunk : kChunk;
+  q.short_chunk = c.soa_short_chunk > 0 ? c.soa_short_chunk : kShortChunk;
+  q.short_bucket = c.soa_short_bucket > 0 ? c.soa_short_bucket : kShortBucket;
+  q.long_bucket = kLongBucket;
+  return q;
+}
+
 struct Work {
+  QueueShape qs;
   double *ws;              // W_NPLANE planes of ncell doubles
   unsigned char *bucket;   // per cell, kNoSoa when the cell has no SOA loop
   int *queue;              // cell indices, longest first
@@ -176,6 +194,16 @@ __device__ __forceinline__ void cell_index(const ChainArgs &a, long c, int &icol
     icol = (int)(c / a.nlev);
     const int k = (int)(c - (long)icol * a.nlev);
     off = (long)icol * a.ldc + k;
+  } else if (a.lpw > 1) {
+    // each run of 64 cells is a tile of lpw levels x 64 / lpw columns
+    const int cpw = 64 / a.lpw;
+    const long t = c >> 6;
+    const int lane = (int)(c & 63);
+    const long tpr = a.ncol / cpw;  // tiles per level row (fill_args: exact)
+    const long tl = t / tpr;
+    icol = (int)((t - tl * tpr) * cpw + lane % cpw);
+    const int k = (int)(tl * a.lpw + lane / cpw);
+    off = (long)k * a.ldc + icol;
   } else {
     const int k = (int)(c / a.ncol);
     icol = (int)(c - (long)k * a.ncol);
@@ -316,9 +344,10 @@ __global__ void __launch_bounds__(kBlock) soa_queue_kernel(Work w, unsigned long
   unsigned nq = 0, nlong = 0, nsplit = 0;
   for (int j = 0; j < kNBucket; ++j) {
     nq += w.ctl->hist[j];
-    if (j >= kLongBucket) nlong += w.ctl->hist[j];
-    if (j >= kShortBucket) nsplit += w.ctl->hist[j];
+    if (j >= w.qs.long_bucket) nlong += w.ctl->hist[j];
+    if (j >= w.qs.short_bucket) nsplit += w.ctl->hist[j];
   }
+  const unsigned qchunk = (unsigned)w.qs.chunk, qschunk = (unsigned)w.qs.short_chunk;
   const unsigned nwave = gridDim.x * (kBlock / 64);
   const unsigned wid = blockIdx.x * (kBlock / 64) + (threadIdx.x >> 6);
   bool first = MAM4X_STATIC_FIRST != 0;
@@ -338,19 +367,19 @@ __global__ void __launch_bounds__(kBlock) soa_queue_kernel(Work w, unsigned long
       unsigned b0 = nsplit, e0 = nsplit;
       if (first) {
         first = false;
-        b0 = wid * (unsigned)kChunk;
+        b0 = wid * qchunk;
       } else if (!lexhausted) {
-        if (lane == 0) b0 = atomicAdd(&w.ctl->head, (unsigned)kChunk);
-        b0 = (MAM4X_STATIC_FIRST ? nwave * (unsigned)kChunk : 0u) +
+        if (lane == 0) b0 = atomicAdd(&w.ctl->head, qchunk);
+        b0 = (MAM4X_STATIC_FIRST ? nwave * qchunk : 0u) +
              __builtin_amdgcn_readfirstlane(b0);
       }
       if (b0 < nsplit) {
-        e0 = min(b0 + (unsigned)kChunk, nsplit);
+        e0 = min(b0 + qchunk, nsplit);
       } else {
         lexhausted = true;
-        if (lane == 0) b0 = atomicAdd(&w.ctl->shead, (unsigned)kShortChunk);
+        if (lane == 0) b0 = atomicAdd(&w.ctl->shead, qschunk);
         b0 = nsplit + __builtin_amdgcn_readfirstlane(b0);
-        e0 = min(b0 + (unsigned)kShortChunk, nq);
+        e0 = min(b0 + qschunk, nq);
       }
       if (MAM4X_SOA_PRIO && b0 < nlong) __builtin_amdgcn_s_setprio(2);
       if (b0 >= nq) {
@@ -474,6 +503,7 @@ constexpr int kLogRows = 32, kLogVals = 6;
 constexpr int kTagIdle = (int)0x80000000;  // tag: lane idle; c >= 0: a sub-step
                                            // of cell c; ~c: cell c ended
 struct AdjWork {
+  QueueShape qs;
   const double *ws;             // the SOA loop's inputs (ws_load_in)
   double *res;                  // R_NPLANE planes
   double *pb;                   // kPbPlanes planes
@@ -520,9 +550,10 @@ __global__ void __launch_bounds__(kBlock) soa_log_kernel(AdjWork g) {
   unsigned nq = 0, nlong = 0, nsplit = 0;
   for (int j = 0; j < kNBucket; ++j) {
     nq += g.ctl->hist[j];
-    if (j >= kLongBucket) nlong += g.ctl->hist[j];
-    if (j >= kShortBucket) nsplit += g.ctl->hist[j];
+    if (j >= g.qs.long_bucket) nlong += g.ctl->hist[j];
+    if (j >= g.qs.short_bucket) nsplit += g.ctl->hist[j];
   }
+  const unsigned qchunk = (unsigned)g.qs.chunk, qschunk = (unsigned)g.qs.short_chunk;
   const unsigned nwave = gridDim.x * (kBlock / 64);
   const unsigned wid = blockIdx.x * (kBlock / 64) + (threadIdx.x >> 6);
   bool first = MAM4X_STATIC_FIRST != 0;
@@ -541,19 +572,19 @@ __global__ void __launch_bounds__(kBlock) soa_log_kernel(AdjWork g) {
       unsigned b0 = nsplit, e0 = nsplit;
       if (first) {
         first = false;
-        b0 = wid * (unsigned)kChunk;
+        b0 = wid * qchunk;
       } else if (!lexhausted) {
-        if (lane == 0) b0 = atomicAdd(&g.ctl->head, (unsigned)kChunk);
-        b0 = (MAM4X_STATIC_FIRST ? nwave * (unsigned)kChunk : 0u) +
+        if (lane == 0) b0 = atomicAdd(&g.ctl->head, qchunk);
+        b0 = (MAM4X_STATIC_FIRST ? nwave * qchunk : 0u) +
              __builtin_amdgcn_readfirstlane(b0);
       }
       if (b0 < nsplit) {
-        e0 = min(b0 + (unsigned)kChunk, nsplit);
+        e0 = min(b0 + qchunk, nsplit);
       } else {
         lexhausted = true;
-        if (lane == 0) b0 = atomicAdd(&g.ctl->shead, (unsigned)kShortChunk);
+        if (lane == 0) b0 = atomicAdd(&g.ctl->shead, qschunk);
         b0 = nsplit + __builtin_amdgcn_readfirstlane(b0);
-        e0 = min(b0 + (unsigned)kShortChunk, nq);
+        e0 = min(b0 + qschunk, nq);
       }
       if (MAM4X_SOA_PRIO && b0 < nlong) __builtin_amdgcn_s_setprio(2);
       if (b0 >= nq) {
@@ -1668,6 +1699,10 @@ static void fill_args(mam4x_handle h, const mam4x_inputs *in, ChainArgs &a) {
   a.cfg.gaexch_h2so4_uptake_optaa = c.newnuc_h2so4_conc_optaa;
   a.cfg.nmono = c.n_so4_monolayers_pcage;
   a.cfg.ntsubstep = c.ntsubstep;
+  // level-major tiles only where they cover the domain exactly
+  const int lpw = c.levels_per_wave > 1 ? c.levels_per_wave : 1;
+  a.lpw = (lpw > 1 && a.layout == MAM4X_LAYOUT_LEV_COL && a.ncol % (64 / lpw) == 0 &&
+           a.nlev % lpw == 0) ? lpw : 1;
   a.do_calcsize = c.do_calcsize;
   a.do_xfer = c.calcsize_do_aitacc_transfer;
   a.do_adjust = c.calcsize_do_adjust;
@@ -1693,6 +1728,10 @@ int mam4x_default_config(mam4x_config *out) {
   out->schedule = MAM4X_SCHED_AUTO;
   out->soa_blocks_per_cu = 0;
   out->ntsubstep = 1;
+  out->soa_chunk = 0;
+  out->soa_short_chunk = 0;
+  out->soa_short_bucket = 0;
+  out->levels_per_wave = 1;
   return MAM4X_OK;
 }
 
@@ -1708,7 +1747,13 @@ int mam4x_create(const mam4x_config *config, mam4x_handle *out) {
   if ((h->cfg.precision != MAM4X_PREC_FP64 && h->cfg.precision != MAM4X_PREC_MIXED) ||
       h->cfg.schedule < MAM4X_SCHED_AUTO || h->cfg.schedule > MAM4X_SCHED_QUEUE ||
       h->cfg.soa_blocks_per_cu < 0 || h->cfg.ntsubstep < 0 ||
-      h->cfg.ntsubstep > MAM4X_MAX_NTSUBSTEP) {
+      h->cfg.ntsubstep > MAM4X_MAX_NTSUBSTEP || h->cfg.soa_chunk < 0 ||
+      h->cfg.soa_chunk > (1 << 20) || h->cfg.soa_short_chunk < 0 ||
+      h->cfg.soa_short_chunk > (1 << 20) || h->cfg.soa_short_bucket < 0 ||
+      h->cfg.soa_short_bucket > kNBucket ||
+      (h->cfg.levels_per_wave != 0 && h->cfg.levels_per_wave != 1 &&
+       h->cfg.levels_per_wave != 2 && h->cfg.levels_per_wave != 4 &&
+       h->cfg.levels_per_wave != 8)) {
     delete h;
     return MAM4X_ERR_ARG;
   }
@@ -1927,6 +1972,7 @@ int mam4x_compute_tangent(mam4x_handle h, double t, double dt, const mam4x_input
   const unsigned grid = (unsigned)((ncell + kBlock - 1) / kBlock);
   Work w;
   std::memset(&w, 0, sizeof(w));
+  w.qs = queue_shape(h->cfg);
   w.ncell = ncell;
   if (h->stats) w.stats = h->d_stats;
   launch_fused<Tangent>(has_pre(in), dev::nsub(a.cfg) > 1, grid, s, OneDomain{a}, w);
@@ -2026,6 +2072,7 @@ int mam4x_compute_adjoint(mam4x_handle h, double t, double dt, const mam4x_input
       if (rc) return rc;
       Work w;
       std::memset(&w, 0, sizeof(w));
+      w.qs = queue_shape(h->cfg);
       w.ncell = ncell;
       w.ctl = h->ctl;
       w.dtfull = dev::dt_sub(a.cfg, dt);
@@ -2072,6 +2119,7 @@ int mam4x_compute_adjoint(mam4x_handle h, double t, double dt, const mam4x_input
     // 1. set-up and queue
     Work w;
     std::memset(&w, 0, sizeof(w));
+    w.qs = queue_shape(h->cfg);
     w.ncell = ncell;
     w.ctl = h->ctl;
     w.dtfull = (double)(int)(dt / 1);  // QUIRK (mam4_amicphys.hpp:1319): int sub-step
@@ -2084,6 +2132,7 @@ int mam4x_compute_adjoint(mam4x_handle h, double t, double dt, const mam4x_input
     hipLaunchKernelGGL(queue_scatter_kernel, dim3(grid), dim3(kBlock), 0, s, w);
     HIP_TRY(h, hipGetLastError());
     AdjWork g;
+    g.qs = queue_shape(h->cfg);
     g.ws = h->ws;
     g.res = h->adjws;
     g.pb = h->adjws + (size_t)R_NPLANE * ncell;
@@ -2274,6 +2323,7 @@ static int compute_step(mam4x_handle h, double dt, const mam4x_inputs *in,
   const bool multi = dev::nsub(a.cfg) > 1;
   Work w;
   std::memset(&w, 0, sizeof(w));
+  w.qs = queue_shape(h->cfg);
   w.ncell = ncell;
   w.ctl = noalloc ? reinterpret_cast<QueueCtl *>(h->scratch + kScratchCtlOff) : h->ctl;
   w.dtfull = dev::dt_sub(a.cfg, dt);  // QUIRK (mam4_amicphys.hpp:1319): int sub-step
@@ -2784,6 +2834,7 @@ int mam4x_batch_run(mam4x_batch b, double t, double dt, void *stream) {
   const SceneBatch src{b->d_args, b->d_blk0, b->nscene};
   Work w;
   std::memset(&w, 0, sizeof(w));
+  w.qs = queue_shape(h->cfg);
   w.ncell = b->ncell;
   w.ctl = b->ctl;
   w.dtfull = (double)(int)(dt / 1);  // QUIRK (mam4_amicphys.hpp:1319): int sub-step

@@ -4,17 +4,22 @@ The results of a step do not depend on its launch configuration (fused and
 work-queue schedules are bitwise identical), only its speed does, and the
 best choice depends on the workload: the number and size of the scenes, the
 share of cells with an SOA loop and the spread of their sub-step counts.
-``tune`` times every candidate on copies of the caller's own scenes and
-returns the fastest configuration.  The candidates are:
+``tune`` times candidates on copies of the caller's own scenes and returns
+the fastest configuration, searching one knob at a time from the best so far:
 
-* schedule: the fused one-lane-per-cell kernel, or the work-queue pipeline;
-* soa_blocks_per_cu: resident blocks per CU of the persistent SOA kernel
-  (fewer waves raise each wave's issue share on the long serial chains,
-  more waves hide latency);
+1. schedule x soa_blocks_per_cu (CANDIDATES): the fused one-lane-per-cell
+   kernel, or the work-queue pipeline with as many resident blocks per CU of
+   the persistent SOA kernel as fit or 1..3 (fewer waves per SIMD raise each
+   wave's issue share on the long serial chains, more waves hide latency);
+2. the queue's granularity (queue schedule only): entries per refill of the
+   long part (soa_chunk) and of the short part (soa_short_chunk), and the
+   bucket that splits them (soa_short_bucket);
+3. levels_per_wave: the per-cell kernels' tiles of 2, 4 or 8 levels per
+   wave instead of 64 columns of one level.
 
-each timed as a batch (one launch sequence for all scenes) over a few
-chained steps, eagerly and replayed from a CUDA graph.  Results are cached
-per workload signature in a JSON file when ``cache`` is given.
+Each candidate is timed as a batch (one launch sequence for all scenes) over
+a few chained steps, eagerly or replayed from a CUDA graph.  Results are
+cached per workload signature in a JSON file when ``cache`` is given.
 """
 import hashlib
 import json
@@ -27,13 +32,14 @@ from . import _abi
 from . import layout as L
 from .chain import Chain, default_config
 
-CANDIDATES = [
-    {"schedule": _abi.SCHED_FUSED, "soa_blocks_per_cu": 0},
-    {"schedule": _abi.SCHED_QUEUE, "soa_blocks_per_cu": 0},
-    {"schedule": _abi.SCHED_QUEUE, "soa_blocks_per_cu": 1},
-    {"schedule": _abi.SCHED_QUEUE, "soa_blocks_per_cu": 2},
-    {"schedule": _abi.SCHED_QUEUE, "soa_blocks_per_cu": 3},
-]
+DEFAULTS = {"soa_chunk": 0, "soa_short_chunk": 0, "soa_short_bucket": 0, "levels_per_wave": 1}
+CANDIDATES = [dict(DEFAULTS, schedule=sc, soa_blocks_per_cu=b) for sc, b in (
+    (_abi.SCHED_FUSED, 0), (_abi.SCHED_QUEUE, 0), (_abi.SCHED_QUEUE, 1), (_abi.SCHED_QUEUE, 2),
+    (_abi.SCHED_QUEUE, 3))]
+# stage 2 (queue schedule) and stage 3: one knob at a time
+QUEUE_KNOBS = [("soa_chunk", (32, 128)), ("soa_short_chunk", (128, 512)),
+               ("soa_short_bucket", (8, 12))]
+TILE_KNOB = ("levels_per_wave", (2, 4, 8))
 
 
 def signature(scenes, dt, precision):
@@ -53,7 +59,7 @@ def _copy(scenes):
 
 
 def _time(cand, scenes, dt, precision, steps, graph):
-    cfg = default_config(schedule=cand["schedule"], soa_blocks_per_cu=cand["soa_blocks_per_cu"])
+    cfg = default_config(**cand)
     ch = Chain(config=cfg, precision=precision)
     ds = _copy(scenes)
     outs = [torch.empty((L.NSTATE,) + tuple(d["temperature"].shape), dtype=torch.float64,
@@ -82,7 +88,8 @@ def _time(cand, scenes, dt, precision, steps, graph):
 
 def tune(scenes, dt, precision=_abi.PREC_FP64, steps=5, graph=True, cache=None,
          candidates=None, verbose=False):
-    """Return (best candidate dict, {candidate index: ms per step}).
+    """Return (best candidate dict, [(candidate, ms per step), ...] in the
+    order tried).  candidates: a list to time instead of the staged search.
 
     scenes: list of scene dicts of CUDA float64 tensors (not modified)."""
     sig = signature(scenes, dt, precision)
@@ -90,20 +97,39 @@ def tune(scenes, dt, precision=_abi.PREC_FP64, steps=5, graph=True, cache=None,
         with open(cache) as f:
             known = json.load(f)
         if sig in known:
-            return known[sig]["best"], {int(k): v for k, v in known[sig]["ms"].items()}
-    cands = candidates or CANDIDATES
-    ms = {}
-    for i, c in enumerate(cands):
-        ms[i] = _time(c, scenes, dt, precision, steps, graph)
+            return known[sig]["best"], [tuple(t) for t in known[sig]["tried"]]
+    tried = []
+
+    def time_it(c):
+        for t in tried:
+            if t[0] == c:
+                return t[1]
+        ms = _time(c, scenes, dt, precision, steps, graph)
+        tried.append((dict(c), ms))
         if verbose:
-            print("tune: %s -> %.3f ms/step" % (c, ms[i]), flush=True)
-    best = cands[min(ms, key=ms.get)]
+            print("tune: %s -> %.3f ms/step" % (c, ms), flush=True)
+        return ms
+
+    if candidates:
+        for c in candidates:
+            time_it(c)
+    else:
+        for c in CANDIDATES:
+            time_it(c)
+        best = min(tried, key=lambda t: t[1])
+        knobs = (QUEUE_KNOBS if best[0]["schedule"] == _abi.SCHED_QUEUE else []) + [TILE_KNOB]
+        for name, values in knobs:
+            base = dict(best[0])
+            for v in values:
+                time_it(dict(base, **{name: v}))
+            best = min(tried, key=lambda t: t[1])
+    best = min(tried, key=lambda t: t[1])[0]
     if cache:
         known = {}
         if os.path.exists(cache):
             with open(cache) as f:
                 known = json.load(f)
-        known[sig] = {"best": best, "ms": ms}
+        known[sig] = {"best": best, "tried": tried}
         with open(cache, "w") as f:
             json.dump(known, f, indent=1)
-    return best, ms
+    return best, tried

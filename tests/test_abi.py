@@ -81,12 +81,13 @@ def test_struct_layouts_match_c(tmp_path):
 
 def test_abi_version_and_default_config():
     lib = _abi.load()
-    assert lib.mam4x_abi_version() == _abi.ABI_VERSION == 8
+    assert lib.mam4x_abi_version() == _abi.ABI_VERSION == 9
     cfg = _abi.Config()
     assert lib.mam4x_default_config(ctypes.byref(cfg)) == _abi.OK
     assert (cfg.do_cond, cfg.do_rename, cfg.do_newnuc, cfg.do_coag) == (1, 1, 1, 1)
     assert cfg.n_so4_monolayers_pcage == 8 and cfg.precision == _abi.PREC_FP64
-    assert cfg.ntsubstep == 1
+    assert cfg.ntsubstep == 1 and cfg.levels_per_wave == 1
+    assert (cfg.soa_chunk, cfg.soa_short_chunk, cfg.soa_short_bucket) == (0, 0, 0)
     assert lib.mam4x_default_config(None) == _abi.ERR_ARG
 
 
