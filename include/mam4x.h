@@ -307,7 +307,11 @@ int mam4x_compute_tangent(mam4x_handle h, double t, double dt, const mam4x_input
  * sub-step records, so the call cannot be captured in a graph.  The records
  * and counters are held by the handle (grown on demand); calls on one handle
  * must be stream-ordered.  A cell whose recomputed loop does not retrace the
- * counted sub-steps gets NaN gradients rather than wrong ones. */
+ * counted sub-steps gets NaN gradients rather than wrong ones.  The SOA
+ * sub-steps are recomputed with the fast FP64 quotient throughout; where the
+ * step itself falls back to IEEE quotients (SOA masses below 2^-400, inputs
+ * out of range) the linearisation point can differ from the step's in the
+ * last place of those masses. */
 int mam4x_compute_adjoint(mam4x_handle h, double t, double dt, const mam4x_inputs *in,
                           const mam4x_adjoint *adj, void *stream);
 

@@ -175,8 +175,8 @@ struct SoaBar {
 // Reverse of one soa_tick from the state c it started at, given the
 // adjoints b of the state after it; on return b holds the adjoints of the
 // state before it (constants' adjoints accumulated).  The forward part
-// reproduces soa_tick's values bit for bit (num::div_r with the shared
-// reciprocal is num::div); every divisor is inverted once and its
+// reproduces soa_tick<FAST = true>'s values bit for bit (num::div_r with the
+// shared reciprocal is num::div_fast); every divisor is inverted once and its
 // reciprocal serves the forward quotient and all reverse uses.
 __host__ __device__ __forceinline__ void soa_tick_rev(const SoaState<double> &k,
                                                       const SoaCkpt &c, double dtfull,
@@ -2405,7 +2405,7 @@ struct SoaTape2 {
     seg = -1;
     while (true) {
       const SoaCkpt c = soa_ckpt(st);
-      if (!soa_tick(st, dtfull)) break;
+      if (!soa_tick<true>(st, dtfull)) break;
       if (nt < C) coarse[nt] = c;
       ++nt;
     }
@@ -2414,7 +2414,7 @@ struct SoaTape2 {
       SoaState<double> t = init;
       for (int i = 0; i < nt; ++i) {
         if (i % s == 0) coarse[i / s] = soa_ckpt(t);
-        soa_tick(t, dtfull);
+        soa_tick<true>(t, dtfull);
       }
     }
     return nt;
@@ -2432,7 +2432,7 @@ struct SoaTape2 {
       const int lo = sg * s, hi = lo + s < nt ? lo + s : nt;
       for (int k = lo; k < hi; ++k) {
         fine[k - lo] = soa_ckpt(t);
-        soa_tick(t, dtfull);
+        soa_tick<true>(t, dtfull);
       }
       seg = sg;
     }
@@ -2498,7 +2498,7 @@ __host__ __device__ __forceinline__ int adjoint_cell(const ChainArgs &a, long of
       if (cfg.do_cond) {
         SoaState<double> st;
         soa_init(st, in);
-        while (soa_tick(st, dtsub)) {
+        while (soa_tick<true>(st, dtsub)) {
         }
         niter += st.niter;
         sg = st.g;

@@ -453,10 +453,7 @@ __device__ __forceinline__ int chain_cell_fused(const ChainArgs &a, long off, in
       if (a.cfg.do_cond) {
         clear_pre<P>(K, X.T, X.p, X.dgw, S, G.g[0], in);
         SoaState<W> st;
-        soa_init(st, in);
-        const double dtfull = (double)(int)(a.dt / 1);
-        while (soa_tick(st, dtfull)) {
-        }
+        soa_run(st, in, (double)(int)(a.dt / 1));
         niter = st.niter;
         capped = st.niter > 1000;
         sg = st.g;
@@ -475,9 +472,7 @@ __device__ __forceinline__ int chain_cell_fused(const ChainArgs &a, long off, in
         if (a.cfg.do_cond) {
           if (j > 0) soa_in_state<P>(S, G.g[0], in);
           SoaState<W> st;
-          soa_init(st, in);
-          while (soa_tick(st, dtsub)) {
-          }
+          soa_run(st, in, dtsub);
           niter += st.niter;
           capped = capped || st.niter > 1000;
           sg = st.g;

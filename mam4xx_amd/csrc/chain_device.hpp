@@ -229,6 +229,12 @@ __device__ __forceinline__ num::Dual soa_term(bool skip, num::Dual x) {
 // One pass of the while loop (gasaerexch_soaexch.hpp:170-335).  Returns false
 // once the loop has ended (time reached or 1000 sub-steps taken).
 //
+// FAST divides with num::div_fast (reciprocal, Newton steps, Markstein
+// correction: IEEE's quotient wherever numerators are 0 or at least 2^-968,
+// quotients normal and divisors in range); otherwise each quotient is the
+// IEEE a / b.  soa_fast_exact below states when every FAST quotient of a
+// cell's loop is the IEEE one; soa_run runs a cell's loop on that condition.
+//
 // Written without branches on the mode.  A skipped mode (uptake <= 1e-15) has
 // uk = 0 and a_soa = 0, so beta = 0, den = 1 and each of its terms
 // (a_soa, uk |phi|, a_soa / den, beta / den) is exactly +0: adding it leaves
@@ -236,7 +242,15 @@ __device__ __forceinline__ num::Dual soa_term(bool skip, num::Dual x) {
 // values the reference's skip produces.  The four modes' divisions are then
 // four independent chains the scheduler interleaves, instead of a branch per
 // mode that serialises them.
-template <class W>
+template <bool FAST, class W>
+__device__ __forceinline__ W soa_div(W a, W b) {
+  if constexpr (FAST)
+    return num::div_fast(a, b);
+  else
+    return a / b;
+}
+
+template <bool FAST = false, class W>
 __device__ __forceinline__ bool soa_tick(SoaState<W> &s, double dtfull) {
   const double alpha_astem = 0.05, a_min1 = 1.0e-20, g_min1 = 1.0e-20;
   if (!(s.tcur < dtfull - 1.0e-3)) return false;
@@ -248,9 +262,9 @@ __device__ __forceinline__ bool soa_tick(SoaState<W> &s, double dtfull) {
 #pragma unroll
   for (int n = 0; n < 4; ++n) {
     a_soa[n] = s.skip[n] ? W(0.0) : kmax(s.a[n], 0.0);
-    sat[n] = num::div(s.g0, kmax(s.aopoa[n] + a_soa[n], a_min1));
+    sat[n] = soa_div<FAST, W>(s.g0, kmax(s.aopoa[n] + a_soa[n], a_min1));
     const W gstar = sat[n] * a_soa[n];
-    const W phi = num::div(g_soa - gstar, kmax(g_soa, kmax(gstar, g_min1)));
+    const W phi = soa_div<FAST, W>(g_soa - gstar, kmax(g_soa, kmax(gstar, g_min1)));
     term[n] = soa_term(s.skip[n], s.uk[n] * fabs(phi));
   }
   // the sums in the reference's mode order
@@ -263,7 +277,7 @@ __device__ __forceinline__ bool soa_tick(SoaState<W> &s, double dtfull) {
   const W tmpa = kmax(0.0, tmpb);
   const W dtmax = dtfull - s.tcur;
   const bool last = dtmax * tmpa <= alpha_astem;
-  const W dtcur = last ? dtmax : num::div(W(alpha_astem), tmpa);
+  const W dtcur = last ? dtmax : soa_div<FAST, W>(W(alpha_astem), tmpa);
   s.tcur = last ? W(dtfull) : s.tcur + dtcur;
   // step 1: explicit estimate of condensing modes' a_soa and sat
   W beta[4];
@@ -273,7 +287,7 @@ __device__ __forceinline__ bool soa_tick(SoaState<W> &s, double dtfull) {
     const W gstar = sat[n] * a_soa[n];
     const W del = g_soa - gstar;
     const W at = a_soa[n] + beta[n] * del;
-    const W sat1 = num::div(s.g0, kmax(s.aopoa[n] + at, a_min1));
+    const W sat1 = soa_div<FAST, W>(s.g0, kmax(s.aopoa[n] + at, a_min1));
     sat[n] = del > 0.0 ? sat1 : sat[n];
   }
   // step 2: implicit in g, semi-implicit in a
@@ -281,16 +295,16 @@ __device__ __forceinline__ bool soa_tick(SoaState<W> &s, double dtfull) {
 #pragma unroll
   for (int n = 0; n < 4; ++n) {
     den[n] = 1.0 + beta[n] * sat[n];
-    sa = sa + soa_term(s.skip[n], num::div(a_soa[n], den[n]));
-    sb = sb + soa_term(s.skip[n], num::div(beta[n], den[n]));
+    sa = sa + soa_term(s.skip[n], soa_div<FAST, W>(a_soa[n], den[n]));
+    sb = sb + soa_term(s.skip[n], soa_div<FAST, W>(beta[n], den[n]));
   }
-  g_soa = num::div(tot - sa, 1.0 + sb);
+  g_soa = soa_div<FAST, W>(tot - sa, 1.0 + sb);
   g_soa = kmax(0.0, g_soa);
 #pragma unroll
   for (int n = 0; n < 4; ++n) {
     // skipped modes: a_soa is never set, so their SOA becomes 0 (quirk of
     // gasaerexch_soaexch.hpp:327-331)
-    s.a[n] = soa_term(s.skip[n], num::div(a_soa[n] + beta[n] * g_soa, den[n]));
+    s.a[n] = soa_term(s.skip[n], soa_div<FAST, W>(a_soa[n] + beta[n] * g_soa, den[n]));
   }
   s.g = g_soa;
   s.avg += dtcur * (gprv + 0.5 * (g_soa - gprv));
@@ -302,6 +316,81 @@ __device__ __forceinline__ bool soa_tick(SoaState<W> &s, double dtfull) {
 template <class W>
 __device__ __forceinline__ W soa_gavg(const SoaState<W> &s) {
   return kmax(0.0, s.avg / s.dtsum);
+}
+
+// ---- when the FAST sub-step is IEEE's, bit for bit
+// soa_fast_exact: with the cell's inputs in the ranges soa_fast_ok checks,
+// and SOAG and each mode's SOA at 0 or >= 2^-400 in every state the loop
+// passes through (soa_state_exp >= kSoaExpMin, the final state included),
+// every kept quotient of a FAST sub-step has a numerator 0 or >= 2^-810, a
+// normal result and a divisor in [2^-67, 2^158], so num::div_fast returns
+// IEEE's quotient.  The bounds, for a state whose masses lie in
+// {0} u [2^-400, 2^11] (the loop conserves g + sum a, which starts below
+// 5 x 2^8; skipped modes' quotients are discarded):
+//   sat = g0 / max(aopoa + a, 1e-20)    g0 in {0} u [2^-300, 2^10]: [2^-312, 2^77]
+//   g* = sat a                          {0} u [2^-712, 2^88]
+//   g - g* (phi's numerator)            multiples of 2^-764; divisor [2^-67, 2^88]
+//   dtcur                               [2^-67, 2^20]: dtmax > 1e-3, or 0.05 / tmpa
+//                                       with tmpa <= sum uk |phi| <= 2^62
+//   beta = dtcur uk                     {0} u [2^-117, 2^80]: uk > 1e-15 unless
+//                                       skipped, uk <= 2^60
+//   sat1 = g0 / max(aopoa + at, 1e-20)  |at| <= 2^169: >= 2^-470
+//   den = 1 + beta sat                  [1, 2^158]
+//   a / den, beta / den                 >= 2^-558, >= 2^-275
+//   tot - sa                            multiples of 2^-610 (every term >= 2^-558)
+//   g' = (tot - sa) / (1 + sb)          {0} u [2^-693, 2^11]
+//   a + beta g'                         {0} u [2^-810, ...)
+//   a' = (a + beta g') / den            {0} u [2^-968, 2^11], normal
+// A cell outside these (a mass below 2^-400 at some sub-step, an input out of
+// range or not finite) runs the IEEE loop from the start instead.
+constexpr int kSoaExpMin = -399;     // frexp exponent of 2^-400
+constexpr int kSoaExpFail = -100000;  // a cell whose inputs are out of range
+
+__device__ __forceinline__ bool soa_fast_ok(const SoaState<double> &s, double dtfull) {
+  bool ok = dtfull <= 0x1p20 && (s.g0 == 0.0 || (s.g0 >= 0x1p-300 && s.g0 <= 0x1p10)) &&
+            ::fabs(s.g) <= 0x1p8;
+#pragma unroll
+  for (int n = 0; n < 4; ++n)
+    ok = ok && s.uk[n] <= 0x1p60 && ::fabs(s.aopoa[n]) <= 0x1p10 && ::fabs(s.a[n]) <= 0x1p8;
+  return ok;
+}
+__device__ __forceinline__ bool soa_fast_ok(const SoaState<num::Dual> &, double) { return true; }
+
+__device__ __forceinline__ int frexp_exp(double x) {
+#if defined(__HIP_DEVICE_COMPILE__)
+  return __builtin_amdgcn_frexp_exp(x);
+#else
+  int e;
+  (void)::frexp(x, &e);
+  return e;
+#endif
+}
+
+// The smallest frexp exponent among SOAG and the modes' SOA (0 for a zero).
+__device__ __forceinline__ int soa_state_exp(const SoaState<double> &s) {
+  int e = frexp_exp(s.g);
+#pragma unroll
+  for (int n = 0; n < 4; ++n) e = kmin(e, frexp_exp(s.a[n]));
+  return e;
+}
+__device__ __forceinline__ int soa_state_exp(const SoaState<num::Dual> &) { return 0; }
+
+// One cell's SOA loop from its inputs, equal bit for bit to the IEEE loop:
+// FAST sub-steps while soa_fast_exact holds, else the IEEE loop from the
+// start.
+template <class W>
+__device__ __forceinline__ void soa_run(SoaState<W> &st, const SoaIn<W> &in, double dtfull) {
+  soa_init(st, in);
+  int emin = soa_fast_ok(st, dtfull) ? 0 : kSoaExpFail;
+  while (true) {
+    emin = kmin(emin, soa_state_exp(st));
+    if (!soa_tick<true>(st, dtfull)) break;
+  }
+  if (emin < kSoaExpMin) {
+    soa_init(st, in);
+    while (soa_tick<false>(st, dtfull)) {
+    }
+  }
 }
 
 // ------------------------------------------------------------------ H2SO4

@@ -88,7 +88,10 @@ constexpr int kShortBucket = MAM4X_SHORT_BUCKET;
 constexpr int kShortChunk = MAM4X_SHORT_CHUNK;
 constexpr unsigned char kNoSoa = 0xff;
 constexpr unsigned char kCapped = 0x80;  // bucket bit: the SOA loop hit its cap
-static_assert(kNBucket <= kCapped, "bucket indices stay below the cap bit");
+// bucket bit: the cell's loop left the range where the queue kernel's fast
+// quotients are IEEE's (dev::soa_fast_exact); soa_ieee_kernel reruns it
+constexpr unsigned char kIeee = 0x40;
+static_assert(kNBucket <= kIeee, "bucket indices stay below the flag bits");
 
 // Workspace: 14 planes of one double per cell.  The set-up kernel writes
 // them in cell order (coalesced); the queue kernel's chunks hold cells of a
@@ -357,6 +360,7 @@ __global__ void __launch_bounds__(kBlock) soa_queue_kernel(Work w, unsigned long
   bool lexhausted = false, exhausted = false;
   bool active = false;
   int cell = 0;
+  int emin = 0;  // the cell's dev::soa_fast_exact record
   dev::SoaState<double> s;
   unsigned long long niter = 0;
   while (true) {
@@ -397,24 +401,61 @@ __global__ void __launch_bounds__(kBlock) soa_queue_kernel(Work w, unsigned long
         dev::SoaIn<double> in;
         ws_load_in(w.ws, n, cell, in);
         dev::soa_init(s, in);
+        emin = dev::soa_fast_ok(s, w.dtfull) ? 0 : dev::kSoaExpFail;
         active = true;
       }
       cbeg += min((unsigned)__popcll(need), avail);
     }
     if (!__any(active)) break;
-    if (active && !dev::soa_tick(s, w.dtfull)) {
-      w.ws[W_G * n + cell] = s.g;
+    if (active) {
+      emin = min(emin, dev::soa_state_exp(s));
+      if (!dev::soa_tick<true>(s, w.dtfull)) {
+        // flags in the cell's bucket byte (buckets are < kIeee; the set-up
+        // kernel rewrites it every step)
+        if (emin < dev::kSoaExpMin) {
+          // inputs left untouched for soa_ieee_kernel
+          w.bucket[cell] = (unsigned char)(w.bucket[cell] | kIeee);
+        } else {
+          w.ws[W_G * n + cell] = s.g;
 #pragma unroll
-      for (int m = 0; m < 4; ++m) w.ws[(W_A + m) * n + cell] = s.a[m];
-      w.ws[W_GAVG * n + cell] = dev::soa_gavg(s);
-      niter += (unsigned long long)s.niter;
-      // the loop stopped at its cap: marked in the cell's bucket byte
-      // (buckets are < kCapped; the set-up kernel rewrites it every step)
-      if (s.niter > 1000) w.bucket[cell] = (unsigned char)(w.bucket[cell] | kCapped);
-      active = false;
+          for (int m = 0; m < 4; ++m) w.ws[(W_A + m) * n + cell] = s.a[m];
+          w.ws[W_GAVG * n + cell] = dev::soa_gavg(s);
+          niter += (unsigned long long)s.niter;
+          // the loop stopped at its cap
+          if (s.niter > 1000) w.bucket[cell] = (unsigned char)(w.bucket[cell] | kCapped);
+        }
+        active = false;
+      }
     }
   }
   add_stats(stats, niter);
+}
+
+// Phase 3b: the cells soa_queue_kernel flagged kIeee (a mass below 2^-400,
+// an input out of range; none in ordinary scenes) rerun with IEEE quotients
+// from their inputs, results written as the queue kernel writes them.
+__global__ void __launch_bounds__(kBlock) soa_ieee_kernel(Work w) {
+  const long c = (long)blockIdx.x * kBlock + threadIdx.x;
+  unsigned long long niter = 0;
+  if (c < w.ncell) {
+    const unsigned char b = w.bucket[c];
+    if (b != kNoSoa && (b & kIeee)) {
+      const long n = w.ncell;
+      dev::SoaIn<double> in;
+      ws_load_in(w.ws, n, c, in);
+      dev::SoaState<double> s;
+      dev::soa_init(s, in);
+      while (dev::soa_tick<false>(s, w.dtfull)) {
+      }
+      w.ws[W_G * n + c] = s.g;
+#pragma unroll
+      for (int m = 0; m < 4; ++m) w.ws[(W_A + m) * n + c] = s.a[m];
+      w.ws[W_GAVG * n + c] = dev::soa_gavg(s);
+      niter = (unsigned long long)s.niter;
+      if (s.niter > 1000) w.bucket[c] = (unsigned char)(b | kCapped);
+    }
+  }
+  add_stats(w.stats, niter);
 }
 
 // Phase 4.
@@ -614,7 +655,7 @@ __global__ void __launch_bounds__(kBlock) soa_log_kernel(AdjWork g) {
     r[5] = s.tcur;
     int tag = kTagIdle;
     if (active) {
-      if (dev::soa_tick(s, g.dtfull)) {
+      if (dev::soa_tick<true>(s, g.dtfull)) {
         tag = cell;
       } else {
         tag = ~cell;
@@ -2364,8 +2405,10 @@ static int compute_step(mam4x_handle h, double dt, const mam4x_inputs *in,
     hipLaunchKernelGGL(queue_scatter_kernel, dim3(grid), dim3(kBlock), 0, s, w);
   }
   if (ev) HIP_TRY(h, hipEventRecord(ev[1], s));
-  if (queue && !multi)
+  if (queue && !multi) {
     hipLaunchKernelGGL(soa_queue_kernel, dim3(h->soa_grid), dim3(kBlock), 0, s, w, w.stats);
+    hipLaunchKernelGGL(soa_ieee_kernel, dim3(grid), dim3(kBlock), 0, s, w);
+  }
   if (ev) HIP_TRY(h, hipEventRecord(ev[2], s));
   if (multi && queue) {
     for (int pass = 0; pass < 2; ++pass) {
@@ -2852,6 +2895,7 @@ int mam4x_batch_run(mam4x_batch b, double t, double dt, void *stream) {
       launch_cells<Fp64>(kPre, b->pre, b->nblock, s, src, w);
     hipLaunchKernelGGL(queue_scatter_kernel, dim3(b->nblock), dim3(kBlock), 0, s, w);
     hipLaunchKernelGGL(soa_queue_kernel, dim3(h->soa_grid), dim3(kBlock), 0, s, w, w.stats);
+    hipLaunchKernelGGL(soa_ieee_kernel, dim3(b->nblock), dim3(kBlock), 0, s, w);
   }
   const int kind = fused ? kFused : kPost;
   const bool multi = h->cfg.ntsubstep > 1;

@@ -132,13 +132,16 @@ __host__ __device__ __forceinline__ Dual round(Dual a) { return Dual(::round(a.v
 // q + r (a - b q) (Markstein).  These are the same operations the general
 // sequence performs when its scaling is the identity, which holds whenever a,
 // b and a/b are normal numbers away from the ends of the exponent range
-// (|a| > 2^-969, 2^-1021 < |b| < 2^1021); a +0 numerator gives +0, a -0
-// numerator +0 as well (IEEE: -0).  For other operands the result may differ
-// from a / b (b = 0 or inf gives NaN), so callers use it only where the
-// operands are bounded and numerators are never -0 (the SOA sub-step).
-// div() in two steps: rcp(b), the reciprocal with its two Newton steps, and
-// div_r(a, b, r), the quotient from it; div_r(a, b, rcp(b)) is div(a, b),
-// so a divisor used more than once is inverted once.
+// (|a| >= 2^-968, 2^-1021 < |b| < 2^1021, |a/b| >= 2^-1021); a +0 numerator
+// gives +0, a -0 numerator +0 as well (IEEE: -0).  Below 2^-968 the residual
+// a - b q is not representable and the quotient can be one unit in the last
+// place off; b = 0 or inf gives NaN.  Callers use it only where they know
+// the operands to be in range (the SOA sub-step, whose loop checks that it
+// is: soa_fast_exact in chain_device.hpp).
+//
+// rcp(b) is the reciprocal with its two Newton steps and div_r(a, b, r) the
+// quotient from it, so a divisor used more than once is inverted once:
+// div_fast(a, b) = div_r(a, b, rcp(b)).  Host builds divide with a / b.
 __host__ __device__ __forceinline__ double rcp(double b) {
 #if defined(__HIP_DEVICE_COMPILE__)
   double r = __builtin_amdgcn_rcp(b);
@@ -160,9 +163,11 @@ __host__ __device__ __forceinline__ double div_r(double a, double b, double r) {
   return a / b;
 #endif
 }
-__host__ __device__ __forceinline__ double div(double a, double b) { return div_r(a, b, rcp(b)); }
-__host__ __device__ __forceinline__ F32 div(F32 a, F32 b) { return a / b; }
-__host__ __device__ __forceinline__ Dual div(Dual a, Dual b) { return a / b; }
+__host__ __device__ __forceinline__ double div_fast(double a, double b) {
+  return div_r(a, b, rcp(b));
+}
+__host__ __device__ __forceinline__ F32 div_fast(F32 a, F32 b) { return a / b; }
+__host__ __device__ __forceinline__ Dual div_fast(Dual a, Dual b) { return a / b; }
 
 // ------------------------------------------------------------------ helpers
 // primal value as a double (branch decisions, integer indices, outputs)

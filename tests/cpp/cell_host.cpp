@@ -33,7 +33,7 @@ struct VecSoaTape {
     v.clear();
     while (true) {
       const dev::rev::SoaCkpt c = dev::rev::soa_ckpt(st);
-      if (!dev::soa_tick(st, dtfull)) break;
+      if (!dev::soa_tick<true>(st, dtfull)) break;
       v.push_back(c);
     }
     return (int)v.size();
