@@ -225,19 +225,15 @@ class Chain:
         self._check(rc, "mam4x_fields_to_tracers")
 
     def compute_gas_chemistry(self, dt, temperature, photolysis_rate, extfrc, invariants,
-                              het_rates, qq, layout=_abi.LAYOUT_LEV_COL, stream=None):
+                              het_rates, qq, layout=_abi.LAYOUT_LEV_COL, stream=None, live=None):
         """Gas-phase chemistry (mam4x_compute_gas_chemistry).  temperature /
         photolysis_rate: planes; extfrc [9][...], invariants [8][...],
         het_rates / qq [31][...] tensors (qq advanced in place); None
-        extfrc / het_rates = 0."""
+        extfrc / het_rates = 0.  live: see _extents."""
         a = _abi.GasChemArgs()
         t = temperature
-        assert t.dim() == 2 and t.dtype == torch.float64 and t.is_cuda and t.is_contiguous()
-        if layout == _abi.LAYOUT_LEV_COL:
-            a.nlev, a.ncol = t.shape
-        else:
-            a.ncol, a.nlev = t.shape
-        a.ldc, a.layout = t.shape[1], layout
+        a.ncol, a.nlev, a.ldc = self._extents(t, layout, live)
+        a.layout = layout
         a.temperature = t.data_ptr()
         assert photolysis_rate.shape == t.shape and photolysis_rate.is_contiguous()
         a.photolysis_rate = photolysis_rate.data_ptr()
@@ -253,21 +249,27 @@ class Chain:
         self._check(rc, "mam4x_compute_gas_chemistry")
 
     @staticmethod
-    def _extents(t, layout):
-        """(ncol, nlev, ldc) of a contiguous float64 CUDA plane"""
+    def _extents(t, layout, live=None):
+        """(ncol, nlev, ldc) of a contiguous float64 CUDA plane.  The plane's
+        fastest dimension is the leading dimension ldc; live (default: all of
+        it) is how many of its entries the domain uses -- ncol for
+        LAYOUT_LEV_COL, nlev for LAYOUT_COL_LEV -- as with planes a host
+        dimensions for its largest chunk (pcols) and fills up to ncol."""
         assert t.dim() == 2 and t.dtype == torch.float64 and t.is_cuda and t.is_contiguous()
+        slow, ldc = t.shape
+        fast = ldc if live is None else int(live)
+        assert 0 <= fast <= ldc, (fast, ldc)
         if layout == _abi.LAYOUT_LEV_COL:
-            nlev, ncol = t.shape
-        else:
-            ncol, nlev = t.shape
-        return ncol, nlev, t.shape[1]
+            return fast, slow, ldc
+        return slow, fast, ldc
 
     def compute_invariants(self, temperature, pressure, qv, cnst_offline, invariants,
-                           layout=_abi.LAYOUT_LEV_COL, stream=None):
+                           layout=_abi.LAYOUT_LEV_COL, stream=None, live=None):
         """Invariant densities (mam4x_compute_invariants, setinv): planes T,
-        p, qv; cnst_offline [4][...] or None; invariants [8][...] output."""
+        p, qv; cnst_offline [4][...] or None; invariants [8][...] output.
+        live: see _extents."""
         a = _abi.SetinvArgs()
-        a.ncol, a.nlev, a.ldc = self._extents(temperature, layout)
+        a.ncol, a.nlev, a.ldc = self._extents(temperature, layout, live)
         a.layout = layout
         for name, t in (("temperature", temperature), ("pressure", pressure),
                         ("vapor_mixing_ratio", qv)):
@@ -286,14 +288,14 @@ class Chain:
     def compute_linoz(self, dt, temperature, pressure, pdel, climatology, zenith_angle,
                       latitude, ltrop, o3, conf, column_above_top=None, o3_column=None,
                       do3_linoz=None, do3_linoz_psc=None, ss_o3=None, o3_sfcsink=None,
-                      layout=_abi.LAYOUT_LEV_COL, stream=None):
+                      layout=_abi.LAYOUT_LEV_COL, stream=None, live=None):
         """Stratospheric ozone (mam4x_compute_linoz): planes T, p, pdel, o3
         (advanced in place) and the optional output planes; climatology
         [8][...]; per column zenith_angle, latitude, ltrop (int32),
         column_above_top and o3_sfcsink; conf: dict of chlorine_loading,
-        psc_T, o3_sfc, o3_tau, o3_lbl."""
+        psc_T, o3_sfc, o3_tau, o3_lbl.  live: see _extents."""
         a = _abi.LinozArgs()
-        a.ncol, a.nlev, a.ldc = self._extents(temperature, layout)
+        a.ncol, a.nlev, a.ldc = self._extents(temperature, layout, live)
         a.layout = layout
         for name, t in (("temperature", temperature), ("pressure", pressure), ("pdel", pdel),
                         ("o3", o3), ("o3_column", o3_column), ("do3_linoz", do3_linoz),
@@ -321,13 +323,14 @@ class Chain:
         self._check(rc, "mam4x_compute_linoz")
 
     def compute_washout(self, dt, planes, latitude, phis, het_rates,
-                        layout=_abi.LAYOUT_LEV_COL, stream=None):
+                        layout=_abi.LAYOUT_LEV_COL, stream=None, live=None):
         """Washout rates (mam4x_compute_washout, sethet): planes: dict of
         pressure, zmid, temperature, cmfdqr, nrain, nevapr, air_density,
-        h2o2 tensors; latitude, phis [ncol]; het_rates [31][...] output."""
+        h2o2 tensors; latitude, phis [ncol]; het_rates [31][...] output.
+        live: see _extents."""
         a = _abi.SethetArgs()
         t = planes["temperature"]
-        a.ncol, a.nlev, a.ldc = self._extents(t, layout)
+        a.ncol, a.nlev, a.ldc = self._extents(t, layout, live)
         a.layout = layout
         for k in ("pressure", "zmid", "temperature", "cmfdqr", "nrain", "nevapr",
                   "air_density", "h2o2"):
@@ -345,14 +348,14 @@ class Chain:
         self._check(rc, "mam4x_compute_washout")
 
     def compute_forcing(self, sectors, extfrc, frc_ndx=None, file_alt_data=None,
-                        layout=_abi.LAYOUT_LEV_COL, stream=None):
+                        layout=_abi.LAYOUT_LEV_COL, stream=None, live=None):
         """External forcing (mam4x_compute_forcing, setext): sectors: 9 lists
         (one per forced species) of up to 4 planes; extfrc [9][...] output;
         frc_ndx: 1-based destination entries (default 1..9); file_alt_data:
-        per species, data stored bottom-up."""
+        per species, data stored bottom-up.  live: see _extents."""
         a = _abi.SetextArgs()
         t = extfrc[0]
-        a.ncol, a.nlev, a.ldc = self._extents(t, layout)
+        a.ncol, a.nlev, a.ldc = self._extents(t, layout, live)
         a.layout = layout
         assert len(sectors) == _abi.EXTCNT
         for m in range(_abi.EXTCNT):
@@ -370,20 +373,17 @@ class Chain:
         self._check(rc, "mam4x_compute_forcing")
 
     def compute_setsox(self, dt, met, qcw, qin, dqdt_aqso4=None, dqdt_aqh2so4=None,
-                       layout=_abi.LAYOUT_LEV_COL, stream=None):
+                       layout=_abi.LAYOUT_LEV_COL, stream=None, live=None):
         """Aqueous sulfur chemistry (mam4x_compute_setsox).  met: dict of
         planes pressure, temperature, liquid_water, cloud_fraction,
         droplet_number, air_number_density; qcw / qin: [31][nlev][ncol]
         tensors (gas_pcnst order) updated in place; dqdt_*: optional
-        [31][nlev][ncol] outputs ([ncol][nlev] planes for LAYOUT_COL_LEV)."""
+        [31][nlev][ncol] outputs ([ncol][nlev] planes for LAYOUT_COL_LEV).
+        live: see _extents."""
         a = _abi.SetsoxArgs()
         t = met["temperature"]
-        assert t.dim() == 2 and t.dtype == torch.float64 and t.is_cuda and t.is_contiguous()
-        if layout == _abi.LAYOUT_LEV_COL:
-            a.nlev, a.ncol = t.shape
-        else:
-            a.ncol, a.nlev = t.shape
-        a.ldc, a.layout = t.shape[1], layout
+        a.ncol, a.nlev, a.ldc = self._extents(t, layout, live)
+        a.layout = layout
         for k in ("pressure", "temperature", "liquid_water", "cloud_fraction", "droplet_number",
                   "air_number_density"):
             v = met[k]
