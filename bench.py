@@ -751,6 +751,109 @@ def bench_linoz(args):
     print(json.dumps(res), flush=True)
 
 
+# table_photo's level pass, per wavelength: two eight-corner sums (8 mul + 7
+# add each), the pressure interpolation and the flux (4), the cross-section
+# interpolation (3), the product and its sum (2); per cell the two corner
+# weight sets (2 x 12) and the five weights' clamps (10)
+PHOTO_OPS_PER_WAVELENGTH = 39
+PHOTO_OPS_PER_CELL = 34
+PHOTO_TABLE = dict(nw=67, nump=51, numsza=10, numcolo3=10, numalb=6, np_xs=15, numj=1)
+
+
+def bench_photo(args):
+    """Photolysis rates (mam4x_compute_photolysis, table_photo; --workload
+    photo) over 21,600 synthetic columns x 72 levels with a table of 67
+    wavelengths x 51 pressures x 10 zenith angles x 10 ozone ratios x 6
+    albedos (synthetic values): one call (column pass + level pass) timed with
+    HIP events on the launch stream, the oracle on the host cores."""
+    import numpy as np
+    import torch
+    from mam4xx_amd import _abi
+    from mam4xx_amd import photo_tables as PT
+    from mam4xx_amd.chain import Chain
+    import oracle
+    rng = np.random.default_rng(0)
+    T = PHOTO_TABLE
+    ax = PT.wide_axes(nump=T["nump"], numsza=T["numsza"], numcolo3=T["numcolo3"],
+                      numalb=T["numalb"], np_xs=T["np_xs"])
+    tab = PT.random_table(rng, ax, nw=T["nw"], numj=T["numj"])
+    cols = PT.columns(rng, NLEV, NCOL, ax)
+    dev = lambda a: torch.as_tensor(np.ascontiguousarray(a, dtype=np.float64)).cuda()
+    dtab = {k: dev(v) for k, v in tab.items()}
+    dtab["lng_index"] = 0
+    names = dict(pressure="pmid", pdel="pdel", temperature="temp", o3_column="colo3",
+                 liquid_water="lwc", cloud_fraction="clouds")
+    planes = {k: dev(cols[v]) for k, v in names.items()}
+    zen, alb = dev(cols["zen_angle"]), dev(cols["srf_alb"])
+    photo0 = dev(cols["photo"])
+    photo = photo0.clone()
+    work = torch.empty((2, NLEV, NCOL), dtype=torch.float64, device="cuda")
+    ch = Chain()
+    s = torch.cuda.current_stream()
+
+    def call():
+        ch.compute_photolysis(dtab, planes, zen, alb, photo, esfact=1.0, work=work)
+
+    for _ in range(args.warmup + 1):
+        photo.copy_(photo0)
+        call()
+    torch.cuda.synchronize()
+    ev = [torch.cuda.Event(enable_timing=True) for _ in range(2)]
+    tot = 0.0
+    for _ in range(args.steps):
+        photo.copy_(photo0)  # every call updates the same rates
+        ev[0].record(s)
+        call()
+        ev[1].record(s)
+        ev[1].synchronize()
+        tot += ev[0].elapsed_time(ev[1])
+    ms = tot / args.steps
+    day = np.rad2deg(cols["zen_angle"]) < 88.85
+    ncell_day = int(day.sum()) * NLEV
+    flops = ncell_day * (PHOTO_OPS_PER_WAVELENGTH * T["nw"] + PHOTO_OPS_PER_CELL)
+    achieved = flops / (ms * 1e-3) / 1e12
+    # HBM values per daylight cell: the column pass reads cloud fraction,
+    # lwc and pdel and writes the two work planes; the level pass reads
+    # pmid, T, the ozone column, the two work planes and the rate and writes
+    # the rate (the tables stay in the caches)
+    bytes_cell = 8 * (3 + 2 + 6 + 1)
+    res = {"metric": "photolysis rates (table_photo): column-steps/sec at 72 lev",
+           "unit": "column-steps/s", "value": NCOL / (ms * 1e-3), "n_gpus": 1,
+           "higher_is_better": True, "dtype": "f64", "data": "synthetic", "steps": args.steps,
+           "warmup": args.warmup,
+           "config": {"workload": "21600 columns x 72 levels (%d in daylight), table %d "
+                                  "wavelengths x %d pressures x %d zenith angles x %d ozone "
+                                  "ratios x %d albedos, one photolysis reaction"
+                                  % (int(day.sum()), T["nw"], T["nump"], T["numsza"],
+                                     T["numcolo3"], T["numalb"]),
+                      "columns": NCOL, "levels": NLEV},
+           "call_ms": ms,
+           "roofline": {"bound": "valu", "achieved": achieved, "peak": FP64_PEAK_TFLOPS,
+                        "unit": "TFLOP/s", "frac": achieved / FP64_PEAK_TFLOPS,
+                        "traffic": _traffic("photo_level_kernel_%dx%d.txt" % (NCOL, NLEV)),
+                        "kernel": "photo_column_kernel + photo_level_kernel",
+                        "algorithmic_gflop_per_call": flops / 1e9,
+                        "hbm_algorithmic_gbs": ncell_day * bytes_cell / (ms * 1e-3) / 1e9,
+                        "algorithmic_bytes_per_cell": bytes_cell}}
+    if not args.no_cpu:
+        orc = oracle.Oracle()
+        nth = min(16, os.cpu_count() or 1)
+        import oracle.pyoracle as P
+        otab = P.photo_table(tab, lng_index=0)
+        run = lambda: orc.table_photo(otab, cols, 1.0)
+        run()
+        best = 1e30
+        for _ in range(3):
+            t0 = time.perf_counter()
+            run()
+            best = min(best, time.perf_counter() - t0)
+        res["cpu_baseline"] = {"value": NCOL / best, "unit": "column-steps/s", "cores": nth,
+                               "kind": "port", "sample": "the full 21600 x 72 domain, best of 3 "
+                               "after a warm-up (OpenMP over columns)", "cpu": _cpu_model()}
+    ch.close()
+    print(json.dumps(res), flush=True)
+
+
 def bench_washout(args):
     """Washout rates (mam4x_compute_washout, sethet; --workload washout) over
     21,600 synthetic columns x 72 levels: kernel time from HIP events on the
@@ -1155,13 +1258,14 @@ def main():
                     help="headline: time the FP64 build only")
     ap.add_argument("--workload", choices=["headline", "batch", "large", "water", "wetsize",
                                            "tracers", "setsox", "gaschem", "linoz", "washout",
-                                           "valu", "adjoint"],
+                                           "photo", "valu", "adjoint"],
                     default="headline",
                     help="headline: the metric's 21,600 x 72 step (default); batch, large, "
                          "water (aerosol water uptake), wetsize (calcsize + water uptake), "
                          "tracers (state_q/qqcw <-> state planes), setsox (aqueous sulfur "
                          "chemistry), gaschem (gas-phase chemistry) and linoz (stratospheric "
-                         "ozone, with the invariant densities), washout (sethet), valu "
+                         "ozone, with the invariant densities), washout (sethet), photo "
+                         "(table_photo), valu "
                          "(scalar FP64 / FP32 VALU rates) and adjoint (tangent and reverse "
                          "sweep against the step) print their own line")
     args = ap.parse_args()
@@ -1191,6 +1295,8 @@ def main():
         return bench_linoz(args)
     if args.workload == "washout":
         return bench_washout(args)
+    if args.workload == "photo":
+        return bench_photo(args)
     if args.workload == "gaschem":
         return bench_gas_chem(args)
     if args.workload == "valu":

@@ -40,7 +40,7 @@
 extern "C" {
 #endif
 
-#define MAM4X_ABI_VERSION 9
+#define MAM4X_ABI_VERSION 10
 #define MAM4X_MAX_NTSUBSTEP 64
 
 /* per-process tendency diagnostics (mam4x_outputs.process_tendencies) */
@@ -592,6 +592,67 @@ typedef struct mam4x_setext_args {
 
 /* setext over a domain.  FP64; stream-ordered; no workspace. */
 int mam4x_compute_forcing(mam4x_handle h, const mam4x_setext_args *args, void *stream);
+
+/* ---------------------------------------------------------------------- */
+/* Photolysis rates: mam4::mo_photo::table_photo (mo_photo.hpp:798-893)    */
+/* for every column -- the photolysis_rate the gas-phase chemistry reads.  */
+/* cloud_mod (:169-427) forms each level's effective albedo and cloud      */
+/* multiplier from the cloud optical depths above and below it; jlong     */
+/* (:647-796) interpolates the radiative source function (interpolate_rsf, */
+/* :498-645) in zenith angle, ozone column, albedo and pressure, and the   */
+/* cross sections in temperature and pressure, and sums their product over */
+/* wavelength for the mechanism's one photolysis reaction (phtcnt = 1);    */
+/* the rate becomes cld_mult esfact (photo + pht_alias_mult j).  Columns   */
+/* in darkness (zenith angle >= 88.85 degrees) are left as they are.       */
+/* Level 0 is the model top.  Planes: extents and layout as in mam4x_inputs. */
+typedef struct mam4x_photo_table {
+  /* PhotoTableData (mo_photo.hpp:28-56) dimensions: wavelengths, the cross
+   * sections' temperatures (>= 201: the index runs to 200) and pressures,
+   * photorates in xsqy, and the source function's pressures, zenith
+   * angles, ozone-column ratios and albedos (each >= 2) */
+  int nw, nt, np_xs, numj, nump, numsza, numcolo3, numalb;
+  /* device arrays, row-major: xsqy [numj][nw][nt][np_xs], rsf_tab [nw][nump]
+   * [numsza][numcolo3][numalb]; axes sza [numsza] (deg, increasing), alb
+   * [numalb] (increasing), press [nump] (hPa, decreasing), colo3 [nump]
+   * (the ozone column at each press), o3rat [numcolo3] (increasing), prs
+   * [np_xs] (hPa, decreasing); del_* and dprs: the reciprocals of each
+   * axis's spacings (n - 1 each, positive); etfphot [nw] */
+  const double *xsqy;
+  const double *sza, *del_sza, *alb, *del_alb, *press, *del_p, *colo3, *o3rat, *del_o3rat;
+  const double *etfphot, *prs, *dprs;
+  const double *rsf_tab;
+  double pht_alias_mult; /* pht_alias_mult_1 of the reaction                  */
+  int lng_index;         /* its lng_indexer entry: the xsqy photorate
+                            (0 .. numj - 1); -1 leaves every rate as it is   */
+} mam4x_photo_table;
+
+typedef struct mam4x_photo_args {
+  int ncol, nlev;
+  int ldc;
+  int layout;
+  const double *pressure;       /* mid-level [Pa]                           */
+  const double *pdel;           /* layer thickness [Pa]                     */
+  const double *temperature;    /* [K]                                      */
+  const double *o3_column;      /* colo3 [molecules/cm2], the overhead ozone
+                                   column (mam4x_compute_linoz's o3_column)  */
+  const double *liquid_water;   /* lwc [kg/kg]                              */
+  const double *cloud_fraction; /* [fraction]                               */
+  const double *zenith_angle;   /* per column [ncol] [rad]                  */
+  const double *surface_albedo; /* per column [ncol]                        */
+  double esfact;                /* earth-sun distance factor               */
+  double *photo;                /* rates [1/s], in and out                  */
+  /* device scratch for the effective albedo and cloud multiplier: two
+   * planes of the domain's layout and ldc, one after the other (2 x nlev x
+   * ldc doubles for LAYOUT_LEV_COL, 2 x ncol x ldc for LAYOUT_COL_LEV) --
+   * the reference's PhotoTableWorkArrays */
+  double *work;
+} mam4x_photo_args;
+
+/* table_photo over a domain: a column kernel (one lane per column) forms
+ * the albedos and multipliers into work, then a level kernel (one lane per
+ * cell) the rates.  FP64; stream-ordered; no allocation. */
+int mam4x_compute_photolysis(mam4x_handle h, const mam4x_photo_table *table,
+                             const mam4x_photo_args *args, void *stream);
 
 /* Batches: many scenes (domains of any column count, each with its own
  * inputs, outputs and layout) stepped by one set of launches.  The scenes'

@@ -6,7 +6,7 @@ The structures below must match the C declarations field for field; the
 import ctypes
 import os
 
-ABI_VERSION = 9
+ABI_VERSION = 10
 
 NSTATE = 52
 NMODE = 4
@@ -245,6 +245,37 @@ class LinozArgs(ctypes.Structure):
     ]
 
 
+PHOTO_TABLE_DIMS = ("nw", "nt", "np_xs", "numj", "nump", "numsza", "numcolo3", "numalb")
+PHOTO_TABLE_ARRAYS = ("xsqy", "sza", "del_sza", "alb", "del_alb", "press", "del_p", "colo3",
+                      "o3rat", "del_o3rat", "etfphot", "prs", "dprs", "rsf_tab")
+
+
+class PhotoTable(ctypes.Structure):
+    _fields_ = [(n, ctypes.c_int) for n in PHOTO_TABLE_DIMS] + \
+        [(n, _vp) for n in PHOTO_TABLE_ARRAYS] + \
+        [("pht_alias_mult", ctypes.c_double), ("lng_index", ctypes.c_int)]
+
+
+class PhotoArgs(ctypes.Structure):
+    _fields_ = [
+        ("ncol", ctypes.c_int),
+        ("nlev", ctypes.c_int),
+        ("ldc", ctypes.c_int),
+        ("layout", ctypes.c_int),
+        ("pressure", _vp),
+        ("pdel", _vp),
+        ("temperature", _vp),
+        ("o3_column", _vp),
+        ("liquid_water", _vp),
+        ("cloud_fraction", _vp),
+        ("zenith_angle", _vp),
+        ("surface_albedo", _vp),
+        ("esfact", ctypes.c_double),
+        ("photo", _vp),
+        ("work", _vp),
+    ]
+
+
 class SethetArgs(ctypes.Structure):
     _fields_ = [
         ("ncol", ctypes.c_int),
@@ -317,6 +348,8 @@ SIGNATURES = {
                                                    ctypes.POINTER(GasChemArgs), _vp]),
     "mam4x_compute_invariants": (ctypes.c_int, [_vp, ctypes.POINTER(SetinvArgs), _vp]),
     "mam4x_compute_linoz": (ctypes.c_int, [_vp, ctypes.c_double, ctypes.POINTER(LinozArgs), _vp]),
+    "mam4x_compute_photolysis": (ctypes.c_int, [_vp, ctypes.POINTER(PhotoTable),
+                                                ctypes.POINTER(PhotoArgs), _vp]),
     "mam4x_compute_washout": (ctypes.c_int, [_vp, ctypes.c_double, ctypes.POINTER(SethetArgs),
                                              _vp]),
     "mam4x_compute_forcing": (ctypes.c_int, [_vp, ctypes.POINTER(SetextArgs), _vp]),

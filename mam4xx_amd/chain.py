@@ -322,6 +322,51 @@ class Chain:
                                           ctypes.c_void_p(s.cuda_stream))
         self._check(rc, "mam4x_compute_linoz")
 
+    def compute_photolysis(self, table, planes, zenith_angle, surface_albedo, photo,
+                           esfact=1.0, work=None, layout=_abi.LAYOUT_LEV_COL, stream=None,
+                           live=None):
+        """Photolysis rates (mam4x_compute_photolysis, table_photo).  table:
+        dict of the mam4x_photo_table arrays as CUDA float64 tensors
+        (rsf_tab [nw, nump, numsza, numcolo3, numalb], xsqy [numj, nw, nt,
+        np_xs], the axes) and optionally pht_alias_mult, lng_index; planes:
+        dict of pressure, pdel, temperature, o3_column, liquid_water,
+        cloud_fraction tensors; zenith_angle, surface_albedo [ncol]; photo:
+        plane updated in place; work: scratch of 2 planes (allocated when
+        None).  live: see _extents."""
+        tb = _abi.PhotoTable()
+        for k in _abi.PHOTO_TABLE_ARRAYS:
+            v = table[k]
+            assert v.dtype == torch.float64 and v.is_cuda and v.is_contiguous(), k
+            setattr(tb, k, v.data_ptr())
+        tb.numj, tb.nw, tb.nt, tb.np_xs = table["xsqy"].shape
+        _, tb.nump, tb.numsza, tb.numcolo3, tb.numalb = table["rsf_tab"].shape
+        tb.pht_alias_mult = float(table.get("pht_alias_mult", 1.0))
+        tb.lng_index = int(table.get("lng_index", 0))
+        a = _abi.PhotoArgs()
+        a.ncol, a.nlev, a.ldc = self._extents(photo, layout, live)
+        a.layout = layout
+        for k in ("pressure", "pdel", "temperature", "o3_column", "liquid_water",
+                  "cloud_fraction"):
+            v = planes[k]
+            assert v.shape == photo.shape and v.dtype == torch.float64 and v.is_contiguous(), k
+            setattr(a, k, v.data_ptr())
+        for name, v in (("zenith_angle", zenith_angle), ("surface_albedo", surface_albedo)):
+            assert v.shape == (a.ncol,) and v.dtype == torch.float64 and v.is_contiguous()
+            setattr(a, name, v.data_ptr())
+        a.esfact = float(esfact)
+        a.photo = photo.data_ptr()
+        if work is None:
+            work = torch.empty((2,) + tuple(photo.shape), dtype=torch.float64,
+                               device=photo.device)
+        assert work.dtype == torch.float64 and work.is_contiguous() and \
+            work.numel() >= 2 * photo.numel()
+        a.work = work.data_ptr()
+        s = stream if stream is not None else torch.cuda.current_stream()
+        rc = self.lib.mam4x_compute_photolysis(self.h, ctypes.byref(tb), ctypes.byref(a),
+                                               ctypes.c_void_p(s.cuda_stream))
+        self._check(rc, "mam4x_compute_photolysis")
+        return work
+
     def compute_washout(self, dt, planes, latitude, phis, het_rates,
                         layout=_abi.LAYOUT_LEV_COL, stream=None, live=None):
         """Washout rates (mam4x_compute_washout, sethet): planes: dict of

@@ -38,6 +38,7 @@
 #include "chain_cell.hpp"
 #include "gas_chem.hpp"
 #include "linoz.hpp"
+#include "photo.hpp"
 #include "sethet.hpp"
 #include "setsox.hpp"
 #include "water_uptake.hpp"
@@ -1506,6 +1507,49 @@ __global__ void __launch_bounds__(kBlock) setext_kernel(SetextArgs a) {
   }
 }
 
+// ------------------------------------------------------------------ photolysis
+// table_photo (photo.hpp): the column pass forms each daylight column's
+// effective albedo and cloud multiplier into the two work planes, the level
+// pass the rates.
+struct PhotoArgs {
+  pht::Table t;
+  const double *pmid, *pdel, *T, *colo3, *lwc, *cld, *zen, *alb;
+  double *photo, *eff, *mult;  // eff, mult: the work planes
+  double esfact;
+  int ncol, nlev, ldc, layout;
+};
+
+constexpr int kPhotoColBlock = 64;
+
+__global__ void __launch_bounds__(kPhotoColBlock) photo_column_kernel(PhotoArgs a) {
+  const int j = blockIdx.x * kPhotoColBlock + threadIdx.x;
+  if (j >= a.ncol) return;
+  const double zen = a.zen[j];
+  if (!pht::daylight(pht::sza_deg(zen))) return;
+  const bool lc = a.layout == MAM4X_LAYOUT_LEV_COL;
+  const long base = lc ? (long)j : (long)j * a.ldc;
+  const long ks = lc ? (long)a.ldc : 1L;
+  pht::cloud_column(a.nlev, ks, zen, a.cld + base, a.lwc + base, a.pdel + base, a.alb[j],
+                    a.eff + base, a.mult + base);
+}
+
+__global__ void __launch_bounds__(kBlock) photo_level_kernel(PhotoArgs a) {
+  const long c = (long)blockIdx.x * kBlock + threadIdx.x;
+  if (c >= (long)a.ncol * a.nlev) return;
+  const bool cl = a.layout == MAM4X_LAYOUT_COL_LEV;
+  const int fast = cl ? a.nlev : a.ncol;
+  const long slow = c / fast;
+  const int f = (int)(c - slow * fast);
+  const int j = cl ? (int)slow : f;
+  const long off = slow * a.ldc + f;
+  const double sza = pht::sza_deg(a.zen[j]);
+  if (!pht::daylight(sza)) return;
+  const pht::SzaPos s = pht::sza_pos(a.t, sza);
+  const double jr =
+      pht::level_rate(a.t, s, a.eff[off], a.pmid[off] * pht::kPaToMb, a.T[off], a.colo3[off]);
+  a.photo[off] = pht::update(a.t, a.photo[off], a.mult[off], a.esfact, jr);
+}
+
 }  // namespace
 
 enum { kPre = 0, kPost = 1, kFused = 2 };
@@ -2720,6 +2764,63 @@ int mam4x_compute_forcing(mam4x_handle h, const mam4x_setext_args *in, void *str
   const long ncell = (long)in->ncol * in->nlev;
   if (ncell == 0) return MAM4X_OK;
   hipLaunchKernelGGL(setext_kernel, dim3((unsigned)((ncell + kBlock - 1) / kBlock)),
+                     dim3(kBlock), 0, (hipStream_t)stream, a);
+  HIP_TRY(h, hipGetLastError());
+  return MAM4X_OK;
+}
+
+int mam4x_compute_photolysis(mam4x_handle h, const mam4x_photo_table *tb,
+                             const mam4x_photo_args *in, void *stream) {
+  if (!h) return MAM4X_ERR_ARG;
+  if (!tb || !in) return fail(h, MAM4X_ERR_ARG, "table or args is NULL");
+  int ld = 0;
+  if (int rc = plane_extents(h, in->ncol, in->nlev, in->layout, in->ldc, ld)) return rc;
+  if (tb->nw < 1 || tb->numj < 1 || tb->nt < pht::kNt || tb->np_xs < 2 || tb->nump < 2 ||
+      tb->numsza < 2 || tb->numcolo3 < 2 || tb->numalb < 2)
+    return fail(h, MAM4X_ERR_ARG,
+                "table dimensions: nw, numj >= 1, nt >= 201, np_xs, nump, numsza, numcolo3, "
+                "numalb >= 2");
+  if (tb->lng_index < -1 || tb->lng_index >= tb->numj)
+    return fail(h, MAM4X_ERR_ARG, "lng_index outside [-1, numj)");
+  const double *tp[14] = {tb->xsqy,  tb->sza,   tb->del_sza, tb->alb,       tb->del_alb,
+                          tb->press, tb->del_p, tb->colo3,   tb->o3rat,     tb->del_o3rat,
+                          tb->etfphot, tb->prs, tb->dprs,    tb->rsf_tab};
+  for (int i = 0; i < 14; ++i)
+    if (!tp[i]) return fail(h, MAM4X_ERR_ARG, "a table array is NULL");
+  PhotoArgs a;
+  std::memset(&a, 0, sizeof(a));
+  a.t = pht::Table{tb->nw,    tb->nt,      tb->np_xs,   tb->numj,     tb->nump,
+                   tb->numsza, tb->numcolo3, tb->numalb, tb->xsqy,     tb->sza,
+                   tb->del_sza, tb->alb,     tb->del_alb, tb->press,    tb->del_p,
+                   tb->colo3,  tb->o3rat,   tb->del_o3rat, tb->etfphot, tb->prs,
+                   tb->dprs,   tb->rsf_tab, tb->pht_alias_mult, tb->lng_index};
+  a.pmid = in->pressure;
+  a.pdel = in->pdel;
+  a.T = in->temperature;
+  a.colo3 = in->o3_column;
+  a.lwc = in->liquid_water;
+  a.cld = in->cloud_fraction;
+  a.zen = in->zenith_angle;
+  a.alb = in->surface_albedo;
+  a.photo = in->photo;
+  if (!a.pmid || !a.pdel || !a.T || !a.colo3 || !a.lwc || !a.cld || !a.zen || !a.alb ||
+      !a.photo || !in->work)
+    return fail(h, MAM4X_ERR_ARG, "a plane, zenith_angle, surface_albedo or work is NULL");
+  const bool lc = in->layout == MAM4X_LAYOUT_LEV_COL;
+  const long slow = lc ? in->nlev : in->ncol;
+  a.eff = in->work;
+  a.mult = in->work + slow * ld;
+  a.esfact = in->esfact;
+  a.ncol = in->ncol;
+  a.nlev = in->nlev;
+  a.ldc = ld;
+  a.layout = in->layout;
+  const long ncell = (long)in->ncol * in->nlev;
+  if (ncell == 0 || tb->lng_index < 0) return MAM4X_OK;
+  hipLaunchKernelGGL(photo_column_kernel,
+                     dim3((unsigned)((in->ncol + kPhotoColBlock - 1) / kPhotoColBlock)),
+                     dim3(kPhotoColBlock), 0, (hipStream_t)stream, a);
+  hipLaunchKernelGGL(photo_level_kernel, dim3((unsigned)((ncell + kBlock - 1) / kBlock)),
                      dim3(kBlock), 0, (hipStream_t)stream, a);
   HIP_TRY(h, hipGetLastError());
   return MAM4X_OK;

@@ -64,6 +64,7 @@ static inline double nz(double v) {
 #define M_erfc(x) nz(erfc(x))
 #define M_sqrt(x) nz(sqrt(x))
 #define M_log10(x) nz(log10(x))
+#define M_cos(x) nz(cos(x))
 
 /* ---------------- constants (mam4_constants.hpp, aero_modes.hpp) -------- */
 static const double PI = 3.14159265358979323846264;
@@ -3784,4 +3785,259 @@ void orc_setext_column(int nlev, const int frc_ndx[9], const int file_alt_data[9
           frcing[(long)nn * nlev + kk] += sectors[mm][isec][kk];
       }
     }
+}
+
+/* ================ photolysis (mo_photo.hpp) ==============================
+ * table_photo (mo_photo.hpp:798-893) for one column: cloud_mod (:169-427)
+ * forms the effective albedo and the cloud multiplier, jlong (:647-796)
+ * interpolates the radiative source function (interpolate_rsf, :498-645,
+ * with find_index :428-450 and calc_sum_wght :452-496) and the cross
+ * sections, and the rate of the one photolysis reaction (phtcnt = 1) is
+ * photo = cld_mult esfact (photo + pht_alias_mult jlong[lng_index]).
+ * Tables are row-major: rsf_tab [nw][nump][numsza][numcolo3][numalb],
+ * xsqy [numj][nw][nt][np_xs].  Level 0 is the model top. */
+
+/* mo_photo.hpp:428-450: the index before the first entry above vmin
+ * (clamped to [0, len - 2]); idx0 when no entry is above vmin */
+int orc_photo_find_index(const double *var, int len, double vmin, int idx0) {
+  for (int ii = 0; ii < len; ++ii)
+    if (var[ii] > vmin) {
+      const int m = ii < len - 1 ? ii : len - 1;
+      return m - 1 > 0 ? m - 1 : 0;
+    }
+  return idx0;
+}
+
+static double pht_bound(double lo, double hi, double x) { return kmax(lo, kmin(hi, x)); }
+
+/* mo_photo.hpp:169-427, the column's scans in level order (the serial
+ * loops its comments give) */
+void orc_photo_cloud_mod(int nlev, double zen_angle, const double *clouds, const double *lwc,
+                         const double *delp, double srf_alb, double *eff_alb,
+                         double *cld_mult) {
+  const double rgrav = 1.0 / 9.80616, f_lwp2tau = .155, tau_min = 5.0;
+  const double C1 = 11.905, C2 = 9.524;
+  double *w = (double *)malloc(sizeof(double) * 5 * (size_t)(nlev > 0 ? nlev : 1));
+  double *del_tau = w, *above_tau = w + nlev, *above_cld = w + 2 * nlev,
+         *below_tau = w + 3 * nlev, *below_cld = w + 4 * nlev;
+  for (int k = 0; k < nlev; ++k) {
+    if (clouds[k] != 0.0) {
+      const double del_lwp = rgrav * lwc[k] * delp[k] * 1000.0 / clouds[k];
+      del_tau[k] = del_lwp * f_lwp2tau * M_pow(clouds[k], 1.5);
+    } else {
+      del_tau[k] = 0.0;
+    }
+  }
+  /* top down */
+  above_tau[0] = 0.0;
+  above_cld[0] = 0.0;
+  for (int k = 0; k < nlev - 1; ++k) {
+    above_tau[k + 1] = del_tau[k] + above_tau[k];
+    above_cld[k + 1] = clouds[k] * del_tau[k] + above_cld[k];
+  }
+  for (int k = 1; k < nlev; ++k)
+    if (above_tau[k] != 0.0) above_cld[k] /= above_tau[k];
+  for (int k = 1; k < nlev; ++k)
+    if (above_tau[k] == 0.0) above_cld[k] = above_cld[k - 1];
+  /* bottom up */
+  below_tau[nlev - 1] = 0.0;
+  below_cld[nlev - 1] = 0.0;
+  for (int k = nlev - 2; k >= 0; --k) {
+    below_tau[k] = del_tau[k + 1] + below_tau[k + 1];
+    below_cld[k] = clouds[k + 1] * del_tau[k + 1] + below_cld[k + 1];
+  }
+  for (int k = nlev - 2; k >= 0; --k)
+    if (below_tau[k] != 0.0) below_cld[k] /= below_tau[k];
+  for (int k = nlev - 2; k >= 0; --k)
+    if (below_tau[k] == 0.0) below_cld[k] = below_cld[k + 1];
+  /* optical depths of the cloudy part (jfm) */
+  for (int k = 1; k < nlev; ++k)
+    if (above_cld[k] != 0.0) above_tau[k] /= above_cld[k];
+  for (int k = 1; k < nlev; ++k)
+    if (above_tau[k] < tau_min) above_cld[k] = 0.0;
+  for (int k = 0; k < nlev - 1; ++k)
+    if (below_cld[k] != 0.0) below_tau[k] /= below_cld[k];
+  for (int k = 0; k < nlev - 1; ++k)
+    if (below_tau[k] < tau_min) below_cld[k] = 0.0;
+  /* transmission factors */
+  const double coschi = kmax(M_cos(zen_angle), 0.5);
+  for (int k = 0; k < nlev; ++k) {
+    const double below_tra = C1 / (C2 + below_tau[k]);
+    eff_alb[k] = srf_alb + below_cld[k] * (1.0 - below_tra) * (1.0 - srf_alb);
+    double del_lwp = 0.0;
+    if (clouds[k] != 0.0) del_lwp = rgrav * lwc[k] * delp[k] * 1000.0 / clouds[k];
+    double fac1 = 0.0;
+    if (del_lwp * f_lwp2tau >= tau_min) fac1 = 1.4 * coschi - 1.0;
+    const double above_tra = C1 / (C2 + above_tau[k]);
+    const double fac2 = kmin(0.0, 1.6 * coschi * above_tra - 1.0);
+    cld_mult[k] = kmax(.05, 1.0 + fac1 * clouds[k] + fac2 * above_cld[k]);
+  }
+  free(w);
+}
+
+/* mo_photo.hpp:452-496: the eight-corner weighted sum for one wavelength */
+static double pht_sum_wght(const orc_photo_table *t, const double dels[3], double wrk0, int iz,
+                           int is, int iv, int ial, int wn) {
+  const long s_al = 1, s_v = t->numalb, s_s = (long)t->numcolo3 * s_v, s_z = (long)t->numsza * s_s,
+             s_w = (long)t->nump * s_z;
+  const double *r = t->rsf_tab + wn * s_w + iz * s_z;
+  double wrk1 = (1.0 - dels[1]) * (1.0 - dels[2]);
+  const double w000 = wrk0 * wrk1, w100 = dels[0] * wrk1;
+  wrk1 = (1.0 - dels[1]) * dels[2];
+  const double w001 = wrk0 * wrk1, w101 = dels[0] * wrk1;
+  wrk1 = dels[1] * (1.0 - dels[2]);
+  const double w010 = wrk0 * wrk1, w110 = dels[0] * wrk1;
+  wrk1 = dels[1] * dels[2];
+  const double w011 = wrk0 * wrk1, w111 = dels[0] * wrk1;
+  const long a0 = is * s_s, a1 = (is + 1) * s_s, v0 = iv * s_v, v1 = (iv + 1) * s_v;
+  const long l0 = ial * s_al, l1 = (ial + 1) * s_al;
+  return w000 * r[a0 + v0 + l0] + w001 * r[a0 + v0 + l1] + w010 * r[a0 + v1 + l0] +
+         w011 * r[a0 + v1 + l1] + w100 * r[a1 + v0 + l0] + w101 * r[a1 + v0 + l1] +
+         w110 * r[a1 + v1 + l0] + w111 * r[a1 + v1 + l1];
+}
+
+/* mo_photo.hpp:498-645: rsf [nw][nlev] */
+void orc_photo_interpolate_rsf(const orc_photo_table *t, int nlev, const double *alb_in,
+                               double sza_in, const double *p_in, const double *colo3_in,
+                               double *rsf) {
+  const int is = orc_photo_find_index(t->sza, t->numsza, sza_in, 0);
+  double dels[3];
+  dels[0] = pht_bound(0.0, 1.0, (sza_in - t->sza[is]) * t->del_sza[is]);
+  const double wrk0 = 1.0 - dels[0];
+  for (int kk = 0; kk < nlev; ++kk) {
+    const int albind = orc_photo_find_index(t->alb, t->numalb, alb_in[kk], 0);
+    int pind;
+    double wght1;
+    if (p_in[kk] > t->press[0]) {
+      pind = 1;
+      wght1 = 1.0;
+    } else if (p_in[kk] <= t->press[t->nump - 1]) {
+      pind = t->nump - 1;
+      wght1 = 0.0;
+    } else {
+      int iz;
+      for (iz = 1; iz < t->nump; ++iz)
+        if (t->press[iz] < p_in[kk]) break;
+      iz = iz < t->nump - 1 ? iz : t->nump - 1;
+      pind = iz > 1 ? iz : 1;
+      wght1 = pht_bound(0.0, 1.0, (p_in[kk] - t->press[pind]) * t->del_p[pind - 1]);
+    }
+    const double v3ratu = colo3_in[kk] / t->colo3[pind - 1];
+    const int ratindu = orc_photo_find_index(t->o3rat, t->numcolo3, v3ratu, 0);
+    double v3ratl;
+    int ratindl;
+    if (t->colo3[pind] != 0.0) {
+      v3ratl = colo3_in[kk] / t->colo3[pind];
+      ratindl = orc_photo_find_index(t->o3rat, t->numcolo3, v3ratl, 0);
+    } else {
+      ratindl = ratindu;
+      v3ratl = t->o3rat[ratindu];
+    }
+    const int ial = albind;
+    dels[2] = pht_bound(0.0, 1.0, (alb_in[kk] - t->alb[ial]) * t->del_alb[ial]);
+    const double d1l = pht_bound(0.0, 1.0, (v3ratl - t->o3rat[ratindl]) * t->del_o3rat[ratindl]);
+    const double d1u = pht_bound(0.0, 1.0, (v3ratu - t->o3rat[ratindu]) * t->del_o3rat[ratindu]);
+    for (int wn = 0; wn < t->nw; ++wn) {
+      double d[3] = {dels[0], d1l, dels[2]};
+      const double psum_l = pht_sum_wght(t, d, wrk0, pind, is, ratindl, ial, wn);
+      d[1] = d1u;
+      const double psum_u = pht_sum_wght(t, d, wrk0, pind - 1, is, ratindu, ial, wn);
+      rsf[(long)wn * nlev + kk] = (psum_l + wght1 * (psum_u - psum_l)) * t->etfphot[wn];
+    }
+  }
+}
+
+/* mo_photo.hpp:647-796: j_long [numj][nlev] */
+void orc_photo_jlong(const orc_photo_table *t, int nlev, double sza_in, const double *alb_in,
+                     const double *p_in, const double *t_in, const double *colo3_in,
+                     double *j_long) {
+  double *rsf = (double *)malloc(sizeof(double) * (size_t)t->nw * (nlev > 0 ? nlev : 1));
+  orc_photo_interpolate_rsf(t, nlev, alb_in, sza_in, p_in, colo3_in, rsf);
+  const long s_t = t->np_xs, s_w = (long)t->nt * s_t, s_j = (long)t->nw * s_w;
+  for (int kk = 0; kk < nlev; ++kk) {
+    const int t_index = (int)(kmin(201.0, kmax(t_in[kk] - 148.5, 1.0)) - 1.0);
+    const double ptarget = p_in[kk];
+    for (int i = 0; i < t->numj; ++i) {
+      double suma = 0.0;
+      for (int wn = 0; wn < t->nw; ++wn) {
+        const double *x = t->xsqy + i * s_j + wn * s_w + t_index * s_t;
+        double xswk;
+        if (ptarget >= t->prs[0]) {
+          xswk = x[0];
+        } else if (ptarget <= t->prs[t->np_xs - 1]) {
+          xswk = x[t->np_xs - 1];
+        } else {
+          double delp = 0.0;
+          int pndx = 0;
+          for (int km = 1; km < t->np_xs; ++km)
+            if (ptarget >= t->prs[km]) {
+              pndx = km - 1;
+              delp = (t->prs[pndx] - ptarget) * t->dprs[pndx];
+              break;
+            }
+          xswk = x[pndx] + delp * (x[pndx + 1] - x[pndx]);
+        }
+        suma += xswk * rsf[(long)wn * nlev + kk];
+      }
+      j_long[(long)i * nlev + kk] = suma;
+    }
+  }
+  free(rsf);
+}
+
+/* mo_photo.hpp:798-893 for one column; photo [nlev] in/out (unchanged at
+ * night, sza >= 88.85 degrees, and when lng_index < 0) */
+void orc_table_photo(const orc_photo_table *t, int nlev, double *photo, const double *pmid,
+                     const double *pdel, const double *temper, const double *colo3_in,
+                     double zen_angle, double srf_alb, const double *lwc, const double *clouds,
+                     double esfact) {
+  const double Pa2mb = 1.e-2, r2d = 180.0 / PI, max_zen_angle = 88.85;
+  const double sza_in = zen_angle * r2d;
+  if (!(sza_in >= 0.0 && sza_in < max_zen_angle)) return;
+  const size_t n = (size_t)(nlev > 0 ? nlev : 1);
+  double *w = (double *)malloc(sizeof(double) * n * (3 + (size_t)t->numj));
+  double *eff_alb = w, *cld_mult = w + n, *parg = w + 2 * n, *jl = w + 3 * n;
+  orc_photo_cloud_mod(nlev, zen_angle, clouds, lwc, pdel, srf_alb, eff_alb, cld_mult);
+  for (int k = 0; k < nlev; ++k) parg[k] = pmid[k] * Pa2mb;
+  orc_photo_jlong(t, nlev, sza_in, eff_alb, parg, temper, colo3_in, jl);
+  const int ind = t->lng_index;
+  if (ind > -1)
+    for (int k = 0; k < nlev; ++k)
+      photo[k] = cld_mult[k] * esfact * (photo[k] + t->pht_alias_mult * jl[(long)ind * nlev + k]);
+  free(w);
+}
+
+/* orc_table_photo over a domain of [nlev][ncol] planes (column fastest);
+ * per column zen_angle, srf_alb.  OpenMP over columns. */
+void orc_table_photo_domain(const orc_photo_table *t, int ncol, int nlev, double *photo,
+                            const double *pmid, const double *pdel, const double *temper,
+                            const double *colo3_in, const double *zen_angle,
+                            const double *srf_alb, const double *lwc, const double *clouds,
+                            double esfact) {
+#ifdef _OPENMP
+#pragma omp parallel
+#endif
+  {
+    double *buf = (double *)malloc(sizeof(double) * (size_t)(nlev > 0 ? nlev : 1) * 7);
+    double *ph = buf, *pm = ph + nlev, *pd = pm + nlev, *te = pd + nlev, *co = te + nlev,
+           *lw = co + nlev, *cl = lw + nlev;
+#ifdef _OPENMP
+#pragma omp for schedule(static)
+#endif
+    for (int j = 0; j < ncol; ++j) {
+      for (int k = 0; k < nlev; ++k) {
+        const long i = (long)k * ncol + j;
+        ph[k] = photo[i];
+        pm[k] = pmid[i];
+        pd[k] = pdel[i];
+        te[k] = temper[i];
+        co[k] = colo3_in[i];
+        lw[k] = lwc[i];
+        cl[k] = clouds[i];
+      }
+      orc_table_photo(t, nlev, ph, pm, pd, te, co, zen_angle[j], srf_alb[j], lw, cl, esfact);
+      for (int k = 0; k < nlev; ++k) photo[(long)k * ncol + j] = ph[k];
+    }
+    free(buf);
+  }
 }

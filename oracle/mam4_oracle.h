@@ -296,6 +296,39 @@ void orc_setext_column(int nlev, const int frc_ndx[9], const int file_alt_data[9
                        const int nsectors[9], const double *const sectors[9][4],
                        double *frcing);
 
+/* ---- photolysis (mo_photo.hpp) ---- */
+/* the photolysis tables (PhotoTableData, mo_photo.hpp:28-56) for the one
+ * reaction (phtcnt = 1): row-major rsf_tab [nw][nump][numsza][numcolo3]
+ * [numalb] and xsqy [numj][nw][nt][np_xs]; pht_alias_mult and lng_index are
+ * the reaction's pht_alias_mult_1 and lng_indexer entries */
+typedef struct {
+  int nw, nt, np_xs, numj, nump, numsza, numcolo3, numalb;
+  const double *xsqy;
+  const double *sza, *del_sza, *alb, *del_alb, *press, *del_p, *colo3, *o3rat, *del_o3rat;
+  const double *etfphot, *prs, *dprs;
+  const double *rsf_tab;
+  double pht_alias_mult;
+  int lng_index;
+} orc_photo_table;
+int orc_photo_find_index(const double *var, int len, double vmin, int idx0);
+void orc_photo_cloud_mod(int nlev, double zen_angle, const double *clouds, const double *lwc,
+                         const double *delp, double srf_alb, double *eff_alb, double *cld_mult);
+void orc_photo_interpolate_rsf(const orc_photo_table *t, int nlev, const double *alb_in,
+                               double sza_in, const double *p_in, const double *colo3_in,
+                               double *rsf);
+void orc_photo_jlong(const orc_photo_table *t, int nlev, double sza_in, const double *alb_in,
+                     const double *p_in, const double *t_in, const double *colo3_in,
+                     double *j_long);
+void orc_table_photo(const orc_photo_table *t, int nlev, double *photo, const double *pmid,
+                     const double *pdel, const double *temper, const double *colo3_in,
+                     double zen_angle, double srf_alb, const double *lwc, const double *clouds,
+                     double esfact);
+void orc_table_photo_domain(const orc_photo_table *t, int ncol, int nlev, double *photo,
+                            const double *pmid, const double *pdel, const double *temper,
+                            const double *colo3_in, const double *zen_angle,
+                            const double *srf_alb, const double *lwc, const double *clouds,
+                            double esfact);
+
 #ifdef __cplusplus
 }
 #endif

@@ -51,6 +51,38 @@ def default_config(**kw):
     return OrcConfig(**c)
 
 
+class PhotoTable(ctypes.Structure):
+    """orc_photo_table: dimensions and pointers into numpy arrays that the
+    instance keeps alive (photo_table())."""
+    _fields_ = [(n, ctypes.c_int) for n in ("nw", "nt", "np_xs", "numj", "nump", "numsza",
+                                             "numcolo3", "numalb")] + \
+        [(n, ctypes.POINTER(ctypes.c_double)) for n in (
+            "xsqy", "sza", "del_sza", "alb", "del_alb", "press", "del_p", "colo3", "o3rat",
+            "del_o3rat", "etfphot", "prs", "dprs", "rsf_tab")] + \
+        [("pht_alias_mult", ctypes.c_double), ("lng_index", ctypes.c_int)]
+
+
+PHOTO_ARRAYS = ("xsqy", "sza", "del_sza", "alb", "del_alb", "press", "del_p", "colo3", "o3rat",
+                "del_o3rat", "etfphot", "prs", "dprs", "rsf_tab")
+
+
+def photo_table(arrays, pht_alias_mult=1.0, lng_index=0):
+    """PhotoTable over a dict of numpy arrays: rsf_tab [nw, nump, numsza,
+    numcolo3, numalb], xsqy [numj, nw, nt, np_xs] and the 1-D axes."""
+    t = PhotoTable()
+    keep = {}
+    for k in PHOTO_ARRAYS:
+        a = np.ascontiguousarray(arrays[k], dtype=np.float64)
+        keep[k] = a
+        setattr(t, k, a.ctypes.data_as(ctypes.POINTER(ctypes.c_double)))
+    t.numj, t.nw, t.nt, t.np_xs = keep["xsqy"].shape
+    _, t.nump, t.numsza, t.numcolo3, t.numalb = keep["rsf_tab"].shape
+    t.pht_alias_mult = pht_alias_mult
+    t.lng_index = lng_index
+    t._keep = keep
+    return t
+
+
 def _p(a):
     if a is None:
         return None
@@ -303,6 +335,73 @@ class Oracle:
         self.lib.orc_gas_phase_chemistry_domain(n, dt, _p(t), _p(ph), _p(e), _p(v), _p(h), _p(q),
                                                 st.ctypes.data_as(ctypes.POINTER(ctypes.c_int)))
         return q.reshape((31,) + tuple(shape)), st.reshape(shape)
+
+    # ---- photolysis (mo_photo.hpp) ----
+    def _pht_sigs(self):
+        L = self.lib
+        if getattr(self, "_pht_ready", False):
+            return
+        D, I, T = ctypes.c_double, ctypes.c_int, ctypes.POINTER(PhotoTable)
+        L.orc_photo_find_index.argtypes = [_dp, I, D, I]
+        L.orc_photo_find_index.restype = I
+        L.orc_photo_cloud_mod.argtypes = [I, D, _dp, _dp, _dp, D, _dp, _dp]
+        L.orc_photo_cloud_mod.restype = None
+        L.orc_photo_interpolate_rsf.argtypes = [T, I, _dp, D, _dp, _dp, _dp]
+        L.orc_photo_interpolate_rsf.restype = None
+        L.orc_photo_jlong.argtypes = [T, I, D, _dp, _dp, _dp, _dp, _dp]
+        L.orc_photo_jlong.restype = None
+        L.orc_table_photo.argtypes = [T, I, _dp, _dp, _dp, _dp, _dp, D, D, _dp, _dp, D]
+        L.orc_table_photo.restype = None
+        L.orc_table_photo_domain.argtypes = [T, I, I] + [_dp] * 9 + [D]
+        L.orc_table_photo_domain.restype = None
+        self._pht_ready = True
+
+    def photo_find_index(self, var, vmin, idx0=0):
+        self._pht_sigs()
+        v = np.ascontiguousarray(var, dtype=np.float64)
+        return self.lib.orc_photo_find_index(_p(v), len(v), vmin, idx0)
+
+    def photo_cloud_mod(self, zen_angle, clouds, lwc, delp, srf_alb):
+        """one column [nlev]: (eff_alb, cld_mult)"""
+        self._pht_sigs()
+        c, w, d = (np.ascontiguousarray(a, dtype=np.float64) for a in (clouds, lwc, delp))
+        ea, cm = np.empty_like(c), np.empty_like(c)
+        self.lib.orc_photo_cloud_mod(len(c), zen_angle, _p(c), _p(w), _p(d), srf_alb, _p(ea),
+                                     _p(cm))
+        return ea, cm
+
+    def photo_interpolate_rsf(self, table, alb_in, sza_in, p_in, colo3_in):
+        """one column: rsf [nw, nlev]; table: a PhotoTable (photo_table())"""
+        self._pht_sigs()
+        a, p, c = (np.ascontiguousarray(x, dtype=np.float64) for x in (alb_in, p_in, colo3_in))
+        out = np.empty((table.nw, len(a)))
+        self.lib.orc_photo_interpolate_rsf(ctypes.byref(table), len(a), _p(a), sza_in, _p(p),
+                                           _p(c), _p(out))
+        return out
+
+    def photo_jlong(self, table, sza_in, alb_in, p_in, t_in, colo3_in):
+        """one column: j_long [numj, nlev]"""
+        self._pht_sigs()
+        a, p, t, c = (np.ascontiguousarray(x, dtype=np.float64)
+                      for x in (alb_in, p_in, t_in, colo3_in))
+        out = np.empty((table.numj, len(a)))
+        self.lib.orc_photo_jlong(ctypes.byref(table), len(a), sza_in, _p(a), _p(p), _p(t),
+                                 _p(c), _p(out))
+        return out
+
+    def table_photo(self, table, cols, esfact=1.0):
+        """columns: dict of photo, pmid, pdel, temp, colo3, lwc, clouds
+        [nlev, ncol] and zen_angle, srf_alb [ncol].  Returns the new photo
+        [nlev, ncol]."""
+        self._pht_sigs()
+        nlev, ncol = np.shape(cols["photo"])
+        f = lambda k: np.ascontiguousarray(cols[k], dtype=np.float64)
+        ph = np.array(cols["photo"], dtype=np.float64)
+        self.lib.orc_table_photo_domain(ctypes.byref(table), ncol, nlev, _p(ph),
+                                        *[_p(f(k)) for k in ("pmid", "pdel", "temp", "colo3",
+                                                             "zen_angle", "srf_alb", "lwc",
+                                                             "clouds")], esfact)
+        return ph
 
     # ---- stratospheric ozone and invariants (lin_strat_chem.hpp, mo_setinv.hpp) ----
     def _lnz_sigs(self):

@@ -81,7 +81,7 @@ def test_struct_layouts_match_c(tmp_path):
 
 def test_abi_version_and_default_config():
     lib = _abi.load()
-    assert lib.mam4x_abi_version() == _abi.ABI_VERSION == 9
+    assert lib.mam4x_abi_version() == _abi.ABI_VERSION == 10
     cfg = _abi.Config()
     assert lib.mam4x_default_config(ctypes.byref(cfg)) == _abi.OK
     assert (cfg.do_cond, cfg.do_rename, cfg.do_newnuc, cfg.do_coag) == (1, 1, 1, 1)
