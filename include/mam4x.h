@@ -641,12 +641,18 @@ typedef struct mam4x_photo_args {
   const double *surface_albedo; /* per column [ncol]                        */
   double esfact;                /* earth-sun distance factor               */
   double *photo;                /* rates [1/s], in and out                  */
-  /* device scratch for the effective albedo and cloud multiplier: two
-   * planes of the domain's layout and ldc, one after the other (2 x nlev x
-   * ldc doubles for LAYOUT_LEV_COL, 2 x ncol x ldc for LAYOUT_COL_LEV) --
-   * the reference's PhotoTableWorkArrays */
+  /* device scratch of mam4x_photo_work_len doubles (the reference's
+   * PhotoTableWorkArrays): two planes of the domain's layout and ldc, one
+   * after the other, for the effective albedo and cloud multiplier */
   double *work;
 } mam4x_photo_args;
+
+/* doubles of work a call needs (get_photo_table_work_len,
+ * mo_photo.hpp:116-122): 2 x slow x ld, with slow the slower extent of the
+ * layout and ld = ldc (0: the fastest extent); -1 for invalid arguments.
+ * No device work. */
+long long mam4x_photo_work_len(const mam4x_photo_table *table, int ncol, int nlev, int ldc,
+                               int layout);
 
 /* table_photo over a domain: a column kernel (one lane per column) forms
  * the albedos and multipliers into work, then a level kernel (one lane per

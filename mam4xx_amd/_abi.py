@@ -350,6 +350,8 @@ SIGNATURES = {
     "mam4x_compute_linoz": (ctypes.c_int, [_vp, ctypes.c_double, ctypes.POINTER(LinozArgs), _vp]),
     "mam4x_compute_photolysis": (ctypes.c_int, [_vp, ctypes.POINTER(PhotoTable),
                                                 ctypes.POINTER(PhotoArgs), _vp]),
+    "mam4x_photo_work_len": (ctypes.c_longlong, [ctypes.POINTER(PhotoTable), ctypes.c_int,
+                                                 ctypes.c_int, ctypes.c_int, ctypes.c_int]),
     "mam4x_compute_washout": (ctypes.c_int, [_vp, ctypes.c_double, ctypes.POINTER(SethetArgs),
                                              _vp]),
     "mam4x_compute_forcing": (ctypes.c_int, [_vp, ctypes.POINTER(SetextArgs), _vp]),

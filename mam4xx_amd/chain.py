@@ -331,8 +331,8 @@ class Chain:
         np_xs], the axes) and optionally pht_alias_mult, lng_index; planes:
         dict of pressure, pdel, temperature, o3_column, liquid_water,
         cloud_fraction tensors; zenith_angle, surface_albedo [ncol]; photo:
-        plane updated in place; work: scratch of 2 planes (allocated when
-        None).  live: see _extents."""
+        plane updated in place; work: scratch of mam4x_photo_work_len doubles
+        (allocated when None; returned for reuse).  live: see _extents."""
         tb = _abi.PhotoTable()
         for k in _abi.PHOTO_TABLE_ARRAYS:
             v = table[k]
@@ -355,11 +355,11 @@ class Chain:
             setattr(a, name, v.data_ptr())
         a.esfact = float(esfact)
         a.photo = photo.data_ptr()
+        need = self.lib.mam4x_photo_work_len(ctypes.byref(tb), a.ncol, a.nlev, a.ldc, layout)
+        assert need >= 0
         if work is None:
-            work = torch.empty((2,) + tuple(photo.shape), dtype=torch.float64,
-                               device=photo.device)
-        assert work.dtype == torch.float64 and work.is_contiguous() and \
-            work.numel() >= 2 * photo.numel()
+            work = torch.empty(need, dtype=torch.float64, device=photo.device)
+        assert work.dtype == torch.float64 and work.is_contiguous() and work.numel() >= need
         a.work = work.data_ptr()
         s = stream if stream is not None else torch.cuda.current_stream()
         rc = self.lib.mam4x_compute_photolysis(self.h, ctypes.byref(tb), ctypes.byref(a),

@@ -6,10 +6,12 @@
 //     walks: the top-down walk forms the cloud above a level and its
 //     multiplier, the bottom-up walk the cloud below it and its albedo, with
 //     the reference's serial running sums;
-//   * the level pass (one lane per cell): interpolate_rsf (mo_photo.hpp:
-//     498-645, find_index :428-450, calc_sum_wght :452-496) and jlong
-//     (:647-796) for the one reaction the rates feed (lng_indexer), then
-//     photo = cld_mult esfact (photo + pht_alias_mult j).
+//   * the level pass: interpolate_rsf (mo_photo.hpp:498-645, find_index
+//     :428-450, calc_sum_wght :452-496) and jlong (:647-796) for the one
+//     reaction the rates feed (lng_indexer), then photo = cld_mult esfact
+//     (photo + pht_alias_mult j): a cell's position in the tables
+//     (level_pos), then the sum over wavelengths of the terms wn_term, in
+//     wavelength order.
 // Tables row-major: rsf_tab [nw][nump][numsza][numcolo3][numalb], xsqy
 // [numj][nw][nt][np_xs].  FP64; every sum in the reference's order.
 #pragma once
@@ -97,6 +99,57 @@ __host__ __device__ __forceinline__ double coschi_of(double zen) {
 #endif
 }
 
+// One layer's values the walks read: its cloud optical depth, the depth
+// weighted by its cloud fraction, and fac1 times the fraction, fac1 the
+// thick-cloud term of the layer's own multiplier (mo_photo.hpp:196-206,
+// :398-411).
+struct Layer {
+  double tau, ctau, f1c;
+};
+__host__ __device__ __forceinline__ Layer layer(double coschi, double cld, double lwc,
+                                                double delp) {
+  Layer l;
+  l.tau = layer_tau(cld, lwc, delp);
+  l.ctau = cld * l.tau;
+  double fac1 = 0.0;
+  if (layer_lwp(cld, lwc, delp) * kLwpToTau >= kTauMin) fac1 = 1.4 * coschi - 1.0;
+  l.f1c = fac1 * cld;
+  return l;
+}
+
+// Running sums over the layers on one side of a level (above from the top,
+// below from the bottom) and the last normalised cloud fraction.
+struct Side {
+  double tau, ctau, prev;
+};
+// the cloud on that side of an interior level: above_tau / above_cld (or
+// below_*) after the normalisation, the copy of the previous fraction where
+// the depth is 0, the cloudy part's depth and the 5.0 threshold
+__host__ __device__ __forceinline__ void side_cloud(Side &s, double &tau, double &cld) {
+  tau = s.tau;
+  cld = s.ctau;
+  cld = tau != 0.0 ? cld / tau : s.prev;
+  s.prev = cld;
+  if (cld != 0.0) tau /= cld;
+  if (tau < kTauMin) cld = 0.0;
+}
+__host__ __device__ __forceinline__ void side_add(Side &s, double tau, double ctau) {
+  s.tau = tau + s.tau;
+  s.ctau = ctau + s.ctau;
+}
+// the cloud multiplier from the cloud above, the effective albedo from the
+// cloud below
+__host__ __device__ __forceinline__ double multiplier(double coschi, double f1c, double a_tau,
+                                                      double a_cld) {
+  const double above_tra = kC1 / (kC2 + a_tau);
+  const double fac2 = kmin(0.0, 1.6 * coschi * above_tra - 1.0);
+  return kmax(.05, 1.0 + f1c + fac2 * a_cld);
+}
+__host__ __device__ __forceinline__ double albedo(double srf_alb, double b_tau, double b_cld) {
+  const double below_tra = kC1 / (kC2 + b_tau);
+  return srf_alb + b_cld * (1.0 - below_tra) * (1.0 - srf_alb);
+}
+
 // cloud_mod for one column whose level k sits at offset k * ks: the
 // multiplier (top-down walk) into cld_mult, the effective albedo (bottom-up
 // walk) into eff_alb, both at the same offsets.
@@ -105,52 +158,23 @@ __host__ __device__ __forceinline__ void cloud_column(int nlev, long ks, double 
                                                       const double *delp, double srf_alb,
                                                       double *eff_alb, double *cld_mult) {
   const double coschi = coschi_of(zen);
-  // top down: the cloud above level k (above_tau / above_cld of the
-  // reference, after its normalisation and the optical-depth threshold)
-  double sum_tau = 0.0, sum_ctau = 0.0, prev_cld = 0.0;
+  Side s{0.0, 0.0, 0.0};
   for (int k = 0; k < nlev; ++k) {
     const long i = k * ks;
     double a_tau = 0.0, a_cld = 0.0;
-    if (k > 0) {
-      a_tau = sum_tau;
-      a_cld = sum_ctau;
-      a_cld = a_tau != 0.0 ? a_cld / a_tau : prev_cld;
-      prev_cld = a_cld;
-      if (a_cld != 0.0) a_tau /= a_cld;
-      if (a_tau < kTauMin) a_cld = 0.0;
-    }
-    const double cld = clouds[i], w = lwc[i], dp = delp[i];
-    const double d_tau = layer_tau(cld, w, dp);
-    const double del_lwp = layer_lwp(cld, w, dp);
-    double fac1 = 0.0;
-    if (del_lwp * kLwpToTau >= kTauMin) fac1 = 1.4 * coschi - 1.0;
-    const double above_tra = kC1 / (kC2 + a_tau);
-    const double fac2 = kmin(0.0, 1.6 * coschi * above_tra - 1.0);
-    cld_mult[i] = kmax(.05, 1.0 + fac1 * cld + fac2 * a_cld);
-    sum_tau = d_tau + sum_tau;
-    sum_ctau = cld * d_tau + sum_ctau;
+    if (k > 0) side_cloud(s, a_tau, a_cld);
+    const Layer l = layer(coschi, clouds[i], lwc[i], delp[i]);
+    cld_mult[i] = multiplier(coschi, l.f1c, a_tau, a_cld);
+    side_add(s, l.tau, l.ctau);
   }
-  // bottom up: the cloud below level k
-  sum_tau = 0.0;
-  sum_ctau = 0.0;
-  prev_cld = 0.0;
+  s = Side{0.0, 0.0, 0.0};
   for (int k = nlev - 1; k >= 0; --k) {
     const long i = k * ks;
     double b_tau = 0.0, b_cld = 0.0;
-    if (k < nlev - 1) {
-      b_tau = sum_tau;
-      b_cld = sum_ctau;
-      b_cld = b_tau != 0.0 ? b_cld / b_tau : prev_cld;
-      prev_cld = b_cld;
-      if (b_cld != 0.0) b_tau /= b_cld;
-      if (b_tau < kTauMin) b_cld = 0.0;
-    }
-    const double below_tra = kC1 / (kC2 + b_tau);
-    eff_alb[i] = srf_alb + b_cld * (1.0 - below_tra) * (1.0 - srf_alb);
-    const double cld = clouds[i];
-    const double d_tau = layer_tau(cld, lwc[i], delp[i]);
-    sum_tau = d_tau + sum_tau;
-    sum_ctau = cld * d_tau + sum_ctau;
+    if (k < nlev - 1) side_cloud(s, b_tau, b_cld);
+    eff_alb[i] = albedo(srf_alb, b_tau, b_cld);
+    const Layer l = layer(coschi, clouds[i], lwc[i], delp[i]);
+    side_add(s, l.tau, l.ctau);
   }
 }
 
@@ -190,92 +214,132 @@ __host__ __device__ __forceinline__ Corners corners(double d0, double wrk0, doub
   return c;
 }
 
-// r: the rsf_tab entry (wn, iz, is, iv, ial); sv / ss: the o3-ratio and
-// zenith-angle strides
-__host__ __device__ __forceinline__ double corner_sum(const Corners &c, const double *r, long sv,
-                                                      long ss) {
-  return c.w000 * r[0] + c.w001 * r[1] + c.w010 * r[sv] + c.w011 * r[sv + 1] +
-         c.w100 * r[ss] + c.w101 * r[ss + 1] + c.w110 * r[ss + sv] +
-         c.w111 * r[ss + sv + 1];
-}
-
-// photolysis rate of reaction t.ind at one level: sza position s, effective
-// albedo alb_in, pressure p_in [hPa], temperature t_in [K], o3 column
-// colo3_in (interpolate_rsf + jlong)
-__host__ __device__ __forceinline__ double level_rate(const Table &t, const SzaPos &s,
-                                                      double alb_in, double p_in, double t_in,
-                                                      double colo3_in) {
-  const int ial = find_index(t.alb, t.numalb, alb_in, 0);
-  int pind;
-  double wght1;
+// Where one level sits in the tables: everything of interpolate_rsf and
+// jlong but the sum over wavelengths.
+struct LevelPos {
+  int pind, is, ial, ratl, ratu;  // rsf_tab indices: layers pind, pind - 1
+  int t_index, pcase, pndx;       // xsqy row; pressure case 0 / 1 / 2 and layer
+  double dels0, wrk0, d1l, d1u, d2, wght1, delp;
+};
+__host__ __device__ __forceinline__ LevelPos level_pos(const Table &t, const SzaPos &s,
+                                                       double alb_in, double p_in, double t_in,
+                                                       double colo3_in) {
+  LevelPos q;
+  q.is = s.is;
+  q.dels0 = s.dels0;
+  q.wrk0 = s.wrk0;
+  q.ial = find_index(t.alb, t.numalb, alb_in, 0);
   if (p_in > t.press[0]) {
-    pind = 1;
-    wght1 = 1.0;
+    q.pind = 1;
+    q.wght1 = 1.0;
   } else if (p_in <= t.press[t.nump - 1]) {
-    pind = t.nump - 1;
-    wght1 = 0.0;
+    q.pind = t.nump - 1;
+    q.wght1 = 0.0;
   } else {
     int iz;
     for (iz = 1; iz < t.nump; ++iz)
       if (t.press[iz] < p_in) break;
     iz = iz < t.nump - 1 ? iz : t.nump - 1;
-    pind = iz > 1 ? iz : 1;
-    wght1 = bound01((p_in - t.press[pind]) * t.del_p[pind - 1]);
+    q.pind = iz > 1 ? iz : 1;
+    q.wght1 = bound01((p_in - t.press[q.pind]) * t.del_p[q.pind - 1]);
   }
-  const double v3ratu = colo3_in / t.colo3[pind - 1];
-  const int ratindu = find_index(t.o3rat, t.numcolo3, v3ratu, 0);
+  const double v3ratu = colo3_in / t.colo3[q.pind - 1];
+  q.ratu = find_index(t.o3rat, t.numcolo3, v3ratu, 0);
   double v3ratl;
-  int ratindl;
-  if (t.colo3[pind] != 0.0) {
-    v3ratl = colo3_in / t.colo3[pind];
-    ratindl = find_index(t.o3rat, t.numcolo3, v3ratl, 0);
+  if (t.colo3[q.pind] != 0.0) {
+    v3ratl = colo3_in / t.colo3[q.pind];
+    q.ratl = find_index(t.o3rat, t.numcolo3, v3ratl, 0);
   } else {
-    ratindl = ratindu;
-    v3ratl = t.o3rat[ratindu];
+    q.ratl = q.ratu;
+    v3ratl = t.o3rat[q.ratu];
   }
-  const double d2 = bound01((alb_in - t.alb[ial]) * t.del_alb[ial]);
-  const double d1l = bound01((v3ratl - t.o3rat[ratindl]) * t.del_o3rat[ratindl]);
-  const double d1u = bound01((v3ratu - t.o3rat[ratindu]) * t.del_o3rat[ratindu]);
-  const Corners cl = corners(s.dels0, s.wrk0, d1l, d2);
-  const Corners cu = corners(s.dels0, s.wrk0, d1u, d2);
-  // the cross-section interpolation in pressure and its temperature row
-  const int t_index = (int)(kmin(201.0, kmax(t_in - 148.5, 1.0)) - 1.0);
-  int pndx = 0, pcase;
-  double delp = 0.0;
+  q.d2 = bound01((alb_in - t.alb[q.ial]) * t.del_alb[q.ial]);
+  q.d1l = bound01((v3ratl - t.o3rat[q.ratl]) * t.del_o3rat[q.ratl]);
+  q.d1u = bound01((v3ratu - t.o3rat[q.ratu]) * t.del_o3rat[q.ratu]);
+  // the cross sections' temperature row and pressure interpolation
+  q.t_index = (int)(kmin(201.0, kmax(t_in - 148.5, 1.0)) - 1.0);
+  q.pndx = 0;
+  q.delp = 0.0;
   if (p_in >= t.prs[0]) {
-    pcase = 0;
+    q.pcase = 0;
   } else if (p_in <= t.prs[t.np_xs - 1]) {
-    pcase = 1;
+    q.pcase = 1;
   } else {
-    pcase = 2;
+    q.pcase = 2;
     for (int km = 1; km < t.np_xs; ++km)
       if (p_in >= t.prs[km]) {
-        pndx = km - 1;
-        delp = (t.prs[pndx] - p_in) * t.dprs[pndx];
+        q.pndx = km - 1;
+        q.delp = (t.prs[q.pndx] - p_in) * t.dprs[q.pndx];
         break;
       }
   }
-  const long sv = t.numalb, ss = (long)t.numcolo3 * sv, sz = (long)t.numsza * ss,
-             sw = (long)t.nump * sz;
-  const double *rl = t.rsf_tab + pind * sz + s.is * ss + ratindl * sv + ial;
-  const double *ru = t.rsf_tab + (pind - 1) * sz + s.is * ss + ratindu * sv + ial;
-  const long xw = (long)t.nt * t.np_xs;
-  const double *x = t.xsqy + (long)t.ind * t.nw * xw + (long)t_index * t.np_xs;
-  double suma = 0.0;
-  for (int wn = 0; wn < t.nw; ++wn) {
-    const double psum_l = corner_sum(cl, rl + wn * sw, sv, ss);
-    const double psum_u = corner_sum(cu, ru + wn * sw, sv, ss);
-    const double rsf = (psum_l + wght1 * (psum_u - psum_l)) * t.etfphot[wn];
-    const double *xr = x + wn * xw;
-    double xswk;
-    if (pcase == 0)
-      xswk = xr[0];
-    else if (pcase == 1)
-      xswk = xr[t.np_xs - 1];
-    else
-      xswk = xr[pndx] + delp * (xr[pndx + 1] - xr[pndx]);
-    suma += xswk * rsf;
+  return q;
+}
+
+// Table memory order (mam4x_photo_table): rsf_tab [nw][nump][numsza]
+// [numcolo3][numalb], xsqy [numj][nw][nt][np_xs].
+struct RowMajor {
+  const double *rsf, *xs;  // xs: xsqy of the reaction
+  long s_al, s_v, s_s, s_z, s_w, x_p, x_t, x_w;
+  __host__ __device__ RowMajor(const Table &t) {
+    rsf = t.rsf_tab;
+    s_al = 1;
+    s_v = t.numalb;
+    s_s = (long)t.numcolo3 * s_v;
+    s_z = (long)t.numsza * s_s;
+    s_w = (long)t.nump * s_z;
+    x_p = 1;
+    x_t = t.np_xs;
+    x_w = (long)t.nt * x_t;
+    xs = t.xsqy + (long)t.ind * t.nw * x_w;
   }
+};
+
+template <class Ix>
+__host__ __device__ __forceinline__ double corner_sum(const Corners &c, const Ix &x,
+                                                      const double *r) {
+  return c.w000 * r[0] + c.w001 * r[x.s_al] + c.w010 * r[x.s_v] + c.w011 * r[x.s_v + x.s_al] +
+         c.w100 * r[x.s_s] + c.w101 * r[x.s_s + x.s_al] + c.w110 * r[x.s_s + x.s_v] +
+         c.w111 * r[x.s_s + x.s_v + x.s_al];
+}
+
+// one wavelength's term of jlong's sum: the cross section times the source
+// function interpolated in zenith angle, ozone ratio, albedo and pressure
+template <class Ix>
+__host__ __device__ __forceinline__ double wn_term(const Table &t, const Ix &x,
+                                                   const LevelPos &q, const Corners &cl,
+                                                   const Corners &cu, int wn) {
+  const double *rl = x.rsf + q.pind * x.s_z + q.is * x.s_s + q.ratl * x.s_v + q.ial * x.s_al +
+                     wn * x.s_w;
+  const double *ru = x.rsf + (q.pind - 1) * x.s_z + q.is * x.s_s + q.ratu * x.s_v +
+                     q.ial * x.s_al + wn * x.s_w;
+  const double psum_l = corner_sum(cl, x, rl);
+  const double psum_u = corner_sum(cu, x, ru);
+  const double rsf = (psum_l + q.wght1 * (psum_u - psum_l)) * t.etfphot[wn];
+  const double *xr = x.xs + (long)q.t_index * x.x_t + wn * x.x_w;
+  double xswk;
+  if (q.pcase == 0) {
+    xswk = xr[0];
+  } else if (q.pcase == 1) {
+    xswk = xr[(t.np_xs - 1) * x.x_p];
+  } else {
+    const double x0 = xr[q.pndx * x.x_p], x1 = xr[(q.pndx + 1) * x.x_p];
+    xswk = x0 + q.delp * (x1 - x0);
+  }
+  return xswk * rsf;
+}
+
+// photolysis rate of reaction t.ind at one level, the wavelengths summed in
+// order (interpolate_rsf + jlong), from the tables as given
+__host__ __device__ __forceinline__ double level_rate(const Table &t, const SzaPos &s,
+                                                      double alb_in, double p_in, double t_in,
+                                                      double colo3_in) {
+  const LevelPos q = level_pos(t, s, alb_in, p_in, t_in, colo3_in);
+  const Corners cl = corners(q.dels0, q.wrk0, q.d1l, q.d2);
+  const Corners cu = corners(q.dels0, q.wrk0, q.d1u, q.d2);
+  const RowMajor x(t);
+  double suma = 0.0;
+  for (int wn = 0; wn < t.nw; ++wn) suma += wn_term(t, x, q, cl, cu, wn);
   return suma;
 }
 

@@ -16,7 +16,8 @@ HEADER = os.path.join(ROOT, "include", "mam4x.h")
 
 def declared_functions():
     src = open(HEADER).read()
-    return sorted(set(re.findall(r"^\s*(?:const\s+)?\w+\s*\*?\s*(mam4x_\w+)\s*\(", src, re.M)))
+    return sorted(set(re.findall(r"^\s*(?:const\s+)?(?:long\s+)?\w+\s*\*?\s*(mam4x_\w+)\s*\(",
+                                 src, re.M)))
 
 
 def test_header_declares_expected_entry_points():
@@ -58,6 +59,10 @@ def test_struct_layouts_match_c(tmp_path):
                     ' offsetof(mam4x_sethet_args, het_rates));'
                     'printf("%zu %zu %zu\\n", sizeof(mam4x_setext_args),'
                     ' offsetof(mam4x_setext_args, sectors), offsetof(mam4x_setext_args, extfrc));'
+                    'printf("%zu %zu %zu\\n", sizeof(mam4x_photo_table),'
+                    ' offsetof(mam4x_photo_table, rsf_tab), offsetof(mam4x_photo_table, lng_index));'
+                    'printf("%zu %zu %zu\\n", sizeof(mam4x_photo_args),'
+                    ' offsetof(mam4x_photo_args, esfact), offsetof(mam4x_photo_args, work));'
                     'return 0;}\n')
     exe = tmp_path / "sz"
     subprocess.run(["gcc", "-I", os.path.join(ROOT, "include"), "-o", str(exe), str(prog)],
@@ -75,7 +80,10 @@ def test_struct_layouts_match_c(tmp_path):
           _abi.LinozArgs.o3_lbl.offset, _abi.LinozArgs.o3_sfcsink.offset,
           ctypes.sizeof(_abi.SethetArgs), _abi.SethetArgs.het_rates.offset,
           ctypes.sizeof(_abi.SetextArgs), _abi.SetextArgs.sectors.offset,
-          _abi.SetextArgs.extfrc.offset]
+          _abi.SetextArgs.extfrc.offset,
+          ctypes.sizeof(_abi.PhotoTable), _abi.PhotoTable.rsf_tab.offset,
+          _abi.PhotoTable.lng_index.offset,
+          ctypes.sizeof(_abi.PhotoArgs), _abi.PhotoArgs.esfact.offset, _abi.PhotoArgs.work.offset]
     assert c == py
 
 

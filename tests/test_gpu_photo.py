@@ -73,9 +73,11 @@ def _case(seed, nlev=72, ncol=500, nw=20, numj=2):
     return t, cols
 
 
-@pytest.mark.parametrize("nlev", [72, 128])
+# the blocked column pass with 32 and 16 columns per block; 400 levels take
+# the lane-per-column walk
+@pytest.mark.parametrize("nlev", [72, 128, 400])
 def test_photolysis_matches_oracle(orc, torch_cuda, nlev):
-    t, cols = _case(40 + nlev, nlev=nlev)
+    t, cols = _case(40 + nlev, nlev=nlev, ncol=500 if nlev < 400 else 120)
     g = _gpu(torch_cuda, t, cols, pht_alias_mult=0.9, lng_index=1)
     ref = orc.table_photo(P.photo_table(t, pht_alias_mult=0.9, lng_index=1), cols, 1.02)
     assert np.isfinite(g).all()
