@@ -830,8 +830,8 @@ def bench_photo(args):
            "call_ms": ms,
            "roofline": {"bound": "valu", "achieved": achieved, "peak": FP64_PEAK_TFLOPS,
                         "unit": "TFLOP/s", "frac": achieved / FP64_PEAK_TFLOPS,
-                        "traffic": _traffic("photo_level_kernel_%dx%d.txt" % (NCOL, NLEV)),
-                        "kernel": "photo_column_kernel + photo_level_kernel",
+                        "traffic": _traffic("photolysis_%dx%d.txt" % (NCOL, NLEV)),
+                        "kernel": "photo_column_block_kernel + photo_level_kernel",
                         "algorithmic_gflop_per_call": flops / 1e9,
                         "hbm_algorithmic_gbs": ncell_day * bytes_cell / (ms * 1e-3) / 1e9,
                         "algorithmic_bytes_per_cell": bytes_cell}}
