@@ -25,9 +25,13 @@ SO = os.path.join(BUILD, "libcellhost.so")
 @pytest.fixture(scope="module")
 def cellhost():
     os.makedirs(BUILD, exist_ok=True)
+    # built under a per-process name and renamed into place, so concurrent
+    # test processes never overwrite a library another one has loaded
+    tmp = SO + ".%d" % os.getpid()
     subprocess.run(["g++", "-O2", "-std=c++17", "-fPIC", "-shared", "-ffp-contract=off",
-                    "-o", SO, os.path.join(HERE, "cpp", "cell_host.cpp"),
+                    "-o", tmp, os.path.join(HERE, "cpp", "cell_host.cpp"),
                     os.path.join(ROOT, "mam4xx_amd", "csrc", "chain_consts.cpp")], check=True)
+    os.replace(tmp, SO)
     lib = ctypes.CDLL(SO)
     lib.cell_host_run.restype = ctypes.c_long
     return lib

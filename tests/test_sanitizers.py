@@ -36,15 +36,20 @@ def driver():
     if not _have_asan():
         pytest.skip("gcc without libasan")
     os.makedirs(BUILD, exist_ok=True)
-    obj = os.path.join(BUILD, "mam4_oracle_san.o")
+    # per-process intermediate names, then an atomic rename: concurrent test
+    # processes never write a file another one is running
+    tag = ".%d" % os.getpid()
+    obj = os.path.join(BUILD, "mam4_oracle_san%s.o" % tag)
     subprocess.run(["gcc", "-c", "-O1", "-g", "-std=gnu11", "-fPIC", "-ffp-contract=off",
                     "-Wno-unknown-pragmas"] + SAN +
                    ["-o", obj, os.path.join(ROOT, "oracle", "mam4_oracle.c")], check=True)
     subprocess.run(["g++", "-O1", "-g", "-std=c++17", "-ffp-contract=off",
                     "-Wno-unknown-pragmas"] + SAN +
-                   ["-o", EXE, os.path.join(HERE, "cpp", "sanitize_driver.cpp"),
+                   ["-o", EXE + tag, os.path.join(HERE, "cpp", "sanitize_driver.cpp"),
                     os.path.join(ROOT, "mam4xx_amd", "csrc", "chain_consts.cpp"), obj, "-lm"],
                    check=True)
+    os.remove(obj)
+    os.replace(EXE + tag, EXE)
     return EXE
 
 
