@@ -942,6 +942,92 @@ def bench_washout(args):
     print(json.dumps(res), flush=True)
 
 
+def bench_budgets(args):
+    """Column budgets of the 21,600 x 72 step (mam4x_outputs.column_budgets;
+    --workload budgets), in both layouts.  The budget kernel runs after the
+    step's finish kernel, so its time is the step's event time less the
+    library's phase times (set-up, SOA, finish), recorded on the same
+    stream; the same steps without budgets are timed beside it.  HBM
+    roofline of the compulsory traffic: 52 tendency planes and dp read, 54
+    values per column written.  Every step starts from the seeded state."""
+    import numpy as np
+    import torch
+    from mam4xx_amd import _abi
+    from mam4xx_amd import layout as L
+    from mam4xx_amd import scenes
+    from mam4xx_amd.chain import Chain, scene_to_device
+    scene = scenes.generate(NCOL, NLEV, seed=0)
+    nb = L.NSTATE + 2
+    legs = {}
+    for name, layout in (("lev_col", _abi.LAYOUT_LEV_COL), ("col_lev", _abi.LAYOUT_COL_LEV)):
+        if layout == _abi.LAYOUT_COL_LEV:
+            sc = {k: (np.ascontiguousarray(np.swapaxes(v, -1, -2)) if v.ndim >= 2 else v)
+                  for k, v in scene.items()}
+        else:
+            sc = scene
+        d = scene_to_device(sc)
+        tend = torch.empty_like(d["state"])
+        bud = torch.empty((nb, NCOL), dtype=torch.float64, device="cuda")
+        ch = Chain(schedule=_abi.SCHED_QUEUE)
+        s = torch.cuda.current_stream()
+
+        def timed(with_budgets):
+            kw = dict(budgets=bud) if with_budgets else {}
+            for _ in range(args.warmup):
+                ch.compute_tendencies(d, DT, tend, layout=layout, **kw)
+            torch.cuda.synchronize()
+            ch.enable_stats(True)
+            ch.last_phase_times()
+            e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+            e0.record(s)
+            for _ in range(args.steps):
+                ch.compute_tendencies(d, DT, tend, layout=layout, **kw)
+            e1.record(s)
+            torch.cuda.synchronize()
+            phases, n = ch.last_phase_times()
+            ch.enable_stats(False)
+            assert n == args.steps
+            step = e0.elapsed_time(e1) / args.steps
+            return step, step - sum(phases.values()) / args.steps
+
+        step_b, budget_ms = timed(True)
+        step_0, rest_0 = timed(False)
+        legs[name] = {"step_ms_with_budgets": step_b, "step_ms_without": step_0,
+                      "budget_kernel_ms": budget_ms, "gap_ms_without": rest_0,
+                      "finite": bool(torch.isfinite(bud).all().item())}
+        if name == "lev_col":
+            tend_h = tend.cpu().numpy()
+            dp_h = scene["hydrostatic_dp"]
+        ch.close()
+    bytes_cell = (L.NSTATE + 1) * 8 + nb * 8.0 / NLEV
+    ms = legs["lev_col"]["budget_kernel_ms"]
+    achieved = NCOL * NLEV * bytes_cell / (ms * 1e-3) / 1e9
+    res = {"metric": "column budgets of the step: columns/sec at 72 lev", "unit": "columns/s",
+           "value": NCOL / (ms * 1e-3), "n_gpus": 1, "higher_is_better": True, "dtype": "f64",
+           "data": "synthetic", "steps": args.steps, "warmup": args.warmup,
+           "config": {"workload": "21600 columns x 72 levels: 54 FP64 column integrals of the "
+                                  "step's tendencies (52 fields, sulfur and organic closures)",
+                      "columns": NCOL, "levels": NLEV, "layouts": legs},
+           "kernel_ms": ms,
+           "roofline": {"bound": "hbm", "kernel": "column_budget_kernel", "achieved": achieved,
+                        "peak": 8000.0, "unit": "GB/s", "frac": achieved / 8000.0,
+                        "traffic": _traffic("column_budget_kernel_%dx%d.txt" % (NCOL, NLEV)),
+                        "algorithmic_bytes_per_cell": bytes_cell}}
+    if not args.no_cpu:
+        # numpy over the same tendencies: 52 weighted level sums per column
+        w = dp_h * (1.0 / 9.80616)
+        best = 1e30
+        for _ in range(5):
+            t0 = time.perf_counter()
+            (tend_h * w[None]).sum(axis=1)
+            best = min(best, time.perf_counter() - t0)
+        res["cpu_baseline"] = {"value": NCOL / best, "unit": "columns/s", "cores": 1,
+                               "kind": "port", "sample": "numpy weighted level sums of the 52 "
+                               "tendency planes of the full scene (no closures), best of 5",
+                               "cpu": _cpu_model()}
+    print(json.dumps(res), flush=True)
+
+
 def bench_valu(args):
     """Scalar VALU rates of the forms the precision policies compile to
     (tests/cpp/valu_rates.hip, built with the library's flags: packed FP32
@@ -1258,14 +1344,14 @@ def main():
                     help="headline: time the FP64 build only")
     ap.add_argument("--workload", choices=["headline", "batch", "large", "water", "wetsize",
                                            "tracers", "setsox", "gaschem", "linoz", "washout",
-                                           "photo", "valu", "adjoint"],
+                                           "photo", "valu", "adjoint", "budgets"],
                     default="headline",
                     help="headline: the metric's 21,600 x 72 step (default); batch, large, "
                          "water (aerosol water uptake), wetsize (calcsize + water uptake), "
                          "tracers (state_q/qqcw <-> state planes), setsox (aqueous sulfur "
                          "chemistry), gaschem (gas-phase chemistry) and linoz (stratospheric "
                          "ozone, with the invariant densities), washout (sethet), photo "
-                         "(table_photo), valu "
+                         "(table_photo), budgets (column budgets of the step), valu "
                          "(scalar FP64 / FP32 VALU rates) and adjoint (tangent and reverse "
                          "sweep against the step) print their own line")
     args = ap.parse_args()
@@ -1303,6 +1389,8 @@ def main():
         return bench_valu(args)
     if args.workload == "adjoint":
         return bench_adjoint(args)
+    if args.workload == "budgets":
+        return bench_budgets(args)
     return bench_headline(args)
 
 if __name__ == "__main__":

@@ -40,7 +40,7 @@
 extern "C" {
 #endif
 
-#define MAM4X_ABI_VERSION 10
+#define MAM4X_ABI_VERSION 11
 #define MAM4X_MAX_NTSUBSTEP 64
 
 /* per-process tendency diagnostics (mam4x_outputs.process_tendencies) */
@@ -51,6 +51,12 @@ extern "C" {
 #define MAM4X_PROC_NUCLEATION 2          /* gas_aero_exchange_nucleation     */
 #define MAM4X_PROC_COAGULATION 3         /* gas_aero_exchange_coagulation    */
 #define MAM4X_PROC_RENAMING_CLOUD_BORNE 4 /* ..._renaming_cloud_borne        */
+
+/* column budgets (mam4x_outputs.column_budgets): entries 0..51 follow the
+ * state fields; then the two closures the microphysics conserves */
+#define MAM4X_NBUDGET 54
+#define MAM4X_BUDGET_SULFUR 52  /* H2SO4 gas + every SO4 species  [mol/m2/s] */
+#define MAM4X_BUDGET_ORGANIC 53 /* SOAG + every SOA species       [mol/m2/s] */
 
 /* status codes */
 #define MAM4X_OK 0
@@ -165,6 +171,10 @@ typedef struct mam4x_inputs {
   const double *pregaschem_h2so4;
   const double *pregaschem_soag;
   const double *precldchem[MAM4X_NSTATE];
+  /* Layer pressure thickness [Pa] (Atmosphere::hydrostatic_dp,
+   * atmosphere.hpp:98).  Read only to form mam4x_outputs.column_budgets;
+   * may be NULL otherwise. */
+  const double *hydrostatic_dp;
 } mam4x_inputs;
 
 typedef struct mam4x_outputs {
@@ -198,6 +208,31 @@ typedef struct mam4x_outputs {
    * int plane in the tendencies' layout and leading dimension; NULL = not
    * formed. */
   int *soa_capped;
+  /* Optional: per column, FP64 column integrals of the step's tendencies
+   * over the levels, sum_k x(k) * dp(k) / g (dp = inputs.hydrostatic_dp,
+   * required when this is set; g = 9.80616 m/s2, mam4_constants.hpp:48).
+   * Entry b of column c at column_budgets[b * ncol + c], MAM4X_NBUDGET
+   * entries:
+   *   b < 52  x = tendency of state field b: kg/m2/s for masses, #/m2/s
+   *           for numbers;
+   *   MAM4X_BUDGET_SULFUR   x = sum over H2SO4 gas and the six SO4 fields
+   *           (interstitial and cloud-borne) of tendency * 1000 / molecular
+   *           weight [g/mol]: mol of sulfur /m2/s;
+   *   MAM4X_BUDGET_ORGANIC  x = the same over SOAG and the SOA fields, in
+   *           the gas-aerosol exchange's SOA basis (mam4_amicphys.hpp:
+   *           131-154: SOAG * 12.011/150, SOA * 0.08, per 12.011 g/mol).
+   * The chain moves sulfur and organics between the gas and the modes
+   * without sources or sinks, so the two closures vanish to rounding for a
+   * pure microphysics step (pregaschem NULL); with gas chemistry upstream
+   * they give its production.  The level sum has a fixed order in both
+   * layouts, so results agree bit for bit across layouts and launches: the
+   * levels k = l, l + 8, l + 16, ... form partial sum l (l = 0..7, fused
+   * multiply-adds in ascending k, weight dp(k) * (1/g)), and the eight
+   * partials are added as ((p0+p4)+(p2+p6))+((p1+p5)+(p3+p7)).  The closure
+   * terms are summed over their fields in state order before the weight.
+   * Single-domain steps (mam4x_compute_tendencies and the allocation-free
+   * variant) in every precision; other entry points reject it. */
+  double *column_budgets;
 } mam4x_outputs;
 
 /* Forward-mode tangent: the direction of the inputs and the derivative of
@@ -256,10 +291,22 @@ int mam4x_set_precision(mam4x_handle h, int precision);
 const char *mam4x_last_error(mam4x_handle h);
 int mam4x_default_config(mam4x_config *out);
 
+/* AeroProcess::init(const ProcessConfig &) (aero_process.hpp:76-80):
+ * re-initializes the handle with a new configuration (NULL = defaults),
+ * validated as by mam4x_create.  Later calls use it; the handle keeps its
+ * workspace.  A handle with live batches (mam4x_batch_create) is refused
+ * with MAM4X_ERR_ARG, since a batch fixes the configuration it was created
+ * with.  On any error the handle keeps its previous configuration.
+ * Host-side: call it between steps, not while one is in flight on another
+ * thread. */
+int mam4x_init(mam4x_handle h, const mam4x_config *config);
+
 /* AeroProcess<Impl>::validate (aero_process.hpp:90-95; checks of
  * GasAerExch::validate, gasaerexch.hpp:96-120): counts cells with T < 0,
- * p < 0, qv < 0 or any negative prognostic.  Synchronizes the stream and
- * writes the count to *violations. */
+ * p < 0, qv < 0 or any negative prognostic.  Reads only the state planes,
+ * temperature, pressure and vapor_mixing_ratio of *in (the reference's
+ * validate takes no diagnostics): the other pointers may be NULL.
+ * Synchronizes the stream and writes the count to *violations. */
 int mam4x_validate(mam4x_handle h, const mam4x_inputs *in, long long *violations,
                    void *stream);
 
@@ -672,7 +719,8 @@ int mam4x_compute_photolysis(mam4x_handle h, const mam4x_photo_table *table,
  *     that captured its run stays valid whatever the handle does later
  *     (larger steps, other batches); it is valid until mam4x_batch_destroy.
  *   mam4x_batch_run: one step of every scene; dt must equal the planned dt.
- * No counterpart in the reference, which steps one column per Kokkos team. */
+ * A run reads the handle's launch settings, so a handle must outlive its
+ * batches.  No counterpart in the reference, which steps one column per Kokkos team. */
 typedef struct mam4x_batch_s *mam4x_batch;
 int mam4x_batch_create(mam4x_handle h, int nscene, const mam4x_inputs *in,
                        const mam4x_outputs *out, double dt, mam4x_batch *batch);

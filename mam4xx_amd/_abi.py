@@ -6,7 +6,7 @@ The structures below must match the C declarations field for field; the
 import ctypes
 import os
 
-ABI_VERSION = 10
+ABI_VERSION = 11
 
 NSTATE = 52
 NMODE = 4
@@ -35,6 +35,9 @@ PACK_STATE = 0
 PACK_TENDENCY = 1
 
 GAS_PCNST = 31
+NBUDGET = 54  # MAM4X_NBUDGET: the 52 state fields, then the closures
+BUDGET_SULFUR = 52
+BUDGET_ORGANIC = 53
 NPROCESS = 5
 NOALLOC_MAX_CELLS = 524288  # MAM4X_NOALLOC_MAX_CELLS
 PROC_CONDENSATION = 0
@@ -89,6 +92,7 @@ class Inputs(ctypes.Structure):
         ("pregaschem_h2so4", _vp),
         ("pregaschem_soag", _vp),
         ("precldchem", _vp * NSTATE),
+        ("hydrostatic_dp", _vp),
     ]
 
 
@@ -140,6 +144,7 @@ class Outputs(ctypes.Structure):
         ("process_tendencies", _vp * NPROCESS),
         ("process_stride", ctypes.c_longlong),
         ("soa_capped", _vp),
+        ("column_budgets", _vp),
     ]
 
 
@@ -321,6 +326,7 @@ SIGNATURES = {
     "mam4x_destroy": (ctypes.c_int, [_vp]),
     "mam4x_get_config": (ctypes.c_int, [_vp, ctypes.POINTER(Config)]),
     "mam4x_set_precision": (ctypes.c_int, [_vp, ctypes.c_int]),
+    "mam4x_init": (ctypes.c_int, [_vp, ctypes.POINTER(Config)]),
     "mam4x_last_error": (ctypes.c_char_p, [_vp]),
     "mam4x_validate": (ctypes.c_int, [_vp, ctypes.POINTER(Inputs),
                                       ctypes.POINTER(ctypes.c_longlong), _vp]),

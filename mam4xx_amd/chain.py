@@ -74,6 +74,13 @@ class Chain:
         self.h = h
         self.config = cfg
 
+    def init(self, config=None, **kw):
+        """Re-initialize the handle with a new configuration (mam4x_init,
+        AeroProcess::init): an _abi.Config, or the defaults updated by kw."""
+        cfg = config or default_config(**kw)
+        self._check(self.lib.mam4x_init(self.h, ctypes.byref(cfg)), "mam4x_init")
+        self.config = cfg
+
     def close(self):
         if getattr(self, "h", None):
             self.lib.mam4x_destroy(self.h)
@@ -91,28 +98,39 @@ class Chain:
             raise ChainError("%s failed (%d): %s" % (what, rc, msg.decode() if msg else ""))
 
     @staticmethod
-    def make_inputs(scene, precldchem=None, pregaschem=None, layout=_abi.LAYOUT_LEV_COL):
+    def make_inputs(scene, precldchem=None, pregaschem=None, layout=_abi.LAYOUT_LEV_COL,
+                    live=None):
         """Build an _abi.Inputs from a dict of CUDA float64 tensors (see
         scenes.generate for the keys): planes [field][nlev][ncol] for
-        LAYOUT_LEV_COL, [field][ncol][nlev] for LAYOUT_COL_LEV."""
+        LAYOUT_LEV_COL, [field][ncol][nlev] for LAYOUT_COL_LEV.  The planes'
+        fastest dimension is the leading dimension; live (default: all of
+        it) is how many of its entries the domain uses."""
         st = scene["state"]
         nf = st.shape[0]
         assert nf == L.NSTATE
         inp = _abi.Inputs()
+        fast = st.shape[2] if live is None else int(live)
+        assert 0 <= fast <= st.shape[2], (fast, st.shape)
         if layout == _abi.LAYOUT_LEV_COL:
-            nlev, ncol = st.shape[1:]
+            nlev, ncol = st.shape[1], fast
         else:
-            ncol, nlev = st.shape[1:]
+            ncol, nlev = st.shape[1], fast
         inp.ncol, inp.nlev, inp.ldc, inp.layout = ncol, nlev, st.shape[2], layout
         for i, p in enumerate(_plane_ptrs(st, L.NSTATE)):
             inp.state[i] = p
+        # a missing key leaves its pointer NULL (the library reports what a
+        # call needs)
         for key, fld in (("temperature", "temperature"), ("pressure", "pressure"),
                          ("height", "height"), ("qv", "vapor_mixing_ratio"),
                          ("cloud_fraction", "cloud_fraction"), ("pblh", "pblh")):
-            t = scene[key]
+            t = scene.get(key)
+            if t is None:
+                continue
             assert t.dtype == torch.float64 and t.is_cuda and t.is_contiguous()
             setattr(inp, fld, t.data_ptr())
         for key in ("dgncur_a", "dgncur_awet", "wetdens"):
+            if scene.get(key) is None:
+                continue
             arr = getattr(inp, key)
             for m, p in enumerate(_plane_ptrs(scene[key], 4)):
                 arr[m] = p
@@ -122,19 +140,29 @@ class Chain:
         if pregaschem is not None:
             pg = _plane_ptrs(pregaschem, 2)
             inp.pregaschem_h2so4, inp.pregaschem_soag = pg[0], pg[1]
+        dp = scene.get("hydrostatic_dp")
+        if dp is not None:
+            assert dp.dtype == torch.float64 and dp.is_cuda and dp.is_contiguous()
+            inp.hydrostatic_dp = dp.data_ptr()
         return inp
 
     def compute_tendencies(self, scene, dt, tend, dgn_i=None, dgn_c=None, state_out=None,
                            precldchem=None, pregaschem=None, t=0.0, stream=None,
                            layout=_abi.LAYOUT_LEV_COL, process=None, noalloc=False,
-                           soa_capped=None):
+                           soa_capped=None, budgets=None, live=None):
         """One step (mam4x_compute_tendencies, or with noalloc the
         allocation-free mam4x_compute_tendencies_noalloc).  process: optional
         per-process tendency diagnostics, [5][31][nlev][ncol] (see
         _process_ptrs).  soa_capped: optional int32 [nlev][ncol], 1 where a
-        SOA loop stopped at its 1000-sub-step cap."""
-        inp = self.make_inputs(scene, precldchem, pregaschem, layout)
+        SOA loop stopped at its 1000-sub-step cap.  budgets: optional float64
+        [NBUDGET][ncol], the column budgets of the step's tendencies (needs
+        scene["hydrostatic_dp"]).  live: see make_inputs."""
+        inp = self.make_inputs(scene, precldchem, pregaschem, layout, live)
         out = _abi.Outputs()
+        if budgets is not None:
+            assert budgets.dtype == torch.float64 and budgets.is_cuda
+            assert budgets.is_contiguous() and budgets.shape == (_abi.NBUDGET, inp.ncol)
+            out.column_budgets = budgets.data_ptr()
         if soa_capped is not None:
             assert soa_capped.dtype == torch.int32 and soa_capped.is_cuda
             assert soa_capped.is_contiguous()

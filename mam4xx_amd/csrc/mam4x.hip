@@ -984,6 +984,157 @@ __global__ void __launch_bounds__(kBlock) validate_kernel(ChainArgs a,
   if ((threadIdx.x & 63) == 0 && bad) atomicAdd(count, bad);
 }
 
+// Column budgets (mam4x_outputs.column_budgets).  Per column and budget,
+// the levels k = l, l + 8, l + 16, ... (l = 0..7) form partial sum l by
+// fused multiply-adds in ascending k, each term weighted by dp(k) * (1/g);
+// the partials are combined as ((p0+p4)+(p2+p6))+((p1+p5)+(p3+p7)).  Both
+// layouts form exactly these operations, so they agree bit for bit:
+//  - LEV_COL (columns contiguous): one lane per column holds the eight
+//    partials; every load is coalesced over 64 columns;
+//  - COL_LEV (levels contiguous): eight lanes per column, lane l walking its
+//    levels (each load covers eight contiguous levels of eight columns), the
+//    partials combined by a butterfly over the eight lanes (xor 4, 2, 1),
+//    which adds the same pairs in the same tree.
+// One wave per (column group, budget), four budgets of a group per block;
+// the linear block index is mapped so that every budget of one column group
+// lands on the same XCD (blocks go round-robin over the eight XCDs), where
+// the group's dp lines are shared through that XCD's L2.
+constexpr int kNBudget = MAM4X_NBUDGET;
+constexpr int kBudgetBlock = 64;  // lanes per budget (one wave)
+constexpr int kBudgetWaves = 4;   // budgets of one column group per block
+constexpr int kBudgetQuads = (kNBudget + kBudgetWaves - 1) / kBudgetWaves;
+constexpr double kInvGravity = 1.0 / 9.80616;  // mam4_constants.hpp:48
+constexpr int kClosureN = 7;  // six SO4 (SOA) fields and the gas
+
+struct BudgetArgs {
+  const double *tend[NSTATE];
+  const double *dp;
+  double *out;
+  int ncol, nlev, ldc, ngroup;
+  // closures (sulfur, organic): their fields in state order and the mol/kg
+  // factor of each
+  int clo_f[2][kClosureN];
+  double clo_c[2][kClosureN];
+};
+
+// the per-level value x(k) of budget b: a tendency, or a closure's sum over
+// its fields in state order
+struct FieldTerm {
+  const double *t;
+  __device__ double operator()(long off) const { return t[off]; }
+};
+struct ClosureTerm {
+  const double *t[kClosureN];
+  double c[kClosureN];
+  __device__ double operator()(long off) const {
+    double v[kClosureN];
+#pragma unroll
+    for (int i = 0; i < kClosureN; ++i) v[i] = t[i][off];
+    double x = 0.0;
+#pragma unroll
+    for (int i = 0; i < kClosureN; ++i) x = fma(v[i], c[i], x);
+    return x;
+  }
+};
+
+// LEV_COL: one lane per column, eight partials; whole chunks of sixteen
+// (then eight) levels issue their loads before the multiply-adds, the last
+// partial chunk is guarded
+template <class Term>
+__device__ double budget_lane(const BudgetArgs &a, const Term &term, int col) {
+  double p[8] = {0.0, 0.0, 0.0, 0.0, 0.0, 0.0, 0.0, 0.0};
+  int k0 = 0;
+  for (; k0 + 16 <= a.nlev; k0 += 16) {
+    double x[16], w[16];
+#pragma unroll
+    for (int l = 0; l < 16; ++l) {
+      const long off = (long)(k0 + l) * a.ldc + col;
+      x[l] = term(off);
+      w[l] = a.dp[off];
+    }
+#pragma unroll
+    for (int l = 0; l < 16; ++l) p[l & 7] = fma(x[l], w[l] * kInvGravity, p[l & 7]);
+  }
+  if (k0 + 8 <= a.nlev) {
+    double x[8], w[8];
+#pragma unroll
+    for (int l = 0; l < 8; ++l) {
+      const long off = (long)(k0 + l) * a.ldc + col;
+      x[l] = term(off);
+      w[l] = a.dp[off];
+    }
+#pragma unroll
+    for (int l = 0; l < 8; ++l) p[l] = fma(x[l], w[l] * kInvGravity, p[l]);
+    k0 += 8;
+  }
+#pragma unroll
+  for (int l = 0; l < 8; ++l)
+    if (k0 + l < a.nlev) {
+      const long off = (long)(k0 + l) * a.ldc + col;
+      p[l] = fma(term(off), a.dp[off] * kInvGravity, p[l]);
+    }
+  return ((p[0] + p[4]) + (p[2] + p[6])) + ((p[1] + p[5]) + (p[3] + p[7]));
+}
+
+// COL_LEV: lane l of a column's eight walks k = l, l + 8, ...; eight levels
+// per pass load together
+template <class Term>
+__device__ double budget_octet(const BudgetArgs &a, const Term &term, int col, int l) {
+  double p = 0.0;
+  const long base = (long)col * a.ldc;
+  int k = l;
+  for (; k + 56 < a.nlev; k += 64) {
+    double x[8], w[8];
+#pragma unroll
+    for (int j = 0; j < 8; ++j) {
+      x[j] = term(base + k + 8 * j);
+      w[j] = a.dp[base + k + 8 * j];
+    }
+#pragma unroll
+    for (int j = 0; j < 8; ++j) p = fma(x[j], w[j] * kInvGravity, p);
+  }
+  for (; k < a.nlev; k += 8) p = fma(term(base + k), a.dp[base + k] * kInvGravity, p);
+  return p;
+}
+
+template <bool COLLEV, class Term>
+__device__ void budget_column(const BudgetArgs &a, const Term &term, int grp, int b) {
+  const int lane = threadIdx.x & (kBudgetBlock - 1);
+  if (!COLLEV) {
+    const int col = grp * kBudgetBlock + lane;
+    if (col < a.ncol) a.out[(long)b * a.ncol + col] = budget_lane(a, term, col);
+  } else {
+    const int col = grp * (kBudgetBlock / 8) + (lane >> 3), l = lane & 7;
+    const bool live = col < a.ncol;
+    double p = live ? budget_octet(a, term, col, l) : 0.0;
+    // every lane takes part in the butterfly
+    p = p + __shfl_xor(p, 4, 64);
+    p = p + __shfl_xor(p, 2, 64);
+    p = p + __shfl_xor(p, 1, 64);
+    if (live && l == 0) a.out[(long)b * a.ncol + col] = p;
+  }
+}
+
+template <bool COLLEV>
+__global__ void __launch_bounds__(kBudgetWaves * kBudgetBlock) column_budget_kernel(BudgetArgs a) {
+  const int xcd = blockIdx.x & 7, i = blockIdx.x >> 3;
+  const int grp = xcd + 8 * (i / kBudgetQuads);
+  const int b = (i % kBudgetQuads) * kBudgetWaves + (int)(threadIdx.x / kBudgetBlock);
+  if (grp >= a.ngroup || b >= kNBudget) return;  // whole waves
+  if (b < NSTATE) {
+    budget_column<COLLEV>(a, FieldTerm{a.tend[b]}, grp, b);
+  } else {
+    const int j = b - NSTATE;
+    ClosureTerm t;
+#pragma unroll
+    for (int q = 0; q < kClosureN; ++q) {
+      t.t[q] = a.tend[a.clo_f[j][q]];
+      t.c[q] = a.clo_c[j][q];
+    }
+    budget_column<COLLEV>(a, t, grp, b);
+  }
+}
+
 // Aerosol water uptake (water_uptake.hpp): one lane per cell.
 struct WaterArgs {
   const double *qi[21];  // interstitial mass planes, state order
@@ -1725,6 +1876,8 @@ struct mam4x_handle_s {
   int *queue = nullptr;
   QueueCtl *ctl = nullptr;
   unsigned soa_grid = 0;  // resident blocks of soa_queue_kernel on the device
+  int ncu = 0, soa_per_cu = 0;  // CUs, and the most SOA blocks one CU holds
+  int nbatch = 0;               // live batches made from this handle
   unsigned log_grid = 0;  // resident blocks of soa_log_kernel on the device
   // the device's allocation-free workspace (`scratch`), NULL if unresolved
   unsigned char *scratch = nullptr;
@@ -1891,6 +2044,26 @@ int mam4x_default_config(mam4x_config *out) {
   return MAM4X_OK;
 }
 
+// the checks mam4x_create and mam4x_init apply to a configuration
+static bool config_ok(const mam4x_config &c) {
+  return (c.precision == MAM4X_PREC_FP64 || c.precision == MAM4X_PREC_MIXED) &&
+         c.schedule >= MAM4X_SCHED_AUTO && c.schedule <= MAM4X_SCHED_QUEUE &&
+         c.soa_blocks_per_cu >= 0 && c.ntsubstep >= 0 && c.ntsubstep <= MAM4X_MAX_NTSUBSTEP &&
+         c.soa_chunk >= 0 && c.soa_chunk <= (1 << 20) && c.soa_short_chunk >= 0 &&
+         c.soa_short_chunk <= (1 << 20) && c.soa_short_bucket >= 0 &&
+         c.soa_short_bucket <= kNBucket &&
+         (c.levels_per_wave == 0 || c.levels_per_wave == 1 || c.levels_per_wave == 2 ||
+          c.levels_per_wave == 4 || c.levels_per_wave == 8);
+}
+
+// resident blocks of the persistent SOA kernel under the handle's cap
+static unsigned soa_grid(mam4x_handle h) {
+  int per_cu = h->soa_per_cu;
+  if (h->cfg.soa_blocks_per_cu > 0 && h->cfg.soa_blocks_per_cu < per_cu)
+    per_cu = h->cfg.soa_blocks_per_cu;
+  return (unsigned)(h->ncu * per_cu);
+}
+
 int mam4x_create(const mam4x_config *config, mam4x_handle *out) {
   if (!out) return MAM4X_ERR_ARG;
   *out = nullptr;
@@ -1900,16 +2073,7 @@ int mam4x_create(const mam4x_config *config, mam4x_handle *out) {
     h->cfg = *config;
   else
     mam4x_default_config(&h->cfg);
-  if ((h->cfg.precision != MAM4X_PREC_FP64 && h->cfg.precision != MAM4X_PREC_MIXED) ||
-      h->cfg.schedule < MAM4X_SCHED_AUTO || h->cfg.schedule > MAM4X_SCHED_QUEUE ||
-      h->cfg.soa_blocks_per_cu < 0 || h->cfg.ntsubstep < 0 ||
-      h->cfg.ntsubstep > MAM4X_MAX_NTSUBSTEP || h->cfg.soa_chunk < 0 ||
-      h->cfg.soa_chunk > (1 << 20) || h->cfg.soa_short_chunk < 0 ||
-      h->cfg.soa_short_chunk > (1 << 20) || h->cfg.soa_short_bucket < 0 ||
-      h->cfg.soa_short_bucket > kNBucket ||
-      (h->cfg.levels_per_wave != 0 && h->cfg.levels_per_wave != 1 &&
-       h->cfg.levels_per_wave != 2 && h->cfg.levels_per_wave != 4 &&
-       h->cfg.levels_per_wave != 8)) {
+  if (!config_ok(h->cfg)) {
     delete h;
     return MAM4X_ERR_ARG;
   }
@@ -1922,10 +2086,9 @@ int mam4x_create(const mam4x_config *config, mam4x_handle *out) {
     delete h;
     return MAM4X_ERR_HIP;
   }
-  if (per_cu < 1) per_cu = 1;
-  if (h->cfg.soa_blocks_per_cu > 0 && h->cfg.soa_blocks_per_cu < per_cu)
-    per_cu = h->cfg.soa_blocks_per_cu;
-  h->soa_grid = (unsigned)(ncu * per_cu);
+  h->ncu = ncu;
+  h->soa_per_cu = per_cu < 1 ? 1 : per_cu;
+  h->soa_grid = soa_grid(h);
   int log_cu = 0;
   if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&log_cu, soa_log_kernel, kBlock, 0) !=
       hipSuccess) {
@@ -1972,6 +2135,22 @@ int mam4x_destroy(mam4x_handle h) {
 int mam4x_get_config(mam4x_handle h, mam4x_config *out) {
   if (!h || !out) return MAM4X_ERR_ARG;
   *out = h->cfg;
+  return MAM4X_OK;
+}
+
+int mam4x_init(mam4x_handle h, const mam4x_config *config) {
+  if (!h) return MAM4X_ERR_ARG;
+  mam4x_config c;
+  if (config)
+    c = *config;
+  else
+    mam4x_default_config(&c);
+  if (!config_ok(c)) return fail(h, MAM4X_ERR_ARG, "invalid configuration");
+  if (h->nbatch > 0)
+    return fail(h, MAM4X_ERR_ARG, "the handle has " + std::to_string(h->nbatch) +
+                " live batch(es); destroy them before re-initializing");
+  h->cfg = c;
+  h->soa_grid = soa_grid(h);
   return MAM4X_OK;
 }
 
@@ -2027,10 +2206,29 @@ int mam4x_last_phase_times(mam4x_handle h, double ms[MAM4X_NPHASE], long long *c
   return MAM4X_OK;
 }
 
+// what mam4x_validate reads: the domain, the state, T, p and qv
+static int check_validate_inputs(mam4x_handle h, const mam4x_inputs *in) {
+  if (!in) return fail(h, MAM4X_ERR_ARG, "inputs is NULL");
+  if (in->ncol < 0 || in->nlev < 0) return fail(h, MAM4X_ERR_ARG, "negative ncol or nlev");
+  if (in->layout != MAM4X_LAYOUT_LEV_COL && in->layout != MAM4X_LAYOUT_COL_LEV)
+    return fail(h, MAM4X_ERR_ARG, "unknown layout");
+  const bool lc = in->layout == MAM4X_LAYOUT_LEV_COL;
+  const int fast = lc ? in->ncol : in->nlev, slow = lc ? in->nlev : in->ncol;
+  const int ld = in->ldc ? in->ldc : fast;
+  if (ld < fast) return fail(h, MAM4X_ERR_ARG, "ldc smaller than the fastest extent");
+  if ((long long)ld * slow > 0x7fffffffLL)
+    return fail(h, MAM4X_ERR_ARG, "domain too large (ldc x slow extent must fit in int32)");
+  for (int f = 0; f < MAM4X_NSTATE; ++f)
+    if (!in->state[f]) return fail(h, MAM4X_ERR_ARG, "state plane " + std::to_string(f) + " is NULL");
+  if (!in->temperature || !in->pressure || !in->vapor_mixing_ratio)
+    return fail(h, MAM4X_ERR_ARG, "temperature, pressure or vapor_mixing_ratio is NULL");
+  return MAM4X_OK;
+}
+
 int mam4x_validate(mam4x_handle h, const mam4x_inputs *in, long long *violations,
                    void *stream) {
   if (!h || !violations) return MAM4X_ERR_ARG;
-  int rc = check_inputs(h, in);
+  int rc = check_validate_inputs(h, in);
   if (rc) return rc;
   ChainArgs a;
   fill_args(h, in, a);
@@ -2112,6 +2310,8 @@ int mam4x_compute_tangent(mam4x_handle h, double t, double dt, const mam4x_input
   if (rc) return rc;
   if (has_proc(a))
     return fail(h, MAM4X_ERR_UNSUPPORTED, "process tendencies are not formed by the tangent step");
+  if (out->column_budgets)
+    return fail(h, MAM4X_ERR_UNSUPPORTED, "column budgets are not formed by the tangent step");
   for (int f = 0; f < NSTATE; ++f) {
     if (!dir->tendencies[f])
       return fail(h, MAM4X_ERR_ARG, "tangent tendency plane " + std::to_string(f) + " is NULL");
@@ -2442,6 +2642,69 @@ int mam4x_compute_wet_size(mam4x_handle h, double dt, const mam4x_inputs *in,
   return MAM4X_OK;
 }
 
+// closures of the column budgets: fields in state order, mol/kg factors
+// (sulfur: 1000 / molecular weight; organic: the gas-aerosol exchange's
+// basis, mam4_amicphys.hpp:131-154)
+// the mode table gives SO4 and SOA to the accumulation, Aitken and coarse
+// modes, interstitial and cloud-borne: six fields and the gas each
+constexpr int closure_fields(int aid) {
+  int n = 1;
+  for (int f = F_Q_I; f < F_H2SO4; ++f) n += mode_spec(field_mode(f), field_spec(f)) == aid;
+  return n;
+}
+static_assert(closure_fields(A_SO4) == kClosureN && closure_fields(A_SOA) == kClosureN,
+              "closure field count");
+
+static void budget_closures(BudgetArgs &b) {
+  std::vector<int> f_s, f_o;
+  std::vector<double> c_s, c_o;
+  for (int f = F_Q_I; f < F_H2SO4; ++f) {
+    const int m = field_mode(f), sp = field_spec(f), aid = mode_spec(m, sp);
+    if (aid == A_SO4) {
+      f_s.push_back(f);
+      c_s.push_back(1000.0 / ADV_MASS[mass_slot(m, sp)]);
+    } else if (aid == A_SOA) {
+      f_o.push_back(f);
+      c_o.push_back(1000.0 * fcvt_aer(A_SOA) / ADV_MASS[mass_slot(m, sp)]);
+    }
+  }
+  f_s.push_back(F_H2SO4);
+  c_s.push_back(1000.0 / 98.078400);
+  f_o.push_back(F_SOAG);
+  c_o.push_back(1000.0 * FCVT_SOAG / 12.011000);
+  for (int i = 0; i < kClosureN; ++i) {
+    b.clo_f[0][i] = f_s[i];
+    b.clo_c[0][i] = c_s[i];
+    b.clo_f[1][i] = f_o[i];
+    b.clo_c[1][i] = c_o[i];
+  }
+}
+
+static int launch_budgets(mam4x_handle h, const mam4x_inputs *in, const mam4x_outputs *out,
+                          hipStream_t s) {
+  if (in->ncol == 0) return MAM4X_OK;
+  BudgetArgs b;
+  std::memset(&b, 0, sizeof(b));
+  for (int f = 0; f < NSTATE; ++f) b.tend[f] = out->tendencies[f];
+  b.dp = in->hydrostatic_dp;
+  b.out = out->column_budgets;
+  b.ncol = in->ncol;
+  b.nlev = in->nlev;
+  const bool collev = in->layout == MAM4X_LAYOUT_COL_LEV;
+  b.ldc = in->ldc ? in->ldc : (collev ? in->nlev : in->ncol);
+  const int per = collev ? kBudgetBlock / 8 : kBudgetBlock;
+  b.ngroup = (in->ncol + per - 1) / per;
+  budget_closures(b);
+  const unsigned nblk = 8u * kBudgetQuads * (unsigned)((b.ngroup + 7) / 8);
+  const dim3 blk(kBudgetWaves * kBudgetBlock);
+  if (collev)
+    hipLaunchKernelGGL(column_budget_kernel<true>, dim3(nblk), blk, 0, s, b);
+  else
+    hipLaunchKernelGGL(column_budget_kernel<false>, dim3(nblk), blk, 0, s, b);
+  HIP_TRY(h, hipGetLastError());
+  return MAM4X_OK;
+}
+
 // One step on one domain.  With noalloc the work-queue workspace and its
 // counters live in the device's `scratch` array and no phase events are
 // recorded: the call allocates nothing and only launches kernels.
@@ -2466,11 +2729,13 @@ static int compute_step(mam4x_handle h, double dt, const mam4x_inputs *in,
   const bool proc = has_proc(a);
   if (proc && mixed)
     return fail(h, MAM4X_ERR_UNSUPPORTED, "process tendencies need FP64 precision");
+  if (out->column_budgets && !in->hydrostatic_dp)
+    return fail(h, MAM4X_ERR_ARG, "column budgets need inputs.hydrostatic_dp");
   a.dt = dt;
   hipStream_t s = (hipStream_t)stream;
   const bool pre = has_pre(in);
   const long ncell = (long)in->ncol * in->nlev;
-  if (ncell == 0) return MAM4X_OK;
+  if (ncell == 0) return out->column_budgets ? launch_budgets(h, in, out, s) : MAM4X_OK;
   const unsigned grid = (unsigned)((ncell + kBlock - 1) / kBlock);
   bool fused = h->cfg.schedule == MAM4X_SCHED_FUSED ||
                (h->cfg.schedule == MAM4X_SCHED_AUTO && ncell < kFusedBelow);
@@ -2545,6 +2810,7 @@ static int compute_step(mam4x_handle h, double dt, const mam4x_inputs *in,
   }
   HIP_TRY(h, hipGetLastError());
   if (ev) HIP_TRY(h, hipEventRecord(ev[3], s));
+  if (out->column_budgets) return launch_budgets(h, in, out, s);
   return MAM4X_OK;
 }
 
@@ -2982,10 +3248,12 @@ struct mam4x_batch_s {
   unsigned char *bucket = nullptr;
   int *wqueue = nullptr;
   QueueCtl *ctl = nullptr;
+  bool counted = false;  // included in h->nbatch
 };
 
 int mam4x_batch_destroy(mam4x_batch b) {
   if (!b) return MAM4X_ERR_ARG;
+  if (b->counted) --b->h->nbatch;
   if (b->d_args) (void)hipFree(b->d_args);
   if (b->d_blk0) (void)hipFree(b->d_blk0);
   if (b->ws) (void)hipFree(b->ws);
@@ -3013,6 +3281,8 @@ int mam4x_batch_create(mam4x_handle h, int nscene, const mam4x_inputs *in,
     if (rc) return rc;
     if (has_proc(args[i]))
       return fail(h, MAM4X_ERR_UNSUPPORTED, "process tendencies need a single-domain call");
+    if (out[i].column_budgets)
+      return fail(h, MAM4X_ERR_UNSUPPORTED, "column budgets need a single-domain call");
     args[i].dt = dt;
     pre = pre || has_pre(&in[i]);
     const long nc = (long)in[i].ncol * in[i].nlev;
@@ -3050,6 +3320,8 @@ int mam4x_batch_create(mam4x_handle h, int nscene, const mam4x_inputs *in,
     mam4x_batch_destroy(b);
     return fail(h, MAM4X_ERR_HIP, "batch upload failed");
   }
+  b->counted = true;
+  ++h->nbatch;
   *batch = b;
   return MAM4X_OK;
 }

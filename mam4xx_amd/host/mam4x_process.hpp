@@ -43,6 +43,7 @@ struct Atmosphere {  // atmosphere.hpp:16-137 (fields the chain reads)
   const double *height = nullptr;              // [m]
   const double *cloud_fraction = nullptr;      // [-]
   const double *planetary_boundary_layer_height = nullptr;  // [m], one per column
+  const double *hydrostatic_dp = nullptr;  // [Pa], read only for column budgets
 };
 
 struct Surface {};  // surface.hpp: the chain reads nothing from it
@@ -109,6 +110,10 @@ class MicrophysicsProcess {
     return c;
   }
 
+  // (re-)initializes the process with the given configuration
+  // (aero_process.hpp:76-80); the handle keeps its device workspace
+  void init(const Config &config) { check(mam4x_init(h_, &config), "mam4x_init"); }
+
   // true when the inputs are physically consistent (aero_process.hpp:90-95):
   // no negative T, p, qv or prognostic.  Synchronizes the stream.
   bool validate(const Domain &d, const Atmosphere &atm, const Surface &,
@@ -129,13 +134,15 @@ class MicrophysicsProcess {
                        stream);
   }
   // the same step, also filling the per-process diagnostic arrays (FP64
-  // precision only)
+  // precision only) and, when column_budgets is set, the MAM4X_NBUDGET x
+  // ncol column budgets of the tendencies (needs atm.hydrostatic_dp)
   void compute_tendencies(const Domain &d, double t, double dt, const Atmosphere &atm,
                           const Surface &, const Prognostics &progs, const Diagnostics &diags,
                           const Tendencies &tends, const MicrophysDiagnosticArrays &diag_arrays,
-                          void *stream = nullptr) const {
+                          void *stream = nullptr, double *column_budgets = nullptr) const {
     mam4x_inputs in = inputs(d, atm, progs, diags);
     mam4x_outputs out = {};
+    out.column_budgets = column_budgets;
     state_table(tends, out.tendencies);
     for (int m = 0; m < num_modes; ++m) {
       out.dgncur_i[m] = diags.dry_geometric_mean_diameter_i[m];
@@ -184,6 +191,7 @@ class MicrophysicsProcess {
     in.vapor_mixing_ratio = atm.vapor_mixing_ratio;
     in.cloud_fraction = atm.cloud_fraction;
     in.pblh = atm.planetary_boundary_layer_height;
+    in.hydrostatic_dp = atm.hydrostatic_dp;
     for (int m = 0; m < num_modes; ++m) {
       in.dgncur_a[m] = diags.dry_geometric_mean_diameter_i[m];
       in.dgncur_awet[m] = diags.wet_geometric_mean_diameter_i[m];

@@ -59,8 +59,11 @@ def generate(ncol, nlev=72, seed=0, ptop=None, clear_frac=0.7,
     """Return a dict of float64 arrays describing one scene.
 
     Keys: ``state`` [NSTATE, nlev, ncol] (kg/kg, #/kg), ``temperature``,
-    ``pressure``, ``height``, ``qv``, ``cloud_fraction`` [nlev, ncol],
-    ``pblh`` [ncol], ``dgncur_a``, ``dgncur_awet``, ``wetdens`` [4, nlev, ncol].
+    ``pressure``, ``height``, ``qv``, ``cloud_fraction``, ``hydrostatic_dp``
+    [nlev, ncol], ``pblh`` [ncol], ``dgncur_a``, ``dgncur_awet``, ``wetdens``
+    [4, nlev, ncol].  Mid levels sit at ptop (ps/ptop)^((k+1/2)/nlev) and the
+    layer interfaces at ptop (ps/ptop)^(k/nlev), whose differences are
+    hydrostatic_dp.
     """
     if ptop is None:
         ptop = 100.0 if nlev <= 72 else 10.0
@@ -69,6 +72,8 @@ def generate(ncol, nlev=72, seed=0, ptop=None, clear_frac=0.7,
     ps = rng.uniform(95000.0, 103000.0, ncol)
     frac = (np.arange(nlev) + 0.5) / nlev
     p = ptop * (ps[None, :] / ptop) ** frac[:, None]
+    pint = ptop * (ps[None, :] / ptop) ** (np.arange(nlev + 1) / nlev)[:, None]
+    dp = np.diff(pint, axis=0)
     tsfc = rng.uniform(250.0, 305.0, ncol)
 
     def temp_of_z(z):
@@ -117,6 +122,7 @@ def generate(ncol, nlev=72, seed=0, ptop=None, clear_frac=0.7,
         "height": z,
         "qv": qv,
         "cloud_fraction": cld,
+        "hydrostatic_dp": dp,
         "pblh": pblh,
         "dgncur_a": dgn,
         "dgncur_awet": dgn.copy(),

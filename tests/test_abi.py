@@ -63,6 +63,9 @@ def test_struct_layouts_match_c(tmp_path):
                     ' offsetof(mam4x_photo_table, rsf_tab), offsetof(mam4x_photo_table, lng_index));'
                     'printf("%zu %zu %zu\\n", sizeof(mam4x_photo_args),'
                     ' offsetof(mam4x_photo_args, esfact), offsetof(mam4x_photo_args, work));'
+                    'printf("%zu %zu %d %d %d\\n", offsetof(mam4x_inputs, hydrostatic_dp),'
+                    ' offsetof(mam4x_outputs, column_budgets), MAM4X_NBUDGET,'
+                    ' MAM4X_BUDGET_SULFUR, MAM4X_BUDGET_ORGANIC);'
                     'return 0;}\n')
     exe = tmp_path / "sz"
     subprocess.run(["gcc", "-I", os.path.join(ROOT, "include"), "-o", str(exe), str(prog)],
@@ -83,13 +86,15 @@ def test_struct_layouts_match_c(tmp_path):
           _abi.SetextArgs.extfrc.offset,
           ctypes.sizeof(_abi.PhotoTable), _abi.PhotoTable.rsf_tab.offset,
           _abi.PhotoTable.lng_index.offset,
-          ctypes.sizeof(_abi.PhotoArgs), _abi.PhotoArgs.esfact.offset, _abi.PhotoArgs.work.offset]
+          ctypes.sizeof(_abi.PhotoArgs), _abi.PhotoArgs.esfact.offset, _abi.PhotoArgs.work.offset,
+          _abi.Inputs.hydrostatic_dp.offset, _abi.Outputs.column_budgets.offset, _abi.NBUDGET,
+          _abi.BUDGET_SULFUR, _abi.BUDGET_ORGANIC]
     assert c == py
 
 
 def test_abi_version_and_default_config():
     lib = _abi.load()
-    assert lib.mam4x_abi_version() == _abi.ABI_VERSION == 10
+    assert lib.mam4x_abi_version() == _abi.ABI_VERSION == 11
     cfg = _abi.Config()
     assert lib.mam4x_default_config(ctypes.byref(cfg)) == _abi.OK
     assert (cfg.do_cond, cfg.do_rename, cfg.do_newnuc, cfg.do_coag) == (1, 1, 1, 1)
@@ -103,6 +108,7 @@ def test_null_handle_is_an_error_not_a_crash():
     lib = _abi.load()
     assert lib.mam4x_destroy(None) == _abi.ERR_ARG
     assert lib.mam4x_set_precision(None, 0) == _abi.ERR_ARG
+    assert lib.mam4x_init(None, None) == _abi.ERR_ARG
     assert lib.mam4x_compute_tendencies(None, 0.0, 1.0, None, None, None) == _abi.ERR_ARG
     assert lib.mam4x_last_error(None) == b"NULL handle"
 

@@ -1,0 +1,115 @@
+"""Column budgets (mam4x_outputs.column_budgets) on the GPU.
+
+* bit for bit the documented operation order (tests/budget_ref.ordered,
+  exact fused multiply-adds) on sampled columns, including the last,
+  partial group of columns in each layout;
+* within 1e-14 of each column's gross of the exact sum (math.fsum) on every
+  column;
+* sulfur and organic closures vanish to 1e-12 of their gross (the chain
+  only moves these between the gas and the modes);
+* the same bits in both layouts, with padded leading dimensions, in both
+  schedules and through the allocation-free entry point; the mixed build's
+  budgets are its own tendencies' budgets;
+* a request without hydrostatic_dp is an error, not a launch."""
+import numpy as np
+import pytest
+
+from mam4xx_amd import layout as L
+from mam4xx_amd import scenes
+
+from . import budget_ref as B
+
+pytestmark = pytest.mark.gpu
+DT = 1800.0
+
+
+@pytest.fixture(scope="module")
+def torch_cuda():
+    import torch
+    if not torch.cuda.is_available():
+        pytest.fail("GPU test run without a GPU")
+    return torch
+
+
+def _step(torch, sc, col_lev=False, pad=0, noalloc=False, **cfg):
+    """(tendencies [52, nlev, ncol], budgets [54, ncol]) of one step."""
+    from mam4xx_amd import _abi
+    from mam4xx_amd.chain import Chain
+    nlev, ncol = sc["temperature"].shape
+    tr = (lambda a: np.ascontiguousarray(np.swapaxes(a, -1, -2))) if col_lev else (lambda a: a)
+
+    def dev(a):
+        a = np.asarray(a, dtype=np.float64)
+        if a.ndim >= 2:
+            a = tr(a)
+        if a.ndim < 2 or not pad:
+            return torch.as_tensor(np.ascontiguousarray(a)).cuda()
+        x = torch.full(a.shape[:-1] + (a.shape[-1] + pad,), float("nan"), dtype=torch.float64,
+                       device="cuda")
+        x[..., :a.shape[-1]] = torch.as_tensor(a)
+        return x
+
+    d = {k: dev(v) for k, v in sc.items()}
+    shape = (L.NSTATE,) + tr(np.empty((nlev, ncol))).shape
+    tend = torch.full(shape[:-1] + (shape[-1] + pad,), float("nan"), dtype=torch.float64,
+                      device="cuda")
+    bud = torch.full((B.NBUDGET, ncol), float("nan"), dtype=torch.float64, device="cuda")
+    ch = Chain(**cfg)
+    ch.compute_tendencies(d, DT, tend, budgets=bud, noalloc=noalloc,
+                          layout=_abi.LAYOUT_COL_LEV if col_lev else _abi.LAYOUT_LEV_COL,
+                          live=(nlev if col_lev else ncol) if pad else None)
+    torch.cuda.synchronize()
+    ch.close()
+    t = tend.cpu().numpy()
+    if pad:
+        assert np.isnan(t[..., -pad:]).all(), "padding written"
+        t = t[..., :-pad]
+    return tr(t), bud.cpu().numpy()
+
+
+def test_budgets_order_exact_sums_and_closures(torch_cuda):
+    from mam4xx_amd import _abi
+    sc = scenes.generate(300, 72, seed=61)  # 300 = 4 x 64 + 44 = 37 x 8 + 4
+    tend, bud = _step(torch_cuda, sc, schedule=_abi.SCHED_QUEUE)
+    assert np.isfinite(bud).all()
+    dp = sc["hydrostatic_dp"]
+    cols = [0, 1, 63, 64, 150, 255, 256, 291, 296, 299]
+    assert np.array_equal(bud[:, cols], B.ordered(tend, dp, cols))
+    ex, gross = B.exact(tend, dp)
+    err = np.abs(bud - ex)
+    assert (err <= 1e-14 * gross).all(), (err / np.maximum(gross, 1e-300)).max()
+    for b in (B.SULFUR, B.ORGANIC):
+        assert (gross[b] > 0).all()
+        assert (np.abs(bud[b]) <= 1e-12 * gross[b]).all(), (np.abs(bud[b]) / gross[b]).max()
+
+
+@pytest.mark.parametrize("nlev", [72, 13])
+def test_budgets_agree_across_layouts_and_entry_points(torch_cuda, nlev):
+    from mam4xx_amd import _abi
+    sc = scenes.generate(77, nlev, seed=62)
+    t0, b0 = _step(torch_cuda, sc, schedule=_abi.SCHED_QUEUE)
+    for kw in (dict(col_lev=True), dict(pad=5), dict(col_lev=True, pad=3),
+               dict(noalloc=True), dict(col_lev=True, noalloc=True)):
+        t1, b1 = _step(torch_cuda, sc, schedule=_abi.SCHED_QUEUE, **kw)
+        assert np.array_equal(t1, t0), kw
+        assert np.array_equal(b1, b0), kw
+    t2, b2 = _step(torch_cuda, sc, schedule=_abi.SCHED_FUSED)
+    assert np.array_equal(t2, t0) and np.array_equal(b2, b0)
+    assert np.array_equal(b0[:, [0, 76]], B.ordered(t0, sc["hydrostatic_dp"], [0, 76]))
+
+
+def test_mixed_budgets_are_its_tendencies_budgets(torch_cuda):
+    from mam4xx_amd import _abi
+    sc = scenes.generate(40, 72, seed=63)
+    tend, bud = _step(torch_cuda, sc, precision=_abi.PREC_MIXED)
+    assert np.array_equal(bud[:, [0, 39]], B.ordered(tend, sc["hydrostatic_dp"], [0, 39]))
+    ex, gross = B.exact(tend, sc["hydrostatic_dp"])
+    assert (np.abs(bud - ex) <= 1e-14 * gross).all()
+
+
+def test_budgets_need_layer_thicknesses(torch_cuda):
+    from mam4xx_amd.chain import ChainError
+    sc = scenes.generate(16, 8, seed=64)
+    del sc["hydrostatic_dp"]
+    with pytest.raises(ChainError, match="hydrostatic_dp"):
+        _step(torch_cuda, sc)

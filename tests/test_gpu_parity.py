@@ -244,4 +244,11 @@ def test_validate_counts_violations(torch_cuda):
     assert ch.validate(scene_to_device(sc)) == 0
     sc["temperature"][0, :3] = -1.0
     sc["state"][5, 2, 10] = -1e-20
-    assert ch.validate(scene_to_device(sc)) == 4
+    d = scene_to_device(sc)
+    assert ch.validate(d) == 4
+    # like the reference's validate, it reads no diagnostics (or height,
+    # cloud fraction, PBL height): those may be absent
+    assert ch.validate({k: d[k] for k in ("state", "temperature", "pressure", "qv")}) == 4
+    from mam4xx_amd.chain import ChainError
+    with pytest.raises(ChainError, match="vapor_mixing_ratio"):
+        ch.validate({k: d[k] for k in ("state", "temperature", "pressure")})
