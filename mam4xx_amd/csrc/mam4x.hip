@@ -2656,28 +2656,22 @@ static_assert(closure_fields(A_SO4) == kClosureN && closure_fields(A_SOA) == kCl
               "closure field count");
 
 static void budget_closures(BudgetArgs &b) {
-  std::vector<int> f_s, f_o;
-  std::vector<double> c_s, c_o;
+  // no allocation: the allocation-free step forms these too
+  int ns = 0, no = 0;
   for (int f = F_Q_I; f < F_H2SO4; ++f) {
     const int m = field_mode(f), sp = field_spec(f), aid = mode_spec(m, sp);
-    if (aid == A_SO4) {
-      f_s.push_back(f);
-      c_s.push_back(1000.0 / ADV_MASS[mass_slot(m, sp)]);
-    } else if (aid == A_SOA) {
-      f_o.push_back(f);
-      c_o.push_back(1000.0 * fcvt_aer(A_SOA) / ADV_MASS[mass_slot(m, sp)]);
+    if (aid == A_SO4 && ns < kClosureN - 1) {
+      b.clo_f[0][ns] = f;
+      b.clo_c[0][ns++] = 1000.0 / ADV_MASS[mass_slot(m, sp)];
+    } else if (aid == A_SOA && no < kClosureN - 1) {
+      b.clo_f[1][no] = f;
+      b.clo_c[1][no++] = 1000.0 * fcvt_aer(A_SOA) / ADV_MASS[mass_slot(m, sp)];
     }
   }
-  f_s.push_back(F_H2SO4);
-  c_s.push_back(1000.0 / 98.078400);
-  f_o.push_back(F_SOAG);
-  c_o.push_back(1000.0 * FCVT_SOAG / 12.011000);
-  for (int i = 0; i < kClosureN; ++i) {
-    b.clo_f[0][i] = f_s[i];
-    b.clo_c[0][i] = c_s[i];
-    b.clo_f[1][i] = f_o[i];
-    b.clo_c[1][i] = c_o[i];
-  }
+  b.clo_f[0][kClosureN - 1] = F_H2SO4;
+  b.clo_c[0][kClosureN - 1] = 1000.0 / 98.078400;
+  b.clo_f[1][kClosureN - 1] = F_SOAG;
+  b.clo_c[1][kClosureN - 1] = 1000.0 * FCVT_SOAG / 12.011000;
 }
 
 static int launch_budgets(mam4x_handle h, const mam4x_inputs *in, const mam4x_outputs *out,
