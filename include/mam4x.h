@@ -222,9 +222,10 @@ typedef struct mam4x_outputs {
    *           the gas-aerosol exchange's SOA basis (mam4_amicphys.hpp:
    *           131-154: SOAG * 12.011/150, SOA * 0.08, per 12.011 g/mol).
    * The chain moves sulfur and organics between the gas and the modes
-   * without sources or sinks, so the two closures vanish to rounding for a
-   * pure microphysics step (pregaschem NULL); with gas chemistry upstream
-   * they give its production.  The level sum has a fixed order in both
+   * without sources or sinks, so the two closures vanish to rounding, also
+   * with the pre-chemistry inputs set (the chemistry's production is already
+   * in the state the tendencies are taken against).  The level sum has a
+   * fixed order in both
    * layouts, so results agree bit for bit across layouts and launches: the
    * levels k = l, l + 8, l + 16, ... form partial sum l (l = 0..7, fused
    * multiply-adds in ascending k, weight dp(k) * (1/g)), and the eight
