@@ -231,8 +231,9 @@ typedef struct mam4x_outputs {
    * multiply-adds in ascending k, weight dp(k) * (1/g)), and the eight
    * partials are added as ((p0+p4)+(p2+p6))+((p1+p5)+(p3+p7)).  The closure
    * terms are summed over their fields in state order before the weight.
-   * Single-domain steps (mam4x_compute_tendencies and the allocation-free
-   * variant) in every precision; other entry points reject it. */
+   * Formed by mam4x_compute_tendencies, the allocation-free variant and
+   * batches (per scene; launches only, so a captured run forms them too), in
+   * every precision; the tangent rejects it. */
   double *column_budgets;
 } mam4x_outputs;
 

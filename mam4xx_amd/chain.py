@@ -489,10 +489,11 @@ class Chain:
         return out
 
     def batch(self, scenes, dt, tends, dgn_i=None, dgn_c=None, state_out=None,
-              layout=_abi.LAYOUT_LEV_COL):
+              layout=_abi.LAYOUT_LEV_COL, budgets=None):
         """Plan one step of many scenes (lists of scene dicts and output
-        tensors, None entries allowed in the optional lists); see Batch."""
-        return Batch(self, scenes, dt, tends, dgn_i, dgn_c, state_out, layout)
+        tensors, None entries allowed in the optional lists; budgets: float64
+        [NBUDGET][ncol] per scene); see Batch."""
+        return Batch(self, scenes, dt, tends, dgn_i, dgn_c, state_out, layout, budgets)
 
     def compute_tangent(self, scene, dt, tend, dtend, dstate=None, dtemperature=None,
                         dpressure=None, dqv=None, dgn_i=None, dgn_c=None, t=0.0, stream=None,
@@ -640,7 +641,7 @@ class Batch:
     bound at creation; run() launches kernels only and can be captured in a
     CUDA graph (torch.cuda.graph) and replayed."""
 
-    def __init__(self, chain, scenes, dt, tends, dgn_i, dgn_c, state_out, layout):
+    def __init__(self, chain, scenes, dt, tends, dgn_i, dgn_c, state_out, layout, budgets=None):
         n = len(scenes)
         pick = lambda lst, i: None if lst is None else lst[i]
         ins = (_abi.Inputs * n)()
@@ -649,7 +650,12 @@ class Batch:
             ins[i] = chain.make_inputs(sc, None, None, layout)
             outs[i] = chain.make_outputs(tends[i], pick(dgn_i, i), pick(dgn_c, i),
                                          pick(state_out, i))
-        self._keep = (scenes, tends, dgn_i, dgn_c, state_out)
+            bud = pick(budgets, i)
+            if bud is not None:
+                assert bud.dtype == torch.float64 and bud.is_cuda and bud.is_contiguous()
+                assert bud.shape == (_abi.NBUDGET, ins[i].ncol), bud.shape
+                outs[i].column_budgets = bud.data_ptr()
+        self._keep = (scenes, tends, dgn_i, dgn_c, state_out, budgets)
         self.chain, self.dt, self.nscene = chain, dt, n
         b = ctypes.c_void_p()
         chain._check(chain.lib.mam4x_batch_create(chain.h, n, ins, outs, dt, ctypes.byref(b)),

@@ -1011,6 +1011,7 @@ struct BudgetArgs {
   const double *dp;
   double *out;
   int ncol, nlev, ldc, ngroup;
+  int collev;  // MAM4X_LAYOUT_COL_LEV
   // closures (sulfur, organic): their fields in state order and the mol/kg
   // factor of each
   int clo_f[2][kClosureN];
@@ -2674,10 +2675,8 @@ static void budget_closures(BudgetArgs &b) {
   b.clo_c[1][kClosureN - 1] = 1000.0 * FCVT_SOAG / 12.011000;
 }
 
-static int launch_budgets(mam4x_handle h, const mam4x_inputs *in, const mam4x_outputs *out,
-                          hipStream_t s) {
-  if (in->ncol == 0) return MAM4X_OK;
-  BudgetArgs b;
+// the column budgets' argument block of one domain (in->ncol > 0)
+static void budget_args(const mam4x_inputs *in, const mam4x_outputs *out, BudgetArgs &b) {
   std::memset(&b, 0, sizeof(b));
   for (int f = 0; f < NSTATE; ++f) b.tend[f] = out->tendencies[f];
   b.dp = in->hydrostatic_dp;
@@ -2688,15 +2687,27 @@ static int launch_budgets(mam4x_handle h, const mam4x_inputs *in, const mam4x_ou
   b.ldc = in->ldc ? in->ldc : (collev ? in->nlev : in->ncol);
   const int per = collev ? kBudgetBlock / 8 : kBudgetBlock;
   b.ngroup = (in->ncol + per - 1) / per;
+  b.collev = collev ? 1 : 0;
   budget_closures(b);
+}
+
+static int launch_budget_args(mam4x_handle h, const BudgetArgs &b, hipStream_t s) {
   const unsigned nblk = 8u * kBudgetQuads * (unsigned)((b.ngroup + 7) / 8);
   const dim3 blk(kBudgetWaves * kBudgetBlock);
-  if (collev)
+  if (b.collev)
     hipLaunchKernelGGL(column_budget_kernel<true>, dim3(nblk), blk, 0, s, b);
   else
     hipLaunchKernelGGL(column_budget_kernel<false>, dim3(nblk), blk, 0, s, b);
   HIP_TRY(h, hipGetLastError());
   return MAM4X_OK;
+}
+
+static int launch_budgets(mam4x_handle h, const mam4x_inputs *in, const mam4x_outputs *out,
+                          hipStream_t s) {
+  if (in->ncol == 0) return MAM4X_OK;
+  BudgetArgs b;
+  budget_args(in, out, b);
+  return launch_budget_args(h, b, s);
 }
 
 // One step on one domain.  With noalloc the work-queue workspace and its
@@ -3243,6 +3254,7 @@ struct mam4x_batch_s {
   int *wqueue = nullptr;
   QueueCtl *ctl = nullptr;
   bool counted = false;  // included in h->nbatch
+  std::vector<BudgetArgs> budgets;  // scenes that form column budgets
 };
 
 int mam4x_batch_destroy(mam4x_batch b) {
@@ -3275,8 +3287,9 @@ int mam4x_batch_create(mam4x_handle h, int nscene, const mam4x_inputs *in,
     if (rc) return rc;
     if (has_proc(args[i]))
       return fail(h, MAM4X_ERR_UNSUPPORTED, "process tendencies need a single-domain call");
-    if (out[i].column_budgets)
-      return fail(h, MAM4X_ERR_UNSUPPORTED, "column budgets need a single-domain call");
+    if (out[i].column_budgets && !in[i].hydrostatic_dp)
+      return fail(h, MAM4X_ERR_ARG, "column budgets need inputs.hydrostatic_dp (scene " +
+                  std::to_string(i) + ")");
     args[i].dt = dt;
     pre = pre || has_pre(&in[i]);
     const long nc = (long)in[i].ncol * in[i].nlev;
@@ -3293,6 +3306,11 @@ int mam4x_batch_create(mam4x_handle h, int nscene, const mam4x_inputs *in,
   b->pre = pre;
   b->nblock = blk0[nscene];
   b->ncell = (long)b->nblock * kBlock;
+  for (int i = 0; i < nscene; ++i)
+    if (out[i].column_budgets && in[i].ncol > 0) {
+      b->budgets.emplace_back();
+      budget_args(&in[i], &out[i], b->budgets.back());
+    }
   b->fused = h->cfg.schedule == MAM4X_SCHED_FUSED ||
              (h->cfg.schedule == MAM4X_SCHED_AUTO && b->ncell < kFusedBelow) ||
              h->cfg.ntsubstep > 1;  // the split phases hold one sub-step
@@ -3325,8 +3343,14 @@ int mam4x_batch_run(mam4x_batch b, double t, double dt, void *stream) {
   if (!b) return MAM4X_ERR_ARG;
   mam4x_handle h = b->h;
   if (dt != b->dt) return fail(h, MAM4X_ERR_ARG, "batch planned for another dt");
-  if (b->ncell == 0) return MAM4X_OK;
   hipStream_t s = (hipStream_t)stream;
+  if (b->ncell == 0) {  // scenes without levels: their budgets are zeros
+    for (const BudgetArgs &ba : b->budgets) {
+      const int rc = launch_budget_args(h, ba, s);
+      if (rc) return rc;
+    }
+    return MAM4X_OK;
+  }
   const bool mixed = h->cfg.precision == MAM4X_PREC_MIXED;
   const SceneBatch src{b->d_args, b->d_blk0, b->nscene};
   Work w;
@@ -3358,6 +3382,10 @@ int mam4x_batch_run(mam4x_batch b, double t, double dt, void *stream) {
   else
     launch_cells<Fp64>(kind, b->pre, b->nblock, s, src, w, multi);
   HIP_TRY(h, hipGetLastError());
+  for (const BudgetArgs &ba : b->budgets) {
+    const int rc = launch_budget_args(h, ba, s);
+    if (rc) return rc;
+  }
   return MAM4X_OK;
 }
 

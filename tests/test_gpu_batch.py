@@ -2,7 +2,7 @@
 
 * one batched launch sequence gives, scene by scene, bitwise the results of
   stepping each scene on its own (same per-cell operations, one shared work
-  queue);
+  queue), column budgets included;
 * a batch run captured in a CUDA graph and replayed over chained steps
   matches an eager batch stepping an identical copy, replay by replay, with
   eager work and synchronisation between the replays;
@@ -44,14 +44,21 @@ def test_batch_equals_scene_by_scene(torch_cuda, sched):
     ds = [scene_to_device(s) for s in scs]
     ch = Chain(schedule=code)
     outs = [_outputs(torch, s) for s in scs]
-    b = ch.batch(ds, DT, [o[0] for o in outs], dgn_i=[o[1] for o in outs])
+    # column budgets for every other scene (None elsewhere)
+    buds = [torch.full((_abi.NBUDGET, s["temperature"].shape[1]), float("nan"),
+                       dtype=torch.float64, device="cuda") if i % 2 == 0 else None
+            for i, s in enumerate(scs)]
+    b = ch.batch(ds, DT, [o[0] for o in outs], dgn_i=[o[1] for o in outs], budgets=buds)
     b.run()
     torch.cuda.synchronize()
-    for d, (tb, gb) in zip(ds, outs):
+    for d, (tb, gb), bb in zip(ds, outs, buds):
         t1, g1 = _outputs(torch, {"temperature": d["temperature"]})
-        ch.compute_tendencies(d, DT, t1, dgn_i=g1)
+        b1 = None if bb is None else torch.empty_like(bb)
+        ch.compute_tendencies(d, DT, t1, dgn_i=g1, budgets=b1)
         torch.cuda.synchronize()
         assert torch.equal(tb, t1) and torch.equal(gb, g1)
+        if bb is not None:
+            assert torch.equal(bb, b1)
     b.close()
     ch.close()
 
@@ -69,10 +76,14 @@ def test_batch_graph_replays_match_eager(torch_cuda):
         copies.append(ds)
     chans = [Chain(schedule=_abi.SCHED_QUEUE), Chain(schedule=_abi.SCHED_QUEUE)]
     batches = []
+    buds = []
     for ch, ds in zip(chans, copies):
         outs = [_outputs(torch, s) for s in scs]
+        bud = [torch.empty((_abi.NBUDGET, s["temperature"].shape[1]), dtype=torch.float64,
+                           device="cuda") for s in scs]
+        buds.append(bud)
         batches.append(ch.batch(ds, DT, [o[0] for o in outs], dgn_i=[d["dgncur_a"] for d in ds],
-                                state_out=[d["state"] for d in ds]))
+                                state_out=[d["state"] for d in ds], budgets=bud))
     graph = torch.cuda.CUDAGraph()
     side = torch.cuda.Stream()
     side.wait_stream(torch.cuda.current_stream())
@@ -84,6 +95,8 @@ def test_batch_graph_replays_match_eager(torch_cuda):
     with torch.cuda.graph(graph):
         batches[0].run()
     for step in range(4):
+        for x in buds[0]:
+            x.fill_(float("nan"))  # each replay writes them anew
         graph.replay()
         batches[1].run()
         torch.cuda.synchronize()
@@ -91,6 +104,8 @@ def test_batch_graph_replays_match_eager(torch_cuda):
         for da, db in zip(copies[0], copies[1]):
             assert torch.equal(da["state"], db["state"]), step
             assert torch.equal(da["dgncur_a"], db["dgncur_a"]), step
+        for xa, xb in zip(buds[0], buds[1]):
+            assert torch.equal(xa, xb), step
     for b in batches:
         b.close()
 
