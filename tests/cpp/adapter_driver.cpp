@@ -51,12 +51,16 @@ bool download(double *plane, const double *dev, int ncol, int nlev) {
 }  // namespace
 
 // Returns 0 on success, 1 on a HIP failure, 2 when validate reports invalid
-// inputs, 3 on an exception (message in err).
+// inputs, 3 on an exception (message in err).  Optional: hdp (layer
+// thicknesses, [nlev][ncol]) with budgets (MAM4X_NBUDGET x ncol out), and
+// do_coag >= 0 to re-initialize the process with that switch
+// (MicrophysicsProcess::init) before the step.
 extern "C" int adapter_run(int ncol, int nlev, double dt, const double *state, const double *T,
                            const double *p, const double *z, const double *qv,
                            const double *cld, const double *pblh, const double *dga,
                            const double *dgw, const double *wd, double *tend, double *dgi,
-                           double *dgc, double *proc, char *err, int errlen) {
+                           double *dgc, double *proc, char *err, int errlen, const double *hdp,
+                           double *budgets, int do_coag) {
   const size_t n = (size_t)ncol * nlev;
   try {
     DevField st[MAM4X_NSTATE], td[MAM4X_NSTATE], met[5], dga_d[4], dgw_d[4], wd_d[4], dgc_d[4];
@@ -126,13 +130,29 @@ extern "C" int adapter_run(int ncol, int nlev, double dt, const double *state, c
       da.gas_aero_exchange_coagulation = blk[3].p;
       da.gas_aero_exchange_renaming_cloud_borne = blk[4].p;
     }
+    DevField dp_d, bud_d;
+    if (hdp) {
+      if (!upload(dp_d, hdp, ncol, nlev)) return 1;
+      atm.hydrostatic_dp = dp_d.p;
+    }
+    if (budgets && hipMalloc(&bud_d.p, (size_t)MAM4X_NBUDGET * ncol * sizeof(double)) != hipSuccess)
+      return 1;
     MicrophysicsProcess process;
+    if (do_coag >= 0) {
+      auto cfg = process.process_config();
+      cfg.do_coag = do_coag;
+      process.init(cfg);
+    }
     if (!process.validate(d, atm, Surface{}, progs, diags)) return 2;
-    if (proc)
-      process.compute_tendencies(d, 0.0, dt, atm, Surface{}, progs, diags, tends, da);
+    if (proc || budgets)
+      process.compute_tendencies(d, 0.0, dt, atm, Surface{}, progs, diags, tends, da, nullptr,
+                                 bud_d.p);
     else
       process.compute_tendencies(d, 0.0, dt, atm, Surface{}, progs, diags, tends);
     if (hipDeviceSynchronize() != hipSuccess) return 1;
+    if (budgets && hipMemcpy(budgets, bud_d.p, (size_t)MAM4X_NBUDGET * ncol * sizeof(double),
+                             hipMemcpyDeviceToHost) != hipSuccess)
+      return 1;
     if (proc)
       for (int k = 0; k < 5; ++k)
         for (int i = 0; i < 31; ++i)

@@ -34,7 +34,7 @@ def hip_runtime_from_torch():
     torch.cuda.init()
 
 
-def run_driver(sc, dt, proc=False):
+def run_driver(sc, dt, proc=False, budgets=False, do_coag=-1):
     if not os.path.exists(DRIVER):
         pytest.fail("%s not built: run __graft_entry__.build()" % DRIVER)
     lib = ctypes.CDLL(DRIVER)
@@ -49,10 +49,16 @@ def run_driver(sc, dt, proc=False):
     dgi = np.empty((4, nlev, ncol))
     dgc = np.empty_like(dgi)
     pr = np.full((5, 31, nlev, ncol), np.nan) if proc else None
+    bud = np.full((54, ncol), np.nan) if budgets else None
+    hdp = np.ascontiguousarray(sc["hydrostatic_dp"]) if budgets else None
     err = ctypes.create_string_buffer(512)
     lib.adapter_run.restype = ctypes.c_int
     rc = lib.adapter_run(ncol, nlev, ctypes.c_double(dt), p(st), *[p(a) for a in arr], p(tend),
-                         p(dgi), p(dgc), None if pr is None else p(pr), err, 512)
+                         p(dgi), p(dgc), None if pr is None else p(pr), err, 512,
+                         None if hdp is None else p(hdp), None if bud is None else p(bud),
+                         int(do_coag))
+    if budgets:
+        return rc, err.value.decode(), tend, bud
     if proc:
         return rc, err.value.decode(), tend, dgi, dgc, pr
     return rc, err.value.decode(), tend, dgi, dgc
@@ -99,3 +105,28 @@ def test_host_interface_fills_diagnostic_arrays(orc):
         orc.set_libm_noise(0)
     _compare(pg, po, env, _scales(sc, to))
     assert np.count_nonzero(pg[:4]) > 1000
+
+
+def test_host_interface_init_and_budgets(orc):
+    """MicrophysicsProcess::init (coagulation switched off on a live process)
+    and the column budgets through the host interface: the tendencies hold
+    the oracle's bar for that configuration, the budgets are the exact
+    column sums of those tendencies and the closures vanish."""
+    from . import budget_ref as B
+    sc = scenes.generate(96, 72, seed=20)
+    dt = 1800.0
+    rc, msg, tg, bud = run_driver(sc, dt, budgets=True, do_coag=0)
+    assert rc == 0, (rc, msg)
+    import oracle.pyoracle as P
+    cfg = P.default_config(do_coag=0)
+    to, env = orc.envelope(sc, dt, config=cfg)
+    _, err = state_rel_err(tg, to, sc["state"], dt)
+    cell_env = env.max(axis=0)
+    bound = np.where(cell_env > 1e-11, np.maximum(1e-10, 10 * cell_env), 1e-10)[None]
+    assert (err <= bound).all(), report(tg, to, sc["state"], dt)
+    with_coag = run_driver(sc, dt)[2]
+    assert not np.array_equal(tg, with_coag)
+    ex, gross = B.exact(tg, sc["hydrostatic_dp"])
+    assert (np.abs(bud - ex) <= 1e-14 * gross).all()
+    for k in (B.SULFUR, B.ORGANIC):
+        assert (np.abs(bud[k]) <= 1e-12 * gross[k]).all()
