@@ -4,8 +4,9 @@ tests/cpp/adapter_driver.cpp stores every field in its own device allocation
 in the EAMxx layout (ncol x nlev, level fastest), fills the AeroProcess-shaped
 containers (aero_config.hpp:74-260, atmosphere.hpp:16-137) species by
 species and calls MicrophysicsProcess::validate and ::compute_tendencies, the
-way a host model would.  Results are held to the bar of test_gpu_parity.py
-against the oracle.
+way a host model would (also after MicrophysicsProcess::init, and with column
+budgets).  Results are held to the bar of test_gpu_parity.py against the
+oracle.
 """
 import ctypes
 import os
