@@ -119,7 +119,8 @@ typedef struct mam4x_config {
   int precision; /* MAM4X_PREC_*                                          */
   int schedule;  /* MAM4X_SCHED_*                                         */
   int soa_blocks_per_cu; /* persistent SOA kernel: resident blocks per CU,
-                            0 = as many as fit (performance only)        */
+                            at most as many as fit; 0 = 3 (performance
+                            only)                                        */
   int ntsubstep; /* condensation sub-steps per step, 1..MAM4X_MAX_NTSUBSTEP
                     (0 is taken as 1).  The reference fixes 1
                     (mam4_amicphys.hpp:1288).  With n > 1 each subarea's

@@ -8,9 +8,10 @@ share of cells with an SOA loop and the spread of their sub-step counts.
 the fastest configuration, searching one knob at a time from the best so far:
 
 1. schedule x soa_blocks_per_cu (CANDIDATES): the fused one-lane-per-cell
-   kernel, or the work-queue pipeline with as many resident blocks per CU of
-   the persistent SOA kernel as fit or 1..3 (fewer waves per SIMD raise each
-   wave's issue share on the long serial chains, more waves hide latency);
+   kernel, or the work-queue pipeline with 1, 2 or 3 (the default) resident
+   blocks per CU of the persistent SOA kernel, or as many as fit (fewer waves
+   per SIMD raise each wave's issue share on the long serial chains, more
+   waves hide latency);
 2. the queue's granularity (queue schedule only): entries per refill of the
    long part (soa_chunk) and of the short part (soa_short_chunk), and the
    bucket that splits them (soa_short_bucket);
@@ -33,9 +34,10 @@ from . import layout as L
 from .chain import Chain, default_config
 
 DEFAULTS = {"soa_chunk": 0, "soa_short_chunk": 0, "soa_short_bucket": 0, "levels_per_wave": 1}
+# soa_blocks_per_cu: 0 is the library's default (3), 8 means as many as fit
 CANDIDATES = [dict(DEFAULTS, schedule=sc, soa_blocks_per_cu=b) for sc, b in (
     (_abi.SCHED_FUSED, 0), (_abi.SCHED_QUEUE, 0), (_abi.SCHED_QUEUE, 1), (_abi.SCHED_QUEUE, 2),
-    (_abi.SCHED_QUEUE, 3))]
+    (_abi.SCHED_QUEUE, 8))]
 # stage 2 (queue schedule) and stage 3: one knob at a time
 QUEUE_KNOBS = [("soa_chunk", (32, 128)), ("soa_short_chunk", (128, 512)),
                ("soa_short_bucket", (8, 12))]

@@ -184,6 +184,7 @@ def test_launch_configurations_are_bitwise_neutral(torch_cuda):
     configs = [dict(tune.DEFAULTS, schedule=_abi.SCHED_QUEUE, soa_blocks_per_cu=0)]
     configs += [dict(configs[0], **{k: v}) for k, vs in tune.QUEUE_KNOBS for v in vs]
     configs += [dict(configs[0], levels_per_wave=v) for v in tune.TILE_KNOB[1]]
+    configs += [dict(configs[0], soa_blocks_per_cu=8)]  # as many as fit
     configs += [dict(configs[0], schedule=_abi.SCHED_FUSED, levels_per_wave=4),
                 dict(configs[0], soa_blocks_per_cu=1, soa_chunk=16, soa_short_chunk=64,
                      soa_short_bucket=4, levels_per_wave=2)]
