@@ -2064,6 +2064,11 @@ static bool config_ok(const mam4x_config &c) {
 // over steps 11-20.
 constexpr int kSoaBlocksDefault = 3;
 
+// Resident blocks per CU of the reverse sweep's SOA log kernel (its replay
+// kernel runs one wave per log wave).  Measured on the 21,600 x 72 sweep of
+// step 1: 2 blocks 12.03 ms per sweep, as many as fit 12.75, 1 block 13.34.
+constexpr int kLogBlocks = 2;
+
 // resident blocks of the persistent SOA kernel under the handle's cap
 static unsigned soa_grid(mam4x_handle h) {
   const int cap = h->cfg.soa_blocks_per_cu > 0 ? h->cfg.soa_blocks_per_cu : kSoaBlocksDefault;
@@ -2102,6 +2107,7 @@ int mam4x_create(const mam4x_config *config, mam4x_handle *out) {
     delete h;
     return MAM4X_ERR_HIP;
   }
+  if (log_cu > kLogBlocks) log_cu = kLogBlocks;
   h->log_grid = (unsigned)(ncu * (log_cu > 0 ? log_cu : 1));
   // only the allocation-free step uses it; that entry point reports a failure
   if (scratch_address(h->device, &h->scratch) != MAM4X_OK) h->scratch = nullptr;
