@@ -236,11 +236,12 @@ __global__ void __launch_bounds__(kBlock) chain_fused_kernel(Src src, Work w) {
   add_stats(w.stats, niter);
 }
 
-// ntsubstep > 1 on the work queue's order: pass 0 runs the queued cells
-// longest first (by the first sub-step's predicted SOA cost), so a wave's
-// lanes run sub-step loops of similar length; pass 1 the cells without a SOA
-// loop.  Every cell's result is independent of the order.
-template <bool PRE, class P>
+// One lane per cell on the work queue's order (ntsubstep > 1, and the
+// tangent): pass 0 runs the queued cells longest first (by the first
+// sub-step's predicted SOA cost), so a wave's lanes run sub-step loops of
+// similar length; pass 1 the cells without a SOA loop.  Every cell's result
+// is independent of the order.
+template <bool PRE, class P, bool MULTI = true>
 __global__ void __launch_bounds__(kBlock) chain_fused_order_kernel(OneDomain src, Work w,
                                                                    int pass) {
   const ChainArgs &a = src.a;
@@ -259,7 +260,7 @@ __global__ void __launch_bounds__(kBlock) chain_fused_order_kernel(OneDomain src
     int icol;
     long off;
     cell_index(a, c, icol, off);
-    niter = (unsigned long long)dev::chain_cell_fused<PRE, P, true>(a, off, icol);
+    niter = (unsigned long long)dev::chain_cell_fused<PRE, P, MULTI>(a, off, icol);
   }
   add_stats(w.stats, niter);
 }
@@ -1815,15 +1816,15 @@ static void launch_cells(int kind, bool pre, unsigned grid, hipStream_t s, const
   }
 }
 
-template <class P>
+template <class P, bool MULTI = true>
 static void launch_order(bool pre, unsigned grid, hipStream_t s, const OneDomain &src,
                          const Work &w, int pass) {
   if (pre)
-    hipLaunchKernelGGL((chain_fused_order_kernel<true, P>), dim3(grid), dim3(kBlock), 0, s, src,
-                       w, pass);
+    hipLaunchKernelGGL((chain_fused_order_kernel<true, P, MULTI>), dim3(grid), dim3(kBlock), 0, s,
+                       src, w, pass);
   else
-    hipLaunchKernelGGL((chain_fused_order_kernel<false, P>), dim3(grid), dim3(kBlock), 0, s, src,
-                       w, pass);
+    hipLaunchKernelGGL((chain_fused_order_kernel<false, P, MULTI>), dim3(grid), dim3(kBlock), 0,
+                       s, src, w, pass);
 }
 
 // Device memory of the allocation-free step (mam4x_compute_tendencies_noalloc):
@@ -2344,7 +2345,30 @@ int mam4x_compute_tangent(mam4x_handle h, double t, double dt, const mam4x_input
   w.qs = queue_shape(h->cfg);
   w.ncell = ncell;
   if (h->stats) w.stats = h->d_stats;
-  launch_fused<Tangent>(has_pre(in), dev::nsub(a.cfg) > 1, grid, s, OneDomain{a}, w);
+  const bool pre = has_pre(in), multi = dev::nsub(a.cfg) > 1;
+  const OneDomain src{a};
+  if (a.cfg.do_cond) {
+    // cells in the work queue's order (the FP64 set-up kernel predicts each
+    // cell's SOA cost): a wave's lanes then run loops of similar length
+    rc = reserve_work(h, ncell, s);
+    if (rc) return rc;
+    w.ws = h->ws;
+    w.bucket = h->bucket;
+    w.queue = h->queue;
+    w.ctl = h->ctl;
+    w.dtfull = dev::dt_sub(a.cfg, dt);
+    hipLaunchKernelGGL(queue_reset_kernel, dim3(1), dim3(64), 0, s, w.ctl);
+    launch_cells<Fp64>(kPre, pre, grid, s, src, w);
+    hipLaunchKernelGGL(queue_scatter_kernel, dim3(grid), dim3(kBlock), 0, s, w);
+    for (int pass = 0; pass < 2; ++pass) {
+      if (multi)
+        launch_order<Tangent, true>(pre, grid, s, src, w, pass);
+      else
+        launch_order<Tangent, false>(pre, grid, s, src, w, pass);
+    }
+  } else {
+    launch_fused<Tangent>(pre, multi, grid, s, src, w);
+  }
   HIP_TRY(h, hipGetLastError());
   return MAM4X_OK;
 }
